@@ -1,0 +1,207 @@
+"""Benchmark of the HIP FDTD hot path on one MI355X.
+
+Workload (the 256^3 FP32 configuration): 256^3 cells, dl = 1 mm, npml = 12;
+εr from 40 random axis-aligned dielectric boxes (sides 8-48 cells, εr in
+[1, 4], numpy seed 2) on vacuum; a 32x32 Jx sheet at z = 128 over
+i, j in [112, 144) with rim taper 0.8 (renormalised), Gaussian-modulated sine
+at f0 = c0/(20 dl), 50 % bandwidth; a Huygens box [32, 223]^3 recording the
+DFT at 8 frequencies; after the run, the near-to-far transform at
+181 θ x 2 φ angles, R = 1 m.  Inputs are synthetic (generated here).
+
+A "step" is one full Yee time step: the fused H + source + E sweep and the
+8-frequency face-DFT accumulation.  ``value`` is Mcell-updates/s over the K
+timed steps.  ``roofline`` is the fused sweep's algorithmic HBM traffic
+(72 B/cell + 8 B per psi entry) divided by its average launch time, taken with
+HIP events on the engine's stream during the timed steps.  ``cpu_baseline``
+times this repository's NumPy restatement of the reference's face-CPML engine
+(one core) on a bounded sample of steps of the same grid.
+
+Usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 20] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "opencl-fdtd-solver_amd"))
+
+C0 = 299_792_458.0
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_epsilon(shape, npml, seed=2, n_boxes=40):
+    """40 random dielectric boxes inside [npml, N - npml)."""
+    rng = np.random.default_rng(seed)
+    eps = np.ones(shape, dtype=np.float32)
+    for _ in range(n_boxes):
+        size = rng.integers(8, 49, size=3)
+        lo = [int(rng.integers(npml, shape[a] - npml - size[a] + 1)) for a in range(3)]
+        er = float(rng.uniform(1.0, 4.0))
+        eps[lo[0] : lo[0] + size[0], lo[1] : lo[1] + size[1], lo[2] : lo[2] + size[2]] = er
+    return eps
+
+
+def gaussian_sine(t, f0, fw):
+    t0 = 5.0 / (math.pi * fw)
+    sig = 1.0 / (math.pi * fw)
+    return math.exp(-0.5 * ((t - t0) / sig) ** 2) * math.sin(2 * math.pi * f0 * t)
+
+
+class Workload:
+    shape = (256, 256, 256)
+    npml = 12
+    dl = 1e-3
+    box = (32, 223)
+    n_freq = 8
+
+    def __init__(self):
+        self.f0 = C0 / (20 * self.dl)
+        self.fw = 0.5 * self.f0
+        self.freqs = np.linspace(self.f0 - self.fw / 2, self.f0 + self.fw / 2, self.n_freq)
+        self.eps = c2_epsilon(self.shape, self.npml)
+        lo, hi = self.box
+        self.center = ((lo + hi) / 2 * self.dl,) * 3
+        self.size = ((hi - lo) * self.dl,) * 3
+
+    def source(self, f):
+        f.add_source_Jx(128, gaussian_sine(f.t, self.f0, self.fw), 112, 144, 112, 144,
+                        rim_taper=True, rim_edge=0.8)  # fmt: skip
+
+    def psi_entries(self):
+        nx, ny, nz = self.shape
+        n = self.npml
+        return 4 * (2 * n * ny * nz + nx * 2 * n * nz + nx * ny * 2 * n)
+
+    def obs_points(self, R=1.0):
+        th = np.deg2rad(np.linspace(0.0, 180.0, 181))
+        pts = []
+        for ph in (0.0, math.pi / 2):
+            pts.append(np.column_stack([R * np.sin(th) * math.cos(ph),
+                                        R * np.sin(th) * math.sin(ph), R * np.cos(th)]))  # fmt: skip
+        return np.concatenate(pts)
+
+
+def run_gpu(args, wl: Workload):
+    import hipfdtd
+
+    sim = hipfdtd.HIPFDTD(wl.shape, wl.dl, npml=wl.npml, dtype=np.float32)
+    sim.set_epsilon(wl.eps)
+    sim.add_source(wl.source)
+    mon = hipfdtd.HIPNear2FarMonitor(sim, wl.center, wl.size, freqs=wl.freqs)
+    assert (mon.ix0, mon.ix1) == wl.box, (mon.ix0, mon.ix1)
+    for _ in range(args.warmup):
+        sim.step()
+    sim.synchronize()
+    sim.kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step()
+    sim.synchronize()
+    t1 = time.perf_counter()
+    kern_ms, launches = sim.kernel_timing_read()
+    sim.kernel_timing(False)
+    print(f"[bench] timed {args.steps} steps in {t1 - t0:.3f} s", file=sys.stderr, flush=True)
+
+    t2 = time.perf_counter()
+    eh = mon.get_farfields(wl.obs_points(), freq_index=wl.n_freq // 2)
+    t3 = time.perf_counter()
+    finite = bool(np.all(np.isfinite(eh)))
+    return dict(wall_s=t1 - t0, kern_ms=kern_ms, launches=launches, n2f_ms=(t3 - t2) * 1e3,
+                finite=finite, device=sim.device_name)  # fmt: skip
+
+
+def run_cpu_baseline(wl: Workload, steps: int):
+    """NumPy face-CPML restatement (one core) on `steps` steps of the same grid."""
+    from oracle import fdtd_oracle as fo
+
+    sim = fo.OracleFDTD(wl.shape, wl.dl, npml=wl.npml)
+    sim.set_epsilon(wl.eps)
+    sim.add_source(wl.source)
+    sim.step()  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    dt = time.perf_counter() - t0
+    cells = float(np.prod(wl.shape))
+    return cells * steps / dt / 1e6, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path "
+              "(x-slab halo exchange) is design-only in this build.", file=sys.stderr)  # fmt: skip
+        sys.exit(2)
+    wl = Workload()
+    res = run_gpu(args, wl)
+    cells = float(np.prod(wl.shape))
+    mcells = cells * args.steps / res["wall_s"] / 1e6
+    ms_step = res["wall_s"] / args.steps * 1e3
+    b_alg = 72.0 * cells + 8.0 * wl.psi_entries()
+    avg_launch_s = res["kern_ms"] / max(res["launches"], 1) / 1e3
+    achieved = b_alg / avg_launch_s / 1e9
+    out = {
+        "metric": "Mcell-updates/s",
+        "value": round(mcells, 1),
+        "unit": "Mcell/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "256^3 FP32 Yee+CPML npml=12, 40 random dielectric boxes, 32x32 Jx "
+                        "sheet, 8-frequency Huygens-box DFT [32,223]^3",
+            "grid": list(wl.shape),
+            "npml": wl.npml,
+            "n2f_angles": "181 theta x 2 phi, R=1 m",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_fused (H + sources + E sweep)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_alg_per_launch": b_alg,
+            "avg_launch_us": round(avg_launch_s * 1e6, 2),
+        },
+        "n2f_ms": round(res["n2f_ms"], 2),
+        "farfield_finite": res["finite"],
+        "device": res["device"],
+    }
+    if not args.no_cpu:
+        cpu_mcells, cpu_s = run_cpu_baseline(wl, args.cpu_steps)
+        out["cpu_baseline"] = {
+            "value": round(cpu_mcells, 3),
+            "unit": "Mcell/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{args.cpu_steps} steps of the same 256^3 workload (NumPy face-CPML "
+                      f"restatement, {cpu_s:.1f} s), host has {os.cpu_count()} CPUs",
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/*
+ * hipfdtd.h — C ABI of the MI355X-native FDTD engine (libhipfdtd.so).
+ *
+ * Plain C types only: opaque handles, pointers and sizes.  Every call returns
+ * an int status (HFDTD_OK = 0, negative on error); hfdtd_last_error() gives a
+ * thread-local message for the last failing call on that thread.
+ *
+ * Each entry point names the reference interface it replaces
+ * (r-bridge/opencl-fdtd-solver, paths relative to the repository root).
+ * Host pointers are borrowed for the duration of the call only and must be
+ * C-contiguous in the engine's dtype unless stated otherwise.  Calls on one
+ * handle are not thread-safe; work is ordered on the handle's HIP stream.
+ */
+#ifndef HIPFDTD_H
+#define HIPFDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFDTD_ABI_VERSION 1
+
+/* status codes */
+#define HFDTD_OK 0
+#define HFDTD_EINVAL (-1)      /* bad argument (maps to ValueError)       */
+#define HFDTD_ENOMEM (-2)      /* device allocation failed (MemoryError)  */
+#define HFDTD_EHIP (-3)        /* HIP runtime error (RuntimeError)        */
+#define HFDTD_EUNSUPPORTED (-4)/* unsupported configuration (ValueError)  */
+#define HFDTD_ENODEV (-5)      /* no usable device (RuntimeError)         */
+
+/* volume ids for read/write: fields Ex..Hz, then ce_x..z, ca_x..z
+ * (engine.py:170-187 buffer set) */
+#define HFDTD_EX 0
+#define HFDTD_EY 1
+#define HFDTD_EZ 2
+#define HFDTD_HX 3
+#define HFDTD_HY 4
+#define HFDTD_HZ 5
+#define HFDTD_CEX 6
+#define HFDTD_CEY 7
+#define HFDTD_CEZ 8
+#define HFDTD_CAX 9
+#define HFDTD_CAY 10
+#define HFDTD_CAZ 11
+
+/* source modes */
+#define HFDTD_SRC_SOFT 0    /* F += amp                 (sources.cl:7-21)  */
+#define HFDTD_SRC_CURRENT 1 /* F += -ce_c * amp * w     (sources.cl:26-49) */
+
+/* N2F accumulator modes */
+#define HFDTD_ACC_REAL 0  /* accumulate in the field dtype (reference default) */
+#define HFDTD_ACC_FP64 1  /* accumulate in double (accum_fp64=True)            */
+#define HFDTD_ACC_KAHAN 2 /* FP32 with compensated (Kahan) summation           */
+
+/* step-path selection for hfdtd_step */
+#define HFDTD_PATH_SPLIT 0 /* update_H, sources, update_E: two sweeps        */
+#define HFDTD_PATH_FUSED 1 /* one fused H+sources+E sweep (ping-pong state)  */
+
+typedef struct hfdtd_engine hfdtd_engine;
+typedef struct hfdtd_n2f hfdtd_n2f;
+
+/* One current / soft source on a box [i0,i1)x[j0,j1)x[k0,k1) of component
+ * comp (0=x,1=y,2=z).  Bounds may extend past the grid (clipped); rim-taper
+ * edge tests use the unclipped bounds, as add_source_Jx does
+ * (engine.py:482-538, sources.cl:26-49).  taper: bit0 i-edges, bit1
+ * j-edges, bit2 k-edges, each multiplying w by rim_edge.  amp is rounded to
+ * the engine dtype before use (engine.py:529 self.real(jx)). */
+typedef struct hfdtd_source {
+  int comp;
+  int mode;
+  int i0, i1, j0, j1, k0, k1;
+  int taper;
+  double rim_edge;
+  double amp;
+} hfdtd_source;
+
+/* ── library / device ─────────────────────────────────────────────────── */
+
+/* thread-local message of the last failed call */
+const char *hfdtd_last_error(void);
+/* HFDTD_ABI_VERSION */
+int hfdtd_version(void);
+/* replaces _default_opencl_runtime device discovery (engine.py:40-82) */
+int hfdtd_device_count(int *count);
+/* replaces device.global_mem_size / device.name (engine.py:337-363) */
+int hfdtd_device_info(int device, uint64_t *total_bytes, uint64_t *free_bytes, char *name,
+                      int name_len);
+
+/* Host-only (no GPU needed): 1-D CFS-CPML profile b, c, kappa of length n in
+ * float (fp64=0) or double (fp64=1); replaces cpml.build_cpml_1d
+ * (cpml.py:84-124).  offset is 0 for E nodes and 0.5 for H nodes. */
+int hfdtd_cpml_profile(int n, int npml, double dl, double dt, double offset, int fp64, void *b,
+                       void *c, void *kappa);
+
+/* ── engine ───────────────────────────────────────────────────────────── */
+
+/* replaces OpenCLFDTD.__init__ (engine.py:99-210): allocates the 12 volume
+ * arrays and the face-stripe psi buffers, fields = 0, ce = dt/eps0, ca = 1,
+ * uploads the CPML profiles, folds dtm_dl, dtm, inv_dl (engine.py:543-544,
+ * 634).  fp64 selects double fields. */
+int hfdtd_create(int nx, int ny, int nz, int npml, double dl, double dt, int fp64, int device,
+                 hfdtd_engine **out);
+/* frees all device memory of the engine and its monitors */
+int hfdtd_destroy(hfdtd_engine *e);
+/* bytes the engine holds on the device */
+int hfdtd_memory_bytes(const hfdtd_engine *e, uint64_t *bytes);
+
+/* replaces set_epsilon (engine.py:212-254): cell-centred eps_r and optional
+ * sigma (NULL = lossless) on the host, in float (src_fp64=0) or double
+ * (src_fp64=1); edge averaging and ce/ca/cb are computed on the device in
+ * FP64 (materials.py:28-89) and stored in the engine dtype. */
+int hfdtd_set_materials(hfdtd_engine *e, const void *eps_r, const void *sigma, int src_fp64);
+
+/* replaces enqueue_copy into a field/coefficient buffer */
+int hfdtd_write_volume(hfdtd_engine *e, int which, const void *host);
+/* replaces _read_field (engine.py:747-751): blocking device->host copy */
+int hfdtd_read_volume(hfdtd_engine *e, int which, void *host);
+/* replaces read_point (engine.py:753-765); index = (i*Ny + j)*Nz + k */
+int hfdtd_read_point(hfdtd_engine *e, int which, int64_t index, double *out);
+/* zero fields and psi, keep coefficients */
+int hfdtd_reset_fields(hfdtd_engine *e);
+
+/* replaces _update_H (engine.py:540-629; yee_update.cl:11-118) */
+int hfdtd_update_H(hfdtd_engine *e);
+/* replaces _update_E (engine.py:631-722; yee_update.cl:120-238) */
+int hfdtd_update_E(hfdtd_engine *e);
+/* replaces add_source_Ex / add_source_Jx (engine.py:448-538) for n sources */
+int hfdtd_add_sources(hfdtd_engine *e, const hfdtd_source *src, int n);
+/* one Yee step H -> sources -> E (engine.py:724-735 minus the callbacks);
+ * path = HFDTD_PATH_SPLIT or HFDTD_PATH_FUSED.  n <= 16 sources. */
+int hfdtd_step(hfdtd_engine *e, const hfdtd_source *src, int n, int path);
+/* queue.finish() */
+int hfdtd_synchronize(hfdtd_engine *e);
+/* Fused-sweep launch shape: rows of j per block (1..16) and planes of i per
+ * block (1..Nx); 0 keeps the current value.  Used by the autotuner. */
+int hfdtd_set_fused_tiling(hfdtd_engine *e, int rows_j, int planes_i);
+
+/* HIP-event timing of the Yee sweep kernels on the engine's stream: while
+ * enabled, every update_H / update_E / fused sweep launch is bracketed by a
+ * pair of events.  enable=1 clears earlier records. */
+int hfdtd_timing(hfdtd_engine *e, int enable);
+/* synchronizes the stream; total milliseconds and number of sweep launches */
+int hfdtd_timing_read(hfdtd_engine *e, double *total_ms, int *launches);
+
+/* ── near-to-far monitor (monitors.py:383-804, dft_farfield.cl:98-635) ── */
+
+/* box = ix0,ix1,iy0,iy1,iz0,iz1 (inclusive, already snapped); nfreq >= 1 */
+int hfdtd_n2f_create(hfdtd_engine *e, const int box[6], int nfreq, int acc_mode,
+                     hfdtd_n2f **out);
+int hfdtd_n2f_destroy(hfdtd_n2f *m);
+/* n_face and the six packed face offsets (monitors.py:396-412) */
+int hfdtd_n2f_layout(const hfdtd_n2f *m, int *n_face, int offsets[6]);
+/* replaces accumulate_dft_faces_fused (dft_farfield.cl:116-230).  phase holds
+ * nfreq x {e.re, e.im, h.re, h.im}, already including dt. */
+int hfdtd_n2f_accumulate(hfdtd_n2f *m, const double *phase);
+/* replaces snapshot_dft (monitors.py:498-503) */
+int hfdtd_n2f_snapshot(hfdtd_n2f *m);
+/* replaces dft_rel_change_partial + host sum (monitors.py:505-569): returns
+ * sum|cur-prev|^2 and sum|cur|^2 over the 6 buffers of one frequency */
+int hfdtd_n2f_rel_change(hfdtd_n2f *m, int freq, double *num, double *den);
+/* face-packed DFT of one component (0..5 = Ex..Hz) and frequency as
+ * n_face complex values in double (re, im interleaved) */
+int hfdtd_n2f_read(hfdtd_n2f *m, int comp, int freq, double *host);
+/* replaces farfield_accumulate_nl + farfield_nl_to_eh
+ * (dft_farfield.cl:372-635; monitors.py:644-731) with a deterministic
+ * two-stage reduction.  obs: n_obs x 3 metres; eh: n_obs x 6 complex (12
+ * doubles each).  k_wave and obs are used as given (the caller rounds them
+ * like monitors.py:653,669 if it wants the reference's rounding); l_sign is
+ * -1 for eta0*N_t - r x L (dft_farfield.cl:617) or +1 (monitors.py:378).
+ * geom_fp64 selects double (1) or the field dtype (0) for r.r' phases. */
+int hfdtd_n2f_farfield(hfdtd_n2f *m, int freq, const double *obs, int n_obs, double k_wave,
+                       double eta0, int l_sign, int geom_fp64, double *eh);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPFDTD_H */

@@ -1,0 +1,94 @@
+// Internal definitions shared by the libhipfdtd translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hipfdtd.h"
+
+namespace hfdtd {
+
+// Thread-local last-error text behind hfdtd_last_error().
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t err, const char *what);
+
+#define HF_HIP(call)                                                                         \
+  do {                                                                                       \
+    hipError_t _e = (call);                                                                  \
+    if (_e != hipSuccess) return ::hfdtd::hip_fail(_e, #call);                               \
+  } while (0)
+
+// Physical constants (constants.py:11-14 restated).
+constexpr double kC0 = 299792458.0;
+constexpr double kMU0 = 4e-7 * 3.14159265358979323846;
+constexpr double kEPS0 = 1.0 / (kMU0 * (kC0 * kC0));
+
+// psi stripe slots, in the reference's naming psi_<field>_<axis>
+// (engine.py:305-318).  Slots 0..5 belong to the H update, 6..11 to E.
+enum PsiSlot {
+  PSI_HY_X = 0, PSI_HZ_X, PSI_HX_Y, PSI_HZ_Y, PSI_HX_Z, PSI_HY_Z,
+  PSI_EY_X = 6, PSI_EZ_X, PSI_EX_Y, PSI_EZ_Y, PSI_EX_Z, PSI_EY_Z,
+  PSI_COUNT = 12
+};
+
+// Axis of the stripe each slot lives in (0 = x, 1 = y, 2 = z).
+constexpr int kPsiAxis[PSI_COUNT] = {0, 0, 1, 1, 2, 2, 0, 0, 1, 1, 2, 2};
+
+struct Engine {
+  int nx = 0, ny = 0, nz = 0, npml = 0;
+  int fp64 = 0;
+  int device = 0;
+  double dl = 0, dt = 0;
+  double dtm_dl = 0, dtm = 0, inv_dl = 0;  // folded on the host in FP64
+  size_t ncell = 0;
+  size_t elem = 4;  // bytes per value
+  hipStream_t stream = nullptr;
+
+  // Current state.  The split path updates these in place; the fused path
+  // reads them and writes the *_alt partners, then swaps.
+  void *field[6] = {};   // Ex Ey Ez Hx Hy Hz
+  void *field_alt[6] = {};
+  void *ce[3] = {};
+  void *ca[3] = {};
+  void *psi[PSI_COUNT] = {};
+  void *psi_alt[PSI_COUNT] = {};
+  size_t psi_len[3] = {};  // entries of an x-, y-, z-stripe buffer
+
+  // CPML 1-D profiles [stagger 0=E,1=H][axis][b, c, ik]
+  void *prof[2][3][3] = {};
+
+  bool lossless = true;  // ca == 1 everywhere (set_epsilon without sigma)
+  int tile_rows = 4;     // fused sweep: j rows per block
+  int tile_planes = 16;  // fused sweep: i planes per block
+
+  uint64_t bytes = 0;
+
+  // HIP-event timing of the Yee sweeps (hfdtd_timing / hfdtd_timing_read):
+  // events are recorded on `stream` around every sweep launch while enabled.
+  bool timing = false;
+  std::vector<hipEvent_t> events;  // pool; [2q, 2q+1] bracket launch q
+  size_t events_used = 0;
+};
+
+// Record the next timing event on the engine stream (no-op when disabled).
+int timing_mark(Engine *e);
+
+// Launch the split H / E sweeps and the source kernel (yee.hip).
+int launch_update_H(Engine *e);
+int launch_update_E(Engine *e);
+int launch_sources(Engine *e, const hfdtd_source *src, int n);
+int launch_fused_step(Engine *e, const hfdtd_source *src, int n);
+// ensure the ping-pong partners exist (engine.hip)
+int ensure_alt(Engine *e);
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace hfdtd
+
+struct hfdtd_engine {
+  hfdtd::Engine s;
+};

@@ -1,0 +1,550 @@
+// Engine lifetime, device memory, materials and host-side CPML profiles.
+//
+// Replaces the allocation / upload half of OpenCLFDTD (engine.py:99-334) and
+// the host coefficient helpers cpml.py:84-124 and materials.py:28-89.  All
+// state lives in device memory owned by the engine; the host only borrows
+// pointers for the duration of a call.
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+namespace hfdtd {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t err, const char *what) {
+  g_last_error = std::string("HIP error ") + hipGetErrorName(err) + " (" +
+                 hipGetErrorString(err) + ") in " + what;
+  return err == hipErrorOutOfMemory || err == hipErrorMemoryAllocation ? HFDTD_ENOMEM
+                                                                          : HFDTD_EHIP;
+}
+
+// ── host-side CPML profile (cpml.py:84-124 restated) ─────────────────────
+//
+// Depth xi into the layer is 1 at the outer wall: lo side (npml-i-off)/npml,
+// hi side (i+1-off)/npml; sigma = sigma_max xi^3, kappa = 1 + 14 xi^3,
+// alpha = alpha_max (1 - xi); b = exp(-(sigma/kappa + alpha) dt/eps0) is
+// rounded to T before c = sigma/kappa (b-1)/(sigma + kappa alpha)/dl uses it.
+template <typename T>
+static void cpml_profile(int n, int npml, double dl, double dt, double off, T *b, T *c, T *kap) {
+  for (int i = 0; i < n; ++i) {
+    b[i] = T(1);
+    c[i] = T(0);
+    kap[i] = T(1);
+  }
+  if (npml <= 0 || n <= 0) return;
+  const double eta0 = std::sqrt(kMU0 / kEPS0);
+  const int m = 3;
+  const double sig_max = 0.8 * (m + 1) / (eta0 * dl);
+  const double alpha_max = 0.05 / eta0;
+  const double kappa_max = 15.0;
+  for (int i = 0; i < npml; ++i) {
+    for (int side = 0; side < 2; ++side) {
+      const bool lo = side == 0;
+      const int idx = lo ? i : n - npml + i;
+      if (idx < 0 || idx >= n) continue;
+      double xi = lo ? (npml - i - off) / npml : (i + 1 - off) / npml;
+      if (xi < 0.0) xi = 0.0;
+      const double xm = std::pow(xi, double(m));
+      const double sig = sig_max * xm;
+      const double kp = 1.0 + (kappa_max - 1.0) * xm;
+      const double al = alpha_max * (1.0 - xi);
+      const double decay = (sig / kp + al) * dt / kEPS0;
+      const T bt = T(std::exp(-decay));
+      b[idx] = bt;
+      const double den = sig + kp * al;
+      c[idx] = den == 0.0 ? T(0) : T(sig / kp * (double(bt) - 1.0) / den / dl);
+      kap[idx] = T(kp);
+    }
+  }
+}
+
+// ── device helpers ────────────────────────────────────────────────────────
+
+template <typename T>
+__global__ void k_fill(T *p, size_t n, T v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+template <typename T>
+static int fill(Engine *e, void *p, size_t n, T v) {
+  if (n == 0) return HFDTD_OK;
+  unsigned blocks = ceil_div(n, 256);
+  if (blocks > 65536u * 4u) blocks = 65536u * 4u;
+  hipLaunchKernelGGL(k_fill<T>, dim3(blocks), dim3(256), 0, e->stream, (T *)p, n, v);
+  HF_HIP(hipGetLastError());
+  return HFDTD_OK;
+}
+
+static int fill_value(Engine *e, void *p, size_t n, double v) {
+  return e->fp64 ? fill<double>(e, p, n, v) : fill<float>(e, p, n, (float)v);
+}
+
+// Edge-averaged material coefficients (materials.py:28-89 restated), one
+// thread per cell, FP64 arithmetic, stored in T.  The edge of component c
+// averages cells idx and idx+stride_c; the last plane keeps the cell value.
+template <typename T, typename S>
+__global__ void k_materials(int nx, int ny, int nz, const S *__restrict__ eps,
+                            const S *__restrict__ sig, double dt, double eps0, T *__restrict__ ce_x,
+                            T *__restrict__ ce_y, T *__restrict__ ce_z, T *__restrict__ ca_x,
+                            T *__restrict__ ca_y, T *__restrict__ ca_z) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int i = blockIdx.z;
+  if (k >= nz) return;
+  const size_t sxy = (size_t)ny * nz;
+  const size_t idx = (size_t)i * sxy + (size_t)j * nz + k;
+  const double e0 = (double)eps[idx];
+  const double ex = i < nx - 1 ? 0.5 * (e0 + (double)eps[idx + sxy]) : e0;
+  const double ey = j < ny - 1 ? 0.5 * (e0 + (double)eps[idx + nz]) : e0;
+  const double ez = k < nz - 1 ? 0.5 * (e0 + (double)eps[idx + 1]) : e0;
+  if (sig == nullptr) {
+    const double scale = dt / eps0;
+    ce_x[idx] = T(scale / ex);
+    ce_y[idx] = T(scale / ey);
+    ce_z[idx] = T(scale / ez);
+    ca_x[idx] = T(1);
+    ca_y[idx] = T(1);
+    ca_z[idx] = T(1);
+    return;
+  }
+  const double s0 = (double)sig[idx];
+  const double sx = i < nx - 1 ? 0.5 * (s0 + (double)sig[idx + sxy]) : s0;
+  const double sy = j < ny - 1 ? 0.5 * (s0 + (double)sig[idx + nz]) : s0;
+  const double sz = k < nz - 1 ? 0.5 * (s0 + (double)sig[idx + 1]) : s0;
+  const double ee[3] = {ex, ey, ez};
+  const double ss[3] = {sx, sy, sz};
+  T *cb_out[3] = {ce_x, ce_y, ce_z};
+  T *ca_out[3] = {ca_x, ca_y, ca_z};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double loss = ss[c] * dt / (2.0 * eps0 * ee[c]);
+    const double den = 1.0 + loss;
+    ca_out[c][idx] = T((1.0 - loss) / den);
+    cb_out[c][idx] = T((dt / (eps0 * ee[c])) / den);
+  }
+}
+
+static int dmalloc(Engine *e, void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return HFDTD_OK;
+  hipError_t err = hipMalloc(p, bytes);
+  if (err != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "device allocation of %.3f GB failed (%s)",
+                  bytes / 1073741824.0, hipGetErrorString(err));
+    return fail(HFDTD_ENOMEM, buf);
+  }
+  e->bytes += bytes;
+  return HFDTD_OK;
+}
+
+static void release(Engine *e) {
+  if (e->device >= 0) (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  auto fr = [](void *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  };
+  for (auto &p : e->field) fr(p);
+  for (auto &p : e->field_alt) fr(p);
+  for (auto &p : e->ce) fr(p);
+  for (auto &p : e->ca) fr(p);
+  for (auto &p : e->psi) fr(p);
+  for (auto &p : e->psi_alt) fr(p);
+  for (auto &s : e->prof)
+    for (auto &a : s)
+      for (auto &p : a) fr(p);
+  for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
+  e->events.clear();
+  e->events_used = 0;
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  e->stream = nullptr;
+}
+
+int timing_mark(Engine *e) {
+  if (!e->timing) return HFDTD_OK;
+  if (e->events_used == e->events.size()) {
+    hipEvent_t ev;
+    HF_HIP(hipEventCreate(&ev));
+    e->events.push_back(ev);
+  }
+  HF_HIP(hipEventRecord(e->events[e->events_used++], e->stream));
+  return HFDTD_OK;
+}
+
+int ensure_alt(Engine *e) {
+  if (e->field_alt[0]) return HFDTD_OK;
+  for (int f = 0; f < 6; ++f) {
+    int rc = dmalloc(e, &e->field_alt[f], e->ncell * e->elem);
+    if (rc) return rc;
+  }
+  for (int s = 0; s < PSI_COUNT; ++s) {
+    int rc = dmalloc(e, &e->psi_alt[s], e->psi_len[kPsiAxis[s]] * e->elem);
+    if (rc) return rc;
+  }
+  return HFDTD_OK;
+}
+
+}  // namespace hfdtd
+
+using namespace hfdtd;
+
+extern "C" {
+
+const char *hfdtd_last_error(void) { return g_last_error.c_str(); }
+
+int hfdtd_version(void) { return HFDTD_ABI_VERSION; }
+
+int hfdtd_device_count(int *count) {
+  if (!count) return fail(HFDTD_EINVAL, "count is NULL");
+  *count = 0;
+  hipError_t err = hipGetDeviceCount(count);
+  if (err != hipSuccess) {
+    (void)hipGetLastError();
+    *count = 0;
+    return fail(HFDTD_ENODEV, std::string("no HIP device: ") + hipGetErrorString(err));
+  }
+  return HFDTD_OK;
+}
+
+int hfdtd_device_info(int device, uint64_t *total_bytes, uint64_t *free_bytes, char *name,
+                      int name_len) {
+  int n = 0;
+  int rc = hfdtd_device_count(&n);
+  if (rc) return rc;
+  if (device < 0 || device >= n) return fail(HFDTD_ENODEV, "device index out of range");
+  HF_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HF_HIP(hipGetDeviceProperties(&prop, device));
+  size_t fr = 0, tot = 0;
+  HF_HIP(hipMemGetInfo(&fr, &tot));
+  if (total_bytes) *total_bytes = tot;
+  if (free_bytes) *free_bytes = fr;
+  if (name && name_len > 0) {
+    std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  return HFDTD_OK;
+}
+
+int hfdtd_cpml_profile(int n, int npml, double dl, double dt, double offset, int fp64, void *b,
+                       void *c, void *kappa) {
+  if (n < 0 || npml < 0 || !(dl > 0) || !(dt > 0) || !b || !c || !kappa)
+    return fail(HFDTD_EINVAL, "hfdtd_cpml_profile: bad argument");
+  if (fp64)
+    cpml_profile<double>(n, npml, dl, dt, offset, (double *)b, (double *)c, (double *)kappa);
+  else
+    cpml_profile<float>(n, npml, dl, dt, offset, (float *)b, (float *)c, (float *)kappa);
+  return HFDTD_OK;
+}
+
+int hfdtd_create(int nx, int ny, int nz, int npml, double dl, double dt, int fp64, int device,
+                 hfdtd_engine **out) {
+  if (!out) return fail(HFDTD_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 1 || npml < 0)
+    return fail(HFDTD_EINVAL, "grid extents must be >= 1 and npml >= 0");
+  if (!(dl > 0.0) || !(dt > 0.0)) return fail(HFDTD_EINVAL, "dl and dt must be positive");
+  if ((double)nx * ny * nz >= 2147483647.0)
+    return fail(HFDTD_EUNSUPPORTED, "grid exceeds 2^31 cells");
+  int ndev = 0;
+  int rc = hfdtd_device_count(&ndev);
+  if (rc) return rc;
+  if (device < 0 || device >= ndev) return fail(HFDTD_ENODEV, "device index out of range");
+
+  hfdtd_engine *h = new hfdtd_engine();
+  Engine *e = &h->s;
+  e->nx = nx;
+  e->ny = ny;
+  e->nz = nz;
+  e->npml = npml;
+  e->fp64 = fp64 ? 1 : 0;
+  e->device = device;
+  e->dl = dl;
+  e->dt = dt;
+  e->elem = fp64 ? 8 : 4;
+  e->ncell = (size_t)nx * ny * nz;
+  // folded constants, evaluated in FP64 as engine.py:543-544, 634 do
+  e->dtm_dl = dt / (kMU0 * dl);
+  e->dtm = dt / kMU0;
+  e->inv_dl = 1.0 / dl;
+  if (npml > 0) {
+    e->psi_len[0] = (size_t)2 * npml * ny * nz;
+    e->psi_len[1] = (size_t)nx * 2 * npml * nz;
+    e->psi_len[2] = (size_t)nx * ny * 2 * npml;
+  }
+
+  auto bail = [&](int code) {
+    release(e);
+    delete h;
+    return code;
+  };
+  hipError_t err = hipSetDevice(device);
+  if (err != hipSuccess) return bail(hip_fail(err, "hipSetDevice"));
+  err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (err != hipSuccess) return bail(hip_fail(err, "hipStreamCreateWithFlags"));
+
+  const size_t vb = e->ncell * e->elem;
+  for (int f = 0; f < 6; ++f)
+    if ((rc = dmalloc(e, &e->field[f], vb))) return bail(rc);
+  for (int c = 0; c < 3; ++c) {
+    if ((rc = dmalloc(e, &e->ce[c], vb))) return bail(rc);
+    if ((rc = dmalloc(e, &e->ca[c], vb))) return bail(rc);
+  }
+  for (int s = 0; s < PSI_COUNT; ++s)
+    if ((rc = dmalloc(e, &e->psi[s], e->psi_len[kPsiAxis[s]] * e->elem))) return bail(rc);
+
+  // CPML profiles, H nodes offset 1/2 and E nodes offset 0 (engine.py:262-283)
+  const int dims[3] = {nx, ny, nz};
+  for (int stag = 0; stag < 2; ++stag) {
+    for (int a = 0; a < 3; ++a) {
+      const int n = dims[a];
+      std::vector<double> bd(n), cd(n), kd(n);
+      std::vector<float> bf(n), cf(n), kf(n);
+      std::vector<unsigned char> host_b(n * e->elem), host_c(n * e->elem), host_ik(n * e->elem);
+      const double off = stag == 0 ? 0.0 : 0.5;
+      if (fp64) {
+        cpml_profile<double>(n, npml, dl, dt, off, bd.data(), cd.data(), kd.data());
+        std::memcpy(host_b.data(), bd.data(), n * 8);
+        std::memcpy(host_c.data(), cd.data(), n * 8);
+        for (int i = 0; i < n; ++i) {
+          double ik = 1.0 / (kd[i] * dl);
+          std::memcpy(host_ik.data() + i * 8, &ik, 8);
+        }
+      } else {
+        cpml_profile<float>(n, npml, dl, dt, off, bf.data(), cf.data(), kf.data());
+        std::memcpy(host_b.data(), bf.data(), n * 4);
+        std::memcpy(host_c.data(), cf.data(), n * 4);
+        for (int i = 0; i < n; ++i) {
+          // 1/(kappa*dl) in FP64 from the stored kappa, then rounded (engine.py:264-266)
+          float ik = (float)(1.0 / ((double)kf[i] * dl));
+          std::memcpy(host_ik.data() + i * 4, &ik, 4);
+        }
+      }
+      const unsigned char *src[3] = {host_b.data(), host_c.data(), host_ik.data()};
+      for (int q = 0; q < 3; ++q) {
+        if ((rc = dmalloc(e, &e->prof[stag][a][q], n * e->elem))) return bail(rc);
+        err = hipMemcpy(e->prof[stag][a][q], src[q], n * e->elem, hipMemcpyHostToDevice);
+        if (err != hipSuccess) return bail(hip_fail(err, "hipMemcpy(cpml)"));
+      }
+    }
+  }
+
+  // defaults: fields 0, ce = dt/eps0, ca = 1 (engine.py:189-204)
+  for (int f = 0; f < 6; ++f)
+    if ((rc = fill_value(e, e->field[f], e->ncell, 0.0))) return bail(rc);
+  for (int c = 0; c < 3; ++c) {
+    if ((rc = fill_value(e, e->ce[c], e->ncell, dt / kEPS0))) return bail(rc);
+    if ((rc = fill_value(e, e->ca[c], e->ncell, 1.0))) return bail(rc);
+  }
+  for (int s = 0; s < PSI_COUNT; ++s)
+    if ((rc = fill_value(e, e->psi[s], e->psi_len[kPsiAxis[s]], 0.0))) return bail(rc);
+  err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return bail(hip_fail(err, "hipStreamSynchronize(create)"));
+  *out = h;
+  return HFDTD_OK;
+}
+
+int hfdtd_destroy(hfdtd_engine *h) {
+  if (!h) return HFDTD_OK;
+  release(&h->s);
+  delete h;
+  return HFDTD_OK;
+}
+
+int hfdtd_memory_bytes(const hfdtd_engine *h, uint64_t *bytes) {
+  if (!h || !bytes) return fail(HFDTD_EINVAL, "NULL argument");
+  *bytes = h->s.bytes;
+  return HFDTD_OK;
+}
+
+static void **volume_slot(Engine *e, int which) {
+  if (which >= HFDTD_EX && which <= HFDTD_HZ) return &e->field[which];
+  if (which >= HFDTD_CEX && which <= HFDTD_CEZ) return &e->ce[which - HFDTD_CEX];
+  if (which >= HFDTD_CAX && which <= HFDTD_CAZ) return &e->ca[which - HFDTD_CAX];
+  return nullptr;
+}
+
+int hfdtd_set_materials(hfdtd_engine *h, const void *eps_r, const void *sigma, int src_fp64) {
+  if (!h || !eps_r) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = &h->s;
+  HF_HIP(hipSetDevice(e->device));
+  const size_t sb = e->ncell * (src_fp64 ? 8 : 4);
+  void *d_eps = nullptr, *d_sig = nullptr;
+  HF_HIP(hipMalloc(&d_eps, sb));
+  if (sigma) {
+    hipError_t err = hipMalloc(&d_sig, sb);
+    if (err != hipSuccess) {
+      (void)hipFree(d_eps);
+      return hip_fail(err, "hipMalloc(sigma)");
+    }
+  }
+  int rc = HFDTD_OK;
+  hipError_t err = hipMemcpyAsync(d_eps, eps_r, sb, hipMemcpyHostToDevice, e->stream);
+  if (err == hipSuccess && sigma)
+    err = hipMemcpyAsync(d_sig, sigma, sb, hipMemcpyHostToDevice, e->stream);
+  if (err == hipSuccess) {
+    dim3 grid(ceil_div(e->nz, 64), e->ny, e->nx);
+#define HF_MAT(T, S)                                                                         \
+  hipLaunchKernelGGL((k_materials<T, S>), grid, dim3(64), 0, e->stream, e->nx, e->ny, e->nz, \
+                     (const S *)d_eps, (const S *)d_sig, e->dt, kEPS0, (T *)e->ce[0],       \
+                     (T *)e->ce[1], (T *)e->ce[2], (T *)e->ca[0], (T *)e->ca[1], (T *)e->ca[2])
+    if (e->fp64) {
+      if (src_fp64) HF_MAT(double, double); else HF_MAT(double, float);
+    } else {
+      if (src_fp64) HF_MAT(float, double); else HF_MAT(float, float);
+    }
+#undef HF_MAT
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) rc = hip_fail(err, "set_materials");
+  (void)hipFree(d_eps);
+  if (d_sig) (void)hipFree(d_sig);
+  e->lossless = sigma == nullptr;
+  return rc;
+}
+
+int hfdtd_write_volume(hfdtd_engine *h, int which, const void *host) {
+  if (!h || !host) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = &h->s;
+  void **slot = volume_slot(e, which);
+  if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
+  HF_HIP(hipSetDevice(e->device));
+  HF_HIP(hipMemcpyAsync(*slot, host, e->ncell * e->elem, hipMemcpyHostToDevice, e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  if (which >= HFDTD_CAX) e->lossless = false;
+  return HFDTD_OK;
+}
+
+int hfdtd_read_volume(hfdtd_engine *h, int which, void *host) {
+  if (!h || !host) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = &h->s;
+  void **slot = volume_slot(e, which);
+  if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
+  HF_HIP(hipSetDevice(e->device));
+  HF_HIP(hipMemcpyAsync(host, *slot, e->ncell * e->elem, hipMemcpyDeviceToHost, e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  return HFDTD_OK;
+}
+
+int hfdtd_read_point(hfdtd_engine *h, int which, int64_t index, double *out) {
+  if (!h || !out) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = &h->s;
+  void **slot = volume_slot(e, which);
+  if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
+  if (index < 0 || (size_t)index >= e->ncell) return fail(HFDTD_EINVAL, "index out of range");
+  HF_HIP(hipSetDevice(e->device));
+  unsigned char buf[8];
+  HF_HIP(hipMemcpyAsync(buf, (const char *)*slot + (size_t)index * e->elem, e->elem,
+                        hipMemcpyDeviceToHost, e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  if (e->fp64) {
+    double v;
+    std::memcpy(&v, buf, 8);
+    *out = v;
+  } else {
+    float v;
+    std::memcpy(&v, buf, 4);
+    *out = v;
+  }
+  return HFDTD_OK;
+}
+
+int hfdtd_reset_fields(hfdtd_engine *h) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = &h->s;
+  HF_HIP(hipSetDevice(e->device));
+  int rc;
+  for (int f = 0; f < 6; ++f)
+    if ((rc = fill_value(e, e->field[f], e->ncell, 0.0))) return rc;
+  for (int s = 0; s < PSI_COUNT; ++s)
+    if ((rc = fill_value(e, e->psi[s], e->psi_len[kPsiAxis[s]], 0.0))) return rc;
+  HF_HIP(hipStreamSynchronize(e->stream));
+  return HFDTD_OK;
+}
+
+int hfdtd_update_H(hfdtd_engine *h) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  return launch_update_H(&h->s);
+}
+
+int hfdtd_update_E(hfdtd_engine *h) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  return launch_update_E(&h->s);
+}
+
+int hfdtd_add_sources(hfdtd_engine *h, const hfdtd_source *src, int n) {
+  if (!h || (n > 0 && !src) || n < 0) return fail(HFDTD_EINVAL, "bad source list");
+  return launch_sources(&h->s, src, n);
+}
+
+int hfdtd_step(hfdtd_engine *h, const hfdtd_source *src, int n, int path) {
+  if (!h || (n > 0 && !src) || n < 0) return fail(HFDTD_EINVAL, "bad source list");
+  Engine *e = &h->s;
+  if (path == HFDTD_PATH_FUSED) return launch_fused_step(e, src, n);
+  if (path != HFDTD_PATH_SPLIT) return fail(HFDTD_EINVAL, "unknown step path");
+  int rc = launch_update_H(e);
+  if (rc) return rc;
+  if (n > 0 && (rc = launch_sources(e, src, n))) return rc;
+  return launch_update_E(e);
+}
+
+int hfdtd_synchronize(hfdtd_engine *h) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  HF_HIP(hipSetDevice(h->s.device));
+  HF_HIP(hipStreamSynchronize(h->s.stream));
+  return HFDTD_OK;
+}
+
+int hfdtd_timing(hfdtd_engine *h, int enable) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = &h->s;
+  if (enable && e->events_used % 2) return fail(HFDTD_EINVAL, "unbalanced timing events");
+  e->timing = enable != 0;
+  if (enable) e->events_used = 0;
+  return HFDTD_OK;
+}
+
+int hfdtd_timing_read(hfdtd_engine *h, double *total_ms, int *launches) {
+  if (!h || !total_ms || !launches) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = &h->s;
+  HF_HIP(hipSetDevice(e->device));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  double tot = 0.0;
+  const size_t n = e->events_used / 2;
+  for (size_t q = 0; q < n; ++q) {
+    float ms = 0.f;
+    HF_HIP(hipEventElapsedTime(&ms, e->events[2 * q], e->events[2 * q + 1]));
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = (int)n;
+  return HFDTD_OK;
+}
+
+int hfdtd_set_fused_tiling(hfdtd_engine *h, int rows_j, int planes_i) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  if (rows_j < 0 || rows_j > 16 || planes_i < 0)
+    return fail(HFDTD_EINVAL, "rows_j must be 0..16 and planes_i >= 0");
+  if (rows_j) h->s.tile_rows = rows_j;
+  if (planes_i) h->s.tile_planes = planes_i;
+  return HFDTD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,609 @@
+// Near-to-far (Huygens box) monitor: per-step multi-frequency DFT of the
+// co-located tangential fields on the six faces, snapshot / relative change,
+// and the surface-integral far-field transform.
+//
+// Replaces OpenCLNear2FarMonitor (monitors.py:383-804) and the kernels
+// accumulate_dft_faces_fused, dft_rel_change_partial, farfield_accumulate_nl
+// and farfield_nl_to_eh (kernels/dft_farfield.cl:98-635).  Differences by
+// design: one launch updates every frequency of the monitor (fields are read
+// once per step, not once per frequency), and the N/L surface sum is a
+// deterministic two-stage reduction in FP64 instead of float CAS atomics.
+
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+struct hfdtd_n2f {
+  hfdtd::Engine *e = nullptr;
+  int box[6] = {};
+  int nfreq = 0;
+  int acc_mode = 0;
+  int n_face = 0;
+  int off[6] = {};
+  size_t acc_elem = 4;           // bytes per real component of an accumulator
+  void *acc = nullptr;           // [6 comps][nfreq][n_face] x {re, im}
+  void *cmp = nullptr;           // Kahan compensation, same layout (float)
+  void *snap = nullptr;          // snapshot of acc
+  double *scratch = nullptr;     // reduction partials
+  size_t scratch_len = 0;
+};
+
+namespace hfdtd {
+
+constexpr int kMaxFreqPerLaunch = 16;
+
+struct Box {
+  int ix0, ix1, iy0, iy1, iz0, iz1;
+  int off[6];
+  int n_face;
+};
+
+template <typename A>
+struct PhaseArg {
+  A v[kMaxFreqPerLaunch][4];  // e.re e.im h.re h.im
+};
+
+// Packed sample -> face id and absolute (i, j, k) (dft_farfield.cl:141-164).
+__device__ __forceinline__ int decode_face(const Box &b, int s, int &i, int &j, int &k) {
+  const int nyf = b.iy1 - b.iy0 + 1, nzf = b.iz1 - b.iz0 + 1;
+  int f, loc;
+  if (s < b.off[2]) {
+    f = s < b.off[1] ? 0 : 1;
+    loc = s - b.off[f];
+    i = f == 0 ? b.ix0 : b.ix1;
+    j = b.iy0 + loc / nzf;
+    k = b.iz0 + loc % nzf;
+  } else if (s < b.off[4]) {
+    f = s < b.off[3] ? 2 : 3;
+    loc = s - b.off[f];
+    i = b.ix0 + loc / nzf;
+    j = f == 2 ? b.iy0 : b.iy1;
+    k = b.iz0 + loc % nzf;
+  } else {
+    f = s < b.off[5] ? 4 : 5;
+    loc = s - b.off[f];
+    i = b.ix0 + loc / nyf;
+    j = b.iy0 + loc % nyf;
+    k = f == 4 ? b.iz0 : b.iz1;
+  }
+  return f;
+}
+
+struct Dims3 {
+  int nx, ny, nz;
+};
+
+template <typename T>
+__device__ __forceinline__ T avg2(T a, T b, bool ok) {
+  return ok ? T(0.5) * (a + b) : a;
+}
+
+template <typename T>
+__device__ __forceinline__ T avg4(T a, T b, T c, T d) {
+  return T(0.25) * (a + b + c + d);
+}
+
+// Tangential E/H of one face sample co-located at the face centre
+// (dft_farfield.cl:171-227): E as 2-point, H as 4-point averages with
+// clamped neighbours.  c[] receives the component ids (0..5 = Ex..Hz).
+template <typename T>
+__device__ __forceinline__ void colocate_impl(const Dims3 &g, int f, int i, int j, int k,
+                                              const T *Ex, const T *Ey, const T *Ez,
+                                              const T *Hx, const T *Hy, const T *Hz, T v[4],
+                                              int c[4]) {
+  auto at = [&](const T *F, int a, int b, int cc) {
+    return F[((size_t)a * g.ny + b) * g.nz + cc];
+  };
+  if (f <= 1) {
+    const int k1 = k + 1 < g.nz ? k + 1 : k;
+    const int j1 = j + 1 < g.ny ? j + 1 : j;
+    const int il = i > 0 ? i - 1 : i;
+    v[0] = avg2(at(Ey, i, j, k), at(Ey, i, j, k1), k1 != k);
+    v[1] = avg2(at(Ez, i, j, k), at(Ez, i, j1, k), j1 != j);
+    v[2] = avg4(at(Hy, i, j, k), at(Hy, il, j, k), at(Hy, i, j1, k), at(Hy, il, j1, k));
+    v[3] = avg4(at(Hz, i, j, k), at(Hz, il, j, k), at(Hz, i, j, k1), at(Hz, il, j, k1));
+    c[0] = 1; c[1] = 2; c[2] = 4; c[3] = 5;
+  } else if (f <= 3) {
+    const int i1 = i + 1 < g.nx ? i + 1 : i;
+    const int k1 = k + 1 < g.nz ? k + 1 : k;
+    const int jl = j > 0 ? j - 1 : j;
+    v[0] = avg2(at(Ex, i, j, k), at(Ex, i, j, k1), k1 != k);
+    v[1] = avg2(at(Ez, i, j, k), at(Ez, i1, j, k), i1 != i);
+    v[2] = avg4(at(Hx, i, j, k), at(Hx, i, jl, k), at(Hx, i1, j, k), at(Hx, i1, jl, k));
+    v[3] = avg4(at(Hz, i, j, k), at(Hz, i, jl, k), at(Hz, i, j, k1), at(Hz, i, jl, k1));
+    c[0] = 0; c[1] = 2; c[2] = 3; c[3] = 5;
+  } else {
+    const int i1 = i + 1 < g.nx ? i + 1 : i;
+    const int j1 = j + 1 < g.ny ? j + 1 : j;
+    const int kl = k > 0 ? k - 1 : k;
+    v[0] = avg2(at(Ex, i, j, k), at(Ex, i, j1, k), j1 != j);
+    v[1] = avg2(at(Ey, i, j, k), at(Ey, i1, j, k), i1 != i);
+    v[2] = avg4(at(Hx, i, j, k), at(Hx, i, j, kl), at(Hx, i, j1, k), at(Hx, i, j1, kl));
+    v[3] = avg4(at(Hy, i, j, k), at(Hy, i, j, kl), at(Hy, i1, j, k), at(Hy, i1, j, kl));
+    c[0] = 0; c[1] = 1; c[2] = 3; c[3] = 4;
+  }
+}
+
+// acc += val * phase, in A, optionally Kahan-compensated (cmp holds the
+// running compensation, sum_true = acc - cmp).
+template <typename A, bool KAHAN>
+__device__ __forceinline__ void dft_add(A *acc, A *cmp, size_t o, A val, A pr, A pi) {
+  if (!KAHAN) {
+    acc[o] += val * pr;
+    acc[o + 1] += val * pi;
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const A term = val * (q == 0 ? pr : pi);
+    const A s = acc[o + q];
+    const A y = term - cmp[o + q];
+    const A t = s + y;
+    cmp[o + q] = (t - s) - y;
+    acc[o + q] = t;
+  }
+}
+
+template <typename T, typename A, bool KAHAN>
+__global__ __launch_bounds__(256) void k_n2f_accumulate(Dims3 g, Box b, int nfreq, int f0,
+                                                        int nf, const T *__restrict__ Ex,
+                                                        const T *__restrict__ Ey,
+                                                        const T *__restrict__ Ez,
+                                                        const T *__restrict__ Hx,
+                                                        const T *__restrict__ Hy,
+                                                        const T *__restrict__ Hz, A *acc,
+                                                        A *cmp, PhaseArg<A> ph) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= b.n_face) return;
+  int i, j, k;
+  const int f = decode_face(b, s, i, j, k);
+  T v[4];
+  int c[4];
+  colocate_impl<T>(g, f, i, j, k, Ex, Ey, Ez, Hx, Hy, Hz, v, c);
+  const size_t nfc = (size_t)b.n_face;
+  for (int q = 0; q < nf; ++q) {
+    const int fr = f0 + q;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const bool is_e = c[m] < 3;
+      const size_t o = (((size_t)c[m] * nfreq + fr) * nfc + s) * 2;
+      dft_add<A, KAHAN>(acc, cmp, o, A(v[m]), ph.v[q][is_e ? 0 : 2], ph.v[q][is_e ? 1 : 3]);
+    }
+  }
+}
+
+// Per-block partial sums of |cur - prev|^2 and |cur|^2 over the 6 buffers of
+// one frequency (dft_farfield.cl:237-290), in FP64.
+template <typename A>
+__global__ __launch_bounds__(256) void k_n2f_relchange(int n_face, int nfreq, int fr,
+                                                      const A *__restrict__ cur,
+                                                      const A *__restrict__ prev,
+                                                      double *__restrict__ part) {
+  __shared__ double sn[256], sd[256];
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  double num = 0.0, den = 0.0;
+  if (s < n_face) {
+    for (int c = 0; c < 6; ++c) {
+      const size_t o = (((size_t)c * nfreq + fr) * n_face + s) * 2;
+      const double cr = cur[o], ci = cur[o + 1];
+      const double dr = cr - (double)prev[o], di = ci - (double)prev[o + 1];
+      num += dr * dr + di * di;
+      den += cr * cr + ci * ci;
+    }
+  }
+  sn[threadIdx.x] = num;
+  sd[threadIdx.x] = den;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      sn[threadIdx.x] += sn[threadIdx.x + st];
+      sd[threadIdx.x] += sd[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sn[0];
+    part[2 * blockIdx.x + 1] = sd[0];
+  }
+}
+
+// Stage 1 of the far-field: block (bx, obs) sums the N, L contributions of
+// samples [bx*chunk, (bx+1)*chunk) for observation obs into 12 doubles.
+// Per-sample terms follow face_sample_NL (dft_farfield.cl:372-474): phase
+// k r.r' in G, e^{j phase} and products in A.
+template <typename G, typename A>
+__global__ __launch_bounds__(256) void k_n2f_nl(Box b, int nfreq, int fr, int chunk,
+                                               const A *__restrict__ acc,
+                                               const double *__restrict__ obs, G k_wave, G dl,
+                                               double *__restrict__ part) {
+  const int ob = blockIdx.y;
+  const int nblk = gridDim.x;
+  const G ox = G(obs[3 * ob]), oy = G(obs[3 * ob + 1]), oz = G(obs[3 * ob + 2]);
+  G r = sqrt(ox * ox + oy * oy + oz * oz);
+  if (r < G(1.0e-30)) r = G(1.0e-30);
+  const G rx = ox / r, ry = oy / r, rz = oz / r;
+  const G dA = dl * dl;
+  double s6[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) s6[q] = 0.0;
+  const size_t nfc = (size_t)b.n_face;
+  auto ld = [&](int comp, int s, A &re, A &im) {
+    const size_t o = (((size_t)comp * nfreq + fr) * nfc + s) * 2;
+    re = acc[o];
+    im = acc[o + 1];
+  };
+  const int s_end = min(b.n_face, (int)((blockIdx.x + 1) * (size_t)chunk));
+  for (int s = blockIdx.x * chunk + threadIdx.x; s < s_end; s += blockDim.x) {
+    int i, j, k;
+    const int f = decode_face(b, s, i, j, k);
+    G xp, yp, zp;
+    if (f <= 1) {
+      xp = G(i) * dl;
+      yp = (G(j) + G(0.5)) * dl;
+      zp = (G(k) + G(0.5)) * dl;
+    } else if (f <= 3) {
+      xp = (G(i) + G(0.5)) * dl;
+      yp = G(j) * dl;
+      zp = (G(k) + G(0.5)) * dl;
+    } else {
+      xp = (G(i) + G(0.5)) * dl;
+      yp = (G(j) + G(0.5)) * dl;
+      zp = G(k) * dl;
+    }
+    const G phase = k_wave * (rx * xp + ry * yp + rz * zp);
+    const A pc = cos(A(phase)), ps = sin(A(phase));
+    G wj, wk;  // trapezoid weights (dft_farfield.cl:431-442)
+    if (f <= 1) {
+      wj = (j == b.iy0 || j == b.iy1) ? G(0.5) : G(1);
+      wk = (k == b.iz0 || k == b.iz1) ? G(0.5) : G(1);
+    } else if (f <= 3) {
+      wj = (i == b.ix0 || i == b.ix1) ? G(0.5) : G(1);
+      wk = (k == b.iz0 || k == b.iz1) ? G(0.5) : G(1);
+    } else {
+      wj = (i == b.ix0 || i == b.ix1) ? G(0.5) : G(1);
+      wk = (j == b.iy0 || j == b.iy1) ? G(0.5) : G(1);
+    }
+    const A w = A(dA * wj * wk);
+    const A nf = (f & 1) ? A(1) : A(-1);
+    // J = n x H, M = -n x E (dft_farfield.cl:446-473); slots N 0..2, L 3..5
+    int jc[2], js[2], mc[2], ms[2];  // component slots and source buffers
+    A jsg[2], msg[2];
+    if (f <= 1) {
+      jc[0] = 1; js[0] = 5; jsg[0] = nf;   // Jy =  n Hz
+      jc[1] = 2; js[1] = 4; jsg[1] = -nf;  // Jz = -n Hy
+      mc[0] = 1; ms[0] = 2; msg[0] = -nf;  // My = -n Ez
+      mc[1] = 2; ms[1] = 1; msg[1] = nf;   // Mz =  n Ey
+    } else if (f <= 3) {
+      jc[0] = 0; js[0] = 5; jsg[0] = -nf;  // Jx = -n Hz
+      jc[1] = 2; js[1] = 3; jsg[1] = nf;   // Jz =  n Hx
+      mc[0] = 0; ms[0] = 2; msg[0] = nf;   // Mx =  n Ez
+      mc[1] = 2; ms[1] = 0; msg[1] = -nf;  // Mz = -n Ex
+    } else {
+      jc[0] = 0; js[0] = 4; jsg[0] = nf;   // Jx =  n Hy
+      jc[1] = 1; js[1] = 3; jsg[1] = -nf;  // Jy = -n Hx
+      mc[0] = 0; ms[0] = 1; msg[0] = -nf;  // Mx = -n Ey
+      mc[1] = 1; ms[1] = 0; msg[1] = nf;   // My =  n Ex
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      A re, im;
+      ld(js[q], s, re, im);
+      re *= jsg[q];
+      im *= jsg[q];
+      const A tr = re * pc - im * ps, ti = re * ps + im * pc;
+      s6[2 * jc[q]] += double(tr * w);
+      s6[2 * jc[q] + 1] += double(ti * w);
+      ld(ms[q], s, re, im);
+      re *= msg[q];
+      im *= msg[q];
+      const A ur = re * pc - im * ps, ui = re * ps + im * pc;
+      s6[6 + 2 * mc[q]] += double(ur * w);
+      s6[6 + 2 * mc[q] + 1] += double(ui * w);
+    }
+  }
+  // block reduction: wave shuffles, then the 4 wave sums in LDS
+  __shared__ double red[4][12];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 12; ++q) {
+    double v = s6[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[wave][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 12) {
+    const int nw = blockDim.x >> 6;
+    double v = 0.0;
+    for (int w2 = 0; w2 < nw; ++w2) v += red[w2][threadIdx.x];
+    part[((size_t)ob * nblk + blockIdx.x) * 12 + threadIdx.x] = v;
+  }
+}
+
+// Stage 2: sum the partials of each observation in block order, then
+// N, L -> far E, H (dft_farfield.cl:559-635) in A:
+//   E = -j k e^{-jkr}/(4 pi r) (eta0 N_t + l_sign r x L),  H = -(r x E)/eta0.
+template <typename A>
+__global__ void k_n2f_eh(int n_obs, int nblk, const double *__restrict__ part,
+                         const double *__restrict__ obs, A k_wave, A eta0, A l_sign,
+                         double *__restrict__ eh) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_obs) return;
+  double nl[12];
+  for (int q = 0; q < 12; ++q) nl[q] = 0.0;
+  for (int bq = 0; bq < nblk; ++bq)
+    for (int q = 0; q < 12; ++q) nl[q] += part[((size_t)p * nblk + bq) * 12 + q];
+  const A ox = A(obs[3 * p]), oy = A(obs[3 * p + 1]), oz = A(obs[3 * p + 2]);
+  A r = sqrt(ox * ox + oy * oy + oz * oz);
+  if (r < A(1.0e-30)) r = A(1.0e-30);
+  const A rh[3] = {ox / r, oy / r, oz / r};
+  A Nr[3], Ni[3], Lr[3], Li[3];
+  for (int c = 0; c < 3; ++c) {
+    Nr[c] = A(nl[2 * c]);
+    Ni[c] = A(nl[2 * c + 1]);
+    Lr[c] = A(nl[6 + 2 * c]);
+    Li[c] = A(nl[6 + 2 * c + 1]);
+  }
+  const A ang = k_wave * r;
+  const A er = cos(ang), ei = -sin(ang);
+  const A scale = k_wave / (A(4) * A(3.14159265358979323846) * r);
+  // pref = (0, -scale) * (er, ei)
+  const A pr = scale * ei, pi = -scale * er;
+  const A xLr[3] = {rh[1] * Lr[2] - rh[2] * Lr[1], rh[2] * Lr[0] - rh[0] * Lr[2],
+                    rh[0] * Lr[1] - rh[1] * Lr[0]};
+  const A xLi[3] = {rh[1] * Li[2] - rh[2] * Li[1], rh[2] * Li[0] - rh[0] * Li[2],
+                    rh[0] * Li[1] - rh[1] * Li[0]};
+  const A dr = rh[0] * Nr[0] + rh[1] * Nr[1] + rh[2] * Nr[2];
+  const A di = rh[0] * Ni[0] + rh[1] * Ni[1] + rh[2] * Ni[2];
+  A Er[3], Ei[3];
+  for (int c = 0; c < 3; ++c) {
+    const A tr = eta0 * (Nr[c] - dr * rh[c]) + l_sign * xLr[c];
+    const A ti = eta0 * (Ni[c] - di * rh[c]) + l_sign * xLi[c];
+    Er[c] = pr * tr - pi * ti;
+    Ei[c] = pr * ti + pi * tr;
+  }
+  const A Hr[3] = {-(rh[1] * Er[2] - rh[2] * Er[1]) / eta0,
+                   -(rh[2] * Er[0] - rh[0] * Er[2]) / eta0,
+                   -(rh[0] * Er[1] - rh[1] * Er[0]) / eta0};
+  const A Hi[3] = {-(rh[1] * Ei[2] - rh[2] * Ei[1]) / eta0,
+                   -(rh[2] * Ei[0] - rh[0] * Ei[2]) / eta0,
+                   -(rh[0] * Ei[1] - rh[1] * Ei[0]) / eta0};
+  double *o = eh + (size_t)p * 12;
+  for (int c = 0; c < 3; ++c) {
+    o[2 * c] = Er[c];
+    o[2 * c + 1] = Ei[c];
+    o[6 + 2 * c] = Hr[c];
+    o[6 + 2 * c + 1] = Hi[c];
+  }
+}
+
+static Box box_of(const hfdtd_n2f *m) {
+  Box b;
+  b.ix0 = m->box[0]; b.ix1 = m->box[1];
+  b.iy0 = m->box[2]; b.iy1 = m->box[3];
+  b.iz0 = m->box[4]; b.iz1 = m->box[5];
+  for (int q = 0; q < 6; ++q) b.off[q] = m->off[q];
+  b.n_face = m->n_face;
+  return b;
+}
+
+static int ensure_scratch(hfdtd_n2f *m, size_t n) {
+  if (m->scratch_len >= n) return HFDTD_OK;
+  if (m->scratch) (void)hipFree(m->scratch);
+  m->scratch = nullptr;
+  m->scratch_len = 0;
+  HF_HIP(hipMalloc((void **)&m->scratch, n * sizeof(double)));
+  m->scratch_len = n;
+  return HFDTD_OK;
+}
+
+template <typename T, typename A, bool KAHAN>
+static int accumulate_t(hfdtd_n2f *m, const double *phase) {
+  Engine *e = m->e;
+  const Dims3 g{e->nx, e->ny, e->nz};
+  const Box b = box_of(m);
+  for (int f0 = 0; f0 < m->nfreq; f0 += kMaxFreqPerLaunch) {
+    const int nf = std::min(kMaxFreqPerLaunch, m->nfreq - f0);
+    PhaseArg<A> ph;
+    for (int q = 0; q < nf; ++q)
+      for (int c = 0; c < 4; ++c) ph.v[q][c] = A(phase[(f0 + q) * 4 + c]);
+    hipLaunchKernelGGL((k_n2f_accumulate<T, A, KAHAN>), dim3(ceil_div(m->n_face, 256)),
+                       dim3(256), 0, e->stream, g, b, m->nfreq, f0, nf,
+                       (const T *)e->field[0], (const T *)e->field[1], (const T *)e->field[2],
+                       (const T *)e->field[3], (const T *)e->field[4], (const T *)e->field[5],
+                       (A *)m->acc, (A *)m->cmp, ph);
+    HF_HIP(hipGetLastError());
+  }
+  return HFDTD_OK;
+}
+
+template <typename G, typename A>
+static int farfield_t(hfdtd_n2f *m, int fr, const double *obs, int n_obs, double k_wave,
+                      double eta0, int l_sign, double *eh) {
+  Engine *e = m->e;
+  const int chunk = 2048;
+  const int nblk = std::max(1, (int)ceil_div(m->n_face, chunk));
+  const size_t need = (size_t)n_obs * nblk * 12 + (size_t)n_obs * 3 + (size_t)n_obs * 12;
+  int rc = ensure_scratch(m, need);
+  if (rc) return rc;
+  double *part = m->scratch;
+  double *d_obs = part + (size_t)n_obs * nblk * 12;
+  double *d_eh = d_obs + (size_t)n_obs * 3;
+  HF_HIP(hipMemcpyAsync(d_obs, obs, (size_t)n_obs * 3 * sizeof(double), hipMemcpyHostToDevice,
+                        e->stream));
+  hipLaunchKernelGGL((k_n2f_nl<G, A>), dim3(nblk, n_obs), dim3(256), 0, e->stream, box_of(m),
+                     m->nfreq, fr, chunk, (const A *)m->acc, d_obs, G(k_wave), G(e->dl), part);
+  HF_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_n2f_eh<A>), dim3(ceil_div(n_obs, 64)), dim3(64), 0, e->stream, n_obs,
+                     nblk, part, d_obs, A(k_wave), A(eta0), A(l_sign), d_eh);
+  HF_HIP(hipGetLastError());
+  HF_HIP(hipMemcpyAsync(eh, d_eh, (size_t)n_obs * 12 * sizeof(double), hipMemcpyDeviceToHost,
+                        e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  return HFDTD_OK;
+}
+
+}  // namespace hfdtd
+
+using namespace hfdtd;
+
+extern "C" {
+
+int hfdtd_n2f_create(hfdtd_engine *h, const int box[6], int nfreq, int acc_mode,
+                     hfdtd_n2f **out) {
+  if (!h || !box || !out) return fail(HFDTD_EINVAL, "NULL argument");
+  *out = nullptr;
+  Engine *e = &h->s;
+  if (nfreq < 1) return fail(HFDTD_EINVAL, "nfreq must be >= 1");
+  if (acc_mode < HFDTD_ACC_REAL || acc_mode > HFDTD_ACC_KAHAN)
+    return fail(HFDTD_EINVAL, "unknown accumulator mode");
+  if (acc_mode == HFDTD_ACC_KAHAN && e->fp64)
+    return fail(HFDTD_EUNSUPPORTED, "Kahan accumulation is for float32 engines");
+  const int lim[3] = {e->nx, e->ny, e->nz};
+  for (int a = 0; a < 3; ++a)
+    if (box[2 * a] < 0 || box[2 * a + 1] >= lim[a] || box[2 * a] > box[2 * a + 1])
+      return fail(HFDTD_EINVAL, "monitor box outside the grid or empty");
+  hfdtd_n2f *m = new hfdtd_n2f();
+  m->e = e;
+  for (int q = 0; q < 6; ++q) m->box[q] = box[q];
+  m->nfreq = nfreq;
+  m->acc_mode = acc_mode;
+  const long nxf = box[1] - box[0] + 1, nyf = box[3] - box[2] + 1, nzf = box[5] - box[4] + 1;
+  const long counts[6] = {nyf * nzf, nyf * nzf, nxf * nzf, nxf * nzf, nxf * nyf, nxf * nyf};
+  long tot = 0;
+  for (int q = 0; q < 6; ++q) {
+    m->off[q] = (int)tot;
+    tot += counts[q];
+  }
+  m->n_face = (int)tot;
+  const bool acc64 = acc_mode == HFDTD_ACC_FP64 || (acc_mode == HFDTD_ACC_REAL && e->fp64);
+  m->acc_elem = acc64 ? 8 : 4;
+  const size_t bytes = (size_t)6 * nfreq * tot * 2 * m->acc_elem;
+  hipError_t err = hipSetDevice(e->device);
+  if (err == hipSuccess) err = hipMalloc(&m->acc, bytes);
+  if (err == hipSuccess && acc_mode == HFDTD_ACC_KAHAN) err = hipMalloc(&m->cmp, bytes);
+  if (err == hipSuccess) err = hipMemsetAsync(m->acc, 0, bytes, e->stream);
+  if (err == hipSuccess && m->cmp) err = hipMemsetAsync(m->cmp, 0, bytes, e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) {
+    int rc = hip_fail(err, "hfdtd_n2f_create");
+    if (m->acc) (void)hipFree(m->acc);
+    if (m->cmp) (void)hipFree(m->cmp);
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return HFDTD_OK;
+}
+
+int hfdtd_n2f_destroy(hfdtd_n2f *m) {
+  if (!m) return HFDTD_OK;
+  (void)hipSetDevice(m->e->device);
+  (void)hipStreamSynchronize(m->e->stream);
+  if (m->acc) (void)hipFree(m->acc);
+  if (m->cmp) (void)hipFree(m->cmp);
+  if (m->snap) (void)hipFree(m->snap);
+  if (m->scratch) (void)hipFree(m->scratch);
+  delete m;
+  return HFDTD_OK;
+}
+
+int hfdtd_n2f_layout(const hfdtd_n2f *m, int *n_face, int offsets[6]) {
+  if (!m) return fail(HFDTD_EINVAL, "NULL argument");
+  if (n_face) *n_face = m->n_face;
+  if (offsets)
+    for (int q = 0; q < 6; ++q) offsets[q] = m->off[q];
+  return HFDTD_OK;
+}
+
+int hfdtd_n2f_accumulate(hfdtd_n2f *m, const double *phase) {
+  if (!m || !phase) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = m->e;
+  HF_HIP(hipSetDevice(e->device));
+  if (e->fp64) return accumulate_t<double, double, false>(m, phase);
+  if (m->acc_mode == HFDTD_ACC_FP64) return accumulate_t<float, double, false>(m, phase);
+  if (m->acc_mode == HFDTD_ACC_KAHAN) return accumulate_t<float, float, true>(m, phase);
+  return accumulate_t<float, float, false>(m, phase);
+}
+
+int hfdtd_n2f_snapshot(hfdtd_n2f *m) {
+  if (!m) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = m->e;
+  HF_HIP(hipSetDevice(e->device));
+  const size_t bytes = (size_t)6 * m->nfreq * m->n_face * 2 * m->acc_elem;
+  if (!m->snap) HF_HIP(hipMalloc(&m->snap, bytes));
+  HF_HIP(hipMemcpyAsync(m->snap, m->acc, bytes, hipMemcpyDeviceToDevice, e->stream));
+  return HFDTD_OK;
+}
+
+int hfdtd_n2f_rel_change(hfdtd_n2f *m, int freq, double *num, double *den) {
+  if (!m || !num || !den) return fail(HFDTD_EINVAL, "NULL argument");
+  if (freq < 0 || freq >= m->nfreq) return fail(HFDTD_EINVAL, "frequency index out of range");
+  if (!m->snap) return fail(HFDTD_EINVAL, "no snapshot");
+  Engine *e = m->e;
+  HF_HIP(hipSetDevice(e->device));
+  const int nblk = (int)ceil_div(m->n_face, 256);
+  int rc = ensure_scratch(m, (size_t)2 * nblk);
+  if (rc) return rc;
+  if (m->acc_elem == 8)
+    hipLaunchKernelGGL((k_n2f_relchange<double>), dim3(nblk), dim3(256), 0, e->stream,
+                       m->n_face, m->nfreq, freq, (const double *)m->acc,
+                       (const double *)m->snap, m->scratch);
+  else
+    hipLaunchKernelGGL((k_n2f_relchange<float>), dim3(nblk), dim3(256), 0, e->stream,
+                       m->n_face, m->nfreq, freq, (const float *)m->acc,
+                       (const float *)m->snap, m->scratch);
+  HF_HIP(hipGetLastError());
+  std::vector<double> host((size_t)2 * nblk);
+  HF_HIP(hipMemcpyAsync(host.data(), m->scratch, host.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  double sn = 0.0, sd = 0.0;
+  for (int q = 0; q < nblk; ++q) {
+    sn += host[2 * q];
+    sd += host[2 * q + 1];
+  }
+  *num = sn;
+  *den = sd;
+  return HFDTD_OK;
+}
+
+int hfdtd_n2f_read(hfdtd_n2f *m, int comp, int freq, double *host) {
+  if (!m || !host) return fail(HFDTD_EINVAL, "NULL argument");
+  if (comp < 0 || comp > 5 || freq < 0 || freq >= m->nfreq)
+    return fail(HFDTD_EINVAL, "component or frequency out of range");
+  Engine *e = m->e;
+  HF_HIP(hipSetDevice(e->device));
+  const size_t n2 = (size_t)m->n_face * 2;
+  const size_t off = ((size_t)comp * m->nfreq + freq) * n2;
+  if (m->acc_elem == 8) {
+    HF_HIP(hipMemcpyAsync(host, (const double *)m->acc + off, n2 * 8, hipMemcpyDeviceToHost,
+                          e->stream));
+    HF_HIP(hipStreamSynchronize(e->stream));
+  } else {
+    std::vector<float> tmp(n2);
+    HF_HIP(hipMemcpyAsync(tmp.data(), (const float *)m->acc + off, n2 * 4,
+                          hipMemcpyDeviceToHost, e->stream));
+    HF_HIP(hipStreamSynchronize(e->stream));
+    for (size_t q = 0; q < n2; ++q) host[q] = tmp[q];
+  }
+  return HFDTD_OK;
+}
+
+int hfdtd_n2f_farfield(hfdtd_n2f *m, int freq, const double *obs, int n_obs, double k_wave,
+                       double eta0, int l_sign, int geom_fp64, double *eh) {
+  if (!m || !obs || !eh) return fail(HFDTD_EINVAL, "NULL argument");
+  if (n_obs < 1) return fail(HFDTD_EINVAL, "n_obs must be >= 1");
+  if (freq < 0 || freq >= m->nfreq) return fail(HFDTD_EINVAL, "frequency index out of range");
+  if (l_sign != 1 && l_sign != -1) return fail(HFDTD_EINVAL, "l_sign must be +1 or -1");
+  Engine *e = m->e;
+  HF_HIP(hipSetDevice(e->device));
+  const bool g64 = geom_fp64 || e->fp64;
+  if (m->acc_elem == 8)
+    return g64 ? farfield_t<double, double>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh)
+               : farfield_t<float, double>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh);
+  return g64 ? farfield_t<double, float>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh)
+             : farfield_t<float, float>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh);
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+"""ctypes binding of ``libhipfdtd.so`` (declared in ``include/hipfdtd.h``).
+
+The library must exist: there is no CPU fallback.  ``load()`` raises
+``RuntimeError`` naming the missing file, so a product path can never run on
+anything but the HIP kernels.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(os.environ.get("HIPFDTD_LIB", Path(__file__).resolve().parent / "libhipfdtd.so"))
+
+OK = 0
+EINVAL = -1
+ENOMEM = -2
+EHIP = -3
+EUNSUPPORTED = -4
+ENODEV = -5
+
+VOLUME_IDS = {
+    "Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5,
+    "ce_x": 6, "ce_y": 7, "ce_z": 8, "ca_x": 9, "ca_y": 10, "ca_z": 11,
+}  # fmt: skip
+
+SRC_SOFT = 0
+SRC_CURRENT = 1
+ACC_REAL = 0
+ACC_FP64 = 1
+ACC_KAHAN = 2
+PATH_SPLIT = 0
+PATH_FUSED = 1
+
+
+class Source(C.Structure):
+    """``hfdtd_source``: one soft or current source on a box."""
+
+    _fields_ = [
+        ("comp", C.c_int),
+        ("mode", C.c_int),
+        ("i0", C.c_int), ("i1", C.c_int),
+        ("j0", C.c_int), ("j1", C.c_int),
+        ("k0", C.c_int), ("k1", C.c_int),
+        ("taper", C.c_int),
+        ("rim_edge", C.c_double),
+        ("amp", C.c_double),
+    ]  # fmt: skip
+
+
+EXPORTS = {
+    "hfdtd_last_error": (C.c_char_p, []),
+    "hfdtd_version": (C.c_int, []),
+    "hfdtd_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "hfdtd_device_info": (
+        C.c_int,
+        [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_char_p, C.c_int],
+    ),
+    "hfdtd_cpml_profile": (
+        C.c_int,
+        [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+         C.c_void_p, C.c_void_p, C.c_void_p],
+    ),  # fmt: skip
+    "hfdtd_create": (
+        C.c_int,
+        [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+         C.POINTER(C.c_void_p)],
+    ),  # fmt: skip
+    "hfdtd_destroy": (C.c_int, [C.c_void_p]),
+    "hfdtd_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "hfdtd_set_materials": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "hfdtd_write_volume": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "hfdtd_read_volume": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "hfdtd_read_point": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_double)]),
+    "hfdtd_reset_fields": (C.c_int, [C.c_void_p]),
+    "hfdtd_update_H": (C.c_int, [C.c_void_p]),
+    "hfdtd_update_E": (C.c_int, [C.c_void_p]),
+    "hfdtd_add_sources": (C.c_int, [C.c_void_p, C.POINTER(Source), C.c_int]),
+    "hfdtd_step": (C.c_int, [C.c_void_p, C.POINTER(Source), C.c_int, C.c_int]),
+    "hfdtd_synchronize": (C.c_int, [C.c_void_p]),
+    "hfdtd_set_fused_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "hfdtd_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfdtd_timing_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "hfdtd_n2f_create": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_void_p)],
+    ),
+    "hfdtd_n2f_destroy": (C.c_int, [C.c_void_p]),
+    "hfdtd_n2f_layout": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "hfdtd_n2f_accumulate": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "hfdtd_n2f_snapshot": (C.c_int, [C.c_void_p]),
+    "hfdtd_n2f_rel_change": (
+        C.c_int,
+        [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    ),
+    "hfdtd_n2f_read": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "hfdtd_n2f_farfield": (
+        C.c_int,
+        [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+         C.c_void_p],
+    ),  # fmt: skip
+}
+
+_LIB = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library once; raise RuntimeError when it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"HIP library {LIB_PATH} is missing: build it with `python -m hipfdtd.build` "
+            "(hipcc --offload-arch=gfx950). There is no CPU fallback."
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load().hfdtd_last_error()
+    return msg.decode(errors="replace") if msg else ""
+
+
+def check(rc: int, what: str = "") -> None:
+    """Map a status code to the reference's exception types."""
+    if rc == OK:
+        return
+    msg = last_error()
+    text = f"{what}: {msg}" if what else msg
+    if rc in (EINVAL, EUNSUPPORTED):
+        raise ValueError(text)
+    if rc == ENOMEM:
+        raise MemoryError(text)
+    raise RuntimeError(text)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = load().hfdtd_device_count(C.byref(n))
+    return int(n.value) if rc == OK else 0
+
+
+def device_info(device: int = 0):
+    """(total_bytes, free_bytes, name) of a HIP device."""
+    tot = C.c_uint64(0)
+    free = C.c_uint64(0)
+    name = C.create_string_buffer(256)
+    check(load().hfdtd_device_info(int(device), C.byref(tot), C.byref(free), name, 256),
+          "hfdtd_device_info")  # fmt: skip
+    return int(tot.value), int(free.value), name.value.decode(errors="replace")
