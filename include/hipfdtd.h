@@ -61,6 +61,8 @@ extern "C" {
 
 typedef struct hfdtd_engine hfdtd_engine;
 typedef struct hfdtd_n2f hfdtd_n2f;
+typedef struct hfdtd_probe hfdtd_probe;
+typedef struct hfdtd_batch hfdtd_batch;
 
 /* One current / soft source on a box [i0,i1)x[j0,j1)x[k0,k1) of component
  * comp (0=x,1=y,2=z).  Bounds may extend past the grid (clipped); rim-taper
@@ -173,6 +175,44 @@ int hfdtd_n2f_read(hfdtd_n2f *m, int comp, int freq, double *host);
  * geom_fp64 selects double (1) or the field dtype (0) for r.r' phases. */
 int hfdtd_n2f_farfield(hfdtd_n2f *m, int freq, const double *obs, int n_obs, double k_wave,
                        double eta0, int l_sign, int geom_fp64, double *eh);
+
+/* ── point probe (device-side DFT of one field sample) ──────────────────── */
+
+/* Replaces the reference harness's per-step read_point + host DFT
+ * (tests/meep_validation/cases.py:179-197): accumulates
+ * sum_n F(i,j,k) * phase_n[f] in FP64 on the device, one launch per step.
+ * comp: 0..5 = Ex..Hz. */
+int hfdtd_probe_create(hfdtd_engine *e, int i, int j, int k, int comp, int nfreq,
+                       hfdtd_probe **out);
+int hfdtd_probe_destroy(hfdtd_probe *p);
+/* phase: nfreq x {re, im} for this step (already including dt) */
+int hfdtd_probe_accumulate(hfdtd_probe *p, const double *phase);
+/* out: nfreq x {re, im} */
+int hfdtd_probe_read(hfdtd_probe *p, double *out);
+
+/* ── batched sweep of independent simulations ─────────────────────────── */
+
+/* No counterpart in the reference (it runs one grid at a time).  Groups n
+ * engines of any sizes (same dl, dtype and device) so one persistent launch
+ * per step advances all of them with the fused sweep; the engines stay usable
+ * for reads afterwards.  Each engine may carry one current/soft source whose
+ * amplitude per step comes from a table, and one point probe whose per-step
+ * phases come from a table. */
+int hfdtd_batch_create(hfdtd_engine *const *engines, int n, hfdtd_batch **out);
+int hfdtd_batch_destroy(hfdtd_batch *b);
+/* src: n descriptors (amp ignored); amp: n x n_steps table (row per engine) */
+int hfdtd_batch_set_sources(hfdtd_batch *b, const hfdtd_source *src, const double *amp,
+                            int n_steps);
+/* ijk: n x 3 cells; comp 0..5; phase: n x n_steps x nfreq x {re, im} */
+int hfdtd_batch_set_probes(hfdtd_batch *b, const int *ijk, int comp, int nfreq,
+                           const double *phase, int n_steps);
+/* advance every engine n_steps steps; graph_steps > 0 captures that many
+ * steps in a HIP graph and replays it (remainder launched directly) */
+int hfdtd_batch_run(hfdtd_batch *b, int n_steps, int graph_steps);
+/* probe DFTs: n x nfreq x {re, im} */
+int hfdtd_batch_read_probes(hfdtd_batch *b, double *out);
+/* persistent grid size (0 = one block per tile) and rows of j per block */
+int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i);
 
 #ifdef __cplusplus
 }

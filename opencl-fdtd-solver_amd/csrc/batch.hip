@@ -1,0 +1,538 @@
+// Point probe and the batched sweep of many independent simulations.
+//
+// The batched sweep advances a set of engines of uneven sizes with one
+// persistent launch per step: a flat work list holds every (engine, tile)
+// pair, each block loops over work items with stride gridDim.x and runs the
+// same fused tile body as the single-engine sweep.  The step index lives in
+// device memory (bumped by a one-thread kernel after each step), so a run of
+// K steps can be captured once as a HIP graph and replayed.  Source
+// amplitudes and probe phases come from per-engine tables indexed by that
+// step, which keeps every launch argument fixed across replays.
+
+#include <algorithm>
+#include <vector>
+
+#include "fdtd_core.h"
+
+struct hfdtd_probe {
+  hfdtd::Engine *e = nullptr;
+  int i = 0, j = 0, k = 0, comp = 0, nfreq = 0;
+  double *acc = nullptr;  // nfreq x {re, im}, device
+};
+
+namespace hfdtd {
+
+constexpr int kProbeFreqPerLaunch = 16;
+
+struct ProbePhase {
+  double v[kProbeFreqPerLaunch][2];
+};
+
+template <typename T>
+__global__ void k_probe_accumulate(const T *__restrict__ F, size_t idx, int f0, int nf,
+                                   ProbePhase ph, double *__restrict__ acc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double v = (double)F[idx];
+  for (int q = 0; q < nf; ++q) {
+    acc[2 * (f0 + q)] += v * ph.v[q][0];
+    acc[2 * (f0 + q) + 1] += v * ph.v[q][1];
+  }
+}
+
+// ── batch ─────────────────────────────────────────────────────────────────
+
+template <typename T>
+struct SimDesc {
+  Geo g;
+  T *st[2];   // state slab, by step parity
+  T *psi[2];  // psi slab, by step parity
+  const T *coef;
+  const T *prof;
+  int tile_begin, tiles_k, tiles_j;
+  int lossy;
+  int has_src;
+  Src<T> src;
+  int has_probe, pi, pj, pk;
+};
+
+template <typename T>
+struct BatchArgs {
+  const SimDesc<T> *sims;
+  const int *tile_sim;
+  int n_tiles;
+  const int *step;  // steps taken since the tables were set
+  int step0;        // value of *step when `sims` was built (parity origin)
+  int planes;
+  T dtm_dl, dtm, inv_dl;
+  const double *amp;  // n x n_table
+  const double *phase;  // n x n_table x nfreq x 2
+  int n_table;
+  int nfreq, comp;
+  double *probe_acc;  // n x nfreq x 2
+};
+
+template <typename T>
+struct BatchSrcP {
+  const Src<T> *s;
+  int n;
+  T a;
+  __device__ int count() const { return n; }
+  __device__ const Src<T> &at(int) const { return *s; }
+  __device__ T amp(int) const { return a; }
+};
+
+template <typename T>
+struct ProbeHook {
+  int on, pi, pj, pk, comp, nf;
+  const double *ph;
+  double *acc;
+  __device__ void operator()(int i, int j, int k, T ex, T ey, T ez) {
+    if (!on || i != pi || j != pj || k != pk) return;
+    const double v = (double)(comp == 0 ? ex : (comp == 1 ? ey : ez));
+    for (int f = 0; f < nf; ++f) {
+      acc[2 * f] += v * ph[2 * f];
+      acc[2 * f + 1] += v * ph[2 * f + 1];
+    }
+  }
+};
+
+// LOSSY selects the update with the ca load; it is exact for lossless
+// engines too (ca = 1), so one variant serves a mixed batch.
+template <typename T, int TJ, bool LOSSY>
+__global__ __launch_bounds__(64 * TJ) void k_batch_step(BatchArgs<T> a) {
+  __shared__ T sE[3][TJ][64];
+  __shared__ T sH[3][TJ + 1][65];
+  const int step = *a.step;
+  const int par = (step - a.step0) & 1;
+  for (int t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+    const int s = a.tile_sim[t];
+    const SimDesc<T> &d = a.sims[s];
+    const Geo g = d.g;
+    const int lt = t - d.tile_begin;
+    const int bx = lt % d.tiles_k;
+    const int rest = lt / d.tiles_k;
+    const int by = rest % d.tiles_j;
+    const int bz = rest / d.tiles_j;
+    const size_t row = (size_t)s * a.n_table + step;
+    BatchSrcP<T> sp{&d.src, d.has_src, d.has_src ? T(a.amp[row]) : T(0)};
+    ProbeHook<T> hk{d.has_probe, d.pi, d.pj, d.pk, a.comp, a.nfreq,
+                    d.has_probe ? a.phase + row * a.nfreq * 2 : nullptr,
+                    a.probe_acc + (size_t)s * a.nfreq * 2};
+    lds_barrier();  // LDS of the previous work item is free
+    fused_tile<T, TJ, LOSSY>(g, a.planes, a.dtm_dl, a.dtm, a.inv_dl, d.st[par], d.st[par ^ 1],
+                             d.coef, d.prof, d.psi[par], d.psi[par ^ 1], sp, hk, bx, by, bz, sE,
+                             sH);
+  }
+}
+
+__global__ void k_step_advance(int *step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(step, 1);
+}
+
+}  // namespace hfdtd
+
+struct hfdtd_batch {
+  std::vector<hfdtd::Engine *> eng;
+  hipStream_t stream = nullptr;
+  int fp64 = 0;
+  void *d_desc = nullptr;
+  int *d_tile_sim = nullptr;
+  int n_tiles = 0;
+  int *d_step = nullptr;
+  int step_host = 0;  // mirror of *d_step
+  std::vector<hfdtd_source> src;
+  std::vector<int> has_src;
+  double *d_amp = nullptr;
+  int n_amp = 0;
+  std::vector<int> probe;  // n x 3, -1 = none
+  int nfreq = 0, comp = 0;
+  double *d_phase = nullptr;
+  int n_phase = 0;
+  double *d_probe_acc = nullptr;
+  int blocks = 0, rows = 4, planes = 16;
+  bool lossy = false;
+  hipGraphExec_t gexec = nullptr;
+};
+
+namespace hfdtd {
+
+static int batch_free_graph(hfdtd_batch *b) {
+  if (b->gexec) {
+    HF_HIP(hipGraphExecDestroy(b->gexec));
+    b->gexec = nullptr;
+  }
+  return HFDTD_OK;
+}
+
+// Build the per-engine descriptors (parity origin = current step) and the
+// tile work list.
+template <typename T>
+static int batch_build(hfdtd_batch *b) {
+  const int n = (int)b->eng.size();
+  const int TJ = b->rows;
+  const int P = b->planes;
+  std::vector<SimDesc<T>> desc(n);
+  std::vector<int> tile_sim;
+  int begin = 0;
+  b->lossy = false;
+  for (int s = 0; s < n; ++s) {
+    Engine *e = b->eng[s];
+    b->lossy = b->lossy || !e->lossless;
+    int rc = ensure_alt(e);
+    if (rc) return rc;
+    SimDesc<T> &d = desc[s];
+    d.g.nx = e->nx;
+    d.g.ny = e->ny;
+    d.g.nz = e->nz;
+    d.g.npml = e->npml;
+    d.g.vs = (long long)e->vstride;
+    d.g.plen = (int)e->prof_len();
+    for (int q = 0; q <= PSI_COUNT; ++q) d.g.psi_off[q] = (int)e->psi_off[q];
+    // parity 0 (relative to step0) is the engine's current state
+    d.st[0] = (T *)e->state[e->cur];
+    d.st[1] = (T *)e->state[e->cur ^ 1];
+    d.psi[0] = (T *)e->psi[e->cur];
+    d.psi[1] = (T *)e->psi[e->cur ^ 1];
+    d.coef = (const T *)e->coef;
+    d.prof = (const T *)e->prof;
+    d.tiles_k = (int)ceil_div(e->nz, 64);
+    d.tiles_j = (int)ceil_div(e->ny, TJ);
+    const int tiles_i = (int)ceil_div(e->nx, P);
+    d.tile_begin = begin;
+    const int nt = d.tiles_k * d.tiles_j * tiles_i;
+    tile_sim.insert(tile_sim.end(), nt, s);
+    begin += nt;
+    d.lossy = e->lossless ? 0 : 1;
+    d.has_src = b->has_src.empty() ? 0 : b->has_src[s];
+    if (d.has_src) {
+      Src<T> o;
+      const hfdtd_source &in = b->src[s];
+      o.comp = in.comp;
+      o.mode = in.mode;
+      o.taper = in.taper & 7;
+      o.i0 = in.i0; o.i1 = in.i1; o.j0 = in.j0; o.j1 = in.j1; o.k0 = in.k0; o.k1 = in.k1;
+      auto clip = [](int a, int m) { return a < 0 ? 0 : (a > m ? m : a); };
+      o.ci0 = clip(in.i0, e->nx); o.ci1 = std::max(o.ci0, clip(in.i1, e->nx));
+      o.cj0 = clip(in.j0, e->ny); o.cj1 = std::max(o.cj0, clip(in.j1, e->ny));
+      o.ck0 = clip(in.k0, e->nz); o.ck1 = std::max(o.ck0, clip(in.k1, e->nz));
+      o.amp = T(0);
+      o.re = T(in.rim_edge);
+      d.src = o;
+    }
+    d.has_probe = 0;
+    d.pi = d.pj = d.pk = -1;
+    if (!b->probe.empty() && b->probe[3 * s] >= 0) {
+      d.has_probe = 1;
+      d.pi = b->probe[3 * s];
+      d.pj = b->probe[3 * s + 1];
+      d.pk = b->probe[3 * s + 2];
+    }
+  }
+  int rc = batch_free_graph(b);
+  if (rc) return rc;
+  if (b->d_desc) HF_HIP(hipFree(b->d_desc));
+  if (b->d_tile_sim) HF_HIP(hipFree(b->d_tile_sim));
+  b->d_desc = nullptr;
+  b->d_tile_sim = nullptr;
+  HF_HIP(hipMalloc(&b->d_desc, sizeof(SimDesc<T>) * n));
+  HF_HIP(hipMalloc((void **)&b->d_tile_sim, sizeof(int) * tile_sim.size()));
+  HF_HIP(hipMemcpyAsync(b->d_desc, desc.data(), sizeof(SimDesc<T>) * n, hipMemcpyHostToDevice,
+                        b->stream));
+  HF_HIP(hipMemcpyAsync(b->d_tile_sim, tile_sim.data(), sizeof(int) * tile_sim.size(),
+                        hipMemcpyHostToDevice, b->stream));
+  HF_HIP(hipStreamSynchronize(b->stream));
+  b->n_tiles = (int)tile_sim.size();
+  return HFDTD_OK;
+}
+
+template <typename T>
+static BatchArgs<T> batch_args(const hfdtd_batch *b) {
+  const Engine *e = b->eng[0];
+  BatchArgs<T> a;
+  a.sims = (const SimDesc<T> *)b->d_desc;
+  a.tile_sim = b->d_tile_sim;
+  a.n_tiles = b->n_tiles;
+  a.step = b->d_step;
+  a.step0 = b->step_host;
+  a.planes = b->planes;
+  a.dtm_dl = T(e->dtm_dl);
+  a.dtm = T(e->dtm);
+  a.inv_dl = T(e->inv_dl);
+  a.amp = b->d_amp;
+  a.phase = b->d_phase;
+  a.n_table = std::max(b->n_amp, b->n_phase);
+  a.nfreq = b->nfreq;
+  a.comp = b->comp;
+  a.probe_acc = b->d_probe_acc;
+  return a;
+}
+
+template <typename T>
+static int batch_launch_step(hfdtd_batch *b, const BatchArgs<T> &a) {
+  const int grid = b->blocks > 0 ? std::min(b->blocks, b->n_tiles) : b->n_tiles;
+#define HF_BATCH(TJ)                                                                          \
+  if (b->lossy)                                                                               \
+    hipLaunchKernelGGL((k_batch_step<T, TJ, true>), dim3(grid), dim3(64, TJ), 0, b->stream, a); \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_batch_step<T, TJ, false>), dim3(grid), dim3(64, TJ), 0, b->stream, a)
+  switch (b->rows) {
+    case 2: HF_BATCH(2); break;
+    case 8: HF_BATCH(8); break;
+    default: HF_BATCH(4); break;
+  }
+#undef HF_BATCH
+  HF_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, b->stream, b->d_step);
+  HF_HIP(hipGetLastError());
+  return HFDTD_OK;
+}
+
+template <typename T>
+static int batch_run_t(hfdtd_batch *b, int n_steps, int graph_steps) {
+  int rc = batch_build<T>(b);
+  if (rc) return rc;
+  const BatchArgs<T> a = batch_args<T>(b);
+  int done = 0;
+  if (graph_steps > 0 && n_steps >= graph_steps) {
+    hipGraph_t graph;
+    HF_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int q = 0; q < graph_steps; ++q) {
+      rc = batch_launch_step<T>(b, a);
+      if (rc) {
+        hipGraph_t junk;
+        (void)hipStreamEndCapture(b->stream, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        return rc;
+      }
+    }
+    HF_HIP(hipStreamEndCapture(b->stream, &graph));
+    hipError_t err = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (err != hipSuccess) return hip_fail(err, "hipGraphInstantiate");
+    for (; done + graph_steps <= n_steps; done += graph_steps)
+      HF_HIP(hipGraphLaunch(b->gexec, b->stream));
+  }
+  for (; done < n_steps; ++done)
+    if ((rc = batch_launch_step<T>(b, a))) return rc;
+  HF_HIP(hipStreamSynchronize(b->stream));
+  b->step_host += n_steps;
+  if (n_steps & 1)
+    for (Engine *e : b->eng) e->cur ^= 1;
+  return HFDTD_OK;
+}
+
+}  // namespace hfdtd
+
+using namespace hfdtd;
+
+extern "C" {
+
+int hfdtd_probe_create(hfdtd_engine *h, int i, int j, int k, int comp, int nfreq,
+                       hfdtd_probe **out) {
+  if (!h || !out) return fail(HFDTD_EINVAL, "NULL argument");
+  *out = nullptr;
+  Engine *e = &h->s;
+  if (i < 0 || i >= e->nx || j < 0 || j >= e->ny || k < 0 || k >= e->nz)
+    return fail(HFDTD_EINVAL, "probe cell outside the grid");
+  if (comp < 0 || comp > 5 || nfreq < 1) return fail(HFDTD_EINVAL, "bad component or nfreq");
+  HF_HIP(hipSetDevice(e->device));
+  hfdtd_probe *p = new hfdtd_probe();
+  p->e = e;
+  p->i = i;
+  p->j = j;
+  p->k = k;
+  p->comp = comp;
+  p->nfreq = nfreq;
+  hipError_t err = hipMalloc((void **)&p->acc, sizeof(double) * 2 * nfreq);
+  if (err == hipSuccess) {
+    std::vector<double> z(2 * nfreq, 0.0);
+    err = hipMemcpy(p->acc, z.data(), sizeof(double) * 2 * nfreq, hipMemcpyHostToDevice);
+  }
+  if (err != hipSuccess) {
+    if (p->acc) (void)hipFree(p->acc);
+    delete p;
+    return hip_fail(err, "hfdtd_probe_create");
+  }
+  *out = p;
+  return HFDTD_OK;
+}
+
+int hfdtd_probe_destroy(hfdtd_probe *p) {
+  if (!p) return HFDTD_OK;
+  (void)hipSetDevice(p->e->device);
+  (void)hipStreamSynchronize(p->e->stream);
+  if (p->acc) (void)hipFree(p->acc);
+  delete p;
+  return HFDTD_OK;
+}
+
+int hfdtd_probe_accumulate(hfdtd_probe *p, const double *phase) {
+  if (!p || !phase) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = p->e;
+  HF_HIP(hipSetDevice(e->device));
+  const size_t idx = ((size_t)p->i * e->ny + p->j) * e->nz + p->k;
+  for (int f0 = 0; f0 < p->nfreq; f0 += kProbeFreqPerLaunch) {
+    const int nf = std::min(kProbeFreqPerLaunch, p->nfreq - f0);
+    ProbePhase ph;
+    for (int q = 0; q < nf; ++q) {
+      ph.v[q][0] = phase[2 * (f0 + q)];
+      ph.v[q][1] = phase[2 * (f0 + q) + 1];
+    }
+    if (e->fp64)
+      hipLaunchKernelGGL(k_probe_accumulate<double>, dim3(1), dim3(64), 0, e->stream,
+                         (const double *)e->field(p->comp), idx, f0, nf, ph, p->acc);
+    else
+      hipLaunchKernelGGL(k_probe_accumulate<float>, dim3(1), dim3(64), 0, e->stream,
+                         (const float *)e->field(p->comp), idx, f0, nf, ph, p->acc);
+    HF_HIP(hipGetLastError());
+  }
+  return HFDTD_OK;
+}
+
+int hfdtd_probe_read(hfdtd_probe *p, double *out) {
+  if (!p || !out) return fail(HFDTD_EINVAL, "NULL argument");
+  HF_HIP(hipSetDevice(p->e->device));
+  HF_HIP(hipMemcpyAsync(out, p->acc, sizeof(double) * 2 * p->nfreq, hipMemcpyDeviceToHost,
+                        p->e->stream));
+  HF_HIP(hipStreamSynchronize(p->e->stream));
+  return HFDTD_OK;
+}
+
+int hfdtd_batch_create(hfdtd_engine *const *engines, int n, hfdtd_batch **out) {
+  if (!engines || !out || n < 1) return fail(HFDTD_EINVAL, "bad engine list");
+  *out = nullptr;
+  const Engine *e0 = &engines[0]->s;
+  for (int s = 0; s < n; ++s) {
+    if (!engines[s]) return fail(HFDTD_EINVAL, "NULL engine");
+    const Engine *e = &engines[s]->s;
+    if (e->fp64 != e0->fp64 || e->dl != e0->dl || e->dt != e0->dt || e->device != e0->device)
+      return fail(HFDTD_EINVAL, "batched engines need the same dl, dt, dtype and device");
+  }
+  hfdtd_batch *b = new hfdtd_batch();
+  for (int s = 0; s < n; ++s) b->eng.push_back(&engines[s]->s);
+  b->fp64 = e0->fp64;
+  hipError_t err = hipSetDevice(e0->device);
+  if (err == hipSuccess) err = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (err == hipSuccess) err = hipMalloc((void **)&b->d_step, sizeof(int));
+  if (err == hipSuccess) {
+    const int zero = 0;
+    err = hipMemcpy(b->d_step, &zero, sizeof(int), hipMemcpyHostToDevice);
+  }
+  // the engines' own streams must be idle before batch work starts
+  for (int s = 0; s < n && err == hipSuccess; ++s) err = hipStreamSynchronize(b->eng[s]->stream);
+  if (err != hipSuccess) {
+    int rc = hip_fail(err, "hfdtd_batch_create");
+    hfdtd_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return HFDTD_OK;
+}
+
+int hfdtd_batch_destroy(hfdtd_batch *b) {
+  if (!b) return HFDTD_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  (void)batch_free_graph(b);
+  for (void *p : {(void *)b->d_desc, (void *)b->d_tile_sim, (void *)b->d_step, (void *)b->d_amp,
+                  (void *)b->d_phase, (void *)b->d_probe_acc})
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return HFDTD_OK;
+}
+
+static int reset_step(hfdtd_batch *b) {
+  const int zero = 0;
+  HF_HIP(hipMemcpy(b->d_step, &zero, sizeof(int), hipMemcpyHostToDevice));
+  b->step_host = 0;
+  return HFDTD_OK;
+}
+
+int hfdtd_batch_set_sources(hfdtd_batch *b, const hfdtd_source *src, const double *amp,
+                            int n_steps) {
+  if (!b || !src || !amp || n_steps < 1) return fail(HFDTD_EINVAL, "bad source tables");
+  const int n = (int)b->eng.size();
+  for (int s = 0; s < n; ++s) {
+    if (src[s].comp < 0 || src[s].comp > 2 ||
+        (src[s].mode != HFDTD_SRC_SOFT && src[s].mode != HFDTD_SRC_CURRENT))
+      return fail(HFDTD_EINVAL, "invalid source descriptor");
+  }
+  if (b->n_phase > 0 && b->n_phase != n_steps)
+    return fail(HFDTD_EINVAL, "source and probe tables must cover the same steps");
+  HF_HIP(hipSetDevice(b->eng[0]->device));
+  b->src.assign(src, src + n);
+  b->has_src.assign(n, 1);
+  if (b->d_amp) HF_HIP(hipFree(b->d_amp));
+  b->d_amp = nullptr;
+  HF_HIP(hipMalloc((void **)&b->d_amp, sizeof(double) * n * n_steps));
+  HF_HIP(hipMemcpy(b->d_amp, amp, sizeof(double) * n * n_steps, hipMemcpyHostToDevice));
+  b->n_amp = n_steps;
+  return reset_step(b);
+}
+
+int hfdtd_batch_set_probes(hfdtd_batch *b, const int *ijk, int comp, int nfreq,
+                           const double *phase, int n_steps) {
+  if (!b || !ijk || !phase || nfreq < 1 || n_steps < 1) return fail(HFDTD_EINVAL, "bad probes");
+  if (comp < 0 || comp > 2) return fail(HFDTD_EINVAL, "batched probes record Ex, Ey or Ez");
+  if (b->n_amp > 0 && b->n_amp != n_steps)
+    return fail(HFDTD_EINVAL, "source and probe tables must cover the same steps");
+  const int n = (int)b->eng.size();
+  for (int s = 0; s < n; ++s) {
+    const Engine *e = b->eng[s];
+    if (ijk[3 * s] < 0) continue;
+    if (ijk[3 * s] >= e->nx || ijk[3 * s + 1] < 0 || ijk[3 * s + 1] >= e->ny ||
+        ijk[3 * s + 2] < 0 || ijk[3 * s + 2] >= e->nz)
+      return fail(HFDTD_EINVAL, "probe cell outside its grid");
+  }
+  HF_HIP(hipSetDevice(b->eng[0]->device));
+  b->probe.assign(ijk, ijk + 3 * n);
+  b->comp = comp;
+  b->nfreq = nfreq;
+  if (b->d_phase) HF_HIP(hipFree(b->d_phase));
+  if (b->d_probe_acc) HF_HIP(hipFree(b->d_probe_acc));
+  b->d_phase = nullptr;
+  b->d_probe_acc = nullptr;
+  const size_t np = (size_t)n * n_steps * nfreq * 2;
+  HF_HIP(hipMalloc((void **)&b->d_phase, sizeof(double) * np));
+  HF_HIP(hipMemcpy(b->d_phase, phase, sizeof(double) * np, hipMemcpyHostToDevice));
+  HF_HIP(hipMalloc((void **)&b->d_probe_acc, sizeof(double) * n * nfreq * 2));
+  std::vector<double> z((size_t)n * nfreq * 2, 0.0);
+  HF_HIP(hipMemcpy(b->d_probe_acc, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
+  b->n_phase = n_steps;
+  return reset_step(b);
+}
+
+int hfdtd_batch_run(hfdtd_batch *b, int n_steps, int graph_steps) {
+  if (!b || n_steps < 0 || graph_steps < 0) return fail(HFDTD_EINVAL, "bad run arguments");
+  const int table = std::max(b->n_amp, b->n_phase);
+  if (table > 0 && b->step_host + n_steps > table)
+    return fail(HFDTD_EINVAL, "run goes past the end of the source/probe tables");
+  if (b->nfreq > 0 && b->d_probe_acc == nullptr) return fail(HFDTD_EINVAL, "probes not set");
+  HF_HIP(hipSetDevice(b->eng[0]->device));
+  for (Engine *e : b->eng) HF_HIP(hipStreamSynchronize(e->stream));
+  return b->fp64 ? batch_run_t<double>(b, n_steps, graph_steps)
+                 : batch_run_t<float>(b, n_steps, graph_steps);
+}
+
+int hfdtd_batch_read_probes(hfdtd_batch *b, double *out) {
+  if (!b || !out) return fail(HFDTD_EINVAL, "NULL argument");
+  if (!b->d_probe_acc) return fail(HFDTD_EINVAL, "probes not set");
+  HF_HIP(hipSetDevice(b->eng[0]->device));
+  const size_t n = b->eng.size() * (size_t)b->nfreq * 2;
+  HF_HIP(hipMemcpyAsync(out, b->d_probe_acc, sizeof(double) * n, hipMemcpyDeviceToHost,
+                        b->stream));
+  HF_HIP(hipStreamSynchronize(b->stream));
+  return HFDTD_OK;
+}
+
+int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i) {
+  if (!b || blocks < 0 || planes_i < 0) return fail(HFDTD_EINVAL, "bad launch shape");
+  if (rows_j != 0 && rows_j != 2 && rows_j != 4 && rows_j != 8)
+    return fail(HFDTD_EINVAL, "rows_j must be 2, 4 or 8");
+  b->blocks = blocks;
+  if (rows_j) b->rows = rows_j;
+  if (planes_i) b->planes = planes_i;
+  return HFDTD_OK;
+}
+
+}  // extern "C"

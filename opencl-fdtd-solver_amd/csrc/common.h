@@ -45,21 +45,19 @@ struct Engine {
   double dl = 0, dt = 0;
   double dtm_dl = 0, dtm = 0, inv_dl = 0;  // folded on the host in FP64
   size_t ncell = 0;
-  size_t elem = 4;  // bytes per value
+  size_t vstride = 0;  // elements between volumes of a slab (ncell rounded up to 64)
+  size_t elem = 4;     // bytes per value
   hipStream_t stream = nullptr;
 
-  // Current state.  The split path updates these in place; the fused path
-  // reads them and writes the *_alt partners, then swaps.
-  void *field[6] = {};   // Ex Ey Ez Hx Hy Hz
-  void *field_alt[6] = {};
-  void *ce[3] = {};
-  void *ca[3] = {};
-  void *psi[PSI_COUNT] = {};
-  void *psi_alt[PSI_COUNT] = {};
-  size_t psi_len[3] = {};  // entries of an x-, y-, z-stripe buffer
-
-  // CPML 1-D profiles [stagger 0=E,1=H][axis][b, c, ik]
-  void *prof[2][3][3] = {};
+  // Slabs.  state[cur] holds Ex Ey Ez Hx Hy Hz (each vstride long); the split
+  // sweeps update it in place, the fused sweep reads state[cur], writes
+  // state[cur ^ 1] and flips cur.  psi[] follows the same parity.
+  void *state[2] = {};
+  void *coef = nullptr;  // ce_x ce_y ce_z ca_x ca_y ca_z
+  void *psi[2] = {};     // 12 face stripes, slot order of PsiSlot
+  size_t psi_off[PSI_COUNT + 1] = {};  // element offset of each slot
+  void *prof = nullptr;  // [stagger 0=E,1=H][b, c, ik][x | y | z]
+  int cur = 0;
 
   bool lossless = true;  // ca == 1 everywhere (set_epsilon without sigma)
   int tile_rows = 4;     // fused sweep: j rows per block
@@ -72,6 +70,13 @@ struct Engine {
   bool timing = false;
   std::vector<hipEvent_t> events;  // pool; [2q, 2q+1] bracket launch q
   size_t events_used = 0;
+
+  char *vol(int slab_parity, int c) const {
+    return (char *)state[slab_parity] + (size_t)c * vstride * elem;
+  }
+  void *field(int c) const { return vol(cur, c); }
+  void *coef_vol(int c) const { return (char *)coef + (size_t)c * vstride * elem; }
+  size_t prof_len() const { return (size_t)nx + ny + nz; }
 };
 
 // Record the next timing event on the engine stream (no-op when disabled).

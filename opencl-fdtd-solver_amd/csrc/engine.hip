@@ -161,15 +161,12 @@ static void release(Engine *e) {
     if (p) (void)hipFree(p);
     p = nullptr;
   };
-  for (auto &p : e->field) fr(p);
-  for (auto &p : e->field_alt) fr(p);
-  for (auto &p : e->ce) fr(p);
-  for (auto &p : e->ca) fr(p);
-  for (auto &p : e->psi) fr(p);
-  for (auto &p : e->psi_alt) fr(p);
-  for (auto &s : e->prof)
-    for (auto &a : s)
-      for (auto &p : a) fr(p);
+  fr(e->state[0]);
+  fr(e->state[1]);
+  fr(e->psi[0]);
+  fr(e->psi[1]);
+  fr(e->coef);
+  fr(e->prof);
   for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
   e->events.clear();
   e->events_used = 0;
@@ -189,16 +186,11 @@ int timing_mark(Engine *e) {
 }
 
 int ensure_alt(Engine *e) {
-  if (e->field_alt[0]) return HFDTD_OK;
-  for (int f = 0; f < 6; ++f) {
-    int rc = dmalloc(e, &e->field_alt[f], e->ncell * e->elem);
-    if (rc) return rc;
-  }
-  for (int s = 0; s < PSI_COUNT; ++s) {
-    int rc = dmalloc(e, &e->psi_alt[s], e->psi_len[kPsiAxis[s]] * e->elem);
-    if (rc) return rc;
-  }
-  return HFDTD_OK;
+  const int other = e->cur ^ 1;
+  if (e->state[other]) return HFDTD_OK;
+  int rc = dmalloc(e, &e->state[other], 6 * e->vstride * e->elem);
+  if (rc) return rc;
+  return dmalloc(e, &e->psi[other], e->psi_off[PSI_COUNT] * e->elem);
 }
 
 }  // namespace hfdtd
@@ -262,6 +254,8 @@ int hfdtd_create(int nx, int ny, int nz, int npml, double dl, double dt, int fp6
   if (!(dl > 0.0) || !(dt > 0.0)) return fail(HFDTD_EINVAL, "dl and dt must be positive");
   if ((double)nx * ny * nz >= 2147483647.0)
     return fail(HFDTD_EUNSUPPORTED, "grid exceeds 2^31 cells");
+  if (npml > 0 && 8.0 * npml * ((double)ny * nz + (double)nx * nz + (double)nx * ny) >= 2147483647.0)
+    return fail(HFDTD_EUNSUPPORTED, "psi stripes exceed 2^31 entries");
   int ndev = 0;
   int rc = hfdtd_device_count(&ndev);
   if (rc) return rc;
@@ -279,14 +273,20 @@ int hfdtd_create(int nx, int ny, int nz, int npml, double dl, double dt, int fp6
   e->dt = dt;
   e->elem = fp64 ? 8 : 4;
   e->ncell = (size_t)nx * ny * nz;
+  e->vstride = (e->ncell + 63) / 64 * 64;
   // folded constants, evaluated in FP64 as engine.py:543-544, 634 do
   e->dtm_dl = dt / (kMU0 * dl);
   e->dtm = dt / kMU0;
   e->inv_dl = 1.0 / dl;
-  if (npml > 0) {
-    e->psi_len[0] = (size_t)2 * npml * ny * nz;
-    e->psi_len[1] = (size_t)nx * 2 * npml * nz;
-    e->psi_len[2] = (size_t)nx * ny * 2 * npml;
+  {
+    size_t len[3] = {0, 0, 0};
+    if (npml > 0) {
+      len[0] = (size_t)2 * npml * ny * nz;
+      len[1] = (size_t)nx * 2 * npml * nz;
+      len[2] = (size_t)nx * ny * 2 * npml;
+    }
+    e->psi_off[0] = 0;
+    for (int q = 0; q < PSI_COUNT; ++q) e->psi_off[q + 1] = e->psi_off[q] + len[kPsiAxis[q]];
   }
 
   auto bail = [&](int code) {
@@ -299,61 +299,47 @@ int hfdtd_create(int nx, int ny, int nz, int npml, double dl, double dt, int fp6
   err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (err != hipSuccess) return bail(hip_fail(err, "hipStreamCreateWithFlags"));
 
-  const size_t vb = e->ncell * e->elem;
-  for (int f = 0; f < 6; ++f)
-    if ((rc = dmalloc(e, &e->field[f], vb))) return bail(rc);
-  for (int c = 0; c < 3; ++c) {
-    if ((rc = dmalloc(e, &e->ce[c], vb))) return bail(rc);
-    if ((rc = dmalloc(e, &e->ca[c], vb))) return bail(rc);
-  }
-  for (int s = 0; s < PSI_COUNT; ++s)
-    if ((rc = dmalloc(e, &e->psi[s], e->psi_len[kPsiAxis[s]] * e->elem))) return bail(rc);
+  if ((rc = dmalloc(e, &e->state[0], 6 * e->vstride * e->elem))) return bail(rc);
+  if ((rc = dmalloc(e, &e->coef, 6 * e->vstride * e->elem))) return bail(rc);
+  if ((rc = dmalloc(e, &e->psi[0], e->psi_off[PSI_COUNT] * e->elem))) return bail(rc);
+  if ((rc = dmalloc(e, &e->prof, 2 * 3 * e->prof_len() * e->elem))) return bail(rc);
 
-  // CPML profiles, H nodes offset 1/2 and E nodes offset 0 (engine.py:262-283)
-  const int dims[3] = {nx, ny, nz};
-  for (int stag = 0; stag < 2; ++stag) {
+  // CPML profiles, H nodes offset 1/2 and E nodes offset 0 (engine.py:262-283),
+  // laid out [stagger][b, c, ik][x | y | z]
+  {
+    const int dims[3] = {nx, ny, nz};
+    const size_t L = e->prof_len();
+    std::vector<unsigned char> host(2 * 3 * L * e->elem);
+    size_t axoff = 0;
     for (int a = 0; a < 3; ++a) {
       const int n = dims[a];
-      std::vector<double> bd(n), cd(n), kd(n);
-      std::vector<float> bf(n), cf(n), kf(n);
-      std::vector<unsigned char> host_b(n * e->elem), host_c(n * e->elem), host_ik(n * e->elem);
-      const double off = stag == 0 ? 0.0 : 0.5;
-      if (fp64) {
-        cpml_profile<double>(n, npml, dl, dt, off, bd.data(), cd.data(), kd.data());
-        std::memcpy(host_b.data(), bd.data(), n * 8);
-        std::memcpy(host_c.data(), cd.data(), n * 8);
-        for (int i = 0; i < n; ++i) {
-          double ik = 1.0 / (kd[i] * dl);
-          std::memcpy(host_ik.data() + i * 8, &ik, 8);
-        }
-      } else {
-        cpml_profile<float>(n, npml, dl, dt, off, bf.data(), cf.data(), kf.data());
-        std::memcpy(host_b.data(), bf.data(), n * 4);
-        std::memcpy(host_c.data(), cf.data(), n * 4);
-        for (int i = 0; i < n; ++i) {
+      for (int stag = 0; stag < 2; ++stag) {
+        const double off = stag == 0 ? 0.0 : 0.5;
+        unsigned char *pb = host.data() + ((size_t)(stag * 3 + 0) * L + axoff) * e->elem;
+        unsigned char *pc = host.data() + ((size_t)(stag * 3 + 1) * L + axoff) * e->elem;
+        unsigned char *pk = host.data() + ((size_t)(stag * 3 + 2) * L + axoff) * e->elem;
+        if (fp64) {
+          std::vector<double> kd(n);
+          cpml_profile<double>(n, npml, dl, dt, off, (double *)pb, (double *)pc, kd.data());
+          for (int i = 0; i < n; ++i) ((double *)pk)[i] = 1.0 / (kd[i] * dl);
+        } else {
+          std::vector<float> kf(n);
+          cpml_profile<float>(n, npml, dl, dt, off, (float *)pb, (float *)pc, kf.data());
           // 1/(kappa*dl) in FP64 from the stored kappa, then rounded (engine.py:264-266)
-          float ik = (float)(1.0 / ((double)kf[i] * dl));
-          std::memcpy(host_ik.data() + i * 4, &ik, 4);
+          for (int i = 0; i < n; ++i) ((float *)pk)[i] = (float)(1.0 / ((double)kf[i] * dl));
         }
       }
-      const unsigned char *src[3] = {host_b.data(), host_c.data(), host_ik.data()};
-      for (int q = 0; q < 3; ++q) {
-        if ((rc = dmalloc(e, &e->prof[stag][a][q], n * e->elem))) return bail(rc);
-        err = hipMemcpy(e->prof[stag][a][q], src[q], n * e->elem, hipMemcpyHostToDevice);
-        if (err != hipSuccess) return bail(hip_fail(err, "hipMemcpy(cpml)"));
-      }
+      axoff += n;
     }
+    err = hipMemcpy(e->prof, host.data(), host.size(), hipMemcpyHostToDevice);
+    if (err != hipSuccess) return bail(hip_fail(err, "hipMemcpy(cpml)"));
   }
 
   // defaults: fields 0, ce = dt/eps0, ca = 1 (engine.py:189-204)
-  for (int f = 0; f < 6; ++f)
-    if ((rc = fill_value(e, e->field[f], e->ncell, 0.0))) return bail(rc);
-  for (int c = 0; c < 3; ++c) {
-    if ((rc = fill_value(e, e->ce[c], e->ncell, dt / kEPS0))) return bail(rc);
-    if ((rc = fill_value(e, e->ca[c], e->ncell, 1.0))) return bail(rc);
-  }
-  for (int s = 0; s < PSI_COUNT; ++s)
-    if ((rc = fill_value(e, e->psi[s], e->psi_len[kPsiAxis[s]], 0.0))) return bail(rc);
+  if ((rc = fill_value(e, e->state[0], 6 * e->vstride, 0.0))) return bail(rc);
+  if ((rc = fill_value(e, e->coef, 3 * e->vstride, dt / kEPS0))) return bail(rc);
+  if ((rc = fill_value(e, e->coef_vol(3), 3 * e->vstride, 1.0))) return bail(rc);
+  if ((rc = fill_value(e, e->psi[0], e->psi_off[PSI_COUNT], 0.0))) return bail(rc);
   err = hipStreamSynchronize(e->stream);
   if (err != hipSuccess) return bail(hip_fail(err, "hipStreamSynchronize(create)"));
   *out = h;
@@ -373,10 +359,9 @@ int hfdtd_memory_bytes(const hfdtd_engine *h, uint64_t *bytes) {
   return HFDTD_OK;
 }
 
-static void **volume_slot(Engine *e, int which) {
-  if (which >= HFDTD_EX && which <= HFDTD_HZ) return &e->field[which];
-  if (which >= HFDTD_CEX && which <= HFDTD_CEZ) return &e->ce[which - HFDTD_CEX];
-  if (which >= HFDTD_CAX && which <= HFDTD_CAZ) return &e->ca[which - HFDTD_CAX];
+static void *volume_ptr(Engine *e, int which) {
+  if (which >= HFDTD_EX && which <= HFDTD_HZ) return e->field(which);
+  if (which >= HFDTD_CEX && which <= HFDTD_CAZ) return e->coef_vol(which - HFDTD_CEX);
   return nullptr;
 }
 
@@ -402,8 +387,9 @@ int hfdtd_set_materials(hfdtd_engine *h, const void *eps_r, const void *sigma, i
     dim3 grid(ceil_div(e->nz, 64), e->ny, e->nx);
 #define HF_MAT(T, S)                                                                         \
   hipLaunchKernelGGL((k_materials<T, S>), grid, dim3(64), 0, e->stream, e->nx, e->ny, e->nz, \
-                     (const S *)d_eps, (const S *)d_sig, e->dt, kEPS0, (T *)e->ce[0],       \
-                     (T *)e->ce[1], (T *)e->ce[2], (T *)e->ca[0], (T *)e->ca[1], (T *)e->ca[2])
+                     (const S *)d_eps, (const S *)d_sig, e->dt, kEPS0, (T *)e->coef_vol(0),   \
+                     (T *)e->coef_vol(1), (T *)e->coef_vol(2), (T *)e->coef_vol(3),            \
+                     (T *)e->coef_vol(4), (T *)e->coef_vol(5))
     if (e->fp64) {
       if (src_fp64) HF_MAT(double, double); else HF_MAT(double, float);
     } else {
@@ -423,10 +409,10 @@ int hfdtd_set_materials(hfdtd_engine *h, const void *eps_r, const void *sigma, i
 int hfdtd_write_volume(hfdtd_engine *h, int which, const void *host) {
   if (!h || !host) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = &h->s;
-  void **slot = volume_slot(e, which);
+  void *slot = volume_ptr(e, which);
   if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
   HF_HIP(hipSetDevice(e->device));
-  HF_HIP(hipMemcpyAsync(*slot, host, e->ncell * e->elem, hipMemcpyHostToDevice, e->stream));
+  HF_HIP(hipMemcpyAsync(slot, host, e->ncell * e->elem, hipMemcpyHostToDevice, e->stream));
   HF_HIP(hipStreamSynchronize(e->stream));
   if (which >= HFDTD_CAX) e->lossless = false;
   return HFDTD_OK;
@@ -435,10 +421,10 @@ int hfdtd_write_volume(hfdtd_engine *h, int which, const void *host) {
 int hfdtd_read_volume(hfdtd_engine *h, int which, void *host) {
   if (!h || !host) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = &h->s;
-  void **slot = volume_slot(e, which);
+  void *slot = volume_ptr(e, which);
   if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
   HF_HIP(hipSetDevice(e->device));
-  HF_HIP(hipMemcpyAsync(host, *slot, e->ncell * e->elem, hipMemcpyDeviceToHost, e->stream));
+  HF_HIP(hipMemcpyAsync(host, slot, e->ncell * e->elem, hipMemcpyDeviceToHost, e->stream));
   HF_HIP(hipStreamSynchronize(e->stream));
   return HFDTD_OK;
 }
@@ -446,12 +432,12 @@ int hfdtd_read_volume(hfdtd_engine *h, int which, void *host) {
 int hfdtd_read_point(hfdtd_engine *h, int which, int64_t index, double *out) {
   if (!h || !out) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = &h->s;
-  void **slot = volume_slot(e, which);
+  void *slot = volume_ptr(e, which);
   if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
   if (index < 0 || (size_t)index >= e->ncell) return fail(HFDTD_EINVAL, "index out of range");
   HF_HIP(hipSetDevice(e->device));
   unsigned char buf[8];
-  HF_HIP(hipMemcpyAsync(buf, (const char *)*slot + (size_t)index * e->elem, e->elem,
+  HF_HIP(hipMemcpyAsync(buf, (const char *)slot + (size_t)index * e->elem, e->elem,
                         hipMemcpyDeviceToHost, e->stream));
   HF_HIP(hipStreamSynchronize(e->stream));
   if (e->fp64) {
@@ -471,10 +457,8 @@ int hfdtd_reset_fields(hfdtd_engine *h) {
   Engine *e = &h->s;
   HF_HIP(hipSetDevice(e->device));
   int rc;
-  for (int f = 0; f < 6; ++f)
-    if ((rc = fill_value(e, e->field[f], e->ncell, 0.0))) return rc;
-  for (int s = 0; s < PSI_COUNT; ++s)
-    if ((rc = fill_value(e, e->psi[s], e->psi_len[kPsiAxis[s]], 0.0))) return rc;
+  if ((rc = fill_value(e, e->state[e->cur], 6 * e->vstride, 0.0))) return rc;
+  if ((rc = fill_value(e, e->psi[e->cur], e->psi_off[PSI_COUNT], 0.0))) return rc;
   HF_HIP(hipStreamSynchronize(e->stream));
   return HFDTD_OK;
 }

@@ -409,8 +409,8 @@ static int accumulate_t(hfdtd_n2f *m, const double *phase) {
       for (int c = 0; c < 4; ++c) ph.v[q][c] = A(phase[(f0 + q) * 4 + c]);
     hipLaunchKernelGGL((k_n2f_accumulate<T, A, KAHAN>), dim3(ceil_div(m->n_face, 256)),
                        dim3(256), 0, e->stream, g, b, m->nfreq, f0, nf,
-                       (const T *)e->field[0], (const T *)e->field[1], (const T *)e->field[2],
-                       (const T *)e->field[3], (const T *)e->field[4], (const T *)e->field[5],
+                       (const T *)e->field(0), (const T *)e->field(1), (const T *)e->field(2),
+                       (const T *)e->field(3), (const T *)e->field(4), (const T *)e->field(5),
                        (A *)m->acc, (A *)m->cmp, ph);
     HF_HIP(hipGetLastError());
   }

@@ -100,6 +100,26 @@ EXPORTS = {
         [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
          C.c_void_p],
     ),  # fmt: skip
+    "hfdtd_probe_create": (
+        C.c_int,
+        [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)],
+    ),
+    "hfdtd_probe_destroy": (C.c_int, [C.c_void_p]),
+    "hfdtd_probe_accumulate": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "hfdtd_probe_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "hfdtd_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "hfdtd_batch_destroy": (C.c_int, [C.c_void_p]),
+    "hfdtd_batch_set_sources": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(Source), C.POINTER(C.c_double), C.c_int],
+    ),
+    "hfdtd_batch_set_probes": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int],
+    ),
+    "hfdtd_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "hfdtd_batch_read_probes": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "hfdtd_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
 }
 
 _LIB = None

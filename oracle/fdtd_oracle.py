@@ -295,6 +295,31 @@ class OracleFDTD:
         term = ((-self.ce[0][sl]) * r(jx)) * w
         self.Ex[sl] = self.Ex[sl] + term.astype(self.dtype)
 
+    def add_source_J(self, component, J, box, *, rim_taper=0, rim_edge=0.8):
+        """Current density on one E component over a half-open box.
+
+        Generalises add_source_Jx (sources.cl:26-49) to any component and box:
+        ``E_c += ((-ce_c) * J) * w`` with w = rim_edge per tapered axis whose
+        edge the cell lies on (bit 0: i, bit 1: j, bit 2: k).  Pinned by the
+        closed form ΔE = -dt/(ε0 εe) J (tests/test_oracle_closed_forms.py).
+        """
+        c = "xyz".index(component) if isinstance(component, str) else int(component)
+        i0, i1, j0, j1, k0, k1 = (int(v) for v in box)
+        r = self.real
+        F = (self.Ex, self.Ey, self.Ez)[c]
+        ce = self.ce[c]
+        for i in range(max(i0, 0), min(i1, self.Nx)):
+            for j in range(max(j0, 0), min(j1, self.Ny)):
+                for k in range(max(k0, 0), min(k1, self.Nz)):
+                    w = r(1.0)
+                    if (rim_taper & 1) and i in (i0, i1 - 1):
+                        w = r(w * r(rim_edge))
+                    if (rim_taper & 2) and j in (j0, j1 - 1):
+                        w = r(w * r(rim_edge))
+                    if (rim_taper & 4) and k in (k0, k1 - 1):
+                        w = r(w * r(rim_edge))
+                    F[i, j, k] = F[i, j, k] + ((-ce[i, j, k]) * r(J)) * w
+
     # ── updates ──
 
     def _interior_mask(self):

