@@ -136,9 +136,11 @@ int hfdtd_add_sources(hfdtd_engine *e, const hfdtd_source *src, int n);
 int hfdtd_step(hfdtd_engine *e, const hfdtd_source *src, int n, int path);
 /* queue.finish() */
 int hfdtd_synchronize(hfdtd_engine *e);
-/* Fused-sweep launch shape: rows of j per block (1..16) and planes of i per
- * block (1..Nx); 0 keeps the current value.  Used by the autotuner. */
-int hfdtd_set_fused_tiling(hfdtd_engine *e, int rows_j, int planes_i);
+/* Fused-sweep launch shape: rows of j per block (1, 2, 4, 8 or 15; a block is
+ * 64 x (rows + 1) threads), planes of i per block (>= 1) and the number of
+ * XCDs the work order is grouped for (1 = hardware order, 8 = MI355X); 0
+ * keeps the current value.  Used by the autotuner. */
+int hfdtd_set_fused_tiling(hfdtd_engine *e, int rows_j, int planes_i, int xcd_groups);
 
 /* HIP-event timing of the Yee sweep kernels on the engine's stream: while
  * enabled, every update_H / update_E / fused sweep launch is bracketed by a

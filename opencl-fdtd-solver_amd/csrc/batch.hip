@@ -3,7 +3,8 @@
 // The batched sweep advances a set of engines of uneven sizes with one
 // persistent launch per step: a flat work list holds every (engine, tile)
 // pair, each block loops over work items with stride gridDim.x and runs the
-// same fused tile body as the single-engine sweep.  The step index lives in
+// same fused tile body as the single-engine sweep; a second small launch then
+// applies each engine's source and records its probe.  The step index lives in
 // device memory (bumped by a one-thread kernel after each step), so a run of
 // K steps can be captured once as a HIP graph and replayed.  Source
 // amplitudes and probe phases come from per-engine tables indexed by that
@@ -69,42 +70,25 @@ struct BatchArgs {
   int n_table;
   int nfreq, comp;
   double *probe_acc;  // n x nfreq x 2
+  int xcd;            // XCD count for the work-order remap (1 = off)
 };
 
 template <typename T>
 struct BatchSrcP {
   const Src<T> *s;
-  int n;
   T a;
-  __device__ int count() const { return n; }
+  __device__ int count() const { return 1; }
   __device__ const Src<T> &at(int) const { return *s; }
   __device__ T amp(int) const { return a; }
-};
-
-template <typename T>
-struct ProbeHook {
-  int on, pi, pj, pk, comp, nf;
-  const double *ph;
-  double *acc;
-  __device__ void operator()(int i, int j, int k, T ex, T ey, T ez) {
-    if (!on || i != pi || j != pj || k != pk) return;
-    const double v = (double)(comp == 0 ? ex : (comp == 1 ? ey : ez));
-    for (int f = 0; f < nf; ++f) {
-      acc[2 * f] += v * ph[2 * f];
-      acc[2 * f + 1] += v * ph[2 * f + 1];
-    }
-  }
 };
 
 // LOSSY selects the update with the ca load; it is exact for lossless
 // engines too (ca = 1), so one variant serves a mixed batch.
 template <typename T, int TJ, bool LOSSY>
-__global__ __launch_bounds__(64 * TJ) void k_batch_step(BatchArgs<T> a) {
-  __shared__ T sE[3][TJ][64];
-  __shared__ T sH[3][TJ + 1][65];
-  const int step = *a.step;
-  const int par = (step - a.step0) & 1;
-  for (int t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+__global__ __launch_bounds__(64 * (TJ + 1)) void k_batch_step(BatchArgs<T> a) {
+  const int par = (*a.step - a.step0) & 1;
+  for (int n = blockIdx.x; n < a.n_tiles; n += gridDim.x) {
+    const int t = xcd_remap(n, a.n_tiles, a.xcd);
     const int s = a.tile_sim[t];
     const SimDesc<T> &d = a.sims[s];
     const Geo g = d.g;
@@ -113,15 +97,51 @@ __global__ __launch_bounds__(64 * TJ) void k_batch_step(BatchArgs<T> a) {
     const int rest = lt / d.tiles_k;
     const int by = rest % d.tiles_j;
     const int bz = rest / d.tiles_j;
-    const size_t row = (size_t)s * a.n_table + step;
-    BatchSrcP<T> sp{&d.src, d.has_src, d.has_src ? T(a.amp[row]) : T(0)};
-    ProbeHook<T> hk{d.has_probe, d.pi, d.pj, d.pk, a.comp, a.nfreq,
-                    d.has_probe ? a.phase + row * a.nfreq * 2 : nullptr,
-                    a.probe_acc + (size_t)s * a.nfreq * 2};
     lds_barrier();  // LDS of the previous work item is free
     fused_tile<T, TJ, LOSSY>(g, a.planes, a.dtm_dl, a.dtm, a.inv_dl, d.st[par], d.st[par ^ 1],
-                             d.coef, d.prof, d.psi[par], d.psi[par ^ 1], sp, hk, bx, by, bz, sE,
-                             sH);
+                             d.coef, d.prof, d.psi[par], d.psi[par ^ 1], bx, by, bz);
+  }
+}
+
+// After the batched sweep: one block per engine redoes E on the cells of its
+// source box (source_fixup_cell), then records its probe cell.
+template <typename T>
+__global__ __launch_bounds__(256) void k_batch_fixup(BatchArgs<T> a) {
+  const int step = *a.step;
+  const int par = (step - a.step0) & 1;
+  const int s = blockIdx.x;
+  const SimDesc<T> &d = a.sims[s];
+  const Geo g = d.g;
+  const size_t row = (size_t)s * a.n_table + step;
+  const T *s0 = d.st[par];
+  T *s1 = d.st[par ^ 1];
+  if (d.has_src) {
+    const BatchSrcP<T> sp{&d.src, T(a.amp[row])};
+    const int ni = d.src.ci1 - d.src.ci0, nj = d.src.cj1 - d.src.cj0;
+    const int nk = d.src.ck1 - d.src.ck0;
+    const long long total = (long long)ni * nj * nk;
+    for (long long t = threadIdx.x; t < total; t += blockDim.x) {
+      const int k = d.src.ck0 + (int)(t % nk);
+      const int j = d.src.cj0 + (int)((t / nk) % nj);
+      const int i = d.src.ci0 + (int)(t / ((long long)nk * nj));
+      if (d.lossy)
+        source_fixup_cell<T, true>(g, a.inv_dl, s0, s1, d.coef, d.prof, d.psi[par],
+                                   d.psi[par ^ 1], sp, i, j, k);
+      else
+        source_fixup_cell<T, false>(g, a.inv_dl, s0, s1, d.coef, d.prof, d.psi[par],
+                                    d.psi[par ^ 1], sp, i, j, k);
+    }
+  }
+  if (!d.has_probe) return;
+  __syncthreads();  // the block's E writes are visible to thread 0
+  if (threadIdx.x != 0) return;
+  const size_t idx = ((size_t)d.pi * g.ny + d.pj) * g.nz + d.pk;
+  const double v = (double)s1[a.comp * g.vs + idx];
+  const double *ph = a.phase + row * a.nfreq * 2;
+  double *acc = a.probe_acc + (size_t)s * a.nfreq * 2;
+  for (int f = 0; f < a.nfreq; ++f) {
+    acc[2 * f] += v * ph[2 * f];
+    acc[2 * f + 1] += v * ph[2 * f + 1];
   }
 }
 
@@ -264,6 +284,7 @@ static BatchArgs<T> batch_args(const hfdtd_batch *b) {
   a.nfreq = b->nfreq;
   a.comp = b->comp;
   a.probe_acc = b->d_probe_acc;
+  a.xcd = e->xcd;
   return a;
 }
 
@@ -272,9 +293,9 @@ static int batch_launch_step(hfdtd_batch *b, const BatchArgs<T> &a) {
   const int grid = b->blocks > 0 ? std::min(b->blocks, b->n_tiles) : b->n_tiles;
 #define HF_BATCH(TJ)                                                                          \
   if (b->lossy)                                                                               \
-    hipLaunchKernelGGL((k_batch_step<T, TJ, true>), dim3(grid), dim3(64, TJ), 0, b->stream, a); \
+    hipLaunchKernelGGL((k_batch_step<T, TJ, true>), dim3(grid), dim3(64, TJ + 1), 0, b->stream, a); \
   else                                                                                        \
-    hipLaunchKernelGGL((k_batch_step<T, TJ, false>), dim3(grid), dim3(64, TJ), 0, b->stream, a)
+    hipLaunchKernelGGL((k_batch_step<T, TJ, false>), dim3(grid), dim3(64, TJ + 1), 0, b->stream, a)
   switch (b->rows) {
     case 2: HF_BATCH(2); break;
     case 8: HF_BATCH(8); break;
@@ -282,6 +303,11 @@ static int batch_launch_step(hfdtd_batch *b, const BatchArgs<T> &a) {
   }
 #undef HF_BATCH
   HF_HIP(hipGetLastError());
+  if (a.amp || a.probe_acc) {
+    hipLaunchKernelGGL(k_batch_fixup<T>, dim3((unsigned)b->eng.size()), dim3(256), 0, b->stream,
+                       a);
+    HF_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, b->stream, b->d_step);
   HF_HIP(hipGetLastError());
   return HFDTD_OK;

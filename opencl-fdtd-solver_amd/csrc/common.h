@@ -60,8 +60,9 @@ struct Engine {
   int cur = 0;
 
   bool lossless = true;  // ca == 1 everywhere (set_epsilon without sigma)
-  int tile_rows = 4;     // fused sweep: j rows per block
-  int tile_planes = 16;  // fused sweep: i planes per block
+  int tile_rows = 8;     // fused sweep: j rows per block
+  int tile_planes = 8;   // fused sweep: i planes per block
+  int xcd = 8;           // XCDs for the work-order remap (1 = hardware order)
 
   uint64_t bytes = 0;
 

@@ -522,12 +522,15 @@ int hfdtd_timing_read(hfdtd_engine *h, double *total_ms, int *launches) {
   return HFDTD_OK;
 }
 
-int hfdtd_set_fused_tiling(hfdtd_engine *h, int rows_j, int planes_i) {
+int hfdtd_set_fused_tiling(hfdtd_engine *h, int rows_j, int planes_i, int xcd_groups) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
-  if (rows_j < 0 || rows_j > 16 || planes_i < 0)
-    return fail(HFDTD_EINVAL, "rows_j must be 0..16 and planes_i >= 0");
+  if ((rows_j != 0 && rows_j != 1 && rows_j != 2 && rows_j != 4 && rows_j != 8 &&
+       rows_j != 15) || planes_i < 0)
+    return fail(HFDTD_EINVAL, "rows_j must be 1, 2, 4, 8 or 15 and planes_i >= 0");
+  if (xcd_groups < 0 || xcd_groups > 64) return fail(HFDTD_EINVAL, "xcd_groups must be 0..64");
   if (rows_j) h->s.tile_rows = rows_j;
   if (planes_i) h->s.tile_planes = planes_i;
+  if (xcd_groups) h->s.xcd = xcd_groups;
   return HFDTD_OK;
 }
 

@@ -14,6 +14,8 @@
 // coefficients and the twelve psi stripes each live in one slab, so a kernel
 // receives a few base pointers instead of dozens.
 
+#include <algorithm>
+
 #include "fdtd_core.h"
 
 namespace hfdtd {
@@ -37,10 +39,11 @@ __global__ __launch_bounds__(64 * kSplitRows) void k_update_H(Geo g, T dtm_dl, T
   const size_t idx = (size_t)i * sxy + (size_t)j * g.nz + k;
   const bool jn = j + 1 < g.ny, kn = k + 1 < g.nz, in = i + 1 < g.nx;
   const StripeIdx s = stripe_idx(g, i, j, k);
-  T p[6];
+  T p[6] = {0, 0, 0, 0, 0, 0};
   psi_load(g, psi, 0, s, p);
   T hx = Hx[idx], hy = Hy[idx], hz = Hz[idx];
-  h_math(g, prof, i, j, k, s, Ex[idx], Ey[idx], Ez[idx], jn ? Ex[idx + g.nz] : T(0),
+  h_math(g, i, j, k, s, prof1(g, prof, 1, 0, i), prof1(g, prof, 1, 1, j), prof1(g, prof, 1, 2, k),
+         Ex[idx], Ey[idx], Ez[idx], jn ? Ex[idx + g.nz] : T(0),
          jn ? Ez[idx + g.nz] : T(0), kn ? Ex[idx + 1] : T(0), kn ? Ey[idx + 1] : T(0),
          in ? Ey[idx + sxy] : T(0), in ? Ez[idx + sxy] : T(0), hx, hy, hz, p, dtm_dl, dtm);
   Hx[idx] = hx;
@@ -64,13 +67,14 @@ __global__ __launch_bounds__(64 * kSplitRows) void k_update_E(Geo g, T inv_dl,
   const size_t sxy = (size_t)g.ny * g.nz;
   const size_t idx = (size_t)i * sxy + (size_t)j * g.nz + k;
   const StripeIdx s = stripe_idx(g, i, j, k);
-  T p[6];
+  T p[6] = {0, 0, 0, 0, 0, 0};
   psi_load(g, psi, 6, s, p);
   T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
   const T ax = LOSSY ? coef[3 * g.vs + idx] : T(1);
   const T ay = LOSSY ? coef[4 * g.vs + idx] : T(1);
   const T az = LOSSY ? coef[5 * g.vs + idx] : T(1);
-  e_math(g, prof, i, j, k, s, Hx[idx], Hy[idx], Hz[idx], j > 0 ? Hx[idx - g.nz] : T(0),
+  e_math(g, i, j, k, s, prof1(g, prof, 0, 0, i), prof1(g, prof, 0, 1, j), prof1(g, prof, 0, 2, k),
+         Hx[idx], Hy[idx], Hz[idx], j > 0 ? Hx[idx - g.nz] : T(0),
          j > 0 ? Hz[idx - g.nz] : T(0), k > 0 ? Hx[idx - 1] : T(0), k > 0 ? Hy[idx - 1] : T(0),
          i > 0 ? Hy[idx - sxy] : T(0), i > 0 ? Hz[idx - sxy] : T(0), ex, ey, ez, ax, ay, az,
          coef[idx], coef[g.vs + idx], coef[2 * g.vs + idx], p, inv_dl);
@@ -118,18 +122,39 @@ static bool make_src(const Engine *e, const hfdtd_source &in, Src<T> &o) {
 // ── fused sweep kernel ────────────────────────────────────────────────────
 
 template <typename T, int TJ, bool LOSSY>
-__global__ __launch_bounds__(64 * TJ) void k_fused(Geo g, int planes, T dtm_dl, T dtm,
+__global__ __launch_bounds__(64 * (TJ + 1)) void k_fused(Geo g, int planes, T dtm_dl, T dtm,
                                                    T inv_dl, const T *__restrict__ s0,
                                                    T *__restrict__ s1,
                                                    const T *__restrict__ coef,
                                                    const T *__restrict__ prof,
                                                    const T *__restrict__ q0,
-                                                   T *__restrict__ q1, SrcList<T> src) {
-  __shared__ T sE[3][TJ][64];
-  __shared__ T sH[3][TJ + 1][65];
-  NoHook hook;
-  fused_tile<T, TJ, LOSSY>(g, planes, dtm_dl, dtm, inv_dl, s0, s1, coef, prof, q0, q1,
-                           SrcListP<T>{src}, hook, blockIdx.x, blockIdx.y, blockIdx.z, sE, sH);
+                                                   T *__restrict__ q1, int tiles_k,
+                                                   int tiles_j, int xcd) {
+  // 1-D grid of tiles, k fastest, then j, then the i chunk
+  const int L = xcd_remap(blockIdx.x, gridDim.x, xcd);
+  const int bx = L % tiles_k, by = (L / tiles_k) % tiles_j, bz = L / (tiles_k * tiles_j);
+  fused_tile<T, TJ, LOSSY>(g, planes, dtm_dl, dtm, inv_dl, s0, s1, coef, prof, q0, q1, bx, by,
+                           bz);
+}
+
+// Source cells after a fused sweep: blockIdx.y picks the source whose box
+// this block walks; each cell gets every source that covers it (a cell in
+// two boxes is recomputed twice, to the same value).
+template <typename T, bool LOSSY>
+__global__ __launch_bounds__(256) void k_source_fixup(Geo g, T inv_dl, const T *__restrict__ s0,
+                                                      T *__restrict__ s1,
+                                                      const T *__restrict__ coef,
+                                                      const T *__restrict__ prof,
+                                                      const T *__restrict__ q0,
+                                                      T *__restrict__ q1, SrcList<T> src) {
+  const Src<T> &b = src.s[blockIdx.y];
+  const int ni = b.ci1 - b.ci0, nj = b.cj1 - b.cj0, nk = b.ck1 - b.ck0;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * nj * nk) return;
+  const int k = b.ck0 + (int)(t % nk);
+  const int j = b.cj0 + (int)((t / nk) % nj);
+  const int i = b.ci0 + (int)(t / ((long long)nk * nj));
+  source_fixup_cell<T, LOSSY>(g, inv_dl, s0, s1, coef, prof, q0, q1, SrcListP<T>{src}, i, j, k);
 }
 
 // ── launchers ─────────────────────────────────────────────────────────────
@@ -188,17 +213,31 @@ static int sources_t(Engine *e, const hfdtd_source *src, int n) {
 template <typename T, int TJ>
 static int fused_tj(Engine *e, const SrcList<T> &sl) {
   const int planes = e->tile_planes < 1 ? 1 : e->tile_planes;
-  dim3 grid(ceil_div(e->nz, 64), ceil_div(e->ny, TJ), ceil_div(e->nx, planes));
+  const int tiles_k = (int)ceil_div(e->nz, 64), tiles_j = (int)ceil_div(e->ny, TJ);
+  const dim3 grid(tiles_k * tiles_j * ceil_div(e->nx, planes));
   auto kern = e->lossless ? k_fused<T, TJ, false> : k_fused<T, TJ, true>;
   int rc = timing_mark(e);
   if (rc) return rc;
   const int o = e->cur ^ 1;
-  hipLaunchKernelGGL(kern, grid, dim3(64, TJ), 0, e->stream, geo_of(e), planes, T(e->dtm_dl),
+  hipLaunchKernelGGL(kern, grid, dim3(64, TJ + 1), 0, e->stream, geo_of(e), planes, T(e->dtm_dl),
                      T(e->dtm), T(e->inv_dl), (const T *)e->state[e->cur], (T *)e->state[o],
+                     (const T *)e->coef, (const T *)e->prof, (const T *)e->psi[e->cur],
+                     (T *)e->psi[o], tiles_k, tiles_j, e->xcd);
+  HF_HIP(hipGetLastError());
+  rc = timing_mark(e);
+  if (rc || sl.n == 0) return rc;
+  long long most = 0;
+  for (int q = 0; q < sl.n; ++q) {
+    const Src<T> &s = sl.s[q];
+    most = std::max(most, (long long)(s.ci1 - s.ci0) * (s.cj1 - s.cj0) * (s.ck1 - s.ck0));
+  }
+  auto fix = e->lossless ? k_source_fixup<T, false> : k_source_fixup<T, true>;
+  hipLaunchKernelGGL(fix, dim3((unsigned)ceil_div(most, 256), sl.n), dim3(256), 0, e->stream,
+                     geo_of(e), T(e->inv_dl), (const T *)e->state[e->cur], (T *)e->state[o],
                      (const T *)e->coef, (const T *)e->prof, (const T *)e->psi[e->cur],
                      (T *)e->psi[o], sl);
   HF_HIP(hipGetLastError());
-  return timing_mark(e);
+  return HFDTD_OK;
 }
 
 template <typename T>
@@ -217,7 +256,7 @@ static int fused_t(Engine *e, const hfdtd_source *src, int n) {
     case 1: rc = fused_tj<T, 1>(e, sl); break;
     case 2: rc = fused_tj<T, 2>(e, sl); break;
     case 8: rc = fused_tj<T, 8>(e, sl); break;
-    case 16: rc = fused_tj<T, 16>(e, sl); break;
+    case 15: rc = fused_tj<T, 15>(e, sl); break;
     default: rc = fused_tj<T, 4>(e, sl); break;
   }
   if (rc) return rc;

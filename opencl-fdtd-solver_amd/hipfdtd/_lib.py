@@ -79,7 +79,7 @@ EXPORTS = {
     "hfdtd_add_sources": (C.c_int, [C.c_void_p, C.POINTER(Source), C.c_int]),
     "hfdtd_step": (C.c_int, [C.c_void_p, C.POINTER(Source), C.c_int, C.c_int]),
     "hfdtd_synchronize": (C.c_int, [C.c_void_p]),
-    "hfdtd_set_fused_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "hfdtd_set_fused_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "hfdtd_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "hfdtd_n2f_create": (

@@ -23,12 +23,16 @@ BUILD_DIR = PKG.parent / "_build"
 ARCH = os.environ.get("HIPFDTD_ARCH", "gfx950")
 
 # -packed-fp32-ops: keep the compiler from emitting v_pk_*_f32 forms.
+# -ffp-contract=off: no implicit multiply-add fusion.  The same cell update is
+# inlined at several call sites (split and fused sweeps, halo recompute,
+# source fix-up, batched sweep); with fusion left to the compiler each site
+# may round differently, and results would depend on the launch shape.
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-ffp-contract=fast",
+    "-ffp-contract=off",
     "-Xclang",
     "-target-feature",
     "-Xclang",

@@ -9,8 +9,9 @@ Step execution.  The reference runs ``update_H``, then the source callbacks
 (``engine.py:724-737``).  ``HIPFDTD`` keeps that order but runs H, sources
 and E as one fused sweep when it can: during the source phase,
 ``add_source_Jx`` / ``add_source_Ex`` / ``add_source_J`` calls are recorded
-instead of launched, and the fused kernel applies them between its H and E
-halves.  A callback that touches field data (``Ex``, ``read_point``, a field
+instead of launched; the fused sweep advances H and E everywhere, then a
+small fix-up kernel recomputes E on the source cells with the recorded
+sources added first, which is the reference's order cell by cell.  A callback that touches field data (``Ex``, ``read_point``, a field
 setter ...) first forces the H half to run on its own, so it sees exactly the
 state the reference would show it; the step then finishes with the split E
 sweep.  ``fused=False`` always uses the split sweeps.
@@ -311,9 +312,11 @@ class HIPFDTD(SourceMonitorMixin):
         self.t = 0.0
         self.step_num = 0
 
-    def set_fused_tiling(self, rows_j: int = 0, planes_i: int = 0) -> None:
-        """Launch shape of the fused sweep (rows of j and planes of i per block)."""
-        _lib.check(self._lib.hfdtd_set_fused_tiling(self._handle, int(rows_j), int(planes_i)))
+    def set_fused_tiling(self, rows_j: int = 0, planes_i: int = 0, xcd_groups: int = 0) -> None:
+        """Launch shape of the fused sweep: rows of j and planes of i per block,
+        and the XCD count the work order is grouped for (0 keeps a value)."""
+        _lib.check(self._lib.hfdtd_set_fused_tiling(self._handle, int(rows_j), int(planes_i),
+                                                    int(xcd_groups)))  # fmt: skip
 
     def kernel_timing(self, enable: bool) -> None:
         """Start (True) or stop (False) HIP-event timing of the sweep kernels."""

@@ -126,15 +126,16 @@ def test_fused_matches_split_and_oracle(shape, npml, lossy):
     assert group_error(out[False], ora.fields()) < 1e-5
 
 
-@pytest.mark.parametrize("rows,planes", [(1, 1), (2, 5), (4, 16), (8, 7), (16, 1000)])
-def test_fused_tilings_are_bit_identical(rows, planes):
+@pytest.mark.parametrize("rows,planes,xcd", [(1, 1, 8), (2, 5, 1), (4, 16, 8), (8, 7, 3),
+                                             (15, 1000, 8)])  # fmt: skip
+def test_fused_tilings_are_bit_identical(rows, planes, xcd):
     shape = (45, 35, 100)
     eps, sig = _random_media(shape, 3, True)
     ref = hip().HIPFDTD(shape, 1e-3, npml=6)
     ref.set_epsilon(eps, sig)
     _drive(ref, shape, 12)
     sim = hip().HIPFDTD(shape, 1e-3, npml=6)
-    sim.set_fused_tiling(rows, planes)
+    sim.set_fused_tiling(rows, planes, xcd)
     sim.set_epsilon(eps, sig)
     _drive(sim, shape, 12)
     a, b = ref.fields(), sim.fields()
