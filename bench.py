@@ -177,7 +177,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_fused (H + sources + E sweep)",
+            "kernel": "k_fused (fused H + E sweep; sources by k_source_fixup)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

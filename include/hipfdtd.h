@@ -136,7 +136,8 @@ int hfdtd_add_sources(hfdtd_engine *e, const hfdtd_source *src, int n);
 int hfdtd_step(hfdtd_engine *e, const hfdtd_source *src, int n, int path);
 /* queue.finish() */
 int hfdtd_synchronize(hfdtd_engine *e);
-/* Fused-sweep launch shape: rows of j per block (1, 2, 4, 8 or 15; a block is
+/* Fused-sweep launch shape: rows of j per block (1, 2, 4, 6 or 8; FP64 engines
+ * use at most 4; a block is
  * 64 x (rows + 1) threads), planes of i per block (>= 1) and the number of
  * XCDs the work order is grouped for (1 = hardware order, 8 = MI355X); 0
  * keeps the current value.  Used by the autotuner. */
@@ -213,7 +214,8 @@ int hfdtd_batch_set_probes(hfdtd_batch *b, const int *ijk, int comp, int nfreq,
 int hfdtd_batch_run(hfdtd_batch *b, int n_steps, int graph_steps);
 /* probe DFTs: n x nfreq x {re, im} */
 int hfdtd_batch_read_probes(hfdtd_batch *b, double *out);
-/* persistent grid size (0 = one block per tile) and rows of j per block */
+/* persistent grid size (0 = one block per tile), rows of j per block (2, 4, 6
+ * or 8; FP64 uses at most 4) and planes of i per block */
 int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i);
 
 #ifdef __cplusplus

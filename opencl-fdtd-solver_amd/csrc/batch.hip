@@ -198,6 +198,8 @@ static int batch_build(hfdtd_batch *b) {
   for (int s = 0; s < n; ++s) {
     Engine *e = b->eng[s];
     b->lossy = b->lossy || !e->lossless;
+    if (!e->fused_fits())
+      return fail(HFDTD_EUNSUPPORTED, "batched engines need field and psi slabs below 4 GB");
     int rc = ensure_alt(e);
     if (rc) return rc;
     SimDesc<T> &d = desc[s];
@@ -296,10 +298,16 @@ static int batch_launch_step(hfdtd_batch *b, const BatchArgs<T> &a) {
     hipLaunchKernelGGL((k_batch_step<T, TJ, true>), dim3(grid), dim3(64, TJ + 1), 0, b->stream, a); \
   else                                                                                        \
     hipLaunchKernelGGL((k_batch_step<T, TJ, false>), dim3(grid), dim3(64, TJ + 1), 0, b->stream, a)
-  switch (b->rows) {
-    case 2: HF_BATCH(2); break;
-    case 8: HF_BATCH(8); break;
-    default: HF_BATCH(4); break;
+  if constexpr (sizeof(T) == 8) {  // FP64: tiles of at most 4 rows fit the LDS
+    if (b->rows == 2) HF_BATCH(2);
+    else HF_BATCH(4);
+  } else {
+    switch (b->rows) {
+      case 2: HF_BATCH(2); break;
+      case 6: HF_BATCH(6); break;
+      case 8: HF_BATCH(8); break;
+      default: HF_BATCH(4); break;
+    }
   }
 #undef HF_BATCH
   HF_HIP(hipGetLastError());
@@ -553,8 +561,8 @@ int hfdtd_batch_read_probes(hfdtd_batch *b, double *out) {
 
 int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i) {
   if (!b || blocks < 0 || planes_i < 0) return fail(HFDTD_EINVAL, "bad launch shape");
-  if (rows_j != 0 && rows_j != 2 && rows_j != 4 && rows_j != 8)
-    return fail(HFDTD_EINVAL, "rows_j must be 2, 4 or 8");
+  if (rows_j != 0 && rows_j != 2 && rows_j != 4 && rows_j != 6 && rows_j != 8)
+    return fail(HFDTD_EINVAL, "rows_j must be 2, 4, 6 or 8");
   b->blocks = blocks;
   if (rows_j) b->rows = rows_j;
   if (planes_i) b->planes = planes_i;

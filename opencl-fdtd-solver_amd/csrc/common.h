@@ -45,7 +45,7 @@ struct Engine {
   double dl = 0, dt = 0;
   double dtm_dl = 0, dtm = 0, inv_dl = 0;  // folded on the host in FP64
   size_t ncell = 0;
-  size_t vstride = 0;  // elements between volumes of a slab (ncell rounded up to 64)
+  size_t vstride = 0;  // elements between volumes of a slab (ncell + 64, rounded up to 64)
   size_t elem = 4;     // bytes per value
   hipStream_t stream = nullptr;
 
@@ -61,7 +61,7 @@ struct Engine {
 
   bool lossless = true;  // ca == 1 everywhere (set_epsilon without sigma)
   int tile_rows = 8;     // fused sweep: j rows per block
-  int tile_planes = 8;   // fused sweep: i planes per block
+  int tile_planes = 64;  // fused sweep: i planes per block
   int xcd = 8;           // XCDs for the work-order remap (1 = hardware order)
 
   uint64_t bytes = 0;
@@ -77,6 +77,11 @@ struct Engine {
   }
   void *field(int c) const { return vol(cur, c); }
   void *coef_vol(int c) const { return (char *)coef + (size_t)c * vstride * elem; }
+  // The fused sweeps address slabs with 32-bit byte offsets.
+  bool fused_fits() const {
+    const double lim = 4294967296.0;
+    return 6.0 * (double)vstride * elem < lim && ((double)psi_off[PSI_COUNT] + 64.0) * elem < lim;
+  }
   size_t prof_len() const { return (size_t)nx + ny + nz; }
 };
 

@@ -190,7 +190,7 @@ int ensure_alt(Engine *e) {
   if (e->state[other]) return HFDTD_OK;
   int rc = dmalloc(e, &e->state[other], 6 * e->vstride * e->elem);
   if (rc) return rc;
-  return dmalloc(e, &e->psi[other], e->psi_off[PSI_COUNT] * e->elem);
+  return dmalloc(e, &e->psi[other], (e->psi_off[PSI_COUNT] + 64) * e->elem);
 }
 
 }  // namespace hfdtd
@@ -229,7 +229,13 @@ int hfdtd_device_info(int device, uint64_t *total_bytes, uint64_t *free_bytes, c
   if (total_bytes) *total_bytes = tot;
   if (free_bytes) *free_bytes = fr;
   if (name && name_len > 0) {
-    std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    // the marketing name can be empty under some drivers: fall back to the
+    // ISA name and compute-unit count
+    if (prop.name[0])
+      std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    else
+      std::snprintf(name, (size_t)name_len, "%s, %d CUs", prop.gcnArchName,
+                    prop.multiProcessorCount);
   }
   return HFDTD_OK;
 }
@@ -273,7 +279,9 @@ int hfdtd_create(int nx, int ny, int nz, int npml, double dl, double dt, int fp6
   e->dt = dt;
   e->elem = fp64 ? 8 : 4;
   e->ncell = (size_t)nx * ny * nz;
-  e->vstride = (e->ncell + 63) / 64 * 64;
+  // >= 64 elements of padding after every volume: the fused sweep's lanes
+  // outside the grid store into it
+  e->vstride = (e->ncell + 64 + 63) / 64 * 64;
   // folded constants, evaluated in FP64 as engine.py:543-544, 634 do
   e->dtm_dl = dt / (kMU0 * dl);
   e->dtm = dt / kMU0;
@@ -301,7 +309,9 @@ int hfdtd_create(int nx, int ny, int nz, int npml, double dl, double dt, int fp6
 
   if ((rc = dmalloc(e, &e->state[0], 6 * e->vstride * e->elem))) return bail(rc);
   if ((rc = dmalloc(e, &e->coef, 6 * e->vstride * e->elem))) return bail(rc);
-  if ((rc = dmalloc(e, &e->psi[0], e->psi_off[PSI_COUNT] * e->elem))) return bail(rc);
+  // + 64 elements of padding: the fused sweep's lanes outside a stripe
+  // store their psi there
+  if ((rc = dmalloc(e, &e->psi[0], (e->psi_off[PSI_COUNT] + 64) * e->elem))) return bail(rc);
   if ((rc = dmalloc(e, &e->prof, 2 * 3 * e->prof_len() * e->elem))) return bail(rc);
 
   // CPML profiles, H nodes offset 1/2 and E nodes offset 0 (engine.py:262-283),
@@ -525,8 +535,8 @@ int hfdtd_timing_read(hfdtd_engine *h, double *total_ms, int *launches) {
 int hfdtd_set_fused_tiling(hfdtd_engine *h, int rows_j, int planes_i, int xcd_groups) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
   if ((rows_j != 0 && rows_j != 1 && rows_j != 2 && rows_j != 4 && rows_j != 8 &&
-       rows_j != 15) || planes_i < 0)
-    return fail(HFDTD_EINVAL, "rows_j must be 1, 2, 4, 8 or 15 and planes_i >= 0");
+       rows_j != 6) || planes_i < 0)
+    return fail(HFDTD_EINVAL, "rows_j must be 1, 2, 4, 6 or 8 and planes_i >= 0");
   if (xcd_groups < 0 || xcd_groups > 64) return fail(HFDTD_EINVAL, "xcd_groups must be 0..64");
   if (rows_j) h->s.tile_rows = rows_j;
   if (planes_i) h->s.tile_planes = planes_i;

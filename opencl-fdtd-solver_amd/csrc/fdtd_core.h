@@ -254,22 +254,35 @@ struct SrcListP {
 // `planes` planes.  State is ping-ponged (reads slab 0, writes slab 1), so a
 // value a neighbouring block needs is never overwritten while it runs.  The
 // block has TJ "row" wavefronts (one per j row of the tile) and one "halo"
-// wavefront.  Per plane i:
-//   0. every wavefront issues the global loads of plane i+1 (software
-//      pipelining: those loads are in flight while plane i is computed);
-//      row waves load E(i+2), H(i+1), ce, ca, psi of their cell and the +1
-//      neighbours outside the tile, the halo wave loads the inputs of the
-//      halo cells (j0-1 row, k0-1 column);
-//   1. E(i) of the tile goes to LDS (E(i) was loaded as E(i+1) of the
-//      previous plane, so every E value comes from HBM once);
-//   2. row waves compute H(i) of the tile and write it out; the halo wave
-//      recomputes H(i) on the j0-1 row and the k0-1 column (not written), so
-//      the E update needs no other block's output;
-//   3. E(i) is advanced from H(i) (LDS) and H(i-1) (registers).  Sources
-//      are left out: source_fixup_cell redoes E on the (few) source cells
-//      once the sweep is complete, so the sweep carries no per-cell branch.
-// The two barriers per plane wait for LDS traffic only, so global loads stay
-// in flight across them.
+// wavefront.
+//
+// Every input reaches the block through LDS, filled by direct global->LDS
+// loads (global_load_lds_dword: the data never occupies VGPRs while in
+// flight):
+//  * a ring of D plane slots, filled D-1 planes ahead of use.  The slot of
+//    plane x holds (SlotLayout):
+//      E   rows j0-1 .. j0+TJ     3 comps: the tile, the halo row, the far row
+//      H   rows j0-1 .. j0+TJ-1   3 comps: old H of the tile and the halo row
+//      ce  rows j0 .. j0+TJ-1     (then ca, for lossy media)
+//      8 column vectors over rows j0-1+r, r < 16: E at k0-1 (3 comps), Ex and
+//      Ey at k0+64, old H at k0-1 (3 comps)
+//  * per wave, 12 rows of old psi for the next plane (stripe cells only).
+// Per plane i:
+//   B1  wait until psi(i) and the slots of planes i, i+1 have landed;
+//       barrier; copy the own psi(i) rows to registers; issue the psi loads
+//       of plane i+1, then the slot loads of plane i+D-1 (into the slot that
+//       plane i-1 used);
+//       row waves: H(i) of the own cell -> HBM and sH (LDS); halo wave: H(i)
+//       of the j0-1 row and the k0-1 column, recomputed, not written -> sH;
+//   B2  barrier; row waves: E(i) of the own cell -> HBM.
+// The compiler neither waits for LDS-DMA data nor counts LDS-DMA loads in
+// its own vmcnt bookkeeping, so the loop holds no compiler-tracked global
+// load at all, and B1 waits explicitly: s_waitcnt vmcnt(N), N = vector
+// memory operations this wave issued after the newest awaited load.  The
+// wave counts its operations exactly: slot and psi loads are issued in
+// wave-uniform branches, and stores are unconditional inside wave-uniform
+// branches (lanes outside the grid or the stripe write into the padding
+// that follows every volume and the psi slab).
 
 // XCD-aware work order.  Workgroup n runs on XCD n % X (round-robin
 // dispatch), and each XCD has its own L2.  Map n to a logical tile so that
@@ -286,282 +299,562 @@ __device__ __forceinline__ void lds_barrier() {
   __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Everything a row wave loads for one plane of its cell.  All loads are
-// unconditional within a wave-uniform branch (addresses clamped to a valid
-// cell), so no masked default write ever targets an in-flight load.
-template <typename T>
-struct RowPlane {
-  T nx, ny, nz;    // E(i+1)
-  T hx, hy, hz;    // H(i)
-  T cx, cy, cz;    // ce(i)
-  T ax, ay, az;    // ca(i) (lossy media)
-  T ph[6], pe[6];  // old psi of the H and E slots (valid in the cell's stripes)
-};
-
-// Index of a cell inside the z-stripe, or 0 when k is not in it (a valid
-// dummy address for the unconditional load).
-__device__ __forceinline__ int zstripe_or0(int k, int n, int nz) {
-  return k < n ? k : (k >= nz - n ? n + k - (nz - n) : 0);
+__device__ __forceinline__ void lgkm_wait() {
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 }
 
-// Loads of plane i for the cell (i, jc, kc) (coordinates already clamped to
-// the grid).  zwave: some lane of this wave lies in the z-stripe.
-template <typename T, bool LOSSY>
-__device__ __forceinline__ void row_load(RowPlane<T> &o, const Geo &g, const T *__restrict__ s0,
-                                         const T *__restrict__ coef, const T *__restrict__ q0,
-                                         int i, int jc, int kc, bool zwave) {
-  const long long vs = g.vs;
-  const int sxy = g.ny * g.nz;
-  const int idx = (i * g.ny + jc) * g.nz + kc;
-  const int in1 = i + 1 < g.nx ? idx + sxy : idx;
-  o.nx = s0[in1];
-  o.ny = s0[vs + in1];
-  o.nz = s0[2 * vs + in1];
-  o.hx = s0[3 * vs + idx];
-  o.hy = s0[4 * vs + idx];
-  o.hz = s0[5 * vs + idx];
-  o.cx = coef[idx];
-  o.cy = coef[vs + idx];
-  o.cz = coef[2 * vs + idx];
-  if (LOSSY) {
-    o.ax = coef[3 * vs + idx];
-    o.ay = coef[4 * vs + idx];
-    o.az = coef[5 * vs + idx];
+// Wait until at most n vector memory operations of this wave are pending;
+// n is rounded down to one of the immediates below.
+__device__ __forceinline__ void vm_wait(int n) {
+  n = __builtin_amdgcn_readfirstlane(n);
+#define HF_VMW(N) __asm__ volatile("s_waitcnt vmcnt(" #N ")" ::: "memory")
+  if (n >= 16) {
+    if (n >= 32) {
+      if (n >= 48) HF_VMW(48);
+      else if (n >= 40) HF_VMW(40);
+      else HF_VMW(32);
+    } else if (n >= 24) {
+      if (n >= 28) HF_VMW(28);
+      else HF_VMW(24);
+    } else if (n >= 20) {
+      HF_VMW(20);
+    } else {
+      HF_VMW(16);
+    }
+  } else if (n >= 8) {
+    if (n >= 12) {
+      if (n >= 14) HF_VMW(14);
+      else HF_VMW(12);
+    } else if (n >= 10) {
+      HF_VMW(10);
+    } else {
+      HF_VMW(8);
+    }
+  } else if (n >= 4) {
+    if (n >= 6) HF_VMW(6);
+    else HF_VMW(4);
+  } else if (n >= 2) {
+    HF_VMW(2);
   } else {
-    o.ax = o.ay = o.az = T(1);
+    HF_VMW(0);
   }
-  const int n = g.npml;
-  if (n <= 0) return;
-  if (i < n || i >= g.nx - n) {  // uniform per plane
-    const int q = (stripe_local(i, n, g.nx) * g.ny + jc) * g.nz + kc;
-    o.ph[0] = q0[g.psi_off[PSI_HY_X] + q];
-    o.ph[1] = q0[g.psi_off[PSI_HZ_X] + q];
-    o.pe[0] = q0[g.psi_off[PSI_EY_X] + q];
-    o.pe[1] = q0[g.psi_off[PSI_EZ_X] + q];
-  }
-  if (jc < n || jc >= g.ny - n) {  // uniform per row wave
-    const int q = (i * 2 * n + stripe_local(jc, n, g.ny)) * g.nz + kc;
-    o.ph[2] = q0[g.psi_off[PSI_HX_Y] + q];
-    o.ph[3] = q0[g.psi_off[PSI_HZ_Y] + q];
-    o.pe[2] = q0[g.psi_off[PSI_EX_Y] + q];
-    o.pe[3] = q0[g.psi_off[PSI_EZ_Y] + q];
-  }
-  if (zwave) {  // uniform per wave; lanes outside the stripe read a dummy
-    const int q = (i * g.ny + jc) * (2 * n) + zstripe_or0(kc, n, g.nz);
-    o.ph[4] = q0[g.psi_off[PSI_HX_Z] + q];
-    o.ph[5] = q0[g.psi_off[PSI_HY_Z] + q];
-    o.pe[4] = q0[g.psi_off[PSI_EX_Z] + q];
-    o.pe[5] = q0[g.psi_off[PSI_EY_Z] + q];
-  }
+#undef HF_VMW
 }
 
-// Inputs of one recomputed halo H cell: E and H at the cell, the two E values
-// at its in-plane forward neighbour outside the tile, E at i+1, old psi.
+// LDS-DMA through buffer resources: the per-lane part of an address is a
+// 32-bit byte offset fixed for the whole tile (VGPR), the per-plane part a
+// wave-uniform byte offset (SGPR), so issuing a load costs no vector ALU
+// work.  Offsets are 32-bit: a slab must stay below 4 GB (checked on the
+// host before a fused sweep is launched).
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ __forceinline__ Rsrc rsrc_of(const void *base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, -1, 0x00020000);
+}
+
+// Per-lane byte offsets of the dwords a 64-element row is made of: dword
+// lane + 64 p of the row (p < sizeof(T)/4) is dword (lane + 64 p) % W of
+// element (lane + 64 p) / W.
 template <typename T>
-struct HaloCell {
-  T ex, ey, ez;
-  T fa, fb;  // row halo: Ex, Ey at k+1; column halo: Ex, Ez at j+1
-  T iy, iz;  // Ey, Ez at i+1
-  T hx, hy, hz;
-  T p[6];
+struct LaneOff {
+  static constexpr int W = (int)(sizeof(T) / 4);
+  unsigned v[W];
 };
 
-// Everything the halo wave loads for one plane: the j0-1 row cell and the
-// k0-1 column cell of this lane, and the E values of the tile's forward
-// neighbours: row j0+TJ (Ex, Ez at k0+lane) and column k0+64 (Ex, Ey at
-// j0+lane).
-template <typename T>
-struct HaloPlane {
-  HaloCell<T> r, c;
-  T fjx, fjz, fkx, fky;
-};
+// element e of the row -> element offset, for the kinds of rows in use
+enum RowKind { kRowK, kRowZ, kColJ, kColY, kColJ2 };
 
-// Halo cell (i, jc, kc), coordinates clamped to the grid; KIND 0 loads the
-// k+1 neighbour, KIND 1 the j+1 neighbour.
 template <typename T, int KIND>
-__device__ __forceinline__ void halo_load(HaloCell<T> &c, const Geo &g, const T *__restrict__ s0,
-                                          const T *__restrict__ q0, int i, int jc, int kc,
-                                          bool zwave) {
-  const long long vs = g.vs;
-  const int sxy = g.ny * g.nz;
-  const int q = (i * g.ny + jc) * g.nz + kc;
-  c.ex = s0[q];
-  c.ey = s0[vs + q];
-  c.ez = s0[2 * vs + q];
-  c.hx = s0[3 * vs + q];
-  c.hy = s0[4 * vs + q];
-  c.hz = s0[5 * vs + q];
-  if (KIND == 0) {
-    const int qk = kc + 1 < g.nz ? q + 1 : q;
-    c.fa = s0[qk];
-    c.fb = s0[vs + qk];
-  } else {
-    const int qj = jc + 1 < g.ny ? q + g.nz : q;
-    c.fa = s0[qj];
-    c.fb = s0[2 * vs + qj];
-  }
-  const int qi = i + 1 < g.nx ? q + sxy : q;
-  c.iy = s0[vs + qi];
-  c.iz = s0[2 * vs + qi];
+__device__ __forceinline__ LaneOff<T> lane_off(const Geo &g, int lane, int k0, int j0) {
+  LaneOff<T> o;
   const int n = g.npml;
-  if (n <= 0) return;
-  if (i < n || i >= g.nx - n) {
-    const int t = (stripe_local(i, n, g.nx) * g.ny + jc) * g.nz + kc;
-    c.p[0] = q0[g.psi_off[PSI_HY_X] + t];
-    c.p[1] = q0[g.psi_off[PSI_HZ_X] + t];
+#pragma unroll
+  for (int p = 0; p < LaneOff<T>::W; ++p) {
+    const int d = lane + 64 * p;
+    const int e = d / LaneOff<T>::W;
+    int el;
+    if (KIND == kRowK) {
+      el = min(k0 + e, g.nz - 1);
+    } else if (KIND == kRowZ) {
+      const int k = min(k0 + e, g.nz - 1);
+      el = k < n ? k : (k >= g.nz - n ? n + k - (g.nz - n) : 0);
+    } else {
+      const int j = min(j0 + e, g.ny - 1);
+      if (KIND == kColJ) el = j * g.nz;
+      else if (KIND == kColJ2) el = j * 2 * n;
+      else el = (j < n ? j : (j >= g.ny - n ? n + j - (g.ny - n) : 0)) * g.nz;
+    }
+    o.v[p] = (unsigned)el * (unsigned)sizeof(T) + 4u * (unsigned)(d % LaneOff<T>::W);
   }
-  // y- and z-stripes: load for every lane with dummy indices outside them
-  {
-    const int jl = jc < n ? jc : (jc >= g.ny - n ? n + jc - (g.ny - n) : 0);
-    const int t = (i * 2 * n + jl) * g.nz + kc;
-    c.p[2] = q0[g.psi_off[PSI_HX_Y] + t];
-    c.p[3] = q0[g.psi_off[PSI_HZ_Y] + t];
-  }
-  if (zwave) {
-    const int t = (i * g.ny + jc) * (2 * n) + zstripe_or0(kc, n, g.nz);
-    c.p[4] = q0[g.psi_off[PSI_HX_Z] + t];
-    c.p[5] = q0[g.psi_off[PSI_HY_Z] + t];
-  }
+  return o;
 }
 
-// LDS of one tile.  sE: E(i) of the TJ rows, with row TJ = row j0+TJ and
-// column 64 = column k0+64 (the forward neighbours outside the tile).  sH:
-// H(i) with row 0 = the j0-1 halo row and column 0 = the k0-1 halo column.
-template <typename T, int TJ>
-struct TileLds {
-  T sE[3][TJ + 1][65];
-  T sH[3][TJ + 1][65];
-};
-
-// Per-wave state of a row wave that persists from plane to plane.
+// Column-vector rows t = 0, 1 of a slot: element c = 64 t + e is vector
+// c / 16 (E x,y,z at k0-1 | Ex, Ey at k0+64 | H x,y,z at k0-1), grid row
+// j0 - 1 + c % 16; offsets relative to the start of the plane.
 template <typename T>
-struct RowState {
-  T ex, ey, ez;  // E(i) of the own cell (pre-source)
-  T hyp, hzp;    // H(i-1) of the own cell, advanced
-};
-
-// One plane of the halo wave: prefetch plane i+1 (PF), publish the forward
-// neighbours of plane i, recompute H(i) on the j0-1 row and k0-1 column.
-// PF is a template argument, not a runtime test: with a load-bearing branch
-// the wait-count bookkeeping at the join would have to assume no prefetch
-// was issued and drain every load before the current plane could be used.
-template <typename T, int TJ, bool PF>
-__device__ __forceinline__ HaloPlane<T> halo_plane(
-    const Geo &g, const T *__restrict__ s0, const T *__restrict__ prof, const T *__restrict__ q0,
-    TileLds<T, TJ> &L, T dtm_dl, T dtm, int i, int j0, int k0, int tk, bool row_on,
-    bool col_on, bool zrow, bool zcol, const Prof1<T> (&rp)[2], const Prof1<T> (&cp)[2],
-    HaloPlane<T> cur) {
-  const int kr = min(k0 + tk, g.nz - 1);       // row cell k (clamped)
-  const int jr = max(j0 - 1, 0);               // row cell j
-  const int jc = min(j0 + tk, g.ny - 1);       // column cell j (clamped)
-  const int kcol = max(k0 - 1, 0);             // column cell k
-  const int jf = min(j0 + TJ, g.ny - 1);       // far row
-  const int kf = min(k0 + 64, g.nz - 1);       // far column
-  HaloPlane<T> nxt;
-  if (PF) {
-    const int i1 = i + 1;
-    halo_load<T, 0>(nxt.r, g, s0, q0, i1, jr, kr, zrow);
-    halo_load<T, 1>(nxt.c, g, s0, q0, i1, jc, kcol, zcol);
-    const long long vs = g.vs;
-    const int qf = (i1 * g.ny + jf) * g.nz + kr;
-    nxt.fjx = s0[qf];
-    nxt.fjz = s0[2 * vs + qf];
-    const int qc = (i1 * g.ny + jc) * g.nz + kf;
-    nxt.fkx = s0[qc];
-    nxt.fky = s0[vs + qc];
+__device__ __forceinline__ LaneOff<T> col_off(const Geo &g, int lane, int k0, int j0, int t) {
+  LaneOff<T> o;
+#pragma unroll
+  for (int p = 0; p < LaneOff<T>::W; ++p) {
+    const int d = lane + 64 * p;
+    const int c = 64 * t + d / LaneOff<T>::W;
+    const int v = c >> 4;
+    const int jr = min(max(j0 - 1 + (c & 15), 0), g.ny - 1);
+    const int comp = v < 3 ? v : (v < 5 ? v - 3 : v - 2);
+    const int kk = (v == 3 || v == 4) ? min(k0 + 64, g.nz - 1) : max(k0 - 1, 0);
+    const unsigned el = (unsigned)comp * (unsigned)g.vs + (unsigned)(jr * g.nz + kk);
+    o.v[p] = el * (unsigned)sizeof(T) + 4u * (unsigned)(d % LaneOff<T>::W);
   }
-  L.sE[0][TJ][tk] = cur.fjx;
-  L.sE[2][TJ][tk] = cur.fjz;
-  if (tk < TJ) {
-    L.sE[0][tk][64] = cur.fkx;
-    L.sE[1][tk][64] = cur.fky;
-  }
-  lds_barrier();  // E(i) tile in LDS
-  const Prof1<T> px = prof1(g, prof, 1, 0, i);
-  if (row_on) {  // E at (i, j0, k) comes from the tile
-    HaloCell<T> &rc = cur.r;
-    const StripeIdx s = stripe_idx(g, i, j0 - 1, k0 + tk);
-    h_math(g, i, j0 - 1, k0 + tk, s, px, rp[0], rp[1], rc.ex, rc.ey, rc.ez, L.sE[0][0][tk],
-           L.sE[2][0][tk], rc.fa, rc.fb, rc.iy, rc.iz, rc.hx, rc.hy, rc.hz, rc.p, dtm_dl, dtm);
-    L.sH[0][0][tk + 1] = rc.hx;
-    L.sH[1][0][tk + 1] = rc.hy;
-    L.sH[2][0][tk + 1] = rc.hz;
-  }
-  if (col_on) {  // E at (i, j, k0) comes from the tile
-    HaloCell<T> &cc = cur.c;
-    const StripeIdx s = stripe_idx(g, i, j0 + tk, k0 - 1);
-    h_math(g, i, j0 + tk, k0 - 1, s, px, cp[0], cp[1], cc.ex, cc.ey, cc.ez, cc.fa, cc.fb,
-           L.sE[0][tk][0], L.sE[1][tk][0], cc.iy, cc.iz, cc.hx, cc.hy, cc.hz, cc.p, dtm_dl, dtm);
-    L.sH[0][tk + 1][0] = cc.hx;
-    L.sH[1][tk + 1][0] = cc.hy;
-    L.sH[2][tk + 1][0] = cc.hz;
-  }
-  lds_barrier();  // halo H in LDS
-  return nxt;
+  return o;
 }
 
-// One plane of a row wave: prefetch plane i+1 (PF, a template argument for
-// the reason given at halo_plane), advance H(i) and E(i) of the own cell from
-// cur.  pyz = {H y, H z, E y, E z} profiles of the own column.
-template <typename T, int TJ, bool LOSSY, bool PF>
-__device__ __forceinline__ RowPlane<T> row_plane(
-    const Geo &g, T dtm_dl, T dtm, T inv_dl, const T *__restrict__ s0, T *__restrict__ s1,
-    const T *__restrict__ coef, const T *__restrict__ prof, const T *__restrict__ q0,
-    T *__restrict__ q1, TileLds<T, TJ> &L, RowState<T> &st, RowPlane<T> cur, int i, int j,
-    int k, int jc, int kc, int w, int tk, bool act, bool zwave, const Prof1<T> (&pyz)[4]) {
-  const long long vs = g.vs;
-  const int idx = (i * g.ny + j) * g.nz + k;
-  // ── 0. loads of plane i+1 ──
-  RowPlane<T> nxt;
-  if (PF) row_load<T, LOSSY>(nxt, g, s0, coef, q0, i + 1, jc, kc, zwave);
-  // ── 1. E(i) tile to LDS ──
-  L.sE[0][w][tk] = st.ex;
-  L.sE[1][w][tk] = st.ey;
-  L.sE[2][w][tk] = st.ez;
-  lds_barrier();
+// One 64-element row into LDS (sizeof(T)/4 loads); soff in elements.
+template <typename T>
+__device__ __forceinline__ int dma(Rsrc rs, T *dst, const LaneOff<T> &vo, unsigned soff) {
+  const unsigned sb = soff * (unsigned)sizeof(T);
+#pragma unroll
+  for (int p = 0; p < LaneOff<T>::W; ++p)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        rs, (__attribute__((address_space(3))) void *)((char *)dst + 256 * p), 4, vo.v[p], sb, 0,
+        0);
+  return LaneOff<T>::W;
+}
 
-  // ── 2. own H(i) ──
-  const StripeIdx so = stripe_idx(g, i, j, k);
-  T hx = cur.hx, hy = cur.hy, hz = cur.hz;
-  if (act) {
-    const Prof1<T> px = prof1(g, prof, 1, 0, i);
-    h_math(g, i, j, k, so, px, pyz[0], pyz[1], st.ex, st.ey, st.ez, L.sE[0][w + 1][tk],
-           L.sE[2][w + 1][tk], L.sE[0][w][tk + 1], L.sE[1][w][tk + 1], cur.ny, cur.nz, hx, hy,
-           hz, cur.ph, dtm_dl, dtm);
-    s1[3 * vs + idx] = hx;
-    s1[4 * vs + idx] = hy;
-    s1[5 * vs + idx] = hz;
-    psi_store(g, q1, 0, so, cur.ph);
+// Element offset of psi slot s (slots are stored back to back, in the
+// order H: Hy_x Hz_x Hx_y Hz_y Hx_z Hy_z, E: Ey_x Ez_x Ex_y Ez_y Ex_z Ey_z;
+// a slot spans its stripe pair).  Cheaper to rebuild from the grid than to
+// keep twelve offsets live in scalar registers.
+__device__ __forceinline__ unsigned psi_slot(const Geo &g, int s) {
+  const unsigned n2 = 2u * (unsigned)g.npml;
+  const unsigned sx = n2 * (unsigned)(g.ny * g.nz), sy = (unsigned)g.nx * n2 * (unsigned)g.nz;
+  const unsigned sz = (unsigned)(g.nx * g.ny) * n2;
+  const int t = s % 6;
+  const unsigned in_group = t == 0 ? 0u : t == 1 ? sx : t == 2 ? 2 * sx : t == 3 ? 2 * sx + sy
+                            : t == 4 ? 2 * (sx + sy) : 2 * (sx + sy) + sz;
+  return (unsigned)(s / 6) * 2 * (sx + sy + sz) + in_group;  // s == 12: the slab end
+}
+
+template <typename T>
+__device__ __forceinline__ void buf_store(Rsrc rs, T v, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, voff, soff, 0);
+  } else {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rs, voff, soff, 0);
   }
+}
+
+// Resources and per-lane offsets of one tile.
+template <typename T>
+struct TileDma {
+  Rsrc st, cf, ps;        // state slab 0, coefficients, old psi
+  Rsrc wst, wps;          // state slab 1, new psi (stores)
+  LaneOff<T> ok, oz;      // k runs: plain, z-stripe local
+  LaneOff<T> oc[2];       // column-vector rows
+  LaneOff<T> ojn, ojy, oj2;  // halo columns over j: x-, y-, z-stripe psi
+};
+
+// LDS layout of one plane slot, in elements (rows of 64).
+template <typename T, int TJ, bool LOSSY>
+struct SlotLayout {
+  static_assert(TJ >= 1 && TJ + 2 <= 16, "column vectors hold 16 rows");
+  static constexpr int kRowsE = 3 * (TJ + 2);
+  static constexpr int kRowsH = 3 * (TJ + 1);
+  static constexpr int kRowsC = (LOSSY ? 6 : 3) * TJ;
+  static constexpr int kRows = kRowsE + kRowsH + kRowsC + 2;
+  static constexpr int E = 0;
+  static constexpr int H = 64 * kRowsE;
+  static constexpr int C = H + 64 * kRowsH;
+  static constexpr int COL = C + 64 * kRowsC;
+  static constexpr int kSize = 64 * kRows;
+  static constexpr int kWaves = TJ + 1;
+  static constexpr unsigned long bytes(int d) {
+    return ((unsigned long)d * kSize + kWaves * 12 * 64 + 3 * (TJ + 1) * 65) * sizeof(T);
+  }
+  // ring depth: 4 while two blocks still fit the 160 KB of a CU, else 3
+  static constexpr int kDepth = bytes(4) <= 80 * 1024 ? 4 : 3;
+};
+
+template <typename T, int TJ, bool LOSSY>
+struct FusedLds {
+  using S = SlotLayout<T, TJ, LOSSY>;
+  T ring[S::kDepth][S::kSize];
+  T psi[TJ + 1][12][64];  // per wave: psi rows of the next plane
+  T sH[3][TJ + 1][65];    // H(i): row 0 = j0-1 halo, column 0 = k0-1 halo
+};
+
+// Slot loads of plane x (clamped to the grid).  Slot row r of the E and H
+// blocks is grid row j0 - 1 + r, of the ce / ca block grid row j0 + r.  Row
+// wave w loads row r = w of every block (3 E, 3 H, 3 or 6 coefficient
+// rows); the halo wave loads E rows TJ and TJ + 1, H row TJ and the two
+// column-vector rows.  Rows outside the grid are clamped to its edge (their
+// values are never used).  Returns the number of loads issued.
+template <typename T, int TJ, bool LOSSY, bool HALO>
+__device__ __forceinline__ int issue_slot(const Geo &g, const TileDma<T> &m, T *slot, int x,
+                                          int j0, int w) {
+  using S = SlotLayout<T, TJ, LOSSY>;
+  constexpr int RE = TJ + 2, RH = TJ + 1;
+  const unsigned vs = (unsigned)g.vs;
+  const unsigned pbase = (unsigned)(min(x, g.nx - 1) * g.ny * g.nz);
+  const int r = HALO ? TJ : w;
+  const unsigned e0 = pbase + (unsigned)(min(max(j0 - 1 + r, 0), g.ny - 1) * g.nz);
+  int n = 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) n += dma(m.st, slot + S::E + (c * RE + r) * 64, m.ok, e0 + c * vs);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    n += dma(m.st, slot + S::H + (c * RH + r) * 64, m.ok, e0 + (3 + c) * vs);
+  if (HALO) {
+    const unsigned e1 = pbase + (unsigned)(min(j0 + TJ, g.ny - 1) * g.nz);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      n += dma(m.st, slot + S::E + (c * RE + TJ + 1) * 64, m.ok, e1 + c * vs);
+    n += dma(m.st, slot + S::COL, m.oc[0], pbase);
+    n += dma(m.st, slot + S::COL + 64, m.oc[1], pbase);
+  } else {
+    const unsigned c0 = pbase + (unsigned)(min(j0 + w, g.ny - 1) * g.nz);
+#pragma unroll
+    for (int c = 0; c < (LOSSY ? 6 : 3); ++c)
+      n += dma(m.cf, slot + S::C + (c * TJ + w) * 64, m.ok, c0 + c * vs);
+  }
+  return n;
+}
+
+// Slot of plane x + the ring bookkeeping of one wave.
+template <int D>
+struct Ring {
+  int issued;       // vector memory operations issued so far
+  int mark[D - 1];  // `issued` right after the slot loads of planes i+1 ..
+  int mark_psi;     // `issued` right after the psi loads of plane i
+  int si;           // slot of plane i
+  __device__ __forceinline__ void next() {
+    si = si + 1 == D ? 0 : si + 1;
+#pragma unroll
+    for (int t = 0; t + 1 < D - 1; ++t) mark[t] = mark[t + 1];
+  }
+  // pending operations allowed while psi(i) and the slots <= i+1 are needed
+  __device__ __forceinline__ int slack() const { return issued - max(mark[0], mark_psi); }
+};
+
+// Kernel arguments of one tile.
+template <typename T>
+struct TileArgs {
+  Geo g;
+  T dtm_dl, dtm, inv_dl;
+  const T *s0;
+  T *s1;
+  const T *coef, *prof, *q0;
+  T *q1;
+  int j0, k0, w, tk;
+  int last;  // last plane whose slot is needed
+  int ie;    // end of the i chunk
+  TileDma<T> m;
+};
+
+// Per-wave constants of a row wave.
+template <typename T>
+struct RowWave {
+  int jc, kc;        // own cell, clamped to the grid
+  unsigned own;      // byte offset of the own cell within its plane
+  unsigned junk;     // byte offset of this lane's volume padding slot
+  bool act;          // own cell inside the grid
+  bool ywave, zwave; // the row lies in a y-stripe / the tile meets a z-stripe
+  bool zin;          // own cell in a z-stripe
+  Prof1<T> pyz[4];   // H y, H z, E y, E z profiles of the own column
+  T hyp, hzp;        // H(i-1) of the own cell, advanced
+};
+
+// Per-wave constants of the halo wave.
+template <typename T>
+struct HaloWave {
+  int jr, kr;      // row-halo cell (j0-1, k0+lane), clamped
+  int kcol;        // column-halo cells (j0+lane, k0-1): k, clamped
+  bool row_on, col_on, yrow, zrow, ycol, zcol;
+  Prof1<T> rp[2], cp[2];  // H y, z profiles of the two cells
+};
+
+// local index inside the stripes of an axis of length N, 0 outside them
+__device__ __forceinline__ int stripe_local_or0(int i, int n, int N) {
+  return i < n ? i : (i >= N - n ? n + i - (N - n) : 0);
+}
+
+// psi rows of plane x for the own cell column of a row wave: rows 0..5 the
+// H slots, 6..11 the E slots, stripes the wave touches only.
+template <typename T>
+__device__ __forceinline__ int issue_psi_row(const Geo &g, const TileDma<T> &m, T (*stage)[64],
+                                             int x, const RowWave<T> &r) {
+  const int n = g.npml;
+  int c = 0;
+  if (n <= 0) return 0;
+  if (x < n || x >= g.nx - n) {
+    const unsigned off = (unsigned)((stripe_local(x, n, g.nx) * g.ny + r.jc) * g.nz);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int s = t < 2 ? t : t + 4;  // Hy_x Hz_x Ey_x Ez_x
+      c += dma(m.ps, stage[s], m.ok, psi_slot(g, s) + off);
+    }
+  }
+  if (r.ywave) {
+    const unsigned off = (unsigned)((x * 2 * n + stripe_local(r.jc, n, g.ny)) * g.nz);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int s = t < 2 ? 2 + t : 6 + t;  // Hx_y Hz_y Ex_y Ez_y
+      c += dma(m.ps, stage[s], m.ok, psi_slot(g, s) + off);
+    }
+  }
+  if (r.zwave) {
+    const unsigned off = (unsigned)((x * g.ny + r.jc) * (2 * n));
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int s = t < 2 ? 4 + t : 8 + t;  // Hx_z Hy_z Ex_z Ey_z
+      c += dma(m.ps, stage[s], m.oz, psi_slot(g, s) + off);
+    }
+  }
+  return c;
+}
+
+// psi rows of plane x for the halo cells: rows 0..5 the H slots of the
+// row-halo cell of each lane, rows 6..11 those of the column-halo cells.
+template <typename T>
+__device__ __forceinline__ int issue_psi_halo(const Geo &g, const TileDma<T> &m, T (*stage)[64],
+                                              int x, const HaloWave<T> &h) {
+  const int n = g.npml;
+  int c = 0;
+  if (n <= 0) return 0;
+  if (x < n || x >= g.nx - n) {
+    const int il = stripe_local(x, n, g.nx);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      c += dma(m.ps, stage[s], m.ok, psi_slot(g, s) + (unsigned)((il * g.ny + h.jr) * g.nz));
+      c += dma(m.ps, stage[6 + s], m.ojn, psi_slot(g, s) + (unsigned)(il * g.ny * g.nz + h.kcol));
+    }
+  }
+  if (h.yrow) {
+    const int off = (x * 2 * n + stripe_local(h.jr, n, g.ny)) * g.nz;
+#pragma unroll
+    for (int s = 2; s < 4; ++s) c += dma(m.ps, stage[s], m.ok, psi_slot(g, s) + (unsigned)off);
+  }
+  if (h.ycol) {
+#pragma unroll
+    for (int s = 2; s < 4; ++s)
+      c += dma(m.ps, stage[6 + s], m.ojy, psi_slot(g, s) + (unsigned)(x * 2 * n * g.nz + h.kcol));
+  }
+  if (h.zrow) {
+    const int off = (x * g.ny + h.jr) * (2 * n);
+#pragma unroll
+    for (int s = 4; s < 6; ++s) c += dma(m.ps, stage[s], m.oz, psi_slot(g, s) + (unsigned)off);
+  }
+  if (h.zcol) {
+    const int kl = stripe_local_or0(h.kcol, n, g.nz);
+#pragma unroll
+    for (int s = 4; s < 6; ++s)
+      c += dma(m.ps, stage[6 + s], m.oj2, psi_slot(g, s) + (unsigned)(x * g.ny * 2 * n + kl));
+  }
+  return c;
+}
+
+// New psi of a row-wave cell, stored for every lane inside wave-uniform
+// branches (lanes outside the grid or the stripe write to the slab padding).
+// Returns the number of stores issued.
+template <typename T>
+__device__ __forceinline__ int store_psi_row(const Geo &g, const TileDma<T> &m, int base,
+                                             const StripeIdx &so, bool xst, const RowWave<T> &r,
+                                             int tk, const T p[6]) {
+  const unsigned es = (unsigned)sizeof(T);
+  const unsigned junk = (psi_slot(g, PSI_COUNT) + (unsigned)tk) * es;
+  int c = 0;
+  if (xst) {
+    buf_store(m.wps, p[0], r.act ? (psi_slot(g, base) + so.qx) * es : junk, 0u);
+    buf_store(m.wps, p[1], r.act ? (psi_slot(g, base + 1) + so.qx) * es : junk, 0u);
+    c += 2;
+  }
+  if (r.ywave) {
+    buf_store(m.wps, p[2], r.act ? (psi_slot(g, base + 2) + so.qy) * es : junk, 0u);
+    buf_store(m.wps, p[3], r.act ? (psi_slot(g, base + 3) + so.qy) * es : junk, 0u);
+    c += 2;
+  }
+  if (r.zwave) {
+    const bool ok = r.act && r.zin;
+    buf_store(m.wps, p[4], ok ? (psi_slot(g, base + 4) + so.qz) * es : junk, 0u);
+    buf_store(m.wps, p[5], ok ? (psi_slot(g, base + 5) + so.qz) * es : junk, 0u);
+    c += 2;
+  }
+  return c;
+}
+
+// Issue the loads of plane i's successors: psi of plane i+1 (own staging,
+// after this wave's reads of it are complete), then the slot of plane i+D-1.
+template <typename T, int TJ, bool LOSSY, bool HALO, class W>
+__device__ __forceinline__ void plane_loads(const TileArgs<T> &a, FusedLds<T, TJ, LOSSY> &L,
+                                            Ring<SlotLayout<T, TJ, LOSSY>::kDepth> &p,
+                                            const W &wv, int i) {
+  using S = SlotLayout<T, TJ, LOSSY>;
+  constexpr int D = S::kDepth;
+  lgkm_wait();  // the staging reads of this wave have completed
+  if (i + 1 < a.ie) {
+    if constexpr (HALO)
+      p.issued += issue_psi_halo(a.g, a.m, L.psi[a.w], i + 1, wv);
+    else
+      p.issued += issue_psi_row(a.g, a.m, L.psi[a.w], i + 1, wv);
+  }
+  p.mark_psi = p.issued;
+  if (i + D - 1 <= a.last) {
+    const int s = p.si == 0 ? D - 1 : p.si - 1;
+    p.issued += issue_slot<T, TJ, LOSSY, HALO>(a.g, a.m, L.ring[s], i + D - 1, a.j0, a.w);
+  }
+  p.mark[D - 2] = p.issued;
+}
+
+// Prologue after the psi loads of plane ib: the slots of planes ib .. ib+D-2.
+template <typename T, int TJ, bool LOSSY, bool HALO>
+__device__ __forceinline__ void prologue_slots(const TileArgs<T> &a, FusedLds<T, TJ, LOSSY> &L,
+                                               Ring<SlotLayout<T, TJ, LOSSY>::kDepth> &p,
+                                               int ib) {
+  constexpr int D = SlotLayout<T, TJ, LOSSY>::kDepth;
+  p.mark_psi = p.issued;
+  p.issued += issue_slot<T, TJ, LOSSY, HALO>(a.g, a.m, L.ring[0], ib, a.j0, a.w);
+#pragma unroll
+  for (int t = 0; t + 2 < D; ++t) {
+    if (ib + 1 + t <= a.last)
+      p.issued +=
+          issue_slot<T, TJ, LOSSY, HALO>(a.g, a.m, L.ring[t + 1], ib + 1 + t, a.j0, a.w);
+    p.mark[t] = p.issued;
+  }
+}
+
+// One plane of a row wave.
+template <typename T, int TJ, bool LOSSY>
+__device__ __forceinline__ void row_plane(const TileArgs<T> &a, FusedLds<T, TJ, LOSSY> &L,
+                                          Ring<SlotLayout<T, TJ, LOSSY>::kDepth> &p,
+                                          RowWave<T> &r, int i) {
+  using S = SlotLayout<T, TJ, LOSSY>;
+  constexpr int D = S::kDepth;
+  constexpr int RE = TJ + 2, RH = TJ + 1;
+  const Geo &g = a.g;
+  const long long vs = g.vs;
+  const int w = a.w, tk = a.tk;
+  vm_wait(p.slack());
+  lds_barrier();  // B1
+  T ph[6], pe[6];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    ph[s] = L.psi[w][s][tk];
+    pe[s] = L.psi[w][6 + s][tk];
+  }
+  plane_loads<T, TJ, LOSSY, false>(a, L, p, r, i);
+
+  const T *X = L.ring[p.si];
+  const T *Y = L.ring[p.si + 1 == D ? 0 : p.si + 1];
+  const int rw = w + 1;  // row of the own cell in the E and H blocks
+  const T ex = X[S::E + (0 * RE + rw) * 64 + tk];
+  const T ey = X[S::E + (1 * RE + rw) * 64 + tk];
+  const T ez = X[S::E + (2 * RE + rw) * 64 + tk];
+  T hx = X[S::H + (0 * RH + rw) * 64 + tk];
+  T hy = X[S::H + (1 * RH + rw) * 64 + tk];
+  T hz = X[S::H + (2 * RH + rw) * 64 + tk];
+  const StripeIdx so = stripe_idx(g, i, r.jc, r.kc);
+  const bool xst = g.npml > 0 && (i < g.npml || i >= g.nx - g.npml);
+  h_math(g, i, r.jc, r.kc, so, prof1(g, a.prof, 1, 0, i), r.pyz[0], r.pyz[1], ex, ey, ez,
+         X[S::E + (0 * RE + rw + 1) * 64 + tk], X[S::E + (2 * RE + rw + 1) * 64 + tk],
+         X[tk < 63 ? S::E + (0 * RE + rw) * 64 + tk + 1 : S::COL + 3 * 16 + rw],
+         X[tk < 63 ? S::E + (1 * RE + rw) * 64 + tk + 1 : S::COL + 4 * 16 + rw],
+         Y[S::E + (1 * RE + rw) * 64 + tk], Y[S::E + (2 * RE + rw) * 64 + tk], hx, hy, hz, ph,
+         a.dtm_dl, a.dtm);
+  const unsigned es = (unsigned)sizeof(T), vsb = (unsigned)vs * es;
+  const unsigned o = r.act ? r.own + (unsigned)(i * g.ny * g.nz) * es : r.junk;
+  buf_store(a.m.wst, hx, o, 3 * vsb);
+  buf_store(a.m.wst, hy, o, 4 * vsb);
+  buf_store(a.m.wst, hz, o, 5 * vsb);
+  p.issued += 3;
+  if (g.npml > 0) p.issued += store_psi_row(g, a.m, 0, so, xst, r, tk, ph);
   L.sH[0][w + 1][tk + 1] = hx;
   L.sH[1][w + 1][tk + 1] = hy;
   L.sH[2][w + 1][tk + 1] = hz;
-  lds_barrier();
+  lds_barrier();  // B2
 
-  // ── 3. E(i) (source cells are redone by source_fixup afterwards) ──
-  if (act) {
-    T ex = st.ex, ey = st.ey, ez = st.ez;
-    const Prof1<T> px = prof1(g, prof, 0, 0, i);
-    e_math(g, i, j, k, so, px, pyz[2], pyz[3], hx, hy, hz, L.sH[0][w][tk + 1],
-           L.sH[2][w][tk + 1], L.sH[0][w + 1][tk], L.sH[1][w + 1][tk], st.hyp, st.hzp, ex, ey,
-           ez, cur.ax, cur.ay, cur.az, cur.cx, cur.cy, cur.cz, cur.pe, inv_dl);
-    s1[idx] = ex;
-    s1[vs + idx] = ey;
-    s1[2 * vs + idx] = ez;
-    psi_store(g, q1, 6, so, cur.pe);
+  T ex2 = ex, ey2 = ey, ez2 = ez;
+  const T ax = LOSSY ? X[S::C + (3 * TJ + w) * 64 + tk] : T(1);
+  const T ay = LOSSY ? X[S::C + (4 * TJ + w) * 64 + tk] : T(1);
+  const T az = LOSSY ? X[S::C + (5 * TJ + w) * 64 + tk] : T(1);
+  e_math(g, i, r.jc, r.kc, so, prof1(g, a.prof, 0, 0, i), r.pyz[2], r.pyz[3], hx, hy, hz,
+         L.sH[0][w][tk + 1], L.sH[2][w][tk + 1], L.sH[0][w + 1][tk], L.sH[1][w + 1][tk], r.hyp,
+         r.hzp, ex2, ey2, ez2, ax, ay, az, X[S::C + (0 * TJ + w) * 64 + tk],
+         X[S::C + (1 * TJ + w) * 64 + tk], X[S::C + (2 * TJ + w) * 64 + tk], pe, a.inv_dl);
+  buf_store(a.m.wst, ex2, o, 0u);
+  buf_store(a.m.wst, ey2, o, vsb);
+  buf_store(a.m.wst, ez2, o, 2 * vsb);
+  p.issued += 3;
+  if (g.npml > 0) p.issued += store_psi_row(g, a.m, 6, so, xst, r, tk, pe);
+  r.hyp = hy;
+  r.hzp = hz;
+  p.next();
+}
+
+// One plane of the halo wave.
+template <typename T, int TJ, bool LOSSY>
+__device__ __forceinline__ void halo_plane(const TileArgs<T> &a, FusedLds<T, TJ, LOSSY> &L,
+                                           Ring<SlotLayout<T, TJ, LOSSY>::kDepth> &p,
+                                           const HaloWave<T> &h, int i) {
+  using S = SlotLayout<T, TJ, LOSSY>;
+  constexpr int D = S::kDepth;
+  constexpr int RE = TJ + 2, RH = TJ + 1;
+  const Geo &g = a.g;
+  const int w = a.w, tk = a.tk;
+  vm_wait(p.slack());
+  lds_barrier();  // B1
+  T pr[6], pc[6];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    pr[s] = L.psi[w][s][tk];
+    pc[s] = L.psi[w][6 + s][tk];
   }
-  st.hyp = hy;
-  st.hzp = hz;
-  st.ex = cur.nx;
-  st.ey = cur.ny;
-  st.ez = cur.nz;
-  return nxt;
+  plane_loads<T, TJ, LOSSY, true>(a, L, p, h, i);
+
+  const T *X = L.ring[p.si];
+  const T *Y = L.ring[p.si + 1 == D ? 0 : p.si + 1];
+  const Prof1<T> px = prof1(g, a.prof, 1, 0, i);
+  if (h.row_on) {  // cell (i, j0-1, k0+lane): E and H rows 0, far column row 0
+    T hx = X[S::H + (0 * RH) * 64 + tk];
+    T hy = X[S::H + (1 * RH) * 64 + tk];
+    T hz = X[S::H + (2 * RH) * 64 + tk];
+    const StripeIdx s = stripe_idx(g, i, h.jr, h.kr);
+    h_math(g, i, h.jr, h.kr, s, px, h.rp[0], h.rp[1], X[S::E + (0 * RE) * 64 + tk],
+           X[S::E + (1 * RE) * 64 + tk], X[S::E + (2 * RE) * 64 + tk],
+           X[S::E + (0 * RE + 1) * 64 + tk], X[S::E + (2 * RE + 1) * 64 + tk],
+           X[tk < 63 ? S::E + (0 * RE) * 64 + tk + 1 : S::COL + 3 * 16],
+           X[tk < 63 ? S::E + (1 * RE) * 64 + tk + 1 : S::COL + 4 * 16],
+           Y[S::E + (1 * RE) * 64 + tk], Y[S::E + (2 * RE) * 64 + tk], hx, hy, hz, pr, a.dtm_dl,
+           a.dtm);
+    L.sH[0][0][tk + 1] = hx;
+    L.sH[1][0][tk + 1] = hy;
+    L.sH[2][0][tk + 1] = hz;
+  }
+  if (h.col_on) {  // cell (i, j0+lane, k0-1): column vectors, row lane+1
+    const int rr = tk + 1;
+    const int jc = a.j0 + tk;
+    T hx = X[S::COL + 5 * 16 + rr];
+    T hy = X[S::COL + 6 * 16 + rr];
+    T hz = X[S::COL + 7 * 16 + rr];
+    const StripeIdx s = stripe_idx(g, i, jc, h.kcol);
+    h_math(g, i, jc, h.kcol, s, px, h.cp[0], h.cp[1], X[S::COL + rr], X[S::COL + 16 + rr],
+           X[S::COL + 32 + rr], X[S::COL + rr + 1], X[S::COL + 32 + rr + 1],
+           X[S::E + (0 * RE + rr) * 64], X[S::E + (1 * RE + rr) * 64], Y[S::COL + 16 + rr],
+           Y[S::COL + 32 + rr], hx, hy, hz, pc, a.dtm_dl, a.dtm);
+    L.sH[0][rr][0] = hx;
+    L.sH[1][rr][0] = hy;
+    L.sH[2][rr][0] = hz;
+  }
+  lds_barrier();  // B2
+  p.next();
 }
 
 // Body of the fused sweep for the tile (bx, by, bz) = (k block, j block,
 // i chunk); the block is 64 x (TJ + 1) threads.  Sources are not applied
-// here: source_fixup recomputes E on the source cells after the sweep.
-// Callers that run several tiles per block separate them with lds_barrier().
+// here: source_fixup_cell redoes E on the source cells after the sweep.
+// When a tile ends, every load it issued has landed (its last plane waits
+// for all of them), so callers running several tiles per block only need
+// an lds_barrier() between tiles.
 template <typename T, int TJ, bool LOSSY>
 __device__ __forceinline__ void fused_tile(const Geo &g, int planes, T dtm_dl, T dtm, T inv_dl,
                                            const T *__restrict__ s0, T *__restrict__ s1,
@@ -569,109 +862,107 @@ __device__ __forceinline__ void fused_tile(const Geo &g, int planes, T dtm_dl, T
                                            const T *__restrict__ prof,
                                            const T *__restrict__ q0, T *__restrict__ q1, int bx,
                                            int by, int bz) {
-  __shared__ TileLds<T, TJ> L;
-  const int tk = threadIdx.x;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one wave per row
-  const int k0 = bx * 64, j0 = by * TJ;
+  using S = SlotLayout<T, TJ, LOSSY>;
+  constexpr int D = S::kDepth;
+  __shared__ FusedLds<T, TJ, LOSSY> L;
+  TileArgs<T> a;
+  a.g = g;
+  a.dtm_dl = dtm_dl;
+  a.dtm = dtm;
+  a.inv_dl = inv_dl;
+  a.s0 = s0;
+  a.s1 = s1;
+  a.coef = coef;
+  a.prof = prof;
+  a.q0 = q0;
+  a.q1 = q1;
+  a.tk = threadIdx.x;
+  a.w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  a.k0 = bx * 64;
+  a.j0 = by * TJ;
+  const int tk = a.tk, w = a.w, k0 = a.k0, j0 = a.j0;
   const int ib = bz * planes;
-  const int ie = min(ib + planes, g.nx);
+  a.ie = min(ib + planes, g.nx);
+  a.last = a.ie;  // E(i+1) of the last plane (clamped to the grid when ie == nx)
   const int n = g.npml;
-  // does a 64-lane k range starting at kb touch the z-stripes?
-  auto zwave_of = [&](int kb, int kl) { return n > 0 && (kb < n || kl >= g.nz - n); };
+  const bool zt = n > 0 && (k0 < n || min(k0 + 63, g.nz - 1) >= g.nz - n);  // tile meets z-stripes
+  a.m.st = rsrc_of(s0);
+  a.m.cf = rsrc_of(coef);
+  a.m.ps = rsrc_of(q0);
+  a.m.wst = rsrc_of(s1);
+  a.m.wps = rsrc_of(q1);
+  a.m.ok = lane_off<T, kRowK>(g, tk, k0, j0);
+  a.m.oz = lane_off<T, kRowZ>(g, tk, k0, j0);
+  a.m.oc[0] = col_off<T>(g, tk, k0, j0, 0);
+  a.m.oc[1] = col_off<T>(g, tk, k0, j0, 1);
+
+  Ring<D> p;
+  p.issued = 0;
+  p.si = 0;
+
 
   if (w == TJ) {
-    // ── halo wave: H on the j0-1 row (lane = k - k0) and on the k0-1 column
-    // (lane = j - j0, lanes < TJ), plus the tile's forward neighbours. ──
-    const bool row_on = j0 > 0 && k0 + tk < g.nz;
-    const bool col_on = tk < TJ && k0 > 0 && j0 + tk < g.ny;
-    const bool zrow = zwave_of(k0, min(k0 + 63, g.nz - 1));
-    const bool zcol = n > 0 && (k0 - 1 < n || k0 - 1 >= g.nz - n);
-    const int kr = min(k0 + tk, g.nz - 1), jr = max(j0 - 1, 0);
-    const int jc = min(j0 + tk, g.ny - 1), kcol = max(k0 - 1, 0);
-    const Prof1<T> rp[2] = {prof1(g, prof, 1, 1, jr), prof1(g, prof, 1, 2, kr)};
-    const Prof1<T> cp[2] = {prof1(g, prof, 1, 1, jc), prof1(g, prof, 1, 2, kcol)};
-    HaloPlane<T> a;
-    {
-      halo_load<T, 0>(a.r, g, s0, q0, ib, jr, kr, zrow);
-      halo_load<T, 1>(a.c, g, s0, q0, ib, jc, kcol, zcol);
-      const long long vs = g.vs;
-      const int qf = (ib * g.ny + min(j0 + TJ, g.ny - 1)) * g.nz + kr;
-      a.fjx = s0[qf];
-      a.fjz = s0[2 * vs + qf];
-      const int qc = (ib * g.ny + jc) * g.nz + min(k0 + 64, g.nz - 1);
-      a.fkx = s0[qc];
-      a.fky = s0[vs + qc];
-    }
-    // planes ib .. ie-2 prefetch their successor, plane ie-1 does not
-    int i = ib;
-    if ((ie - 1 - ib) & 1) {  // odd count of prefetching planes: peel one
-      a = halo_plane<T, TJ, true>(g, s0, prof, q0, L, dtm_dl, dtm, i, j0, k0, tk, row_on, col_on,
-                                  zrow, zcol, rp, cp, a);
-      ++i;
-    }
-    for (; i < ie - 1; i += 2) {
-      const HaloPlane<T> b = halo_plane<T, TJ, true>(g, s0, prof, q0, L, dtm_dl, dtm, i, j0, k0,
-                                                     tk, row_on, col_on, zrow, zcol, rp, cp, a);
-      a = halo_plane<T, TJ, true>(g, s0, prof, q0, L, dtm_dl, dtm, i + 1, j0, k0, tk, row_on,
-                                  col_on, zrow, zcol, rp, cp, b);
-    }
-    halo_plane<T, TJ, false>(g, s0, prof, q0, L, dtm_dl, dtm, i, j0, k0, tk, row_on, col_on,
-                             zrow, zcol, rp, cp, a);
+    // ── halo wave ──
+    HaloWave<T> h;
+    h.jr = max(j0 - 1, 0);
+    h.kr = min(k0 + tk, g.nz - 1);
+    h.kcol = max(k0 - 1, 0);
+    h.row_on = j0 > 0 && k0 + tk < g.nz;
+    h.col_on = tk < TJ && k0 > 0 && j0 + tk < g.ny;
+    h.yrow = n > 0 && (h.jr < n || h.jr >= g.ny - n);
+    h.zrow = zt;
+    h.ycol = n > 0 && (j0 < n || min(j0 + TJ - 1, g.ny - 1) >= g.ny - n);
+    h.zcol = n > 0 && (h.kcol < n || h.kcol >= g.nz - n);
+    h.rp[0] = prof1(g, prof, 1, 1, h.jr);
+    h.rp[1] = prof1(g, prof, 1, 2, h.kr);
+    h.cp[0] = prof1(g, prof, 1, 1, min(j0 + tk, g.ny - 1));
+    h.cp[1] = prof1(g, prof, 1, 2, h.kcol);
+    a.m.ojn = lane_off<T, kColJ>(g, tk, k0, j0);
+    a.m.ojy = lane_off<T, kColY>(g, tk, k0, j0);
+    a.m.oj2 = lane_off<T, kColJ2>(g, tk, k0, j0);
+    p.issued += issue_psi_halo(g, a.m, L.psi[w], ib, h);
+    prologue_slots<T, TJ, LOSSY, true>(a, L, p, ib);
+    for (int i = ib; i < a.ie; ++i) halo_plane<T, TJ, LOSSY>(a, L, p, h, i);
     return;
   }
 
   // ── row waves ──
   const long long vs = g.vs;
-  const int sxy = g.ny * g.nz;
   const int k = k0 + tk, j = j0 + w;
-  const bool act = k < g.nz && j < g.ny;
-  const int kc = min(k, g.nz - 1), jc = min(j, g.ny - 1);
-  const bool zwave = zwave_of(k0, min(k0 + 63, g.nz - 1));
-  const Prof1<T> pyz[4] = {prof1(g, prof, 1, 1, jc), prof1(g, prof, 1, 2, kc),
-                           prof1(g, prof, 0, 1, jc), prof1(g, prof, 0, 2, kc)};
+  RowWave<T> r;
+  r.act = k < g.nz && j < g.ny;
+  r.kc = min(k, g.nz - 1);
+  r.jc = min(j, g.ny - 1);
+  r.own = (unsigned)(r.jc * g.nz + r.kc) * (unsigned)sizeof(T);
+  r.junk = (unsigned)(g.nx * g.ny * g.nz + tk) * (unsigned)sizeof(T);
+  r.ywave = n > 0 && (r.jc < n || r.jc >= g.ny - n);
+  r.zwave = zt;
+  r.zin = n > 0 && (r.kc < n || r.kc >= g.nz - n);
+  r.pyz[0] = prof1(g, prof, 1, 1, r.jc);
+  r.pyz[1] = prof1(g, prof, 1, 2, r.kc);
+  r.pyz[2] = prof1(g, prof, 0, 1, r.jc);
+  r.pyz[3] = prof1(g, prof, 0, 2, r.kc);
+  r.hyp = r.hzp = T(0);
 
-  RowState<T> st;
-  {
-    const int idx = (ib * g.ny + jc) * g.nz + kc;
-    st.ex = s0[idx];
-    st.ey = s0[vs + idx];
-    st.ez = s0[2 * vs + idx];
-    st.hyp = st.hzp = T(0);
-    if (ib > 0 && act) {  // H(ib-1) of the own cell, recomputed (not written)
-      const int q = idx - sxy;
-      const bool jn = j + 1 < g.ny, kn = k + 1 < g.nz;
-      const StripeIdx s = stripe_idx(g, ib - 1, j, k);
-      T p[6] = {0, 0, 0, 0, 0, 0};
-      if (!is_interior(g, ib - 1, j, k)) psi_load(g, q0, 0, s, p);
-      T hx = s0[3 * vs + q], hy = s0[4 * vs + q], hz = s0[5 * vs + q];
-      h_math(g, ib - 1, j, k, s, prof1(g, prof, 1, 0, ib - 1), pyz[0], pyz[1], s0[q],
-             s0[vs + q], s0[2 * vs + q], jn ? s0[q + g.nz] : T(0), jn ? s0[2 * vs + q + g.nz] : T(0),
-             kn ? s0[q + 1] : T(0), kn ? s0[vs + q + 1] : T(0), st.ey, st.ez, hx, hy, hz, p,
-             dtm_dl, dtm);
-      st.hyp = hy;
-      st.hzp = hz;
-    }
+  p.issued += issue_psi_row(g, a.m, L.psi[w], ib, r);
+  prologue_slots<T, TJ, LOSSY, false>(a, L, p, ib);
+  if (ib > 0 && r.act) {
+    // H(ib-1) of the own cell, recomputed (not written) for E(ib)
+    const int sxy = g.ny * g.nz;
+    const int q = ((ib - 1) * g.ny + r.jc) * g.nz + r.kc;
+    const bool jn = r.jc + 1 < g.ny, kn = r.kc + 1 < g.nz;
+    const StripeIdx s = stripe_idx(g, ib - 1, r.jc, r.kc);
+    T pm[6] = {0, 0, 0, 0, 0, 0};
+    if (!is_interior(g, ib - 1, r.jc, r.kc)) psi_load(g, q0, 0, s, pm);
+    T hx = T(0), hy = s0[4 * vs + q], hz = s0[5 * vs + q];
+    h_math(g, ib - 1, r.jc, r.kc, s, prof1(g, prof, 1, 0, ib - 1), r.pyz[0], r.pyz[1], s0[q],
+           s0[vs + q], s0[2 * vs + q], jn ? s0[q + g.nz] : T(0),
+           jn ? s0[2 * vs + q + g.nz] : T(0), kn ? s0[q + 1] : T(0), kn ? s0[vs + q + 1] : T(0),
+           s0[vs + q + sxy], s0[2 * vs + q + sxy], hx, hy, hz, pm, dtm_dl, dtm);
+    r.hyp = hy;
+    r.hzp = hz;
   }
-  // Plane data flows by value through an unrolled pair of planes, so each
-  // prefetch lands in registers that are not touched again until the next
-  // plane uses them (no register copies that would wait on the loads).
-  RowPlane<T> a;
-  row_load<T, LOSSY>(a, g, s0, coef, q0, ib, jc, kc, zwave);
-  int i = ib;
-  if ((ie - 1 - ib) & 1) {  // odd count of prefetching planes: peel one
-    a = row_plane<T, TJ, LOSSY, true>(g, dtm_dl, dtm, inv_dl, s0, s1, coef, prof, q0, q1, L, st,
-                                      a, i, j, k, jc, kc, w, tk, act, zwave, pyz);
-    ++i;
-  }
-  for (; i < ie - 1; i += 2) {
-    const RowPlane<T> b = row_plane<T, TJ, LOSSY, true>(g, dtm_dl, dtm, inv_dl, s0, s1, coef,
-                                                        prof, q0, q1, L, st, a, i, j, k, jc, kc,
-                                                        w, tk, act, zwave, pyz);
-    a = row_plane<T, TJ, LOSSY, true>(g, dtm_dl, dtm, inv_dl, s0, s1, coef, prof, q0, q1, L, st,
-                                      b, i + 1, j, k, jc, kc, w, tk, act, zwave, pyz);
-  }
-  row_plane<T, TJ, LOSSY, false>(g, dtm_dl, dtm, inv_dl, s0, s1, coef, prof, q0, q1, L, st, a, i,
-                                 j, k, jc, kc, w, tk, act, zwave, pyz);
+  for (int i = ib; i < a.ie; ++i) row_plane<T, TJ, LOSSY>(a, L, p, r, i);
 }
 
 // Source fix-up for one cell after a fused sweep.  The sweep advanced E as if

@@ -250,14 +250,26 @@ static int fused_t(Engine *e, const hfdtd_source *src, int n) {
     if (!make_src<T>(e, src[q], s)) return fail(HFDTD_EINVAL, "invalid source descriptor");
     if (s.ci1 > s.ci0 && s.cj1 > s.cj0 && s.ck1 > s.ck0) sl.s[sl.n++] = s;
   }
+  if (!e->fused_fits())
+    return fail(HFDTD_EUNSUPPORTED, "the fused sweep needs field and psi slabs below 4 GB; "
+                                    "use the split sweeps for this grid");
   int rc = ensure_alt(e);
   if (rc) return rc;
-  switch (e->tile_rows) {
-    case 1: rc = fused_tj<T, 1>(e, sl); break;
-    case 2: rc = fused_tj<T, 2>(e, sl); break;
-    case 8: rc = fused_tj<T, 8>(e, sl); break;
-    case 15: rc = fused_tj<T, 15>(e, sl); break;
-    default: rc = fused_tj<T, 4>(e, sl); break;
+  // FP64 slots are twice as large: tiles of at most 4 rows fit the LDS
+  if constexpr (sizeof(T) == 8) {
+    switch (e->tile_rows) {
+      case 1: rc = fused_tj<T, 1>(e, sl); break;
+      case 2: rc = fused_tj<T, 2>(e, sl); break;
+      default: rc = fused_tj<T, 4>(e, sl); break;
+    }
+  } else {
+    switch (e->tile_rows) {
+      case 1: rc = fused_tj<T, 1>(e, sl); break;
+      case 2: rc = fused_tj<T, 2>(e, sl); break;
+      case 6: rc = fused_tj<T, 6>(e, sl); break;
+      case 8: rc = fused_tj<T, 8>(e, sl); break;
+      default: rc = fused_tj<T, 4>(e, sl); break;
+    }
   }
   if (rc) return rc;
   e->cur ^= 1;

@@ -11,10 +11,12 @@ and E as one fused sweep when it can: during the source phase,
 ``add_source_Jx`` / ``add_source_Ex`` / ``add_source_J`` calls are recorded
 instead of launched; the fused sweep advances H and E everywhere, then a
 small fix-up kernel recomputes E on the source cells with the recorded
-sources added first, which is the reference's order cell by cell.  A callback that touches field data (``Ex``, ``read_point``, a field
-setter ...) first forces the H half to run on its own, so it sees exactly the
-state the reference would show it; the step then finishes with the split E
-sweep.  ``fused=False`` always uses the split sweeps.
+sources added first, which is the reference's order cell by cell.  A
+callback that touches field data (``Ex``, ``read_point``, a field setter ...)
+first forces the H half to run on its own, so it sees exactly the state the
+reference would show it; the step then finishes with the split E sweep.
+``fused=False`` always uses the split sweeps, and so do grids whose field
+slab (six volumes) reaches 4 GB, beyond the fused sweep's 32-bit offsets.
 """
 
 from __future__ import annotations
@@ -78,7 +80,11 @@ class HIPFDTD(SourceMonitorMixin):
         self.psi_y_size = (self.Nx * 2 * n * self.Nz) if n > 0 else 0
         self.psi_z_size = (self.Nx * self.Ny * 2 * n) if n > 0 else 0
         self.device = int(device)
-        self.fused = bool(fused)
+        # the fused sweep addresses its slabs with 32-bit byte offsets
+        item = np.dtype(self.dtype).itemsize
+        vstride = (self.Nx * self.Ny * self.Nz + 64 + 63) // 64 * 64
+        psi_total = 4 * (self.psi_x_size + self.psi_y_size + self.psi_z_size) + 64
+        self.fused = bool(fused) and max(6 * vstride, psi_total) * item < 2**32
         self._sources = []
         self._monitors = []
         self._handle = None
