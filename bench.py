@@ -8,15 +8,22 @@ at f0 = c0/(20 dl), 50 % bandwidth; a Huygens box [32, 223]^3 recording the
 DFT at 8 frequencies; after the run, the near-to-far transform at
 181 θ x 2 φ angles, R = 1 m.  Inputs are synthetic (generated here).
 
-A "step" is one full Yee time step: the fused H + source + E sweep and the
-8-frequency face-DFT accumulation.  ``value`` is Mcell-updates/s over the K
+A "step" is one full Yee time step: the fused H + E sweep, the source fix-up
+on the sheet cells and the 8-frequency face-DFT accumulation.  ``value`` is Mcell-updates/s over the K
 timed steps.  ``roofline`` is the fused sweep's algorithmic HBM traffic
 (72 B/cell + 8 B per psi entry) divided by its average launch time, taken with
 HIP events on the engine's stream during the timed steps.  ``cpu_baseline``
 times this repository's NumPy restatement of the reference's face-CPML engine
 (one core) on a bounded sample of steps of the same grid.
 
+``roofline.traffic`` is the measured HBM traffic of one k_fused launch,
+taken from ``profiles/k_fused_256cube_hbm_traffic.txt`` (two rocprofv3 PMC
+passes, FETCH_SIZE and WRITE_SIZE, each scaled by what the same counter
+reads for a plain device copy of known size; ``--copy-calibration`` runs that
+copy).
+
 Usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 20] [--no-cpu]
+       python bench.py --copy-calibration
 """
 
 from __future__ import annotations
@@ -135,6 +142,33 @@ def run_cpu_baseline(wl: Workload, steps: int):
     return cells * steps / dt / 1e6, dt
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt"
+
+
+def measured_traffic():
+    """HBM bytes per k_fused launch recorded in the profile summary, or None."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(n_bytes=1 << 30, reps=5):
+    """A plain device-to-device copy of n_bytes, reps times (the reference
+    kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+
+    src = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n_bytes, "copy_bytes_written": n_bytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,7 +176,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path "
               "(x-slab halo exchange) is design-only in this build.", file=sys.stderr)  # fmt: skip
@@ -182,7 +220,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "bytes_alg_per_launch": b_alg,
             "avg_launch_us": round(avg_launch_s * 1e6, 2),
         },
