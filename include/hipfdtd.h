@@ -55,6 +55,10 @@ extern "C" {
 #define HFDTD_ACC_FP64 1  /* accumulate in double (accum_fp64=True)            */
 #define HFDTD_ACC_KAHAN 2 /* FP32 with compensated (Kahan) summation           */
 
+/* plane-DFT modes */
+#define HFDTD_DFT_DIRECT 0   /* per-step accumulation (FP32, Kahan or FP64)          */
+#define HFDTD_DFT_BUFFERED 1 /* K steps buffered, one FP32 matrix-core contraction   */
+
 /* step-path selection for hfdtd_step */
 #define HFDTD_PATH_SPLIT 0 /* update_H, sources, update_E: two sweeps        */
 #define HFDTD_PATH_FUSED 1 /* one fused H+sources+E sweep (ping-pong state)  */
@@ -63,6 +67,7 @@ typedef struct hfdtd_engine hfdtd_engine;
 typedef struct hfdtd_n2f hfdtd_n2f;
 typedef struct hfdtd_probe hfdtd_probe;
 typedef struct hfdtd_batch hfdtd_batch;
+typedef struct hfdtd_dft hfdtd_dft;
 
 /* One current / soft source on a box [i0,i1)x[j0,j1)x[k0,k1) of component
  * comp (0=x,1=y,2=z).  Bounds may extend past the grid (clipped); rim-taper
@@ -192,6 +197,29 @@ int hfdtd_probe_destroy(hfdtd_probe *p);
 int hfdtd_probe_accumulate(hfdtd_probe *p, const double *phase);
 /* out: nfreq x {re, im} */
 int hfdtd_probe_read(hfdtd_probe *p, double *out);
+
+/* ── multi-frequency plane DFT ────────────────────────────────────────────
+ * Replaces the host-side plane DFT of the reference's test harness
+ * (tests/harness cases.py:179-197: acc += Ex(t_{n+1}) e^{jw t_{n+1}} dt over a
+ * z-plane) with a device monitor over any axis-aligned plane and any subset
+ * of Ex..Hz.  Samples p = c*(n0*n1) + a*n1 + b, c over the selected
+ * components in Ex..Hz order; plane axes (a, b) = (j, k) for x-planes,
+ * (i, k) for y-planes, (i, j) for z-planes. */
+/* comp_mask: bit c selects component c (HFDTD_EX..HFDTD_HZ); 1..16
+ * frequencies; mode HFDTD_DFT_DIRECT or HFDTD_DFT_BUFFERED (K even, 2..256,
+ * FP32 engines); accum HFDTD_ACC_REAL / _FP64 / _KAHAN. */
+int hfdtd_dft_plane_create(hfdtd_engine *e, int axis, int index, int comp_mask, int nfreq, int K,
+                           int mode, int accum, hfdtd_dft **out);
+int hfdtd_dft_destroy(hfdtd_dft *d);
+/* one step: phase = nfreq x {cos, sin}(w t) * dt, applied to the current fields */
+int hfdtd_dft_accumulate(hfdtd_dft *d, const double *phase);
+/* contract a partially filled block now (read does this itself) */
+int hfdtd_dft_flush(hfdtd_dft *d);
+/* zero the sums */
+int hfdtd_dft_reset(hfdtd_dft *d);
+int hfdtd_dft_shape(const hfdtd_dft *d, int *ncomp, int *n0, int *n1, int *nfreq);
+/* sums as double: [ncomp][n0][n1][nfreq] x {re, im} */
+int hfdtd_dft_read(hfdtd_dft *d, double *out);
 
 /* ── batched sweep of independent simulations ─────────────────────────── */
 

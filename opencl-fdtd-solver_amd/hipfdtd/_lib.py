@@ -32,6 +32,8 @@ ACC_FP64 = 1
 ACC_KAHAN = 2
 PATH_SPLIT = 0
 PATH_FUSED = 1
+DFT_DIRECT = 0
+DFT_BUFFERED = 1
 
 
 class Source(C.Structure):
@@ -107,6 +109,21 @@ EXPORTS = {
     "hfdtd_probe_destroy": (C.c_int, [C.c_void_p]),
     "hfdtd_probe_accumulate": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "hfdtd_probe_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "hfdtd_dft_plane_create": (
+        C.c_int,
+        [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+         C.POINTER(C.c_void_p)],
+    ),  # fmt: skip
+    "hfdtd_dft_destroy": (C.c_int, [C.c_void_p]),
+    "hfdtd_dft_accumulate": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "hfdtd_dft_flush": (C.c_int, [C.c_void_p]),
+    "hfdtd_dft_reset": (C.c_int, [C.c_void_p]),
+    "hfdtd_dft_shape": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+         C.POINTER(C.c_int)],
+    ),  # fmt: skip
+    "hfdtd_dft_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "hfdtd_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     "hfdtd_batch_destroy": (C.c_int, [C.c_void_p]),
     "hfdtd_batch_set_sources": (
