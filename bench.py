@@ -22,8 +22,17 @@ passes, FETCH_SIZE and WRITE_SIZE, each scaled by what the same counter
 reads for a plain device copy of known size; ``--copy-calibration`` runs that
 copy).
 
+``--workload plane-dft`` (not the default line) measures the 512^3 plane-DFT
+configuration: smooth Gaussian random εr (seed 3, σ = 8 cells, clipped to
+[1, 6]), three orthogonal point dipoles with staggered pulses, npml 16, and a
+16-frequency DFT of Ex, Ey, Ez on the three mid-planes.  It reports the
+per-step cost of each DFT mode (K = 32 buffered matrix-core contraction,
+direct FP32, direct Kahan, direct FP64) and, after the full run, each mode's
+error against the FP64 sums.
+
 Usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 20] [--no-cpu]
        python bench.py --copy-calibration
+       python bench.py --workload plane-dft [--steps 1500]
 """
 
 from __future__ import annotations
@@ -142,6 +151,96 @@ def run_cpu_baseline(wl: Workload, steps: int):
     return cells * steps / dt / 1e6, dt
 
 
+def c3_epsilon(shape, seed=3, sigma=8.0):
+    """Smooth Gaussian random field: white noise low-passed by a Gaussian of
+    `sigma` cells (FFT), standardised; εr = clip(3.5 + 1.25 g, 1, 6)."""
+    import scipy.fft as sf
+
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal(shape, dtype=np.float32)
+    spec = sf.rfftn(g, workers=-1)
+    kx = sf.fftfreq(shape[0]).astype(np.float32)[:, None, None]
+    ky = sf.fftfreq(shape[1]).astype(np.float32)[None, :, None]
+    kz = sf.rfftfreq(shape[2]).astype(np.float32)[None, None, :]
+    c = np.float32(-2.0 * (math.pi * sigma) ** 2)
+    spec *= np.exp(c * (kx * kx + ky * ky + kz * kz))
+    g = sf.irfftn(spec, s=shape, workers=-1).astype(np.float32)
+    g = (g - g.mean()) / g.std()
+    return np.clip(3.5 + 1.25 * g, 1.0, 6.0).astype(np.float32)
+
+
+def run_plane_dft(args):
+    """512^3 16-frequency mid-plane DFT: cost per mode and error vs FP64."""
+    import hipfdtd
+    import hipfdtd.dft
+
+    shape, npml, dl = (512, 512, 512), 16, 1e-3
+    f0 = C0 / (20 * dl)
+    fw = 0.5 * f0
+    freqs = np.linspace(f0 - fw / 2, f0 + fw / 2, 16)
+    eps = c3_epsilon(shape)
+    rng = np.random.default_rng(3)
+    pos = [tuple(int(rng.integers(npml + 16, s - npml - 16)) for s in shape) for _ in range(3)]
+
+    def make():
+        sim = hipfdtd.HIPFDTD(shape, dl, npml=npml)
+        sim.set_epsilon(eps)
+        for comp, p, lag in zip("xyz", pos, (0, 20, 40)):
+            box = (p[0], p[0] + 1, p[1], p[1] + 1, p[2], p[2] + 1)
+            sim.add_source(lambda f, comp=comp, box=box, lag=lag:
+                           f.add_source_J(comp, gaussian_sine(f.t - lag * f.dt, f0, fw), box))  # fmt: skip
+        return sim
+
+    modes = {"buffered_k32": dict(mode="buffered", K=32), "direct_fp32": dict(mode="direct"),
+             "direct_kahan": dict(mode="direct", accum="kahan"),
+             "direct_fp64": dict(mode="direct", accum="fp64")}  # fmt: skip
+
+    def attach(sim, kw):
+        return [hipfdtd.dft.HIPPlaneDFTMonitor(sim, ax, 256, ("Ex", "Ey", "Ez"), freqs, **kw)
+                for ax in "xyz"]  # fmt: skip
+
+    def timed(sim, n):
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.run(n)
+        sim.synchronize()
+        return (time.perf_counter() - t0) / n * 1e6
+
+    sim = make()
+    sim.run(32)
+    base = timed(sim, 96)
+    cost = {}
+    for name, kw in modes.items():
+        mons = attach(sim, kw)
+        sim.run(32)
+        cost[name] = round(timed(sim, 96) - base, 1)
+        for m in mons:
+            sim._monitors.remove(m)
+            m._release()
+    sim.close()
+    print(f"[plane-dft] per-step cost {cost}, sweep {base:.1f} us", file=sys.stderr, flush=True)
+
+    sim = make()
+    allm = {name: attach(sim, kw) for name, kw in modes.items()}
+    sim.run(args.steps)
+    ref = [m.dft() for m in allm["direct_fp64"]]
+    err = {}
+    for name in ("buffered_k32", "direct_fp32", "direct_kahan"):
+        err[name] = max(float(np.max(np.abs(m.dft() - r)) / np.max(np.abs(r)))
+                        for m, r in zip(allm[name], ref))  # fmt: skip
+    print(json.dumps({
+        "metric": "plane_dft_us_per_step", "unit": "us", "higher_is_better": False,
+        "steps": args.steps, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "512^3 FP32 npml 16, smooth random eps in [1,6], 3 dipoles; "
+                               "16-frequency DFT of Ex,Ey,Ez on the 3 mid-planes",
+                   "samples_per_step": 3 * 3 * 512 * 512},
+        "sweep_us_per_step": round(base, 1),
+        "dft_us_per_step": cost,
+        "max_rel_err_vs_fp64": err,
+        "device": sim.device_name,
+    }))  # fmt: skip
+
+
 TRAFFIC_FILE = ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt"
 
 
@@ -177,9 +276,15 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--workload", default="yee-256", choices=["yee-256", "plane-dft"])
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "plane-dft":
+        if args.steps == 2000:
+            args.steps = 1500
+        run_plane_dft(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path "
