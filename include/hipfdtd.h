@@ -184,6 +184,14 @@ int hfdtd_n2f_read(hfdtd_n2f *m, int comp, int freq, double *host);
 int hfdtd_n2f_farfield(hfdtd_n2f *m, int freq, const double *obs, int n_obs, double k_wave,
                        double eta0, int l_sign, int geom_fp64, double *eh);
 
+/* K-step buffering of the face DFT (extension; per-step accumulation is the
+ * reference's monitors.py:571-624).  K = 0: per-step sums; K even, 2..256:
+ * the co-located samples of K steps are kept on the device and folded into
+ * the sums by one FP32 matrix-core contraction per K steps (float32 engines,
+ * at most 16 frequencies).  Readers (snapshot, rel change, read, farfield)
+ * fold a partial block in first. */
+int hfdtd_n2f_set_buffering(hfdtd_n2f *m, int K);
+
 /* ── point probe (device-side DFT of one field sample) ──────────────────── */
 
 /* Replaces the reference harness's per-step read_point + host DFT

@@ -27,6 +27,12 @@ struct hfdtd_n2f {
   void *snap = nullptr;          // snapshot of acc
   double *scratch = nullptr;     // reduction partials
   size_t scratch_len = 0;
+  // K-step buffering (0 = per-step accumulation): co-located tangential
+  // samples of K steps, rows [E: 2 per face sample | H: 2 per face sample]
+  int K = 0, kfill = 0;
+  long long pe = 0, ppad = 0;    // start of the H rows, padded row count
+  float *buf = nullptr;          // [K][ppad]
+  float *we = nullptr, *wh = nullptr;  // [K][32] phase rows (E, H)
 };
 
 namespace hfdtd {
@@ -169,6 +175,98 @@ __global__ __launch_bounds__(256) void k_n2f_accumulate(Dims3 g, Box b, int nfre
       const bool is_e = c[m] < 3;
       const size_t o = (((size_t)c[m] * nfreq + fr) * nfc + s) * 2;
       dft_add<A, KAHAN>(acc, cmp, o, A(v[m]), ph.v[q][is_e ? 0 : 2], ph.v[q][is_e ? 1 : 3]);
+    }
+  }
+}
+
+// ── K-step buffering ───────────────────────────────────────────────────────
+// Over K steps the face sums are the contraction
+//   acc[c][f][s] += sum_k v_k(c, s) * phase_k(f)   (E phases or H phases),
+// done once per K steps on the matrix cores (as in dft.hip): k_n2f_gather
+// writes the co-located tangential samples of one step into row k of the
+// buffer, k_n2f_contract folds the block into the sums.
+
+struct PhaseRowEH {
+  float e[32], h[32];  // cos, sin pairs (times dt); zero beyond 2 nfreq
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_n2f_gather(Dims3 g, Box b, const T *__restrict__ Ex,
+                                                    const T *__restrict__ Ey,
+                                                    const T *__restrict__ Ez,
+                                                    const T *__restrict__ Hx,
+                                                    const T *__restrict__ Hy,
+                                                    const T *__restrict__ Hz,
+                                                    float *__restrict__ row, long long pe,
+                                                    PhaseRowEH ph, float *__restrict__ we,
+                                                    float *__restrict__ wh) {
+  if (blockIdx.x == 0 && threadIdx.x < 32) {
+    we[threadIdx.x] = ph.e[threadIdx.x];
+    wh[threadIdx.x] = ph.h[threadIdx.x];
+  }
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= b.n_face) return;
+  int i, j, k;
+  const int f = decode_face(b, s, i, j, k);
+  T v[4];
+  int c[4];
+  colocate_impl<T>(g, f, i, j, k, Ex, Ey, Ez, Hx, Hy, Hz, v, c);
+  row[2 * s] = (float)v[0];
+  row[2 * s + 1] = (float)v[1];
+  row[pe + 2 * s] = (float)v[2];
+  row[pe + 2 * s + 1] = (float)v[3];
+}
+
+typedef float n2f_f32x16 __attribute__((ext_vector_type(16)));
+
+// One wavefront per 32 buffer rows (all E rows or all H rows: pe is a
+// multiple of 32); v_mfma_f32_32x32x2f32 as in k_dft_contract, output
+// D[i][j] in register r of lane l with i = (r/4)*8 + (l/32)*4 + r%4,
+// j = l%32 = 2 f + {re, im}.  W rows at or beyond kvalid count as zero.
+template <typename A, bool KAHAN>
+__global__ __launch_bounds__(256) void k_n2f_contract(Box b, const float *__restrict__ buf,
+                                                      long long ppad, long long pe,
+                                                      const float *__restrict__ we,
+                                                      const float *__restrict__ wh, int kvalid,
+                                                      int nfreq, A *__restrict__ acc,
+                                                      A *__restrict__ cmp) {
+  const int lane = threadIdx.x & 63;
+  const long long p0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+  if (p0 >= ppad) return;  // wave-uniform
+  const bool hrow = p0 >= pe;
+  const float *W = hrow ? wh : we;
+  const int half = lane >> 5, col = lane & 31;
+  n2f_f32x16 d = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int pairs = (kvalid + 1) / 2;
+  for (int t = 0; t < pairs; ++t) {
+    const int k = 2 * t + half;
+    const float a = buf[(long long)k * ppad + p0 + col];
+    const float w = k < kvalid ? W[k * 32 + col] : 0.0f;
+    d = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w, d, 0, 0, 0);
+  }
+  if (col >= 2 * nfreq) return;
+  const int fr = col >> 1, q = col & 1;
+  const size_t nfc = (size_t)b.n_face;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const long long local = p0 + (r / 4) * 8 + half * 4 + (r % 4) - (hrow ? pe : 0);
+    if (local >= 2 * (long long)b.n_face) continue;
+    const int s = (int)(local >> 1), tg = (int)(local & 1);
+    int i, j, k;
+    const int f = decode_face(b, s, i, j, k);
+    // tangential components of the face: E (t=0,1) then H (t=0,1)
+    const int ce = f <= 1 ? 1 + tg : (f <= 3 ? 2 * tg : tg);
+    const int comp = hrow ? ce + 3 : ce;
+    const size_t o = (((size_t)comp * nfreq + fr) * nfc + s) * 2 + q;
+    const A term = (A)d[r];
+    if (!KAHAN) {
+      acc[o] += term;
+    } else {
+      const A sv = acc[o];
+      const A y = term - cmp[o];
+      const A tt = sv + y;
+      cmp[o] = (tt - sv) - y;
+      acc[o] = tt;
     }
   }
 }
@@ -417,6 +515,53 @@ static int accumulate_t(hfdtd_n2f *m, const double *phase) {
   return HFDTD_OK;
 }
 
+template <typename A, bool KAHAN>
+static int contract_t(hfdtd_n2f *m) {
+  const long long waves = m->ppad / 32;
+  hipLaunchKernelGGL((k_n2f_contract<A, KAHAN>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0,
+                     m->e->stream, box_of(m), m->buf, m->ppad, m->pe, m->we, m->wh, m->kfill,
+                     m->nfreq, (A *)m->acc, (A *)m->cmp);
+  HF_HIP(hipGetLastError());
+  m->kfill = 0;
+  return HFDTD_OK;
+}
+
+// Fold a partially filled block into the sums (before anything reads them).
+static int n2f_flush(hfdtd_n2f *m) {
+  if (m->K == 0 || m->kfill == 0) return HFDTD_OK;
+  HF_HIP(hipSetDevice(m->e->device));
+  if (m->acc_mode == HFDTD_ACC_FP64) return contract_t<double, false>(m);
+  if (m->acc_mode == HFDTD_ACC_KAHAN) return contract_t<float, true>(m);
+  return contract_t<float, false>(m);
+}
+
+static int gather_step(hfdtd_n2f *m, const double *phase) {
+  Engine *e = m->e;
+  PhaseRowEH ph;
+  for (int q = 0; q < 32; ++q) {
+    const int fr = q >> 1, c = q & 1;
+    ph.e[q] = fr < m->nfreq ? (float)phase[fr * 4 + c] : 0.0f;
+    ph.h[q] = fr < m->nfreq ? (float)phase[fr * 4 + 2 + c] : 0.0f;
+  }
+  const Dims3 g{e->nx, e->ny, e->nz};
+  hipLaunchKernelGGL(k_n2f_gather<float>, dim3(ceil_div(m->n_face, 256)), dim3(256), 0,
+                     e->stream, g, box_of(m), (const float *)e->field(0),
+                     (const float *)e->field(1), (const float *)e->field(2),
+                     (const float *)e->field(3), (const float *)e->field(4),
+                     (const float *)e->field(5), m->buf + (size_t)m->kfill * m->ppad, m->pe, ph,
+                     m->we + (size_t)m->kfill * 32, m->wh + (size_t)m->kfill * 32);
+  HF_HIP(hipGetLastError());
+  if (++m->kfill == m->K) return n2f_flush(m);
+  return HFDTD_OK;
+}
+
+static void free_buffering(hfdtd_n2f *m) {
+  for (float *p : {m->buf, m->we, m->wh})
+    if (p) (void)hipFree(p);
+  m->buf = m->we = m->wh = nullptr;
+  m->K = m->kfill = 0;
+}
+
 template <typename G, typename A>
 static int farfield_t(hfdtd_n2f *m, int fr, const double *obs, int n_obs, double k_wave,
                       double eta0, int l_sign, double *eh) {
@@ -504,6 +649,7 @@ int hfdtd_n2f_destroy(hfdtd_n2f *m) {
   if (m->cmp) (void)hipFree(m->cmp);
   if (m->snap) (void)hipFree(m->snap);
   if (m->scratch) (void)hipFree(m->scratch);
+  free_buffering(m);
   delete m;
   return HFDTD_OK;
 }
@@ -520,6 +666,7 @@ int hfdtd_n2f_accumulate(hfdtd_n2f *m, const double *phase) {
   if (!m || !phase) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = m->e;
   HF_HIP(hipSetDevice(e->device));
+  if (m->K > 0) return gather_step(m, phase);
   if (e->fp64) return accumulate_t<double, double, false>(m, phase);
   if (m->acc_mode == HFDTD_ACC_FP64) return accumulate_t<float, double, false>(m, phase);
   if (m->acc_mode == HFDTD_ACC_KAHAN) return accumulate_t<float, float, true>(m, phase);
@@ -528,6 +675,8 @@ int hfdtd_n2f_accumulate(hfdtd_n2f *m, const double *phase) {
 
 int hfdtd_n2f_snapshot(hfdtd_n2f *m) {
   if (!m) return fail(HFDTD_EINVAL, "NULL argument");
+  int rc = n2f_flush(m);
+  if (rc) return rc;
   Engine *e = m->e;
   HF_HIP(hipSetDevice(e->device));
   const size_t bytes = (size_t)6 * m->nfreq * m->n_face * 2 * m->acc_elem;
@@ -540,10 +689,12 @@ int hfdtd_n2f_rel_change(hfdtd_n2f *m, int freq, double *num, double *den) {
   if (!m || !num || !den) return fail(HFDTD_EINVAL, "NULL argument");
   if (freq < 0 || freq >= m->nfreq) return fail(HFDTD_EINVAL, "frequency index out of range");
   if (!m->snap) return fail(HFDTD_EINVAL, "no snapshot");
+  int rc = n2f_flush(m);
+  if (rc) return rc;
   Engine *e = m->e;
   HF_HIP(hipSetDevice(e->device));
   const int nblk = (int)ceil_div(m->n_face, 256);
-  int rc = ensure_scratch(m, (size_t)2 * nblk);
+  rc = ensure_scratch(m, (size_t)2 * nblk);
   if (rc) return rc;
   if (m->acc_elem == 8)
     hipLaunchKernelGGL((k_n2f_relchange<double>), dim3(nblk), dim3(256), 0, e->stream,
@@ -572,6 +723,8 @@ int hfdtd_n2f_read(hfdtd_n2f *m, int comp, int freq, double *host) {
   if (!m || !host) return fail(HFDTD_EINVAL, "NULL argument");
   if (comp < 0 || comp > 5 || freq < 0 || freq >= m->nfreq)
     return fail(HFDTD_EINVAL, "component or frequency out of range");
+  int rc = n2f_flush(m);
+  if (rc) return rc;
   Engine *e = m->e;
   HF_HIP(hipSetDevice(e->device));
   const size_t n2 = (size_t)m->n_face * 2;
@@ -596,6 +749,8 @@ int hfdtd_n2f_farfield(hfdtd_n2f *m, int freq, const double *obs, int n_obs, dou
   if (n_obs < 1) return fail(HFDTD_EINVAL, "n_obs must be >= 1");
   if (freq < 0 || freq >= m->nfreq) return fail(HFDTD_EINVAL, "frequency index out of range");
   if (l_sign != 1 && l_sign != -1) return fail(HFDTD_EINVAL, "l_sign must be +1 or -1");
+  int rc = n2f_flush(m);
+  if (rc) return rc;
   Engine *e = m->e;
   HF_HIP(hipSetDevice(e->device));
   const bool g64 = geom_fp64 || e->fp64;
@@ -604,6 +759,36 @@ int hfdtd_n2f_farfield(hfdtd_n2f *m, int freq, const double *obs, int n_obs, dou
                : farfield_t<float, double>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh);
   return g64 ? farfield_t<double, float>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh)
              : farfield_t<float, float>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh);
+}
+
+int hfdtd_n2f_set_buffering(hfdtd_n2f *m, int K) {
+  if (!m) return fail(HFDTD_EINVAL, "NULL argument");
+  if (K != 0 && (K < 2 || K > 256 || (K & 1))) return fail(HFDTD_EINVAL, "K must be 0 or even, 2..256");
+  if (K != 0 && m->e->fp64)
+    return fail(HFDTD_EUNSUPPORTED, "buffered face DFT needs a float32 engine");
+  if (K != 0 && m->nfreq > 16) return fail(HFDTD_EUNSUPPORTED, "buffered face DFT: at most 16 frequencies");
+  int rc = n2f_flush(m);
+  if (rc) return rc;
+  HF_HIP(hipSetDevice(m->e->device));
+  HF_HIP(hipStreamSynchronize(m->e->stream));
+  free_buffering(m);
+  if (K == 0) return HFDTD_OK;
+  const long long rows = 2LL * m->n_face;
+  m->pe = (rows + 31) / 32 * 32;
+  m->ppad = 2 * m->pe;
+  hipError_t err = hipMalloc((void **)&m->buf, (size_t)K * m->ppad * sizeof(float));
+  if (err == hipSuccess) err = hipMalloc((void **)&m->we, (size_t)K * 32 * sizeof(float));
+  if (err == hipSuccess) err = hipMalloc((void **)&m->wh, (size_t)K * 32 * sizeof(float));
+  if (err == hipSuccess) err = hipMemsetAsync(m->buf, 0, (size_t)K * m->ppad * sizeof(float), m->e->stream);
+  if (err == hipSuccess) err = hipMemsetAsync(m->we, 0, (size_t)K * 32 * sizeof(float), m->e->stream);
+  if (err == hipSuccess) err = hipMemsetAsync(m->wh, 0, (size_t)K * 32 * sizeof(float), m->e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(m->e->stream);
+  if (err != hipSuccess) {
+    free_buffering(m);
+    return hip_fail(err, "hfdtd_n2f_set_buffering");
+  }
+  m->K = K;
+  return HFDTD_OK;
 }
 
 }  // extern "C"
