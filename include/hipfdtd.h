@@ -68,6 +68,7 @@ typedef struct hfdtd_n2f hfdtd_n2f;
 typedef struct hfdtd_probe hfdtd_probe;
 typedef struct hfdtd_batch hfdtd_batch;
 typedef struct hfdtd_dft hfdtd_dft;
+typedef struct hfdtd_adjoint hfdtd_adjoint;
 
 /* One current / soft source on a box [i0,i1)x[j0,j1)x[k0,k1) of component
  * comp (0=x,1=y,2=z).  Bounds may extend past the grid (clipped); rim-taper
@@ -253,6 +254,56 @@ int hfdtd_batch_read_probes(hfdtd_batch *b, double *out);
 /* persistent grid size (0 = one block per tile), rows of j per block (2, 4, 6
  * or 8; FP64 uses at most 4) and planes of i per block */
 int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i);
+
+/* ── adjoint gradient of probe-DFT objectives ─────────────────────────────
+ * No counterpart in the reference (it has no differentiation).  Objective:
+ * any real L(F) of probe DFTs F_p = sum_{n=1..N} E_p(t_n) e^{j w t_n} dt,
+ * t_n = n dt, on Ex/Ey/Ez cells.  The reverse sweep runs the transposed
+ * Yee + CPML step (exact discrete adjoint of hfdtd_step on a lossless grid)
+ * and sums G_c = sum_n E~_c^n (E_c^n - E_c^{n-1}) in FP64 over the edges of
+ * the design box grown one cell toward lower indices; dL/d(eps_edge) =
+ * -G/eps_edge.  Forward states are rebuilt per segment from checkpoints
+ * (checkpoint-every-K recomputation); E over that box is recorded per step
+ * into history slots. */
+/* design_box: i0,i1,j0,j1,k0,k1 (half-open, inside the grid); hist_slots >= 2
+ * (segment length + 1); n_ckpt full-state checkpoints.  Lossless grids only. */
+int hfdtd_adjoint_create(hfdtd_engine *e, const int *design_box, int hist_slots, int n_ckpt,
+                         hfdtd_adjoint **out);
+int hfdtd_adjoint_destroy(hfdtd_adjoint *a);
+int hfdtd_adjoint_memory_bytes(const hfdtd_adjoint *a, uint64_t *bytes);
+/* box[6] of the gradient sums (design box grown toward lower indices) */
+int hfdtd_adjoint_grad_box(const hfdtd_adjoint *a, int *box);
+/* ijkc: n x {i, j, k, component 0..2}, distinct; omega: nfreq angular freqs */
+int hfdtd_adjoint_set_probes(hfdtd_adjoint *a, int n, const int *ijkc, int nfreq,
+                             const double *omega);
+int hfdtd_adjoint_probe_reset(hfdtd_adjoint *a);
+/* F: n x nfreq x {re, im} (FP64 sums) */
+int hfdtd_adjoint_probe_read(hfdtd_adjoint *a, double *out);
+/* conj(dL/dF*)-style coefficients a_pf (n x nfreq x {re, im}): the adjoint
+ * source after step n is sum_f Re(a_pf e^{j w_f t_n}) dt; for torch's
+ * gradient g of a real L, a = conj(g) */
+int hfdtd_adjoint_set_objective_grad(hfdtd_adjoint *a, const double *coef);
+/* full forward state (E, H, psi) to / from checkpoint c (device copies) */
+int hfdtd_adjoint_save(hfdtd_adjoint *a, int c);
+int hfdtd_adjoint_restore(hfdtd_adjoint *a, int c);
+/* E over the gradient box into history slot */
+int hfdtd_adjoint_record(hfdtd_adjoint *a, int slot);
+/* n_steps forward steps of the engine (fused sweep when it fits, else
+ * split), step q using src[r].amp = amp[q*nsrc + r]; the step taking the
+ * grid to t_n, n = step0 + q + 1, accumulates the probe DFT if probe != 0 and
+ * records E into slot slot0 + q + 1 if slot0 >= 0 */
+int hfdtd_adjoint_forward(hfdtd_adjoint *a, const hfdtd_source *src, int nsrc,
+                          const double *amp, int n_steps, int64_t step0, int slot0, int probe);
+/* zero E~, H~, psi~ and G; add the adjoint source of step n_last */
+int hfdtd_adjoint_begin(hfdtd_adjoint *a, int64_t n_last);
+/* reverse steps n = n_hi .. n_lo+1: G += E~ (E^n - E^{n-1}) from slots
+ * slot(n) = slot_hi - (n_hi - n) and slot(n) - 1, transposed step, adjoint
+ * source of step n - 1 */
+int hfdtd_adjoint_reverse(hfdtd_adjoint *a, int64_t n_hi, int64_t n_lo, int slot_hi);
+/* G: 3 x box cells (component-major, k fastest) */
+int hfdtd_adjoint_read_grad(hfdtd_adjoint *a, double *out);
+/* E~x..H~z (0..5) as a volume in the engine dtype */
+int hfdtd_adjoint_read_volume(hfdtd_adjoint *a, int which, void *host);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,674 @@
+// Discrete adjoint of the Yee + CPML step, probe-DFT objectives and the
+// checkpointed reverse sweep that gives dL/d(eps) for inverse design.
+//
+// The reference has no adjoint.  With the lossless forward step
+// (kernels/yee_update.cl:11-238, step order engine.py:724-737)
+//
+//   H half:  psi_H' = b psi_H + c D E,   H' = H - dtm (ik D E + S psi_H')
+//   E half:  psi_E' = b psi_E + c D'H,   E' = E + ce (ik D'H + S psi_E')
+//
+// (D forward difference zero on the last index, D' backward difference zero
+// on index 0), the transposed step, taken in reverse order, is
+//
+//   E half:  w = s ce E~,   g = psi~_E + w,  H~ += D'^T (ik w + c g),  psi~_E = b g
+//   H half:  w = -s dtm H~, g = psi~_H + w,  E~ += D^T  (ik w + c g),  psi~_H = b g
+//
+// with s = +-1 the sign of the term in the curl, (D^T u)_m = [m>=1] u_{m-1} -
+// [m<=N-2] u_m and (D'^T u)_m = [m>=1] u_m - [m<=N-2] u_{m+1}.  Each half is
+// one gather sweep shaped like the forward split sweeps:
+//
+//   pass 1 (one thread per H cell): H~ += D'^T(...) from E~ and psi~_E;
+//   pass 2 (one thread per E cell): E~ += D^T(...)  from H~ and psi~_H.
+//
+// A thread may update psi~ only where no other thread of the same pass reads
+// it, so each psi~ update is deferred to the pass that does not read it:
+// pass 2 settles psi~_E of its own cell (from the E~ it is about to
+// overwrite) and pass 1 of the next reverse step settles psi~_H of its own
+// cell (from the H~ it is about to add to).  Between reverse steps E~, H~
+// and psi~_E are final and psi~_H holds the value one step behind, which
+// pass 1 completes as  psi~_H = b (psi~_H - s dtm H~).
+//
+// Gradient: current sources add -ce J, so for a lossless step
+// dE^n/dce = (E^n - E^{n-1})/ce and dL/d(eps_edge) = -G/eps_edge with
+// G = sum_n E~^n (E^n - E^{n-1}); G is summed in FP64 over a box of edges
+// (the design box grown one cell toward lower indices, so every edge that
+// touches a design cell is in it) from E snapshots recorded per step.
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fdtd_core.h"
+
+namespace hfdtd {
+
+namespace {
+
+constexpr int kAdjRows = 4;
+
+struct Box {
+  int i0, j0, k0, ni, nj, nk;
+  long long n;
+};
+
+__device__ __forceinline__ int coord_of(int a, int i, int j, int k) {
+  return a == 0 ? i : (a == 1 ? j : k);
+}
+
+__device__ __forceinline__ bool in_stripe(const Geo &g, int a, int x) {
+  const int N = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
+  return x < g.npml || x >= N - g.npml;
+}
+
+// element offset of cell (i,j,k) inside the face stripe normal to axis a
+__device__ __forceinline__ int stripe_off(const Geo &g, int a, int i, int j, int k) {
+  const int n = g.npml;
+  if (a == 0) return (stripe_local(i, n, g.nx) * g.ny + j) * g.nz + k;
+  if (a == 1) return (i * 2 * n + stripe_local(j, n, g.ny)) * g.nz + k;
+  return (i * g.ny + j) * (2 * n) + stripe_local(k, n, g.nz);
+}
+
+__device__ __forceinline__ size_t cell(const Geo &g, int i, int j, int k) {
+  return ((size_t)i * g.ny + j) * g.nz + k;
+}
+
+// ik w + c g of E term (component e, derivative axis a, psi slot, sign s) at
+// E cell (i,j,k): the coefficient its E~ hands to the H~ of the cells whose
+// backward difference it enters.  Interior cells: s ce inv_dl E~.
+template <typename T>
+__device__ __forceinline__ T u_e(const Geo &g, const T *__restrict__ adj,
+                                 const T *__restrict__ coef, const T *__restrict__ prof,
+                                 const T *__restrict__ apsi, int e, int a, int slot, T s, int i,
+                                 int j, int k, T inv_dl) {
+  const size_t idx = cell(g, i, j, k);
+  const T w = s * coef[e * g.vs + idx] * adj[e * g.vs + idx];
+  if (is_interior(g, i, j, k)) return w * inv_dl;
+  const int x = coord_of(a, i, j, k);
+  T u = prof_at(g, prof, 0, 2, a, x) * w;
+  if (in_stripe(g, a, x))
+    u += prof_at(g, prof, 0, 1, a, x) * (apsi[g.psi_off[slot] + stripe_off(g, a, i, j, k)] + w);
+  return u;
+}
+
+// ik w + c g of H term (component h, axis a, psi slot, sign s) at H cell
+// (i,j,k), with w = -s dtm H~.  Interior cells: -s dtm_dl H~.
+template <typename T>
+__device__ __forceinline__ T u_h(const Geo &g, const T *__restrict__ adj,
+                                 const T *__restrict__ prof, const T *__restrict__ apsi, int h,
+                                 int a, int slot, T s, int i, int j, int k, T dtm, T dtm_dl) {
+  const T hb = adj[(3 + h) * g.vs + cell(g, i, j, k)];
+  if (is_interior(g, i, j, k)) return -s * dtm_dl * hb;
+  const T w = -s * dtm * hb;
+  const int x = coord_of(a, i, j, k);
+  T u = prof_at(g, prof, 1, 2, a, x) * w;
+  if (in_stripe(g, a, x))
+    u += prof_at(g, prof, 1, 1, a, x) * (apsi[g.psi_off[slot] + stripe_off(g, a, i, j, k)] + w);
+  return u;
+}
+
+// psi~ <- b (psi~ + w) at the cell's own stripe entry (stagger st)
+template <typename T>
+__device__ __forceinline__ void settle(const Geo &g, const T *__restrict__ prof,
+                                       T *__restrict__ apsi, int st, int a, int slot, int i,
+                                       int j, int k, T w) {
+  const int x = coord_of(a, i, j, k);
+  if (!in_stripe(g, a, x)) return;
+  T *p = apsi + g.psi_off[slot] + stripe_off(g, a, i, j, k);
+  *p = prof_at(g, prof, st, 0, a, x) * (*p + w);
+}
+
+// Pass 1: settle psi~_H of the previous reverse step, then the transposed E
+// half into H~.  adj = [E~x E~y E~z H~x H~y H~z], psi~ slab in PsiSlot order.
+template <typename T>
+__global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass1(Geo g, T inv_dl, T dtm,
+                                                             T *__restrict__ adj,
+                                                             const T *__restrict__ coef,
+                                                             const T *__restrict__ prof,
+                                                             T *__restrict__ apsi) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * kAdjRows + threadIdx.y;
+  const int i = blockIdx.z;
+  if (k >= g.nz || j >= g.ny) return;
+  const size_t idx = cell(g, i, j, k);
+  T hx = adj[3 * g.vs + idx], hy = adj[4 * g.vs + idx], hz = adj[5 * g.vs + idx];
+  if (!is_interior(g, i, j, k)) {
+    // H terms: Hx (+ y, - z), Hy (+ z, - x), Hz (+ x, - y)
+    settle(g, prof, apsi, 1, 1, PSI_HX_Y, i, j, k, -dtm * hx);
+    settle(g, prof, apsi, 1, 2, PSI_HX_Z, i, j, k, dtm * hx);
+    settle(g, prof, apsi, 1, 2, PSI_HY_Z, i, j, k, -dtm * hy);
+    settle(g, prof, apsi, 1, 0, PSI_HY_X, i, j, k, dtm * hy);
+    settle(g, prof, apsi, 1, 0, PSI_HZ_X, i, j, k, -dtm * hz);
+    settle(g, prof, apsi, 1, 1, PSI_HZ_Y, i, j, k, dtm * hz);
+  }
+  const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
+  const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
+  const T P = T(1), M = T(-1);
+  // Ey (+ z: Hx), Ez (- y: Hx)
+  if (k0) hx += u_e(g, adj, coef, prof, apsi, 1, 2, PSI_EY_Z, P, i, j, k, inv_dl);
+  if (k1) hx -= u_e(g, adj, coef, prof, apsi, 1, 2, PSI_EY_Z, P, i, j, k + 1, inv_dl);
+  if (j0) hx += u_e(g, adj, coef, prof, apsi, 2, 1, PSI_EZ_Y, M, i, j, k, inv_dl);
+  if (j1) hx -= u_e(g, adj, coef, prof, apsi, 2, 1, PSI_EZ_Y, M, i, j + 1, k, inv_dl);
+  // Ex (- z: Hy), Ez (+ x: Hy)
+  if (k0) hy += u_e(g, adj, coef, prof, apsi, 0, 2, PSI_EX_Z, M, i, j, k, inv_dl);
+  if (k1) hy -= u_e(g, adj, coef, prof, apsi, 0, 2, PSI_EX_Z, M, i, j, k + 1, inv_dl);
+  if (i0) hy += u_e(g, adj, coef, prof, apsi, 2, 0, PSI_EZ_X, P, i, j, k, inv_dl);
+  if (i1) hy -= u_e(g, adj, coef, prof, apsi, 2, 0, PSI_EZ_X, P, i + 1, j, k, inv_dl);
+  // Ex (+ y: Hz), Ey (- x: Hz)
+  if (j0) hz += u_e(g, adj, coef, prof, apsi, 0, 1, PSI_EX_Y, P, i, j, k, inv_dl);
+  if (j1) hz -= u_e(g, adj, coef, prof, apsi, 0, 1, PSI_EX_Y, P, i, j + 1, k, inv_dl);
+  if (i0) hz += u_e(g, adj, coef, prof, apsi, 1, 0, PSI_EY_X, M, i, j, k, inv_dl);
+  if (i1) hz -= u_e(g, adj, coef, prof, apsi, 1, 0, PSI_EY_X, M, i + 1, j, k, inv_dl);
+  adj[3 * g.vs + idx] = hx;
+  adj[4 * g.vs + idx] = hy;
+  adj[5 * g.vs + idx] = hz;
+}
+
+// Pass 2: settle psi~_E of this cell from the E~ about to be replaced, then
+// the transposed H half into E~ (lossless: ca = 1).
+template <typename T>
+__global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass2(Geo g, T dtm, T dtm_dl,
+                                                             T *__restrict__ adj,
+                                                             const T *__restrict__ coef,
+                                                             const T *__restrict__ prof,
+                                                             T *__restrict__ apsi) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * kAdjRows + threadIdx.y;
+  const int i = blockIdx.z;
+  if (k >= g.nz || j >= g.ny) return;
+  const size_t idx = cell(g, i, j, k);
+  T ex = adj[idx], ey = adj[g.vs + idx], ez = adj[2 * g.vs + idx];
+  if (!is_interior(g, i, j, k)) {
+    const T wx = coef[idx] * ex, wy = coef[g.vs + idx] * ey, wz = coef[2 * g.vs + idx] * ez;
+    // E terms: Ex (+ y, - z), Ey (+ z, - x), Ez (+ x, - y)
+    settle(g, prof, apsi, 0, 1, PSI_EX_Y, i, j, k, wx);
+    settle(g, prof, apsi, 0, 2, PSI_EX_Z, i, j, k, -wx);
+    settle(g, prof, apsi, 0, 2, PSI_EY_Z, i, j, k, wy);
+    settle(g, prof, apsi, 0, 0, PSI_EY_X, i, j, k, -wy);
+    settle(g, prof, apsi, 0, 0, PSI_EZ_X, i, j, k, wz);
+    settle(g, prof, apsi, 0, 1, PSI_EZ_Y, i, j, k, -wz);
+  }
+  const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
+  const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
+  const T P = T(1), M = T(-1);
+  // Hy (+ z: Ex), Hz (- y: Ex)
+  if (k0) ex += u_h(g, adj, prof, apsi, 1, 2, PSI_HY_Z, P, i, j, k - 1, dtm, dtm_dl);
+  if (k1) ex -= u_h(g, adj, prof, apsi, 1, 2, PSI_HY_Z, P, i, j, k, dtm, dtm_dl);
+  if (j0) ex += u_h(g, adj, prof, apsi, 2, 1, PSI_HZ_Y, M, i, j - 1, k, dtm, dtm_dl);
+  if (j1) ex -= u_h(g, adj, prof, apsi, 2, 1, PSI_HZ_Y, M, i, j, k, dtm, dtm_dl);
+  // Hx (- z: Ey), Hz (+ x: Ey)
+  if (k0) ey += u_h(g, adj, prof, apsi, 0, 2, PSI_HX_Z, M, i, j, k - 1, dtm, dtm_dl);
+  if (k1) ey -= u_h(g, adj, prof, apsi, 0, 2, PSI_HX_Z, M, i, j, k, dtm, dtm_dl);
+  if (i0) ey += u_h(g, adj, prof, apsi, 2, 0, PSI_HZ_X, P, i - 1, j, k, dtm, dtm_dl);
+  if (i1) ey -= u_h(g, adj, prof, apsi, 2, 0, PSI_HZ_X, P, i, j, k, dtm, dtm_dl);
+  // Hx (+ y: Ez), Hy (- x: Ez)
+  if (j0) ez += u_h(g, adj, prof, apsi, 0, 1, PSI_HX_Y, P, i, j - 1, k, dtm, dtm_dl);
+  if (j1) ez -= u_h(g, adj, prof, apsi, 0, 1, PSI_HX_Y, P, i, j, k, dtm, dtm_dl);
+  if (i0) ez += u_h(g, adj, prof, apsi, 1, 0, PSI_HY_X, M, i - 1, j, k, dtm, dtm_dl);
+  if (i1) ez -= u_h(g, adj, prof, apsi, 1, 0, PSI_HY_X, M, i, j, k, dtm, dtm_dl);
+  adj[idx] = ex;
+  adj[g.vs + idx] = ey;
+  adj[2 * g.vs + idx] = ez;
+}
+
+__device__ __forceinline__ size_t box_cell(const Geo &g, const Box &b, long long q) {
+  const int bk = (int)(q % b.nk);
+  const int bj = (int)((q / b.nk) % b.nj);
+  const int bi = (int)(q / ((long long)b.nk * b.nj));
+  return cell(g, b.i0 + bi, b.j0 + bj, b.k0 + bk);
+}
+
+// hist[c][q] = E_c at box cell q
+template <typename T>
+__global__ void k_adj_record(Geo g, Box b, const T *__restrict__ st, T *__restrict__ hist) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= b.n) return;
+  const size_t idx = box_cell(g, b, q);
+  hist[q] = st[idx];
+  hist[b.n + q] = st[g.vs + idx];
+  hist[2 * b.n + q] = st[2 * g.vs + idx];
+}
+
+// G_c += E~_c (E^n_c - E^{n-1}_c) over the box, in FP64
+template <typename T>
+__global__ void k_adj_grad(Geo g, Box b, const T *__restrict__ adj, const T *__restrict__ h1,
+                           const T *__restrict__ h0, double *__restrict__ G) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= b.n) return;
+  const size_t idx = box_cell(g, b, q);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const long long o = c * b.n + q;
+    G[o] += (double)adj[c * g.vs + idx] * ((double)h1[o] - (double)h0[o]);
+  }
+}
+
+// F[p][f] += E_p e^{j w_f t} dt (FP64), one thread per (probe, frequency)
+template <typename T>
+__global__ void k_adj_probe_dft(Geo g, int np, int nf, const int *__restrict__ comp,
+                                const long long *__restrict__ where, const double *__restrict__ om,
+                                double t, double dt, const T *__restrict__ st,
+                                double *__restrict__ F) {
+  const int th = blockIdx.x * blockDim.x + threadIdx.x;
+  if (th >= np * nf) return;
+  const int p = th / nf, f = th % nf;
+  const double v = (double)st[comp[p] * g.vs + where[p]] * dt;
+  double s, c;
+  sincos(om[f] * t, &s, &c);
+  F[2 * th] += v * c;
+  F[2 * th + 1] += v * s;
+}
+
+// E~_p += sum_f Re(coef_pf e^{j w_f t}) dt, one thread per probe (probes are
+// distinct cells, checked on the host)
+template <typename T>
+__global__ void k_adj_inject(Geo g, int np, int nf, const int *__restrict__ comp,
+                             const long long *__restrict__ where, const double *__restrict__ om,
+                             const double *__restrict__ coef, double t, double dt,
+                             T *__restrict__ adj) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double acc = 0.0;
+  for (int f = 0; f < nf; ++f) {
+    double s, c;
+    sincos(om[f] * t, &s, &c);
+    acc += coef[2 * (p * nf + f)] * c - coef[2 * (p * nf + f) + 1] * s;
+  }
+  adj[comp[p] * g.vs + where[p]] += T(acc * dt);
+}
+
+template <typename T>
+__global__ void k_adj_fill(T *p, size_t n, T v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace
+
+}  // namespace hfdtd
+
+using namespace hfdtd;
+
+struct hfdtd_adjoint {
+  Engine *e = nullptr;
+  Box box{};
+  int design[6] = {};
+  int hist_slots = 0;
+  int n_ckpt = 0;
+  void *adj = nullptr;   // E~ H~ slab (engine layout)
+  void *apsi = nullptr;  // psi~ slab (engine layout)
+  void *hist = nullptr;  // hist_slots x 3 x box.n
+  double *G = nullptr;   // 3 x box.n
+  void *ckpt = nullptr;  // n_ckpt x (state slab + psi slab)
+  size_t ckpt_state = 0, ckpt_psi = 0;
+  int np = 0, nf = 0;
+  int *pcomp = nullptr;
+  long long *pidx = nullptr;
+  double *om = nullptr;
+  double *F = nullptr;     // np x nf x {re, im}
+  double *coef = nullptr;  // np x nf x {re, im}
+  bool coef_set = false;
+  uint64_t bytes = 0;
+  uint64_t probe_bytes = 0;
+};
+
+namespace {
+
+int amalloc(hfdtd_adjoint *a, void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return HFDTD_OK;
+  hipError_t err = hipMalloc(p, bytes);
+  if (err != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "adjoint: device allocation of %.3f GB failed (%s)",
+                  bytes / 1073741824.0, hipGetErrorString(err));
+    return fail(HFDTD_ENOMEM, buf);
+  }
+  a->bytes += bytes;
+  return HFDTD_OK;
+}
+
+void release(hfdtd_adjoint *a) {
+  if (a->e) {
+    (void)hipSetDevice(a->e->device);
+    (void)hipStreamSynchronize(a->e->stream);
+  }
+  void **ptrs[] = {&a->adj, &a->apsi, &a->hist, (void **)&a->G, &a->ckpt, (void **)&a->pcomp,
+                   (void **)&a->pidx, (void **)&a->om, (void **)&a->F, (void **)&a->coef};
+  for (void **p : ptrs) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+}
+
+template <typename T>
+int fill_t(Engine *e, void *p, size_t n, T v) {
+  if (n == 0) return HFDTD_OK;
+  unsigned blocks = ceil_div(n, 256);
+  if (blocks > 65536u * 4u) blocks = 65536u * 4u;
+  hipLaunchKernelGGL(k_adj_fill<T>, dim3(blocks), dim3(256), 0, e->stream, (T *)p, n, v);
+  HF_HIP(hipGetLastError());
+  return HFDTD_OK;
+}
+
+size_t psi_elems(const Engine *e) { return e->psi_off[PSI_COUNT] + 64; }
+
+template <typename T>
+int record_t(hfdtd_adjoint *a, int slot) {
+  Engine *e = a->e;
+  T *dst = (T *)a->hist + (size_t)slot * 3 * a->box.n;
+  hipLaunchKernelGGL(k_adj_record<T>, dim3(ceil_div(a->box.n, 256)), dim3(256), 0, e->stream,
+                     geo_of(e), a->box, (const T *)e->state[e->cur], dst);
+  HF_HIP(hipGetLastError());
+  return HFDTD_OK;
+}
+
+int record(hfdtd_adjoint *a, int slot) {
+  return a->e->fp64 ? record_t<double>(a, slot) : record_t<float>(a, slot);
+}
+
+template <typename T>
+int probe_t(hfdtd_adjoint *a, long long n) {
+  Engine *e = a->e;
+  const int th = a->np * a->nf;
+  hipLaunchKernelGGL(k_adj_probe_dft<T>, dim3(ceil_div(th, 64)), dim3(64), 0, e->stream,
+                     geo_of(e), a->np, a->nf, a->pcomp, a->pidx, a->om, (double)n * e->dt, e->dt,
+                     (const T *)e->state[e->cur], a->F);
+  HF_HIP(hipGetLastError());
+  return HFDTD_OK;
+}
+
+template <typename T>
+int inject_t(hfdtd_adjoint *a, long long n) {
+  Engine *e = a->e;
+  hipLaunchKernelGGL(k_adj_inject<T>, dim3(ceil_div(a->np, 64)), dim3(64), 0, e->stream,
+                     geo_of(e), a->np, a->nf, a->pcomp, a->pidx, a->om, a->coef,
+                     (double)n * e->dt, e->dt, (T *)a->adj);
+  HF_HIP(hipGetLastError());
+  return HFDTD_OK;
+}
+
+template <typename T>
+int reverse_step_t(hfdtd_adjoint *a, long long n, int slot) {
+  Engine *e = a->e;
+  const Geo g = geo_of(e);
+  const size_t hs = (size_t)3 * a->box.n;
+  const T *h1 = (const T *)a->hist + (size_t)slot * hs;
+  const T *h0 = (const T *)a->hist + (size_t)(slot - 1) * hs;
+  hipLaunchKernelGGL(k_adj_grad<T>, dim3(ceil_div(a->box.n, 256)), dim3(256), 0, e->stream, g,
+                     a->box, (const T *)a->adj, h1, h0, a->G);
+  HF_HIP(hipGetLastError());
+  const dim3 grid(ceil_div(e->nz, 64), ceil_div(e->ny, kAdjRows), e->nx);
+  hipLaunchKernelGGL(k_adj_pass1<T>, grid, dim3(64, kAdjRows), 0, e->stream, g, T(e->inv_dl),
+                     T(e->dtm), (T *)a->adj, (const T *)e->coef, (const T *)e->prof,
+                     (T *)a->apsi);
+  HF_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_adj_pass2<T>, grid, dim3(64, kAdjRows), 0, e->stream, g, T(e->dtm),
+                     T(e->dtm_dl), (T *)a->adj, (const T *)e->coef, (const T *)e->prof,
+                     (T *)a->apsi);
+  HF_HIP(hipGetLastError());
+  if (n - 1 >= 1) return inject_t<T>(a, n - 1);
+  return HFDTD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hfdtd_adjoint_create(hfdtd_engine *h, const int *design_box, int hist_slots, int n_ckpt,
+                         hfdtd_adjoint **out) {
+  if (!h || !design_box || !out) return fail(HFDTD_EINVAL, "NULL argument");
+  *out = nullptr;
+  Engine *e = &h->s;
+  const int ext[3] = {e->nx, e->ny, e->nz};
+  for (int a = 0; a < 3; ++a) {
+    const int lo = design_box[2 * a], hi = design_box[2 * a + 1];
+    if (lo < 0 || hi > ext[a] || hi <= lo)
+      return fail(HFDTD_EINVAL, "design box must be a non-empty box inside the grid");
+  }
+  if (hist_slots < 2) return fail(HFDTD_EINVAL, "hist_slots must be >= 2");
+  if (n_ckpt < 0) return fail(HFDTD_EINVAL, "n_ckpt must be >= 0");
+  if (!e->lossless)
+    return fail(HFDTD_EUNSUPPORTED, "the adjoint gradient needs a lossless grid (no sigma)");
+  HF_HIP(hipSetDevice(e->device));
+  hfdtd_adjoint *a = new hfdtd_adjoint();
+  a->e = e;
+  for (int q = 0; q < 6; ++q) a->design[q] = design_box[q];
+  a->box.i0 = std::max(design_box[0] - 1, 0);
+  a->box.j0 = std::max(design_box[2] - 1, 0);
+  a->box.k0 = std::max(design_box[4] - 1, 0);
+  a->box.ni = design_box[1] - a->box.i0;
+  a->box.nj = design_box[3] - a->box.j0;
+  a->box.nk = design_box[5] - a->box.k0;
+  a->box.n = (long long)a->box.ni * a->box.nj * a->box.nk;
+  a->hist_slots = hist_slots;
+  a->n_ckpt = n_ckpt;
+  a->ckpt_state = 6 * e->vstride * e->elem;
+  a->ckpt_psi = psi_elems(e) * e->elem;
+  int rc;
+  auto bail = [&](int code) {
+    release(a);
+    delete a;
+    return code;
+  };
+  if ((rc = amalloc(a, &a->adj, 6 * e->vstride * e->elem))) return bail(rc);
+  if ((rc = amalloc(a, &a->apsi, psi_elems(e) * e->elem))) return bail(rc);
+  if ((rc = amalloc(a, &a->hist, (size_t)hist_slots * 3 * a->box.n * e->elem))) return bail(rc);
+  if ((rc = amalloc(a, (void **)&a->G, (size_t)3 * a->box.n * sizeof(double)))) return bail(rc);
+  if ((rc = amalloc(a, &a->ckpt, (size_t)n_ckpt * (a->ckpt_state + a->ckpt_psi)))) return bail(rc);
+  *out = a;
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_destroy(hfdtd_adjoint *a) {
+  if (!a) return HFDTD_OK;
+  release(a);
+  delete a;
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_memory_bytes(const hfdtd_adjoint *a, uint64_t *bytes) {
+  if (!a || !bytes) return fail(HFDTD_EINVAL, "NULL argument");
+  *bytes = a->bytes;
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_grad_box(const hfdtd_adjoint *a, int *box) {
+  if (!a || !box) return fail(HFDTD_EINVAL, "NULL argument");
+  box[0] = a->box.i0;
+  box[1] = a->box.i0 + a->box.ni;
+  box[2] = a->box.j0;
+  box[3] = a->box.j0 + a->box.nj;
+  box[4] = a->box.k0;
+  box[5] = a->box.k0 + a->box.nk;
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_set_probes(hfdtd_adjoint *a, int n, const int *ijkc, int nfreq,
+                             const double *omega) {
+  if (!a || n < 1 || !ijkc || nfreq < 1 || !omega) return fail(HFDTD_EINVAL, "bad probe list");
+  Engine *e = a->e;
+  std::vector<int> comp(n);
+  std::vector<long long> where(n);
+  for (int p = 0; p < n; ++p) {
+    const int i = ijkc[4 * p], j = ijkc[4 * p + 1], k = ijkc[4 * p + 2], c = ijkc[4 * p + 3];
+    if (i < 0 || i >= e->nx || j < 0 || j >= e->ny || k < 0 || k >= e->nz || c < 0 || c > 2)
+      return fail(HFDTD_EINVAL, "probe cell outside the grid or component not Ex/Ey/Ez");
+    comp[p] = c;
+    where[p] = ((long long)i * e->ny + j) * e->nz + k;
+    for (int q = 0; q < p; ++q)
+      if (comp[q] == c && where[q] == where[p]) return fail(HFDTD_EINVAL, "duplicate probe");
+  }
+  HF_HIP(hipSetDevice(e->device));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  void **ptrs[] = {(void **)&a->pcomp, (void **)&a->pidx, (void **)&a->om, (void **)&a->F,
+                   (void **)&a->coef};
+  for (void **p : ptrs) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  a->bytes -= a->probe_bytes;
+  a->np = a->nf = 0;
+  const uint64_t before = a->bytes;
+  int rc;
+  const size_t nfc = (size_t)n * nfreq * 2 * sizeof(double);
+  if ((rc = amalloc(a, (void **)&a->pcomp, n * sizeof(int)))) return rc;
+  if ((rc = amalloc(a, (void **)&a->pidx, n * sizeof(long long)))) return rc;
+  if ((rc = amalloc(a, (void **)&a->om, nfreq * sizeof(double)))) return rc;
+  if ((rc = amalloc(a, (void **)&a->F, nfc))) return rc;
+  if ((rc = amalloc(a, (void **)&a->coef, nfc))) return rc;
+  HF_HIP(hipMemcpy(a->pcomp, comp.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  HF_HIP(hipMemcpy(a->pidx, where.data(), n * sizeof(long long), hipMemcpyHostToDevice));
+  HF_HIP(hipMemcpy(a->om, omega, nfreq * sizeof(double), hipMemcpyHostToDevice));
+  HF_HIP(hipMemset(a->F, 0, nfc));
+  HF_HIP(hipMemset(a->coef, 0, nfc));
+  a->probe_bytes = a->bytes - before;
+  a->np = n;
+  a->nf = nfreq;
+  a->coef_set = false;
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_probe_reset(hfdtd_adjoint *a) {
+  if (!a || !a->np) return fail(HFDTD_EINVAL, "no probes set");
+  HF_HIP(hipSetDevice(a->e->device));
+  return fill_t<double>(a->e, a->F, (size_t)a->np * a->nf * 2, 0.0);
+}
+
+int hfdtd_adjoint_probe_read(hfdtd_adjoint *a, double *out) {
+  if (!a || !out || !a->np) return fail(HFDTD_EINVAL, "no probes set");
+  Engine *e = a->e;
+  HF_HIP(hipSetDevice(e->device));
+  HF_HIP(hipMemcpyAsync(out, a->F, (size_t)a->np * a->nf * 2 * sizeof(double),
+                        hipMemcpyDeviceToHost, e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_set_objective_grad(hfdtd_adjoint *a, const double *coef) {
+  if (!a || !coef || !a->np) return fail(HFDTD_EINVAL, "no probes set");
+  Engine *e = a->e;
+  HF_HIP(hipSetDevice(e->device));
+  HF_HIP(hipMemcpyAsync(a->coef, coef, (size_t)a->np * a->nf * 2 * sizeof(double),
+                        hipMemcpyHostToDevice, e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  a->coef_set = true;
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_save(hfdtd_adjoint *a, int c) {
+  if (!a || c < 0 || c >= a->n_ckpt) return fail(HFDTD_EINVAL, "checkpoint index out of range");
+  Engine *e = a->e;
+  HF_HIP(hipSetDevice(e->device));
+  char *dst = (char *)a->ckpt + (size_t)c * (a->ckpt_state + a->ckpt_psi);
+  HF_HIP(hipMemcpyAsync(dst, e->state[e->cur], a->ckpt_state, hipMemcpyDeviceToDevice,
+                        e->stream));
+  HF_HIP(hipMemcpyAsync(dst + a->ckpt_state, e->psi[e->cur], a->ckpt_psi,
+                        hipMemcpyDeviceToDevice, e->stream));
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_restore(hfdtd_adjoint *a, int c) {
+  if (!a || c < 0 || c >= a->n_ckpt) return fail(HFDTD_EINVAL, "checkpoint index out of range");
+  Engine *e = a->e;
+  HF_HIP(hipSetDevice(e->device));
+  const char *src = (const char *)a->ckpt + (size_t)c * (a->ckpt_state + a->ckpt_psi);
+  HF_HIP(hipMemcpyAsync(e->state[e->cur], src, a->ckpt_state, hipMemcpyDeviceToDevice,
+                        e->stream));
+  HF_HIP(hipMemcpyAsync(e->psi[e->cur], src + a->ckpt_state, a->ckpt_psi,
+                        hipMemcpyDeviceToDevice, e->stream));
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_record(hfdtd_adjoint *a, int slot) {
+  if (!a || slot < 0 || slot >= a->hist_slots) return fail(HFDTD_EINVAL, "slot out of range");
+  HF_HIP(hipSetDevice(a->e->device));
+  return record(a, slot);
+}
+
+int hfdtd_adjoint_forward(hfdtd_adjoint *a, const hfdtd_source *src, int nsrc,
+                          const double *amp, int n_steps, int64_t step0, int slot0, int probe) {
+  if (!a || n_steps < 0 || nsrc < 0 || nsrc > kMaxSources || (nsrc > 0 && (!src || !amp)))
+    return fail(HFDTD_EINVAL, "bad forward arguments (at most 16 sources)");
+  if (slot0 >= 0 && slot0 + n_steps >= a->hist_slots)
+    return fail(HFDTD_EINVAL, "recorded steps exceed the history slots");
+  if (probe && !a->np) return fail(HFDTD_EINVAL, "no probes set");
+  Engine *e = a->e;
+  HF_HIP(hipSetDevice(e->device));
+  std::vector<hfdtd_source> s(src, src + nsrc);
+  const bool fused = e->fused_fits();
+  for (int q = 0; q < n_steps; ++q) {
+    for (int r = 0; r < nsrc; ++r) s[r].amp = amp[(size_t)q * nsrc + r];
+    int rc;
+    if (fused) {
+      rc = launch_fused_step(e, s.data(), nsrc);
+    } else {
+      rc = launch_update_H(e);
+      if (!rc && nsrc) rc = launch_sources(e, s.data(), nsrc);
+      if (!rc) rc = launch_update_E(e);
+    }
+    if (rc) return rc;
+    const long long n = step0 + q + 1;
+    if (probe && (rc = e->fp64 ? probe_t<double>(a, n) : probe_t<float>(a, n))) return rc;
+    if (slot0 >= 0 && (rc = record(a, slot0 + q + 1))) return rc;
+  }
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_begin(hfdtd_adjoint *a, int64_t n_last) {
+  if (!a || !a->np || !a->coef_set)
+    return fail(HFDTD_EINVAL, "set the probes and the objective gradient first");
+  Engine *e = a->e;
+  if (!e->lossless)
+    return fail(HFDTD_EUNSUPPORTED, "the adjoint gradient needs a lossless grid (no sigma)");
+  HF_HIP(hipSetDevice(e->device));
+  int rc;
+  if (e->fp64) {
+    if ((rc = fill_t<double>(e, a->adj, 6 * e->vstride, 0.0))) return rc;
+    if ((rc = fill_t<double>(e, a->apsi, psi_elems(e), 0.0))) return rc;
+  } else {
+    if ((rc = fill_t<float>(e, a->adj, 6 * e->vstride, 0.f))) return rc;
+    if ((rc = fill_t<float>(e, a->apsi, psi_elems(e), 0.f))) return rc;
+  }
+  if ((rc = fill_t<double>(e, a->G, (size_t)3 * a->box.n, 0.0))) return rc;
+  if (n_last >= 1) return e->fp64 ? inject_t<double>(a, n_last) : inject_t<float>(a, n_last);
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_reverse(hfdtd_adjoint *a, int64_t n_hi, int64_t n_lo, int slot_hi) {
+  if (!a || n_lo < 0 || n_hi < n_lo) return fail(HFDTD_EINVAL, "need 0 <= n_lo <= n_hi");
+  if (slot_hi >= a->hist_slots || slot_hi - (n_hi - n_lo) < 0)
+    return fail(HFDTD_EINVAL, "history slots do not cover the reverse range");
+  Engine *e = a->e;
+  HF_HIP(hipSetDevice(e->device));
+  for (long long n = n_hi; n > n_lo; --n) {
+    const int slot = (int)(slot_hi - (n_hi - n));
+    const int rc = e->fp64 ? reverse_step_t<double>(a, n, slot) : reverse_step_t<float>(a, n, slot);
+    if (rc) return rc;
+  }
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_read_grad(hfdtd_adjoint *a, double *out) {
+  if (!a || !out) return fail(HFDTD_EINVAL, "NULL argument");
+  Engine *e = a->e;
+  HF_HIP(hipSetDevice(e->device));
+  HF_HIP(hipMemcpyAsync(out, a->G, (size_t)3 * a->box.n * sizeof(double), hipMemcpyDeviceToHost,
+                        e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  return HFDTD_OK;
+}
+
+int hfdtd_adjoint_read_volume(hfdtd_adjoint *a, int which, void *host) {
+  if (!a || !host || which < 0 || which > 5) return fail(HFDTD_EINVAL, "bad volume id");
+  Engine *e = a->e;
+  HF_HIP(hipSetDevice(e->device));
+  HF_HIP(hipMemcpyAsync(host, (const char *)a->adj + (size_t)which * e->vstride * e->elem,
+                        e->ncell * e->elem, hipMemcpyDeviceToHost, e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  return HFDTD_OK;
+}
+
+}  // extern "C"

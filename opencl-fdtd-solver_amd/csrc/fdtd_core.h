@@ -20,6 +20,18 @@ struct Geo {
   int psi_off[PSI_COUNT + 1];  // element offsets of the psi slots
 };
 
+inline Geo geo_of(const Engine *e) {
+  Geo g;
+  g.nx = e->nx;
+  g.ny = e->ny;
+  g.nz = e->nz;
+  g.npml = e->npml;
+  g.vs = (long long)e->vstride;
+  g.plen = (int)e->prof_len();
+  for (int q = 0; q <= PSI_COUNT; ++q) g.psi_off[q] = (int)e->psi_off[q];
+  return g;
+}
+
 // CPML profile accessor: stagger s (0 = E nodes, 1 = H nodes), array q
 // (0 = b, 1 = c, 2 = 1/(kappa dl)), axis a, index along the axis.
 template <typename T>

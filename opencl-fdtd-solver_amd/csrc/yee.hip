@@ -159,18 +159,6 @@ __global__ __launch_bounds__(256) void k_source_fixup(Geo g, T inv_dl, const T *
 
 // ── launchers ─────────────────────────────────────────────────────────────
 
-static Geo geo_of(const Engine *e) {
-  Geo g;
-  g.nx = e->nx;
-  g.ny = e->ny;
-  g.nz = e->nz;
-  g.npml = e->npml;
-  g.vs = (long long)e->vstride;
-  g.plen = (int)e->prof_len();
-  for (int q = 0; q <= PSI_COUNT; ++q) g.psi_off[q] = (int)e->psi_off[q];
-  return g;
-}
-
 template <typename T>
 static int update_H_t(Engine *e) {
   dim3 grid(ceil_div(e->nz, 64), ceil_div(e->ny, kSplitRows), e->nx);

@@ -138,6 +138,32 @@ EXPORTS = {
     "hfdtd_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "hfdtd_batch_read_probes": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "hfdtd_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "hfdtd_adjoint_create": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_void_p)],
+    ),
+    "hfdtd_adjoint_destroy": (C.c_int, [C.c_void_p]),
+    "hfdtd_adjoint_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "hfdtd_adjoint_grad_box": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "hfdtd_adjoint_set_probes": (
+        C.c_int,
+        [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_double)],
+    ),
+    "hfdtd_adjoint_probe_reset": (C.c_int, [C.c_void_p]),
+    "hfdtd_adjoint_probe_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "hfdtd_adjoint_set_objective_grad": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "hfdtd_adjoint_save": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfdtd_adjoint_restore": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfdtd_adjoint_record": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfdtd_adjoint_forward": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(Source), C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int64,
+         C.c_int, C.c_int],
+    ),  # fmt: skip
+    "hfdtd_adjoint_begin": (C.c_int, [C.c_void_p, C.c_int64]),
+    "hfdtd_adjoint_reverse": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int]),
+    "hfdtd_adjoint_read_grad": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "hfdtd_adjoint_read_volume": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
 }
 
 _LIB = None
