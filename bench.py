@@ -22,6 +22,14 @@ passes, FETCH_SIZE and WRITE_SIZE, each scaled by what the same counter
 reads for a plain device copy of known size; ``--copy-calibration`` runs that
 copy).
 
+``--workload adjoint`` measures the 160^3 inverse-design gradient: a
+64x64x32 design box (εr ~ U[1, 12], seed 5), a Jx point source and a 4x4 Ex
+probe patch 40 cells away along +z, J = Σ|DFT(Ex)|² at one frequency, 3000
+steps with a checkpoint every 100.  It reports the adjoint sweep's wall time
+over the forward run's (recomputation of the checkpointed segments plus the
+transposed steps) and checks the FP32 gradient on 16 random design cells
+against central differences (δε = 1e-3) of FP64 runs.
+
 ``--workload plane-dft`` (not the default line) measures the 512^3 plane-DFT
 configuration: smooth Gaussian random εr (seed 3, σ = 8 cells, clipped to
 [1, 6]), three orthogonal point dipoles with staggered pulses, npml 16, and a
@@ -33,6 +41,7 @@ error against the FP64 sums.
 Usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 20] [--no-cpu]
        python bench.py --copy-calibration
        python bench.py --workload plane-dft [--steps 1500]
+       python bench.py --workload adjoint [--steps 3000] [--repeat 2]
 """
 
 from __future__ import annotations
@@ -241,6 +250,81 @@ def run_plane_dft(args):
     }))  # fmt: skip
 
 
+def run_adjoint(args):
+    """160^3 inverse-design gradient: forward and checkpointed adjoint sweep
+    timed on the FP32 engine, gradient checked against central differences of
+    FP64 runs on 16 random design cells."""
+    from hipfdtd.adjoint import ProbeDFTAdjoint
+
+    shape, npml, dl = (160, 160, 160), 12, 1e-3
+    f0 = C0 / (30 * dl)
+    fw = 0.5 * f0
+    rng = np.random.default_rng(5)
+    box = (48, 112, 48, 112, 64, 96)
+    design = rng.uniform(1.0, 12.0, (64, 64, 32))
+    src = [("x", (80, 81, 80, 81, 58, 59), lambda t: gaussian_sine(t, f0, fw))]
+    probes = [(0, i, j, 98) for i in range(78, 82) for j in range(78, 82)]
+    steps, K = args.steps, 100
+    pb = ProbeDFTAdjoint(shape, dl, npml, box, src, probes, [f0], steps, checkpoint=K)
+    for _ in range(args.warmup):
+        pb.value_and_grad(design)
+    tf, tb, trc, trv = [], [], [], []
+    for _ in range(args.repeat):
+        F = pb.forward(design)
+        g = pb.backward(2.0 * F)
+        tf.append(pb.timing["forward_s"])
+        tb.append(pb.timing["backward_s"])
+        trc.append(pb.timing["recompute_s"])
+        trv.append(pb.timing["reverse_s"])
+    J = float(np.sum(np.abs(F) ** 2))
+    mem32 = pb.memory_bytes()
+    dev = pb.fdtd.device_name
+    pb.close()
+    t_f, t_b = float(np.median(tf)), float(np.median(tb))
+    print(f"[adjoint] forward {t_f:.3f} s backward {t_b:.3f} s", file=sys.stderr, flush=True)
+
+    p64 = ProbeDFTAdjoint(shape, dl, npml, box, src, probes, [f0], steps, checkpoint=K,
+                          dtype=np.float64)  # fmt: skip
+    J64, g64 = p64.value_and_grad(design)
+    cells = [tuple(int(rng.integers(0, n)) for n in design.shape) for _ in range(16)]
+    d = 1e-3
+    fd = []
+    for c in cells:
+        js = []
+        for sgn in (+1, -1):
+            d2 = design.copy()
+            d2[c] += sgn * d
+            js.append(float(np.sum(np.abs(p64.forward(d2)) ** 2)))
+        fd.append((js[0] - js[1]) / (2 * d))
+    p64.close()
+    fd = np.array(fd)
+    a32 = np.array([g[c] for c in cells])
+    a64 = np.array([g64[c] for c in cells])
+    cells_n = float(np.prod(shape))
+    print(json.dumps({
+        "metric": "adjoint/forward time", "value": round(t_b / t_f, 3), "unit": "x",
+        "higher_is_better": False, "steps": steps, "warmup": args.warmup,
+        "repeat": args.repeat, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "160^3 FP32 npml 12, design box 64x64x32 centred with eps_r "
+                               "~ U[1,12] (seed 5), Jx point source, 4x4 Ex probe patch 40 "
+                               "cells away along +z, J = sum |DFT(Ex)|^2 at f0 = c0/(30 dl)",
+                   "checkpoint_every": K, "design_cells": int(design.size)},
+        "forward_s": round(t_f, 4), "backward_s": round(t_b, 4),
+        "backward_recompute_s": round(float(np.median(trc)), 4),
+        "backward_reverse_s": round(float(np.median(trv)), 4),
+        "forward_mcell_per_s": round(cells_n * steps / t_f / 1e6, 1),
+        "reverse_mcell_per_s": round(cells_n * steps / float(np.median(trv)) / 1e6, 1),
+        "device_bytes_fp32": mem32,
+        "objective_fp32_vs_fp64_rel": abs(J - J64) / J64,
+        "fd_check": {"cells": 16, "delta_eps": d, "fd_engine": "f64",
+                     "max_rel_err_f32_adjoint": float(np.max(np.abs(a32 - fd) / np.abs(fd))),
+                     "max_rel_err_f64_adjoint": float(np.max(np.abs(a64 - fd) / np.abs(fd))),
+                     "peak_norm_err_f32_adjoint": float(np.max(np.abs(a32 - fd)) / np.max(np.abs(fd)))},
+        "grad_f32_vs_f64_peak_norm": float(np.max(np.abs(g - g64)) / np.max(np.abs(g64))),
+        "device": dev,
+    }))  # fmt: skip
+
+
 TRAFFIC_FILE = ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt"
 
 
@@ -276,7 +360,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", default="yee-256", choices=["yee-256", "plane-dft"])
+    ap.add_argument("--workload", default="yee-256", choices=["yee-256", "plane-dft", "adjoint"])
+    ap.add_argument("--repeat", type=int, default=2, help="adjoint: timed forward+backward runs")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -285,6 +370,13 @@ def main():
         if args.steps == 2000:
             args.steps = 1500
         run_plane_dft(args)
+        return
+    if args.workload == "adjoint":
+        if args.steps == 2000:
+            args.steps = 3000
+        if args.warmup == 20:
+            args.warmup = 1
+        run_adjoint(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path "

@@ -144,7 +144,8 @@ int hfdtd_step(hfdtd_engine *e, const hfdtd_source *src, int n, int path);
 int hfdtd_synchronize(hfdtd_engine *e);
 /* Fused-sweep launch shape: rows of j per block (1, 2, 4, 6 or 8; FP64 engines
  * use at most 4; a block is
- * 64 x (rows + 1) threads), planes of i per block (>= 1) and the number of
+ * 64 x (rows + 1) threads), planes of i per block (>= 1; -1 = chosen from the
+ * grid: up to 64, two blocks per CU) and the number of
  * XCDs the work order is grouped for (1 = hardware order, 8 = MI355X); 0
  * keeps the current value.  Used by the autotuner. */
 int hfdtd_set_fused_tiling(hfdtd_engine *e, int rows_j, int planes_i, int xcd_groups);
@@ -302,7 +303,8 @@ int hfdtd_adjoint_begin(hfdtd_adjoint *a, int64_t n_last);
 int hfdtd_adjoint_reverse(hfdtd_adjoint *a, int64_t n_hi, int64_t n_lo, int slot_hi);
 /* G: 3 x box cells (component-major, k fastest) */
 int hfdtd_adjoint_read_grad(hfdtd_adjoint *a, double *out);
-/* E~x..H~z (0..5) as a volume in the engine dtype */
+/* adjoint volumes 0..5 in the engine dtype: ce_c E~_c for c = x, y, z (the
+ * stored, ce-scaled E adjoint), then H~x, H~y, H~z */
 int hfdtd_adjoint_read_volume(hfdtd_adjoint *a, int which, void *host);
 
 #ifdef __cplusplus

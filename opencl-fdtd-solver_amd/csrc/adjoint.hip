@@ -20,6 +20,10 @@
 //   pass 1 (one thread per H cell): H~ += D'^T(...) from E~ and psi~_E;
 //   pass 2 (one thread per E cell): E~ += D^T(...)  from H~ and psi~_H.
 //
+// The stored E adjoint is scaled, ce E~ (with E~ the adjoint above), which
+// drops ce from pass 1; in the interior the two passes are then the forward
+// H and E stencils with -inv_dl for dtm_dl and -ce dtm_dl for ce inv_dl.
+//
 // A thread may update psi~ only where no other thread of the same pass reads
 // it, so each psi~ update is deferred to the pass that does not read it:
 // pass 2 settles psi~_E of its own cell (from the E~ it is about to
@@ -73,15 +77,14 @@ __device__ __forceinline__ size_t cell(const Geo &g, int i, int j, int k) {
 }
 
 // ik w + c g of E term (component e, derivative axis a, psi slot, sign s) at
-// E cell (i,j,k): the coefficient its E~ hands to the H~ of the cells whose
-// backward difference it enters.  Interior cells: s ce inv_dl E~.
+// E cell (i,j,k), w = s ce E~ (the stored scaled adjoint): the coefficient
+// its E~ hands to the H~ of the cells whose backward difference it enters.
+// Interior cells: s inv_dl ce E~.
 template <typename T>
 __device__ __forceinline__ T u_e(const Geo &g, const T *__restrict__ adj,
-                                 const T *__restrict__ coef, const T *__restrict__ prof,
-                                 const T *__restrict__ apsi, int e, int a, int slot, T s, int i,
-                                 int j, int k, T inv_dl) {
-  const size_t idx = cell(g, i, j, k);
-  const T w = s * coef[e * g.vs + idx] * adj[e * g.vs + idx];
+                                 const T *__restrict__ prof, const T *__restrict__ apsi, int e,
+                                 int a, int slot, T s, int i, int j, int k, T inv_dl) {
+  const T w = s * adj[e * g.vs + cell(g, i, j, k)];
   if (is_interior(g, i, j, k)) return w * inv_dl;
   const int x = coord_of(a, i, j, k);
   T u = prof_at(g, prof, 0, 2, a, x) * w;
@@ -117,12 +120,18 @@ __device__ __forceinline__ void settle(const Geo &g, const T *__restrict__ prof,
   *p = prof_at(g, prof, st, 0, a, x) * (*p + w);
 }
 
+// lo <= c <= hi on all three axes
+__device__ __forceinline__ bool within(int i, int j, int k, int lo, int hx, int hy, int hz) {
+  return i >= lo && j >= lo && k >= lo && i <= hx && j <= hy && k <= hz;
+}
+
 // Pass 1: settle psi~_H of the previous reverse step, then the transposed E
-// half into H~.  adj = [E~x E~y E~z H~x H~y H~z], psi~ slab in PsiSlot order.
+// half into H~.  adj = [ce E~x, ce E~y, ce E~z, H~x, H~y, H~z], psi~ slab in
+// PsiSlot order.  Where the cell and its +x/+y/+z neighbours are interior
+// the update is the forward H stencil with -inv_dl for dtm_dl.
 template <typename T>
 __global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass1(Geo g, T inv_dl, T dtm,
                                                              T *__restrict__ adj,
-                                                             const T *__restrict__ coef,
                                                              const T *__restrict__ prof,
                                                              T *__restrict__ apsi) {
   const int k = blockIdx.x * 64 + threadIdx.x;
@@ -130,41 +139,53 @@ __global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass1(Geo g, T inv_dl, T 
   const int i = blockIdx.z;
   if (k >= g.nz || j >= g.ny) return;
   const size_t idx = cell(g, i, j, k);
+  const size_t sxy = (size_t)g.ny * g.nz;
+  const T *Ex = adj, *Ey = adj + g.vs, *Ez = adj + 2 * g.vs;
   T hx = adj[3 * g.vs + idx], hy = adj[4 * g.vs + idx], hz = adj[5 * g.vs + idx];
-  if (!is_interior(g, i, j, k)) {
-    // H terms: Hx (+ y, - z), Hy (+ z, - x), Hz (+ x, - y)
-    settle(g, prof, apsi, 1, 1, PSI_HX_Y, i, j, k, -dtm * hx);
-    settle(g, prof, apsi, 1, 2, PSI_HX_Z, i, j, k, dtm * hx);
-    settle(g, prof, apsi, 1, 2, PSI_HY_Z, i, j, k, -dtm * hy);
-    settle(g, prof, apsi, 1, 0, PSI_HY_X, i, j, k, dtm * hy);
-    settle(g, prof, apsi, 1, 0, PSI_HZ_X, i, j, k, -dtm * hz);
-    settle(g, prof, apsi, 1, 1, PSI_HZ_Y, i, j, k, dtm * hz);
+  const int n = g.npml;
+  if (n > 0 && within(i, j, k, n, g.nx - n - 2, g.ny - n - 2, g.nz - n - 2)) {
+    const T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
+    hx += inv_dl * ((ey - Ey[idx + 1]) - (ez - Ez[idx + g.nz]));
+    hy += inv_dl * ((ez - Ez[idx + sxy]) - (ex - Ex[idx + 1]));
+    hz += inv_dl * ((ex - Ex[idx + g.nz]) - (ey - Ey[idx + sxy]));
+  } else {
+    if (!is_interior(g, i, j, k)) {
+      // H terms: Hx (+ y, - z), Hy (+ z, - x), Hz (+ x, - y)
+      settle(g, prof, apsi, 1, 1, PSI_HX_Y, i, j, k, -dtm * hx);
+      settle(g, prof, apsi, 1, 2, PSI_HX_Z, i, j, k, dtm * hx);
+      settle(g, prof, apsi, 1, 2, PSI_HY_Z, i, j, k, -dtm * hy);
+      settle(g, prof, apsi, 1, 0, PSI_HY_X, i, j, k, dtm * hy);
+      settle(g, prof, apsi, 1, 0, PSI_HZ_X, i, j, k, -dtm * hz);
+      settle(g, prof, apsi, 1, 1, PSI_HZ_Y, i, j, k, dtm * hz);
+    }
+    const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
+    const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
+    const T P = T(1), M = T(-1);
+    // Ey (+ z: Hx), Ez (- y: Hx)
+    if (k0) hx += u_e(g, adj, prof, apsi, 1, 2, PSI_EY_Z, P, i, j, k, inv_dl);
+    if (k1) hx -= u_e(g, adj, prof, apsi, 1, 2, PSI_EY_Z, P, i, j, k + 1, inv_dl);
+    if (j0) hx += u_e(g, adj, prof, apsi, 2, 1, PSI_EZ_Y, M, i, j, k, inv_dl);
+    if (j1) hx -= u_e(g, adj, prof, apsi, 2, 1, PSI_EZ_Y, M, i, j + 1, k, inv_dl);
+    // Ex (- z: Hy), Ez (+ x: Hy)
+    if (k0) hy += u_e(g, adj, prof, apsi, 0, 2, PSI_EX_Z, M, i, j, k, inv_dl);
+    if (k1) hy -= u_e(g, adj, prof, apsi, 0, 2, PSI_EX_Z, M, i, j, k + 1, inv_dl);
+    if (i0) hy += u_e(g, adj, prof, apsi, 2, 0, PSI_EZ_X, P, i, j, k, inv_dl);
+    if (i1) hy -= u_e(g, adj, prof, apsi, 2, 0, PSI_EZ_X, P, i + 1, j, k, inv_dl);
+    // Ex (+ y: Hz), Ey (- x: Hz)
+    if (j0) hz += u_e(g, adj, prof, apsi, 0, 1, PSI_EX_Y, P, i, j, k, inv_dl);
+    if (j1) hz -= u_e(g, adj, prof, apsi, 0, 1, PSI_EX_Y, P, i, j + 1, k, inv_dl);
+    if (i0) hz += u_e(g, adj, prof, apsi, 1, 0, PSI_EY_X, M, i, j, k, inv_dl);
+    if (i1) hz -= u_e(g, adj, prof, apsi, 1, 0, PSI_EY_X, M, i + 1, j, k, inv_dl);
   }
-  const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
-  const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
-  const T P = T(1), M = T(-1);
-  // Ey (+ z: Hx), Ez (- y: Hx)
-  if (k0) hx += u_e(g, adj, coef, prof, apsi, 1, 2, PSI_EY_Z, P, i, j, k, inv_dl);
-  if (k1) hx -= u_e(g, adj, coef, prof, apsi, 1, 2, PSI_EY_Z, P, i, j, k + 1, inv_dl);
-  if (j0) hx += u_e(g, adj, coef, prof, apsi, 2, 1, PSI_EZ_Y, M, i, j, k, inv_dl);
-  if (j1) hx -= u_e(g, adj, coef, prof, apsi, 2, 1, PSI_EZ_Y, M, i, j + 1, k, inv_dl);
-  // Ex (- z: Hy), Ez (+ x: Hy)
-  if (k0) hy += u_e(g, adj, coef, prof, apsi, 0, 2, PSI_EX_Z, M, i, j, k, inv_dl);
-  if (k1) hy -= u_e(g, adj, coef, prof, apsi, 0, 2, PSI_EX_Z, M, i, j, k + 1, inv_dl);
-  if (i0) hy += u_e(g, adj, coef, prof, apsi, 2, 0, PSI_EZ_X, P, i, j, k, inv_dl);
-  if (i1) hy -= u_e(g, adj, coef, prof, apsi, 2, 0, PSI_EZ_X, P, i + 1, j, k, inv_dl);
-  // Ex (+ y: Hz), Ey (- x: Hz)
-  if (j0) hz += u_e(g, adj, coef, prof, apsi, 0, 1, PSI_EX_Y, P, i, j, k, inv_dl);
-  if (j1) hz -= u_e(g, adj, coef, prof, apsi, 0, 1, PSI_EX_Y, P, i, j + 1, k, inv_dl);
-  if (i0) hz += u_e(g, adj, coef, prof, apsi, 1, 0, PSI_EY_X, M, i, j, k, inv_dl);
-  if (i1) hz -= u_e(g, adj, coef, prof, apsi, 1, 0, PSI_EY_X, M, i + 1, j, k, inv_dl);
   adj[3 * g.vs + idx] = hx;
   adj[4 * g.vs + idx] = hy;
   adj[5 * g.vs + idx] = hz;
 }
 
 // Pass 2: settle psi~_E of this cell from the E~ about to be replaced, then
-// the transposed H half into E~ (lossless: ca = 1).
+// the transposed H half into E~ (lossless: ca = 1).  Where the cell and its
+// -x/-y/-z neighbours are interior this is the forward E stencil with
+// -ce dtm_dl for ce inv_dl.
 template <typename T>
 __global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass2(Geo g, T dtm, T dtm_dl,
                                                              T *__restrict__ adj,
@@ -176,38 +197,49 @@ __global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass2(Geo g, T dtm, T dtm
   const int i = blockIdx.z;
   if (k >= g.nz || j >= g.ny) return;
   const size_t idx = cell(g, i, j, k);
-  T ex = adj[idx], ey = adj[g.vs + idx], ez = adj[2 * g.vs + idx];
-  if (!is_interior(g, i, j, k)) {
-    const T wx = coef[idx] * ex, wy = coef[g.vs + idx] * ey, wz = coef[2 * g.vs + idx] * ez;
-    // E terms: Ex (+ y, - z), Ey (+ z, - x), Ez (+ x, - y)
-    settle(g, prof, apsi, 0, 1, PSI_EX_Y, i, j, k, wx);
-    settle(g, prof, apsi, 0, 2, PSI_EX_Z, i, j, k, -wx);
-    settle(g, prof, apsi, 0, 2, PSI_EY_Z, i, j, k, wy);
-    settle(g, prof, apsi, 0, 0, PSI_EY_X, i, j, k, -wy);
-    settle(g, prof, apsi, 0, 0, PSI_EZ_X, i, j, k, wz);
-    settle(g, prof, apsi, 0, 1, PSI_EZ_Y, i, j, k, -wz);
+  const size_t sxy = (size_t)g.ny * g.nz;
+  const T *Hx = adj + 3 * g.vs, *Hy = adj + 4 * g.vs, *Hz = adj + 5 * g.vs;
+  const T ex = adj[idx], ey = adj[g.vs + idx], ez = adj[2 * g.vs + idx];
+  T ax, ay, az;
+  const int n = g.npml;
+  if (n > 0 && within(i, j, k, n + 1, g.nx - n - 1, g.ny - n - 1, g.nz - n - 1)) {
+    const T hx = Hx[idx], hy = Hy[idx], hz = Hz[idx];
+    ax = dtm_dl * ((hy - Hy[idx - 1]) - (hz - Hz[idx - g.nz]));
+    ay = dtm_dl * ((hz - Hz[idx - sxy]) - (hx - Hx[idx - 1]));
+    az = dtm_dl * ((hx - Hx[idx - g.nz]) - (hy - Hy[idx - sxy]));
+  } else {
+    if (!is_interior(g, i, j, k)) {
+      // E terms: Ex (+ y, - z), Ey (+ z, - x), Ez (+ x, - y)
+      settle(g, prof, apsi, 0, 1, PSI_EX_Y, i, j, k, ex);
+      settle(g, prof, apsi, 0, 2, PSI_EX_Z, i, j, k, -ex);
+      settle(g, prof, apsi, 0, 2, PSI_EY_Z, i, j, k, ey);
+      settle(g, prof, apsi, 0, 0, PSI_EY_X, i, j, k, -ey);
+      settle(g, prof, apsi, 0, 0, PSI_EZ_X, i, j, k, ez);
+      settle(g, prof, apsi, 0, 1, PSI_EZ_Y, i, j, k, -ez);
+    }
+    const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
+    const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
+    const T P = T(1), M = T(-1);
+    ax = ay = az = T(0);
+    // Hy (+ z: Ex), Hz (- y: Ex)
+    if (k0) ax += u_h(g, adj, prof, apsi, 1, 2, PSI_HY_Z, P, i, j, k - 1, dtm, dtm_dl);
+    if (k1) ax -= u_h(g, adj, prof, apsi, 1, 2, PSI_HY_Z, P, i, j, k, dtm, dtm_dl);
+    if (j0) ax += u_h(g, adj, prof, apsi, 2, 1, PSI_HZ_Y, M, i, j - 1, k, dtm, dtm_dl);
+    if (j1) ax -= u_h(g, adj, prof, apsi, 2, 1, PSI_HZ_Y, M, i, j, k, dtm, dtm_dl);
+    // Hx (- z: Ey), Hz (+ x: Ey)
+    if (k0) ay += u_h(g, adj, prof, apsi, 0, 2, PSI_HX_Z, M, i, j, k - 1, dtm, dtm_dl);
+    if (k1) ay -= u_h(g, adj, prof, apsi, 0, 2, PSI_HX_Z, M, i, j, k, dtm, dtm_dl);
+    if (i0) ay += u_h(g, adj, prof, apsi, 2, 0, PSI_HZ_X, P, i - 1, j, k, dtm, dtm_dl);
+    if (i1) ay -= u_h(g, adj, prof, apsi, 2, 0, PSI_HZ_X, P, i, j, k, dtm, dtm_dl);
+    // Hx (+ y: Ez), Hy (- x: Ez)
+    if (j0) az += u_h(g, adj, prof, apsi, 0, 1, PSI_HX_Y, P, i, j - 1, k, dtm, dtm_dl);
+    if (j1) az -= u_h(g, adj, prof, apsi, 0, 1, PSI_HX_Y, P, i, j, k, dtm, dtm_dl);
+    if (i0) az += u_h(g, adj, prof, apsi, 1, 0, PSI_HY_X, M, i - 1, j, k, dtm, dtm_dl);
+    if (i1) az -= u_h(g, adj, prof, apsi, 1, 0, PSI_HY_X, M, i, j, k, dtm, dtm_dl);
   }
-  const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
-  const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
-  const T P = T(1), M = T(-1);
-  // Hy (+ z: Ex), Hz (- y: Ex)
-  if (k0) ex += u_h(g, adj, prof, apsi, 1, 2, PSI_HY_Z, P, i, j, k - 1, dtm, dtm_dl);
-  if (k1) ex -= u_h(g, adj, prof, apsi, 1, 2, PSI_HY_Z, P, i, j, k, dtm, dtm_dl);
-  if (j0) ex += u_h(g, adj, prof, apsi, 2, 1, PSI_HZ_Y, M, i, j - 1, k, dtm, dtm_dl);
-  if (j1) ex -= u_h(g, adj, prof, apsi, 2, 1, PSI_HZ_Y, M, i, j, k, dtm, dtm_dl);
-  // Hx (- z: Ey), Hz (+ x: Ey)
-  if (k0) ey += u_h(g, adj, prof, apsi, 0, 2, PSI_HX_Z, M, i, j, k - 1, dtm, dtm_dl);
-  if (k1) ey -= u_h(g, adj, prof, apsi, 0, 2, PSI_HX_Z, M, i, j, k, dtm, dtm_dl);
-  if (i0) ey += u_h(g, adj, prof, apsi, 2, 0, PSI_HZ_X, P, i - 1, j, k, dtm, dtm_dl);
-  if (i1) ey -= u_h(g, adj, prof, apsi, 2, 0, PSI_HZ_X, P, i, j, k, dtm, dtm_dl);
-  // Hx (+ y: Ez), Hy (- x: Ez)
-  if (j0) ez += u_h(g, adj, prof, apsi, 0, 1, PSI_HX_Y, P, i, j - 1, k, dtm, dtm_dl);
-  if (j1) ez -= u_h(g, adj, prof, apsi, 0, 1, PSI_HX_Y, P, i, j, k, dtm, dtm_dl);
-  if (i0) ez += u_h(g, adj, prof, apsi, 1, 0, PSI_HY_X, M, i - 1, j, k, dtm, dtm_dl);
-  if (i1) ez -= u_h(g, adj, prof, apsi, 1, 0, PSI_HY_X, M, i, j, k, dtm, dtm_dl);
-  adj[idx] = ex;
-  adj[g.vs + idx] = ey;
-  adj[2 * g.vs + idx] = ez;
+  adj[idx] = ex + coef[idx] * ax;
+  adj[g.vs + idx] = ey + coef[g.vs + idx] * ay;
+  adj[2 * g.vs + idx] = ez + coef[2 * g.vs + idx] * az;
 }
 
 __device__ __forceinline__ size_t box_cell(const Geo &g, const Box &b, long long q) {
@@ -228,17 +260,19 @@ __global__ void k_adj_record(Geo g, Box b, const T *__restrict__ st, T *__restri
   hist[2 * b.n + q] = st[2 * g.vs + idx];
 }
 
-// G_c += E~_c (E^n_c - E^{n-1}_c) over the box, in FP64
+// G_c += E~_c (E^n_c - E^{n-1}_c) over the box, in FP64 (adj holds ce E~)
 template <typename T>
-__global__ void k_adj_grad(Geo g, Box b, const T *__restrict__ adj, const T *__restrict__ h1,
-                           const T *__restrict__ h0, double *__restrict__ G) {
+__global__ void k_adj_grad(Geo g, Box b, const T *__restrict__ adj, const T *__restrict__ coef,
+                           const T *__restrict__ h1, const T *__restrict__ h0,
+                           double *__restrict__ G) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= b.n) return;
   const size_t idx = box_cell(g, b, q);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const long long o = c * b.n + q;
-    G[o] += (double)adj[c * g.vs + idx] * ((double)h1[o] - (double)h0[o]);
+    const double eb = (double)adj[c * g.vs + idx] / (double)coef[c * g.vs + idx];
+    G[o] += eb * ((double)h1[o] - (double)h0[o]);
   }
 }
 
@@ -259,12 +293,12 @@ __global__ void k_adj_probe_dft(Geo g, int np, int nf, const int *__restrict__ c
 }
 
 // E~_p += sum_f Re(coef_pf e^{j w_f t}) dt, one thread per probe (probes are
-// distinct cells, checked on the host)
+// distinct cells, checked on the host); adj holds ce E~
 template <typename T>
 __global__ void k_adj_inject(Geo g, int np, int nf, const int *__restrict__ comp,
                              const long long *__restrict__ where, const double *__restrict__ om,
                              const double *__restrict__ coef, double t, double dt,
-                             T *__restrict__ adj) {
+                             const T *__restrict__ ce, T *__restrict__ adj) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   double acc = 0.0;
@@ -273,7 +307,8 @@ __global__ void k_adj_inject(Geo g, int np, int nf, const int *__restrict__ comp
     sincos(om[f] * t, &s, &c);
     acc += coef[2 * (p * nf + f)] * c - coef[2 * (p * nf + f) + 1] * s;
   }
-  adj[comp[p] * g.vs + where[p]] += T(acc * dt);
+  const size_t at = comp[p] * g.vs + where[p];
+  adj[at] += T((double)ce[at] * (acc * dt));
 }
 
 template <typename T>
@@ -385,7 +420,7 @@ int inject_t(hfdtd_adjoint *a, long long n) {
   Engine *e = a->e;
   hipLaunchKernelGGL(k_adj_inject<T>, dim3(ceil_div(a->np, 64)), dim3(64), 0, e->stream,
                      geo_of(e), a->np, a->nf, a->pcomp, a->pidx, a->om, a->coef,
-                     (double)n * e->dt, e->dt, (T *)a->adj);
+                     (double)n * e->dt, e->dt, (const T *)e->coef, (T *)a->adj);
   HF_HIP(hipGetLastError());
   return HFDTD_OK;
 }
@@ -398,12 +433,11 @@ int reverse_step_t(hfdtd_adjoint *a, long long n, int slot) {
   const T *h1 = (const T *)a->hist + (size_t)slot * hs;
   const T *h0 = (const T *)a->hist + (size_t)(slot - 1) * hs;
   hipLaunchKernelGGL(k_adj_grad<T>, dim3(ceil_div(a->box.n, 256)), dim3(256), 0, e->stream, g,
-                     a->box, (const T *)a->adj, h1, h0, a->G);
+                     a->box, (const T *)a->adj, (const T *)e->coef, h1, h0, a->G);
   HF_HIP(hipGetLastError());
   const dim3 grid(ceil_div(e->nz, 64), ceil_div(e->ny, kAdjRows), e->nx);
   hipLaunchKernelGGL(k_adj_pass1<T>, grid, dim3(64, kAdjRows), 0, e->stream, g, T(e->inv_dl),
-                     T(e->dtm), (T *)a->adj, (const T *)e->coef, (const T *)e->prof,
-                     (T *)a->apsi);
+                     T(e->dtm), (T *)a->adj, (const T *)e->prof, (T *)a->apsi);
   HF_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_adj_pass2<T>, grid, dim3(64, kAdjRows), 0, e->stream, g, T(e->dtm),
                      T(e->dtm_dl), (T *)a->adj, (const T *)e->coef, (const T *)e->prof,

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -61,8 +62,9 @@ struct Engine {
 
   bool lossless = true;  // ca == 1 everywhere (set_epsilon without sigma)
   int tile_rows = 8;     // fused sweep: j rows per block
-  int tile_planes = 64;  // fused sweep: i planes per block
+  int tile_planes = 0;   // fused sweep: i planes per block (0 = chosen from the grid)
   int xcd = 8;           // XCDs for the work-order remap (1 = hardware order)
+  int cus = 256;         // compute units of the device
 
   uint64_t bytes = 0;
 
@@ -83,6 +85,30 @@ struct Engine {
     return 6.0 * (double)vstride * elem < lim && ((double)psi_off[PSI_COUNT] + 64.0) * elem < lim;
   }
   size_t prof_len() const { return (size_t)nx + ny + nz; }
+  // Resident fused blocks per CU, by rows (index) and lossy (0/1); 0 = not
+  // yet queried.
+  int fused_occ[2][9] = {};
+  // i planes per fused block: the set value, or the count P in [8, 64] that
+  // minimises the modelled sweep time (waves of resident blocks) x (P + 2):
+  // each wave lasts as long as its blocks' P planes plus the two halo planes
+  // every block re-reads; slots = resident blocks per CU x CUs.  Ties go to
+  // more planes per block.
+  int fused_planes(int rows, int per_cu) const {
+    if (tile_planes > 0) return tile_planes;
+    const long long slots = (long long)(per_cu > 0 ? per_cu : 1) * cus;
+    const long long tiles = (long long)((nz + 63) / 64) * ((ny + rows - 1) / rows);
+    int best = 64;
+    long long best_t = -1;
+    for (int P = 64; P >= 8; --P) {
+      const long long blocks = tiles * ((nx + P - 1) / P);
+      const long long t = (blocks + slots - 1) / slots * (P + 2);
+      if (best_t < 0 || t < best_t) {
+        best_t = t;
+        best = P;
+      }
+    }
+    return best;
+  }
 };
 
 // Record the next timing event on the engine stream (no-op when disabled).

@@ -306,6 +306,12 @@ int hfdtd_create(int nx, int ny, int nz, int npml, double dl, double dt, int fp6
   if (err != hipSuccess) return bail(hip_fail(err, "hipSetDevice"));
   err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (err != hipSuccess) return bail(hip_fail(err, "hipStreamCreateWithFlags"));
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus > 0)
+      e->cus = cus;
+  }
 
   if ((rc = dmalloc(e, &e->state[0], 6 * e->vstride * e->elem))) return bail(rc);
   if ((rc = dmalloc(e, &e->coef, 6 * e->vstride * e->elem))) return bail(rc);
@@ -535,11 +541,11 @@ int hfdtd_timing_read(hfdtd_engine *h, double *total_ms, int *launches) {
 int hfdtd_set_fused_tiling(hfdtd_engine *h, int rows_j, int planes_i, int xcd_groups) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
   if ((rows_j != 0 && rows_j != 1 && rows_j != 2 && rows_j != 4 && rows_j != 8 &&
-       rows_j != 6) || planes_i < 0)
-    return fail(HFDTD_EINVAL, "rows_j must be 1, 2, 4, 6 or 8 and planes_i >= 0");
+       rows_j != 6) || planes_i < -1)
+    return fail(HFDTD_EINVAL, "rows_j must be 1, 2, 4, 6 or 8 and planes_i >= -1");
   if (xcd_groups < 0 || xcd_groups > 64) return fail(HFDTD_EINVAL, "xcd_groups must be 0..64");
   if (rows_j) h->s.tile_rows = rows_j;
-  if (planes_i) h->s.tile_planes = planes_i;
+  if (planes_i) h->s.tile_planes = planes_i < 0 ? 0 : planes_i;
   if (xcd_groups) h->s.xcd = xcd_groups;
   return HFDTD_OK;
 }

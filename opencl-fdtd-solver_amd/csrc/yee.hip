@@ -200,10 +200,19 @@ static int sources_t(Engine *e, const hfdtd_source *src, int n) {
 
 template <typename T, int TJ>
 static int fused_tj(Engine *e, const SrcList<T> &sl) {
-  const int planes = e->tile_planes < 1 ? 1 : e->tile_planes;
+  auto kern = e->lossless ? k_fused<T, TJ, false> : k_fused<T, TJ, true>;
+  int &occ = e->fused_occ[e->lossless ? 0 : 1][TJ];
+  if (occ == 0) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * (TJ + 1), 0) !=
+            hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    occ = per_cu;
+  }
+  const int planes = e->fused_planes(TJ, occ);
   const int tiles_k = (int)ceil_div(e->nz, 64), tiles_j = (int)ceil_div(e->ny, TJ);
   const dim3 grid(tiles_k * tiles_j * ceil_div(e->nx, planes));
-  auto kern = e->lossless ? k_fused<T, TJ, false> : k_fused<T, TJ, true>;
   int rc = timing_mark(e);
   if (rc) return rc;
   const int o = e->cur ^ 1;

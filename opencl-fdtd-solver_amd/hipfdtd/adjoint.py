@@ -269,11 +269,14 @@ class ProbeDFTAdjoint:
         return int(v.value) + self.fdtd.device_memory_bytes()
 
     def read_adjoint_volume(self, name: str) -> np.ndarray:
-        """Ẽx..H̃z of the reverse sweep as a volume (for tests)."""
+        """Ẽx..H̃z of the reverse sweep as a volume (for tests).  The library
+        stores the E adjoint scaled by ce; it is divided out here."""
         which = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz").index(name)
         out = np.empty(self.shape, dtype=self.fdtd.dtype)
         _lib.check(self._lib.hfdtd_adjoint_read_volume(self._handle, which,
                                                        out.ctypes.data_as(C.c_void_p)))  # fmt: skip
+        if which < 3:
+            return out.astype(np.float64) / self.fdtd.read_volume("ce_" + "xyz"[which])
         return out
 
     def close(self) -> None:

@@ -320,7 +320,8 @@ class HIPFDTD(SourceMonitorMixin):
 
     def set_fused_tiling(self, rows_j: int = 0, planes_i: int = 0, xcd_groups: int = 0) -> None:
         """Launch shape of the fused sweep: rows of j and planes of i per block,
-        and the XCD count the work order is grouped for (0 keeps a value)."""
+        and the XCD count the work order is grouped for (0 keeps a value;
+        planes_i=-1 returns to the grid-derived plane count)."""
         _lib.check(self._lib.hfdtd_set_fused_tiling(self._handle, int(rows_j), int(planes_i),
                                                     int(xcd_groups)))  # fmt: skip
 
