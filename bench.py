@@ -30,6 +30,12 @@ over the forward run's (recomputation of the checkpointed segments plus the
 transposed steps) and checks the FP32 gradient on 16 random design cells
 against central differences (δε = 1e-3) of FP64 runs.
 
+``--workload batch`` measures 64 simulations of uneven sizes (sides
+U{48..160}, seed 4, npml 8, a centred dielectric sphere, a centred Jz dipole
+and a 4-frequency Ez probe each, 1200 steps) advanced together by the
+batched persistent sweep (schedule picked by the autotuner, HIP-graph
+replays), against the same 64 runs one after another on the plain engine.
+
 ``--workload plane-dft`` (not the default line) measures the 512^3 plane-DFT
 configuration: smooth Gaussian random εr (seed 3, σ = 8 cells, clipped to
 [1, 6]), three orthogonal point dipoles with staggered pulses, npml 16, and a
@@ -42,6 +48,7 @@ Usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 20] [--no-cpu]
        python bench.py --copy-calibration
        python bench.py --workload plane-dft [--steps 1500]
        python bench.py --workload adjoint [--steps 3000] [--repeat 2]
+       python bench.py --workload batch [--steps 1200] [--graph-steps 100]
 """
 
 from __future__ import annotations
@@ -325,6 +332,124 @@ def run_adjoint(args):
     }))  # fmt: skip
 
 
+def c4_sims(n=64, seed=4, dl=1e-3, npml=8):
+    """64 uneven grids: sides U{48..160}, a centred sphere (radius U[5, 30]
+    cells, clipped to fit; εr U[1.5, 8]), a centred Jz dipole at
+    f ~ U(c0/(30 dl), c0/(12 dl)), an Ez probe beside the sphere, 4 DFT
+    frequencies around f."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        shape = tuple(int(v) for v in rng.integers(48, 161, 3))
+        r = float(rng.uniform(5.0, 30.0))
+        er = float(rng.uniform(1.5, 8.0))
+        f = float(rng.uniform(C0 / (30 * dl), C0 / (12 * dl)))
+        c = [v // 2 for v in shape]
+        r = min(r, min(shape) // 2 - npml - 2)
+        pi = min(c[0] + int(r) + 3, shape[0] - npml - 2)
+        out.append({"shape": shape, "r": r, "er": er, "f": f, "centre": c,
+                    "probe": (pi, c[1], c[2]), "freqs": list(f * np.array([0.8, 14 / 15, 16 / 15, 1.2]))})  # fmt: skip
+    return out, dl, npml
+
+
+def c4_eps(sim):
+    nx, ny, nz = sim["shape"]
+    I, J, K = np.ogrid[:nx, :ny, :nz]
+    c = sim["centre"]
+    eps = np.ones(sim["shape"], dtype=np.float32)
+    eps[(I - c[0]) ** 2 + (J - c[1]) ** 2 + (K - c[2]) ** 2 < sim["r"] ** 2] = sim["er"]
+    return eps
+
+
+def run_batch(args):
+    """64 uneven grids: one persistent batched launch per step (graph-replayed)
+    against the same 64 runs one after another on the plain engine."""
+    import hipfdtd
+    from hipfdtd.autotune import tune_batch
+    from hipfdtd.batch import BatchedFDTD
+    from hipfdtd.probes import HIPPointProbe
+
+    cfg, dl, npml = c4_sims()
+    steps = args.steps
+    cells = float(sum(np.prod(c["shape"]) for c in cfg))
+    psi = float(sum(4 * 2 * npml * (c["shape"][1] * c["shape"][2] + c["shape"][0] * c["shape"][2]
+                                     + c["shape"][0] * c["shape"][1]) for c in cfg))  # fmt: skip
+    epss = [c4_eps(c) for c in cfg]
+
+    def wave(c):
+        f = c["f"]
+        return lambda t: gaussian_sine(t, f, 0.5 * f)
+
+    def box(c):
+        x, y, z = c["centre"]
+        return (x, x + 1, y, y + 1, z, z + 1)
+
+    def serial_run(fused):
+        """The plain engine, one grid after another, source and probe as callbacks."""
+        total, dfts, dev = 0.0, [], ""
+        for c, eps in zip(cfg, epss):
+            sim = hipfdtd.HIPFDTD(c["shape"], dl, npml=npml, fused=fused)
+            dev = sim.device_name
+            sim.set_epsilon(eps)
+            w, bx = wave(c), box(c)
+            sim.add_source(lambda f, w=w, bx=bx: f.add_source_J("z", w(f.t), bx))
+            pr = HIPPointProbe(sim, c["probe"], "Ez", c["freqs"])
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.run(steps)
+            sim.synchronize()
+            total += time.perf_counter() - t0
+            dfts.append(pr.dft())
+            sim.close()
+        return total, dfts, dev
+
+    t_serial, serial, dev = serial_run(True)
+    t_split, _, _ = serial_run(False)
+    print(f"[batch] serial {t_serial:.2f} s (split sweeps {t_split:.2f} s)", file=sys.stderr,
+          flush=True)  # fmt: skip
+
+    sims = []
+    for c, eps in zip(cfg, epss):
+        sim = hipfdtd.HIPFDTD(c["shape"], dl, npml=npml)
+        sim.set_epsilon(eps)
+        sims.append(sim)
+    batch = BatchedFDTD(sims)
+    tuned = tune_batch(batch, use_cache=False, write_cache=False)
+    batch.set_sources([("z", box(c)) for c in cfg], [wave(c) for c in cfg], steps)
+    batch.set_probes([c["probe"] for c in cfg], "Ez", [c["freqs"] for c in cfg], steps)
+    sims[0].synchronize()
+    t0 = time.perf_counter()
+    batch.run(steps, graph_steps=args.graph_steps)
+    t_batch = time.perf_counter() - t0
+    got = batch.probe_dfts()
+    ref = np.array(serial)
+    err = float(np.max(np.abs(got - ref) / np.max(np.abs(ref), axis=1, keepdims=True)))
+    batch.close()
+    for s in sims:
+        s.close()
+    b_alg = 72.0 * cells + 8.0 * psi
+    step_s = t_batch / steps
+    print(json.dumps({
+        "metric": "batched/serial throughput", "value": round(t_serial / t_batch, 2), "unit": "x",
+        "higher_is_better": True, "steps": steps, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "64 grids, sides U{48..160} (seed 4), npml 8, centred sphere "
+                               "r U[5,30] eps_r U[1.5,8], centred Jz dipole f ~ U(c0/30dl, "
+                               "c0/12dl), Ez probe, 4-frequency DFT",
+                   "total_cells": int(cells), "graph_steps": args.graph_steps},
+        "batch_mcell_per_s": round(cells * steps / t_batch / 1e6, 1),
+        "serial_mcell_per_s": round(cells * steps / t_serial / 1e6, 1),
+        "batch_s": round(t_batch, 3), "serial_s": round(t_serial, 3),
+        "serial_split_sweeps_s": round(t_split, 3),
+        "batched_vs_serial_split_sweeps": round(t_split / t_batch, 2),
+        "batch_ms_per_step": round(step_s * 1e3, 3),
+        "batch_hbm_gbs_alg": round(b_alg / step_s / 1e9, 1),
+        "batch_hbm_frac": round(b_alg / step_s / 1e9 / HBM_PEAK_GBS, 4),
+        "probe_dft_max_rel_err_vs_serial": err,
+        "autotune": {k: tuned[k] for k in ("blocks", "rows", "planes", "ms")},
+        "device": dev,
+    }))  # fmt: skip
+
+
 TRAFFIC_FILE = ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt"
 
 
@@ -360,7 +485,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", default="yee-256", choices=["yee-256", "plane-dft", "adjoint"])
+    ap.add_argument("--workload", default="yee-256",
+                    choices=["yee-256", "plane-dft", "adjoint", "batch"])
+    ap.add_argument("--graph-steps", type=int, default=100, help="batch: steps per HIP graph")
     ap.add_argument("--repeat", type=int, default=2, help="adjoint: timed forward+backward runs")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -370,6 +497,11 @@ def main():
         if args.steps == 2000:
             args.steps = 1500
         run_plane_dft(args)
+        return
+    if args.workload == "batch":
+        if args.steps == 2000:
+            args.steps = 1200
+        run_batch(args)
         return
     if args.workload == "adjoint":
         if args.steps == 2000:

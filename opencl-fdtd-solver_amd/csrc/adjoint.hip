@@ -23,6 +23,8 @@
 // The stored E adjoint is scaled, ce E~ (with E~ the adjoint above), which
 // drops ce from pass 1; in the interior the two passes are then the forward
 // H and E stencils with -inv_dl for dtm_dl and -ce dtm_dl for ce inv_dl.
+// Every cell uses the kappa-scaled form (ik = 1/dl and c = 0 off the PML),
+// so a wave runs one code path whether or not it touches a face stripe.
 //
 // A thread may update psi~ only where no other thread of the same pass reads
 // it, so each psi~ update is deferred to the pass that does not read it:
@@ -76,39 +78,6 @@ __device__ __forceinline__ size_t cell(const Geo &g, int i, int j, int k) {
   return ((size_t)i * g.ny + j) * g.nz + k;
 }
 
-// ik w + c g of E term (component e, derivative axis a, psi slot, sign s) at
-// E cell (i,j,k), w = s ce E~ (the stored scaled adjoint): the coefficient
-// its E~ hands to the H~ of the cells whose backward difference it enters.
-// Interior cells: s inv_dl ce E~.
-template <typename T>
-__device__ __forceinline__ T u_e(const Geo &g, const T *__restrict__ adj,
-                                 const T *__restrict__ prof, const T *__restrict__ apsi, int e,
-                                 int a, int slot, T s, int i, int j, int k, T inv_dl) {
-  const T w = s * adj[e * g.vs + cell(g, i, j, k)];
-  if (is_interior(g, i, j, k)) return w * inv_dl;
-  const int x = coord_of(a, i, j, k);
-  T u = prof_at(g, prof, 0, 2, a, x) * w;
-  if (in_stripe(g, a, x))
-    u += prof_at(g, prof, 0, 1, a, x) * (apsi[g.psi_off[slot] + stripe_off(g, a, i, j, k)] + w);
-  return u;
-}
-
-// ik w + c g of H term (component h, axis a, psi slot, sign s) at H cell
-// (i,j,k), with w = -s dtm H~.  Interior cells: -s dtm_dl H~.
-template <typename T>
-__device__ __forceinline__ T u_h(const Geo &g, const T *__restrict__ adj,
-                                 const T *__restrict__ prof, const T *__restrict__ apsi, int h,
-                                 int a, int slot, T s, int i, int j, int k, T dtm, T dtm_dl) {
-  const T hb = adj[(3 + h) * g.vs + cell(g, i, j, k)];
-  if (is_interior(g, i, j, k)) return -s * dtm_dl * hb;
-  const T w = -s * dtm * hb;
-  const int x = coord_of(a, i, j, k);
-  T u = prof_at(g, prof, 1, 2, a, x) * w;
-  if (in_stripe(g, a, x))
-    u += prof_at(g, prof, 1, 1, a, x) * (apsi[g.psi_off[slot] + stripe_off(g, a, i, j, k)] + w);
-  return u;
-}
-
 // psi~ <- b (psi~ + w) at the cell's own stripe entry (stagger st)
 template <typename T>
 __device__ __forceinline__ void settle(const Geo &g, const T *__restrict__ prof,
@@ -120,18 +89,48 @@ __device__ __forceinline__ void settle(const Geo &g, const T *__restrict__ prof,
   *p = prof_at(g, prof, st, 0, a, x) * (*p + w);
 }
 
-// lo <= c <= hi on all three axes
-__device__ __forceinline__ bool within(int i, int j, int k, int lo, int hx, int hy, int hz) {
-  return i >= lo && j >= lo && k >= lo && i <= hx && j <= hy && k <= hz;
+// Profile values of one stagger along one axis at one index: s = ik + c and
+// c (c = 0 off the PML, where ik = 1/dl), and whether the index lies in the
+// axis' face stripes.
+template <typename T>
+struct AxisP {
+  T s, c;
+  bool in;
+};
+
+template <typename T>
+__device__ __forceinline__ AxisP<T> axis_p(const Geo &g, const T *__restrict__ prof, int st,
+                                           int a, int x) {
+  const int N = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
+  AxisP<T> p;
+  p.s = T(0);
+  p.c = T(0);
+  p.in = false;
+  if (x < 0 || x >= N) return p;
+  p.c = prof_at(g, prof, st, 1, a, x);
+  p.s = prof_at(g, prof, st, 2, a, x) + p.c;
+  p.in = in_stripe(g, a, x);
+  return p;
+}
+
+// u = ik w + c (psi~ + w) = w (ik + c) + c psi~ of one curl term at cell
+// (i,j,k), psi~ read only on the term's stripe.
+template <typename T>
+__device__ __forceinline__ T term_u(const Geo &g, const T *__restrict__ apsi, int slot, int a,
+                                    const AxisP<T> &p, T w, int i, int j, int k) {
+  T u = w * p.s;
+  if (p.in) u += p.c * apsi[g.psi_off[slot] + stripe_off(g, a, i, j, k)];
+  return u;
 }
 
 // Pass 1: settle psi~_H of the previous reverse step, then the transposed E
 // half into H~.  adj = [ce E~x, ce E~y, ce E~z, H~x, H~y, H~z], psi~ slab in
-// PsiSlot order.  Where the cell and its +x/+y/+z neighbours are interior
-// the update is the forward H stencil with -inv_dl for dtm_dl.
+// PsiSlot order.  For E term (component e, axis a, sign s) with w = s ce E~
+// at cell c, u(c) = w (ik + c) + c psi~ and H~ += [m_a >= 1] u(m) -
+// [m_a <= N-2] u(m + e_a).  In the interior (c = 0, ik = 1/dl) this is the
+// forward H stencil with -inv_dl in place of dtm_dl.
 template <typename T>
-__global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass1(Geo g, T inv_dl, T dtm,
-                                                             T *__restrict__ adj,
+__global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass1(Geo g, T dtm, T *__restrict__ adj,
                                                              const T *__restrict__ prof,
                                                              T *__restrict__ apsi) {
   const int k = blockIdx.x * 64 + threadIdx.x;
@@ -142,53 +141,49 @@ __global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass1(Geo g, T inv_dl, T 
   const size_t sxy = (size_t)g.ny * g.nz;
   const T *Ex = adj, *Ey = adj + g.vs, *Ez = adj + 2 * g.vs;
   T hx = adj[3 * g.vs + idx], hy = adj[4 * g.vs + idx], hz = adj[5 * g.vs + idx];
-  const int n = g.npml;
-  if (n > 0 && within(i, j, k, n, g.nx - n - 2, g.ny - n - 2, g.nz - n - 2)) {
-    const T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
-    hx += inv_dl * ((ey - Ey[idx + 1]) - (ez - Ez[idx + g.nz]));
-    hy += inv_dl * ((ez - Ez[idx + sxy]) - (ex - Ex[idx + 1]));
-    hz += inv_dl * ((ex - Ex[idx + g.nz]) - (ey - Ey[idx + sxy]));
-  } else {
-    if (!is_interior(g, i, j, k)) {
-      // H terms: Hx (+ y, - z), Hy (+ z, - x), Hz (+ x, - y)
-      settle(g, prof, apsi, 1, 1, PSI_HX_Y, i, j, k, -dtm * hx);
-      settle(g, prof, apsi, 1, 2, PSI_HX_Z, i, j, k, dtm * hx);
-      settle(g, prof, apsi, 1, 2, PSI_HY_Z, i, j, k, -dtm * hy);
-      settle(g, prof, apsi, 1, 0, PSI_HY_X, i, j, k, dtm * hy);
-      settle(g, prof, apsi, 1, 0, PSI_HZ_X, i, j, k, -dtm * hz);
-      settle(g, prof, apsi, 1, 1, PSI_HZ_Y, i, j, k, dtm * hz);
-    }
-    const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
-    const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
-    const T P = T(1), M = T(-1);
-    // Ey (+ z: Hx), Ez (- y: Hx)
-    if (k0) hx += u_e(g, adj, prof, apsi, 1, 2, PSI_EY_Z, P, i, j, k, inv_dl);
-    if (k1) hx -= u_e(g, adj, prof, apsi, 1, 2, PSI_EY_Z, P, i, j, k + 1, inv_dl);
-    if (j0) hx += u_e(g, adj, prof, apsi, 2, 1, PSI_EZ_Y, M, i, j, k, inv_dl);
-    if (j1) hx -= u_e(g, adj, prof, apsi, 2, 1, PSI_EZ_Y, M, i, j + 1, k, inv_dl);
-    // Ex (- z: Hy), Ez (+ x: Hy)
-    if (k0) hy += u_e(g, adj, prof, apsi, 0, 2, PSI_EX_Z, M, i, j, k, inv_dl);
-    if (k1) hy -= u_e(g, adj, prof, apsi, 0, 2, PSI_EX_Z, M, i, j, k + 1, inv_dl);
-    if (i0) hy += u_e(g, adj, prof, apsi, 2, 0, PSI_EZ_X, P, i, j, k, inv_dl);
-    if (i1) hy -= u_e(g, adj, prof, apsi, 2, 0, PSI_EZ_X, P, i + 1, j, k, inv_dl);
-    // Ex (+ y: Hz), Ey (- x: Hz)
-    if (j0) hz += u_e(g, adj, prof, apsi, 0, 1, PSI_EX_Y, P, i, j, k, inv_dl);
-    if (j1) hz -= u_e(g, adj, prof, apsi, 0, 1, PSI_EX_Y, P, i, j + 1, k, inv_dl);
-    if (i0) hz += u_e(g, adj, prof, apsi, 1, 0, PSI_EY_X, M, i, j, k, inv_dl);
-    if (i1) hz -= u_e(g, adj, prof, apsi, 1, 0, PSI_EY_X, M, i + 1, j, k, inv_dl);
-  }
+  // H terms: Hx (+ y, - z), Hy (+ z, - x), Hz (+ x, - y); w = -s dtm H~
+  settle(g, prof, apsi, 1, 1, PSI_HX_Y, i, j, k, -dtm * hx);
+  settle(g, prof, apsi, 1, 2, PSI_HX_Z, i, j, k, dtm * hx);
+  settle(g, prof, apsi, 1, 2, PSI_HY_Z, i, j, k, -dtm * hy);
+  settle(g, prof, apsi, 1, 0, PSI_HY_X, i, j, k, dtm * hy);
+  settle(g, prof, apsi, 1, 0, PSI_HZ_X, i, j, k, -dtm * hz);
+  settle(g, prof, apsi, 1, 1, PSI_HZ_Y, i, j, k, dtm * hz);
+  const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
+  const AxisP<T> x0 = axis_p(g, prof, 0, 0, i), x1 = axis_p(g, prof, 0, 0, i + 1);
+  const AxisP<T> y0 = axis_p(g, prof, 0, 1, j), y1 = axis_p(g, prof, 0, 1, j + 1);
+  const AxisP<T> z0 = axis_p(g, prof, 0, 2, k), z1 = axis_p(g, prof, 0, 2, k + 1);
+  const T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
+  const T ex_j1 = j1 ? Ex[idx + g.nz] : T(0), ex_k1 = k1 ? Ex[idx + 1] : T(0);
+  const T ey_i1 = i1 ? Ey[idx + sxy] : T(0), ey_k1 = k1 ? Ey[idx + 1] : T(0);
+  const T ez_i1 = i1 ? Ez[idx + sxy] : T(0), ez_j1 = j1 ? Ez[idx + g.nz] : T(0);
+  const T Z = T(0);
+  // Hx: Ey (+ z), Ez (- y)
+  hx += (k >= 1 ? term_u(g, apsi, PSI_EY_Z, 2, z0, ey, i, j, k) : Z) -
+        (k1 ? term_u(g, apsi, PSI_EY_Z, 2, z1, ey_k1, i, j, k + 1) : Z) +
+        (j >= 1 ? term_u(g, apsi, PSI_EZ_Y, 1, y0, -ez, i, j, k) : Z) -
+        (j1 ? term_u(g, apsi, PSI_EZ_Y, 1, y1, -ez_j1, i, j + 1, k) : Z);
+  // Hy: Ex (- z), Ez (+ x)
+  hy += (k >= 1 ? term_u(g, apsi, PSI_EX_Z, 2, z0, -ex, i, j, k) : Z) -
+        (k1 ? term_u(g, apsi, PSI_EX_Z, 2, z1, -ex_k1, i, j, k + 1) : Z) +
+        (i >= 1 ? term_u(g, apsi, PSI_EZ_X, 0, x0, ez, i, j, k) : Z) -
+        (i1 ? term_u(g, apsi, PSI_EZ_X, 0, x1, ez_i1, i + 1, j, k) : Z);
+  // Hz: Ex (+ y), Ey (- x)
+  hz += (j >= 1 ? term_u(g, apsi, PSI_EX_Y, 1, y0, ex, i, j, k) : Z) -
+        (j1 ? term_u(g, apsi, PSI_EX_Y, 1, y1, ex_j1, i, j + 1, k) : Z) +
+        (i >= 1 ? term_u(g, apsi, PSI_EY_X, 0, x0, -ey, i, j, k) : Z) -
+        (i1 ? term_u(g, apsi, PSI_EY_X, 0, x1, -ey_i1, i + 1, j, k) : Z);
   adj[3 * g.vs + idx] = hx;
   adj[4 * g.vs + idx] = hy;
   adj[5 * g.vs + idx] = hz;
 }
 
 // Pass 2: settle psi~_E of this cell from the E~ about to be replaced, then
-// the transposed H half into E~ (lossless: ca = 1).  Where the cell and its
-// -x/-y/-z neighbours are interior this is the forward E stencil with
-// -ce dtm_dl for ce inv_dl.
+// the transposed H half into E~ (lossless: ca = 1).  For H term (component
+// h, axis a, sign s) with w = -s dtm H~ at cell c, u(c) = w (ik + c) + c psi~
+// and E~ += ce ([m_a >= 1] u(m - e_a) - [m_a <= N-2] u(m)).  In the interior
+// this is the forward E stencil with -ce dtm_dl in place of ce inv_dl.
 template <typename T>
-__global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass2(Geo g, T dtm, T dtm_dl,
-                                                             T *__restrict__ adj,
+__global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass2(Geo g, T dtm, T *__restrict__ adj,
                                                              const T *__restrict__ coef,
                                                              const T *__restrict__ prof,
                                                              T *__restrict__ apsi) {
@@ -200,43 +195,38 @@ __global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass2(Geo g, T dtm, T dtm
   const size_t sxy = (size_t)g.ny * g.nz;
   const T *Hx = adj + 3 * g.vs, *Hy = adj + 4 * g.vs, *Hz = adj + 5 * g.vs;
   const T ex = adj[idx], ey = adj[g.vs + idx], ez = adj[2 * g.vs + idx];
-  T ax, ay, az;
-  const int n = g.npml;
-  if (n > 0 && within(i, j, k, n + 1, g.nx - n - 1, g.ny - n - 1, g.nz - n - 1)) {
-    const T hx = Hx[idx], hy = Hy[idx], hz = Hz[idx];
-    ax = dtm_dl * ((hy - Hy[idx - 1]) - (hz - Hz[idx - g.nz]));
-    ay = dtm_dl * ((hz - Hz[idx - sxy]) - (hx - Hx[idx - 1]));
-    az = dtm_dl * ((hx - Hx[idx - g.nz]) - (hy - Hy[idx - sxy]));
-  } else {
-    if (!is_interior(g, i, j, k)) {
-      // E terms: Ex (+ y, - z), Ey (+ z, - x), Ez (+ x, - y)
-      settle(g, prof, apsi, 0, 1, PSI_EX_Y, i, j, k, ex);
-      settle(g, prof, apsi, 0, 2, PSI_EX_Z, i, j, k, -ex);
-      settle(g, prof, apsi, 0, 2, PSI_EY_Z, i, j, k, ey);
-      settle(g, prof, apsi, 0, 0, PSI_EY_X, i, j, k, -ey);
-      settle(g, prof, apsi, 0, 0, PSI_EZ_X, i, j, k, ez);
-      settle(g, prof, apsi, 0, 1, PSI_EZ_Y, i, j, k, -ez);
-    }
-    const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
-    const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
-    const T P = T(1), M = T(-1);
-    ax = ay = az = T(0);
-    // Hy (+ z: Ex), Hz (- y: Ex)
-    if (k0) ax += u_h(g, adj, prof, apsi, 1, 2, PSI_HY_Z, P, i, j, k - 1, dtm, dtm_dl);
-    if (k1) ax -= u_h(g, adj, prof, apsi, 1, 2, PSI_HY_Z, P, i, j, k, dtm, dtm_dl);
-    if (j0) ax += u_h(g, adj, prof, apsi, 2, 1, PSI_HZ_Y, M, i, j - 1, k, dtm, dtm_dl);
-    if (j1) ax -= u_h(g, adj, prof, apsi, 2, 1, PSI_HZ_Y, M, i, j, k, dtm, dtm_dl);
-    // Hx (- z: Ey), Hz (+ x: Ey)
-    if (k0) ay += u_h(g, adj, prof, apsi, 0, 2, PSI_HX_Z, M, i, j, k - 1, dtm, dtm_dl);
-    if (k1) ay -= u_h(g, adj, prof, apsi, 0, 2, PSI_HX_Z, M, i, j, k, dtm, dtm_dl);
-    if (i0) ay += u_h(g, adj, prof, apsi, 2, 0, PSI_HZ_X, P, i - 1, j, k, dtm, dtm_dl);
-    if (i1) ay -= u_h(g, adj, prof, apsi, 2, 0, PSI_HZ_X, P, i, j, k, dtm, dtm_dl);
-    // Hx (+ y: Ez), Hy (- x: Ez)
-    if (j0) az += u_h(g, adj, prof, apsi, 0, 1, PSI_HX_Y, P, i, j - 1, k, dtm, dtm_dl);
-    if (j1) az -= u_h(g, adj, prof, apsi, 0, 1, PSI_HX_Y, P, i, j, k, dtm, dtm_dl);
-    if (i0) az += u_h(g, adj, prof, apsi, 1, 0, PSI_HY_X, M, i - 1, j, k, dtm, dtm_dl);
-    if (i1) az -= u_h(g, adj, prof, apsi, 1, 0, PSI_HY_X, M, i, j, k, dtm, dtm_dl);
-  }
+  // E terms: Ex (+ y, - z), Ey (+ z, - x), Ez (+ x, - y); w = s ce E~
+  settle(g, prof, apsi, 0, 1, PSI_EX_Y, i, j, k, ex);
+  settle(g, prof, apsi, 0, 2, PSI_EX_Z, i, j, k, -ex);
+  settle(g, prof, apsi, 0, 2, PSI_EY_Z, i, j, k, ey);
+  settle(g, prof, apsi, 0, 0, PSI_EY_X, i, j, k, -ey);
+  settle(g, prof, apsi, 0, 0, PSI_EZ_X, i, j, k, ez);
+  settle(g, prof, apsi, 0, 1, PSI_EZ_Y, i, j, k, -ez);
+  const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
+  const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
+  const AxisP<T> x0 = axis_p(g, prof, 1, 0, i), xm = axis_p(g, prof, 1, 0, i - 1);
+  const AxisP<T> y0 = axis_p(g, prof, 1, 1, j), ym = axis_p(g, prof, 1, 1, j - 1);
+  const AxisP<T> z0 = axis_p(g, prof, 1, 2, k), zm = axis_p(g, prof, 1, 2, k - 1);
+  const T hx = -dtm * Hx[idx], hy = -dtm * Hy[idx], hz = -dtm * Hz[idx];
+  const T hx_jm = j0 ? -dtm * Hx[idx - g.nz] : T(0), hx_km = k0 ? -dtm * Hx[idx - 1] : T(0);
+  const T hy_im = i0 ? -dtm * Hy[idx - sxy] : T(0), hy_km = k0 ? -dtm * Hy[idx - 1] : T(0);
+  const T hz_im = i0 ? -dtm * Hz[idx - sxy] : T(0), hz_jm = j0 ? -dtm * Hz[idx - g.nz] : T(0);
+  const T Z = T(0);
+  // Ex: Hy (+ z), Hz (- y)
+  const T ax = (k0 ? term_u(g, apsi, PSI_HY_Z, 2, zm, hy_km, i, j, k - 1) : Z) -
+               (k1 ? term_u(g, apsi, PSI_HY_Z, 2, z0, hy, i, j, k) : Z) +
+               (j0 ? term_u(g, apsi, PSI_HZ_Y, 1, ym, -hz_jm, i, j - 1, k) : Z) -
+               (j1 ? term_u(g, apsi, PSI_HZ_Y, 1, y0, -hz, i, j, k) : Z);
+  // Ey: Hx (- z), Hz (+ x)
+  const T ay = (k0 ? term_u(g, apsi, PSI_HX_Z, 2, zm, -hx_km, i, j, k - 1) : Z) -
+               (k1 ? term_u(g, apsi, PSI_HX_Z, 2, z0, -hx, i, j, k) : Z) +
+               (i0 ? term_u(g, apsi, PSI_HZ_X, 0, xm, hz_im, i - 1, j, k) : Z) -
+               (i1 ? term_u(g, apsi, PSI_HZ_X, 0, x0, hz, i, j, k) : Z);
+  // Ez: Hx (+ y), Hy (- x)
+  const T az = (j0 ? term_u(g, apsi, PSI_HX_Y, 1, ym, hx_jm, i, j - 1, k) : Z) -
+               (j1 ? term_u(g, apsi, PSI_HX_Y, 1, y0, hx, i, j, k) : Z) +
+               (i0 ? term_u(g, apsi, PSI_HY_X, 0, xm, -hy_im, i - 1, j, k) : Z) -
+               (i1 ? term_u(g, apsi, PSI_HY_X, 0, x0, -hy, i, j, k) : Z);
   adj[idx] = ex + coef[idx] * ax;
   adj[g.vs + idx] = ey + coef[g.vs + idx] * ay;
   adj[2 * g.vs + idx] = ez + coef[2 * g.vs + idx] * az;
@@ -436,12 +426,11 @@ int reverse_step_t(hfdtd_adjoint *a, long long n, int slot) {
                      a->box, (const T *)a->adj, (const T *)e->coef, h1, h0, a->G);
   HF_HIP(hipGetLastError());
   const dim3 grid(ceil_div(e->nz, 64), ceil_div(e->ny, kAdjRows), e->nx);
-  hipLaunchKernelGGL(k_adj_pass1<T>, grid, dim3(64, kAdjRows), 0, e->stream, g, T(e->inv_dl),
-                     T(e->dtm), (T *)a->adj, (const T *)e->prof, (T *)a->apsi);
+  hipLaunchKernelGGL(k_adj_pass1<T>, grid, dim3(64, kAdjRows), 0, e->stream, g, T(e->dtm),
+                     (T *)a->adj, (const T *)e->prof, (T *)a->apsi);
   HF_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_adj_pass2<T>, grid, dim3(64, kAdjRows), 0, e->stream, g, T(e->dtm),
-                     T(e->dtm_dl), (T *)a->adj, (const T *)e->coef, (const T *)e->prof,
-                     (T *)a->apsi);
+                     (T *)a->adj, (const T *)e->coef, (const T *)e->prof, (T *)a->apsi);
   HF_HIP(hipGetLastError());
   if (n - 1 >= 1) return inject_t<T>(a, n - 1);
   return HFDTD_OK;

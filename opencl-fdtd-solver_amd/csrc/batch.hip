@@ -563,6 +563,7 @@ int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i)
   if (!b || blocks < 0 || planes_i < 0) return fail(HFDTD_EINVAL, "bad launch shape");
   if (rows_j != 0 && rows_j != 2 && rows_j != 4 && rows_j != 6 && rows_j != 8)
     return fail(HFDTD_EINVAL, "rows_j must be 2, 4, 6 or 8");
+  if (b->fp64 && rows_j > 4) return fail(HFDTD_EINVAL, "FP64 batches use rows_j 2 or 4");
   b->blocks = blocks;
   if (rows_j) b->rows = rows_j;
   if (planes_i) b->planes = planes_i;

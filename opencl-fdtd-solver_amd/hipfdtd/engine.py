@@ -87,6 +87,7 @@ class HIPFDTD(SourceMonitorMixin):
         self.fused = bool(fused) and max(6 * vstride, psi_total) * item < 2**32
         self._sources = []
         self._monitors = []
+        self.lossy = False  # σ or ca volumes set: the sweeps read ca
         self._handle = None
         self._deferring = False
         self._h_done = False
@@ -205,6 +206,7 @@ class HIPFDTD(SourceMonitorMixin):
                                           int(src64)),
             "set_epsilon",
         )  # fmt: skip
+        self.lossy = sig is not None
 
     # ── sources (engine.py:448-538) ──
 
@@ -363,6 +365,8 @@ class HIPFDTD(SourceMonitorMixin):
         if name not in _lib.VOLUME_IDS:
             raise ValueError(f"unknown volume {name!r}")
         self._write(name, value)
+        if name.startswith("ca_"):
+            self.lossy = True
 
     def read_point(self, field_name, i, j, k):
         """One value of a field / coefficient volume."""

@@ -1,0 +1,68 @@
+"""GPU: the autotuner picks a launch shape without changing results.
+
+* ``tune_engine`` on a fresh engine leaves its fields zero, applies a
+  candidate from its table, is served from the cache the second time, and
+  the tuned engine steps bit-identically to an untuned one (the tile shape
+  only reorders independent cell updates);
+* ``tune_batch`` on a fresh batch applies a schedule and refuses a batch
+  whose tables are set.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import fdtd_oracle as fo
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (40, 48, 72)
+DL = 1e-3
+
+
+def src(f):
+    f.add_source_J("z", fo.gaussian_sine(f.t, 3e10, 3e10), (20, 21, 24, 25, 36, 37))
+
+
+def test_tune_engine_applies_and_caches(tmp_path):
+    import hipfdtd
+    from hipfdtd.autotune import tune_engine
+
+    cache = tmp_path / "tune.json"
+    sim = hipfdtd.HIPFDTD(SHAPE, DL, npml=6)
+    res = tune_engine(sim, planes=(-1, 8, 24), steps=5, warmup=1, cache_file=cache)
+    print("[autotune] engine", res["rows"], res["planes"], res["us"], flush=True)
+    assert not res["cached"]
+    assert [res["rows"], res["planes"], res["us"]] in res["table"]
+    assert cache.exists()
+    again = tune_engine(sim, cache_file=cache)
+    assert again["cached"] and again["rows"] == res["rows"] and again["planes"] == res["planes"]
+    assert not np.any(sim.read_volume("Ex"))
+    ref = hipfdtd.HIPFDTD(SHAPE, DL, npml=6)
+    for s in (sim, ref):
+        s.add_source(src)
+        s.run(25)
+    for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        assert np.array_equal(sim.read_volume(name), ref.read_volume(name)), name
+
+
+def test_tune_batch_applies_schedule(tmp_path):
+    import hipfdtd
+    from hipfdtd.autotune import tune_batch
+    from hipfdtd.batch import BatchedFDTD
+
+    sims = [hipfdtd.HIPFDTD(s, DL, npml=4) for s in ((24, 30, 40), (32, 20, 64), (20, 20, 20))]
+    b = BatchedFDTD(sims)
+    res = tune_batch(b, rows=(2, 4), planes=(8, 16), steps=3, warmup=1,
+                     cache_file=tmp_path / "tune.json")  # fmt: skip
+    print("[autotune] batch", res["blocks"], res["rows"], res["planes"], res["ms"], flush=True)
+    assert (res["rows"], res["planes"]) in {(2, 8), (2, 16), (4, 8), (4, 16)}
+    for s in sims:
+        assert not np.any(s.read_volume("Ez"))
+    b.set_sources([("z", (10, 11, 10, 11, 10, 11))] * 3, [lambda t: 1.0] * 3, 4)
+    with pytest.raises(ValueError):
+        tune_batch(b, use_cache=False, write_cache=False)
+    b.run(4)
+    assert all(np.any(s.read_volume("Ez")) for s in sims)
+    b.close()
