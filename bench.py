@@ -8,19 +8,22 @@ at f0 = c0/(20 dl), 50 % bandwidth; a Huygens box [32, 223]^3 recording the
 DFT at 8 frequencies; after the run, the near-to-far transform at
 181 θ x 2 φ angles, R = 1 m.  Inputs are synthetic (generated here).
 
-A "step" is one full Yee time step: the fused H + E sweep, the source fix-up
-on the sheet cells and the 8-frequency face-DFT accumulation.  ``value`` is Mcell-updates/s over the K
-timed steps.  ``roofline`` is the fused sweep's algorithmic HBM traffic
-(72 B/cell + 8 B per psi entry) divided by its average launch time, taken with
-HIP events on the engine's stream during the timed steps.  ``cpu_baseline``
+A "step" is one full Yee time step on the engine's default FP32 path: the
+split H and E sweeps with the sheet source between them (``--path fused``:
+the fused H + E sweep and the source fix-up), and the 8-frequency face-DFT
+accumulation.  ``value`` is Mcell-updates/s over the K timed steps.
+``roofline`` is the step's algorithmic HBM traffic (72 B/cell + 8 B per psi
+entry) divided by the time of the step's sweep kernels, taken with HIP
+events on the engine's stream around every sweep launch of the timed steps.  ``cpu_baseline``
 times this repository's NumPy restatement of the reference's face-CPML engine
 (one core) on a bounded sample of steps of the same grid.
 
-``roofline.traffic`` is the measured HBM traffic of one k_fused launch,
-taken from ``profiles/k_fused_256cube_hbm_traffic.txt`` (two rocprofv3 PMC
-passes, FETCH_SIZE and WRITE_SIZE, each scaled by what the same counter
-reads for a plain device copy of known size; ``--copy-calibration`` runs that
-copy).
+``roofline.traffic`` is the measured HBM traffic of the step's sweep
+kernels, taken from ``profiles/split_sweeps_256cube_hbm_traffic.txt`` (or
+``profiles/k_fused_256cube_hbm_traffic.txt`` for the fused path): two
+rocprofv3 PMC passes, FETCH_SIZE and WRITE_SIZE, each scaled by what the same
+counter reads for a plain device copy of known size (``--copy-calibration``
+runs that copy).
 
 ``--workload adjoint`` measures the 160^3 inverse-design gradient: a
 64x64x32 design box (εr ~ U[1, 12], seed 5), a Jx point source and a 4x4 Ex
@@ -125,7 +128,8 @@ class Workload:
 def run_gpu(args, wl: Workload):
     import hipfdtd
 
-    sim = hipfdtd.HIPFDTD(wl.shape, wl.dl, npml=wl.npml, dtype=np.float32)
+    fused = {"auto": None, "fused": True, "split": False}[args.path]
+    sim = hipfdtd.HIPFDTD(wl.shape, wl.dl, npml=wl.npml, dtype=np.float32, fused=fused)
     sim.set_epsilon(wl.eps)
     sim.add_source(wl.source)
     mon = hipfdtd.HIPNear2FarMonitor(sim, wl.center, wl.size, freqs=wl.freqs)
@@ -148,7 +152,7 @@ def run_gpu(args, wl: Workload):
     t3 = time.perf_counter()
     finite = bool(np.all(np.isfinite(eh)))
     return dict(wall_s=t1 - t0, kern_ms=kern_ms, launches=launches, n2f_ms=(t3 - t2) * 1e3,
-                finite=finite, device=sim.device_name)  # fmt: skip
+                finite=finite, device=sim.device_name, fused=sim.fused)  # fmt: skip
 
 
 def run_cpu_baseline(wl: Workload, steps: int):
@@ -445,19 +449,22 @@ def run_batch(args):
         "batch_hbm_gbs_alg": round(b_alg / step_s / 1e9, 1),
         "batch_hbm_frac": round(b_alg / step_s / 1e9 / HBM_PEAK_GBS, 4),
         "probe_dft_max_rel_err_vs_serial": err,
-        "autotune": {k: tuned[k] for k in ("blocks", "rows", "planes", "ms")},
+        "autotune": {k: tuned[k] for k in ("path", "blocks", "rows", "planes", "ms")},
+        "autotune_table": tuned["table"],
         "device": dev,
     }))  # fmt: skip
 
 
-TRAFFIC_FILE = ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt"
+TRAFFIC_FILES = {True: ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt",
+                 False: ROOT / "profiles" / "split_sweeps_256cube_hbm_traffic.txt"}  # fmt: skip
 
 
-def measured_traffic():
-    """HBM bytes per k_fused launch recorded in the profile summary, or None."""
+def measured_traffic(fused):
+    """HBM bytes per step of the sweep kernels recorded in the profile
+    summary for this path, or None."""
     try:
-        for line in TRAFFIC_FILE.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch:"):
+        for line in TRAFFIC_FILES[fused].read_text().splitlines():
+            if line.startswith("traffic_bytes_per_step:"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -489,6 +496,8 @@ def main():
                     choices=["yee-256", "plane-dft", "adjoint", "batch"])
     ap.add_argument("--graph-steps", type=int, default=100, help="batch: steps per HIP graph")
     ap.add_argument("--repeat", type=int, default=2, help="adjoint: timed forward+backward runs")
+    ap.add_argument("--path", default="auto", choices=["auto", "fused", "split"],
+                    help="256^3 workload: engine step path (auto = the engine's default)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -520,8 +529,11 @@ def main():
     mcells = cells * args.steps / res["wall_s"] / 1e6
     ms_step = res["wall_s"] / args.steps * 1e3
     b_alg = 72.0 * cells + 8.0 * wl.psi_entries()
-    avg_launch_s = res["kern_ms"] / max(res["launches"], 1) / 1e3
-    achieved = b_alg / avg_launch_s / 1e9
+    # sweep kernels of one step: k_fused, or k_update_H + k_update_E
+    step_kern_s = res["kern_ms"] / args.steps / 1e3
+    achieved = b_alg / step_kern_s / 1e9
+    kernel = ("k_fused (fused H + E sweep; sources by k_source_fixup)" if res["fused"] else
+              "k_update_H + k_update_E (split sweeps, one step; sources by k_add_source)")  # fmt: skip
     out = {
         "metric": "Mcell-updates/s",
         "value": round(mcells, 1),
@@ -544,14 +556,15 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_fused (fused H + E sweep; sources by k_source_fixup)",
+            "kernel": kernel,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(),
-            "bytes_alg_per_launch": b_alg,
-            "avg_launch_us": round(avg_launch_s * 1e6, 2),
+            "traffic": measured_traffic(res["fused"]),
+            "bytes_alg_per_step": b_alg,
+            "sweep_us_per_step": round(step_kern_s * 1e6, 2),
+            "launches_per_step": res["launches"] // args.steps,
         },
         "n2f_ms": round(res["n2f_ms"], 2),
         "farfield_finite": res["finite"],

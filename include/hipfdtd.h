@@ -142,13 +142,17 @@ int hfdtd_add_sources(hfdtd_engine *e, const hfdtd_source *src, int n);
 int hfdtd_step(hfdtd_engine *e, const hfdtd_source *src, int n, int path);
 /* queue.finish() */
 int hfdtd_synchronize(hfdtd_engine *e);
-/* Fused-sweep launch shape: rows of j per block (1, 2, 4, 6 or 8; FP64 engines
- * use at most 4; a block is
+/* Fused-sweep launch shape: rows of j per block (1, 2, 4, 6, 8, 10 or 12;
+ * FP64 engines use at most 4, lossy FP32 engines at most 10; a block is
  * 64 x (rows + 1) threads), planes of i per block (>= 1; -1 = chosen from the
  * grid: up to 64, two blocks per CU) and the number of
  * XCDs the work order is grouped for (1 = hardware order, 8 = MI355X); 0
  * keeps the current value.  Used by the autotuner. */
 int hfdtd_set_fused_tiling(hfdtd_engine *e, int rows_j, int planes_i, int xcd_groups);
+/* Split-sweep work order: each XCD runs runs of run_blocks consecutive
+ * 256-cell blocks (1 = hardware round robin; 0 keeps the current value).
+ * Used by the autotuner. */
+int hfdtd_set_split_schedule(hfdtd_engine *e, int run_blocks);
 
 /* HIP-event timing of the Yee sweep kernels on the engine's stream: while
  * enabled, every update_H / update_E / fused sweep launch is bracketed by a
@@ -255,6 +259,11 @@ int hfdtd_batch_read_probes(hfdtd_batch *b, double *out);
 /* persistent grid size (0 = one block per tile), rows of j per block (2, 4, 6
  * or 8; FP64 uses at most 4) and planes of i per block */
 int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i);
+/* HFDTD_PATH_FUSED (default): one persistent fused-tile launch per step;
+ * HFDTD_PATH_SPLIT: persistent split sweeps over blocks of 256 cells of the
+ * flattened (j, k) planes (H, sources, E, probes), which leave no lanes idle
+ * on extents that are not multiples of 64 */
+int hfdtd_batch_set_path(hfdtd_batch *b, int path);
 
 /* ── adjoint gradient of probe-DFT objectives ─────────────────────────────
  * No counterpart in the reference (it has no differentiation).  Objective:
@@ -289,12 +298,14 @@ int hfdtd_adjoint_save(hfdtd_adjoint *a, int c);
 int hfdtd_adjoint_restore(hfdtd_adjoint *a, int c);
 /* E over the gradient box into history slot */
 int hfdtd_adjoint_record(hfdtd_adjoint *a, int slot);
-/* n_steps forward steps of the engine (fused sweep when it fits, else
- * split), step q using src[r].amp = amp[q*nsrc + r]; the step taking the
- * grid to t_n, n = step0 + q + 1, accumulates the probe DFT if probe != 0 and
- * records E into slot slot0 + q + 1 if slot0 >= 0 */
+/* n_steps forward steps of the engine (path HFDTD_PATH_FUSED when the grid
+ * allows it, else the split sweeps), step q using src[r].amp =
+ * amp[q*nsrc + r]; the step taking the grid to t_n, n = step0 + q + 1,
+ * accumulates the probe DFT if probe != 0 and records E into slot
+ * slot0 + q + 1 if slot0 >= 0 */
 int hfdtd_adjoint_forward(hfdtd_adjoint *a, const hfdtd_source *src, int nsrc,
-                          const double *amp, int n_steps, int64_t step0, int slot0, int probe);
+                          const double *amp, int n_steps, int64_t step0, int slot0, int probe,
+                          int path);
 /* zero E~, H~, psi~ and G; add the adjoint source of step n_last */
 int hfdtd_adjoint_begin(hfdtd_adjoint *a, int64_t n_last);
 /* reverse steps n = n_hi .. n_lo+1: G += E~ (E^n - E^{n-1}) from slots

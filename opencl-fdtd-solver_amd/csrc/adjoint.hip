@@ -18,7 +18,10 @@
 // one gather sweep shaped like the forward split sweeps:
 //
 //   pass 1 (one thread per H cell): H~ += D'^T(...) from E~ and psi~_E;
-//   pass 2 (one thread per E cell): E~ += D^T(...)  from H~ and psi~_H.
+//   pass 2 (one thread per E cell): E~ += D^T(...)  from H~ and psi~_H,
+//
+// with threads over the flattened (j, k) plane and XCD-contiguous plane
+// slabs as in the split sweeps.
 //
 // The stored E adjoint is scaled, ce E~ (with E~ the adjoint above), which
 // drops ce from pass 1; in the interior the two passes are then the forward
@@ -50,8 +53,6 @@ namespace hfdtd {
 
 namespace {
 
-constexpr int kAdjRows = 4;
-
 struct Box {
   int i0, j0, k0, ni, nj, nk;
   long long n;
@@ -78,49 +79,57 @@ __device__ __forceinline__ size_t cell(const Geo &g, int i, int j, int k) {
   return ((size_t)i * g.ny + j) * g.nz + k;
 }
 
-// psi~ <- b (psi~ + w) at the cell's own stripe entry (stagger st)
+// All loads of the two passes are unconditional: a neighbour outside the
+// grid reads the cell itself (and is masked out of the sum), a psi~ entry
+// off its stripe reads a zero element of the slab's padding (its c is 0
+// anyway), a profile index outside the axis reads the axis' edge.  The
+// compiler can then issue every load of a cell at once instead of waiting at
+// the join of each divergent branch.  psi~ stores stay predicated on the
+// stripe.
+
+// first element of the padding after the psi~ slab that nothing writes
+__device__ __forceinline__ int zero_slot(const Geo &g) { return g.psi_off[PSI_COUNT] + 32; }
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// profile entry q (0 = b, 1 = c, 2 = ik) of stagger st along axis a at index
+// x clamped to the axis
 template <typename T>
-__device__ __forceinline__ void settle(const Geo &g, const T *__restrict__ prof,
-                                       T *__restrict__ apsi, int st, int a, int slot, int i,
-                                       int j, int k, T w) {
-  const int x = coord_of(a, i, j, k);
-  if (!in_stripe(g, a, x)) return;
-  T *p = apsi + g.psi_off[slot] + stripe_off(g, a, i, j, k);
-  *p = prof_at(g, prof, st, 0, a, x) * (*p + w);
+__device__ __forceinline__ T prof_c(const Geo &g, const T *__restrict__ prof, int st, int q, int a,
+                                    int x) {
+  const int N = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
+  return prof_at(g, prof, st, q, a, clampi(x, 0, N - 1));
 }
 
-// Profile values of one stagger along one axis at one index: s = ik + c and
-// c (c = 0 off the PML, where ik = 1/dl), and whether the index lies in the
-// axis' face stripes.
+// Per-axis data of one cell index: s = ik + c, c, b (stagger st) and the
+// psi~ element offsets of two slots of this axis' stripe (zero_slot off it).
 template <typename T>
-struct AxisP {
-  T s, c;
+struct AxisT {
+  T s, c, b;
   bool in;
+  int o0, o1;
 };
 
 template <typename T>
-__device__ __forceinline__ AxisP<T> axis_p(const Geo &g, const T *__restrict__ prof, int st,
-                                           int a, int x) {
+__device__ __forceinline__ AxisT<T> axis_t(const Geo &g, const T *__restrict__ prof, int st, int a,
+                                           int slot0, int slot1, int i, int j, int k) {
+  const int x = coord_of(a, i, j, k);
+  AxisT<T> p;
+  p.c = prof_c(g, prof, st, 1, a, x);
+  p.s = prof_c(g, prof, st, 2, a, x) + p.c;
+  p.b = prof_c(g, prof, st, 0, a, x);
   const int N = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
-  AxisP<T> p;
-  p.s = T(0);
-  p.c = T(0);
-  p.in = false;
-  if (x < 0 || x >= N) return p;
-  p.c = prof_at(g, prof, st, 1, a, x);
-  p.s = prof_at(g, prof, st, 2, a, x) + p.c;
-  p.in = in_stripe(g, a, x);
+  p.in = x >= 0 && x < N && in_stripe(g, a, x);
+  const int so = p.in ? stripe_off(g, a, i, j, k) : 0;
+  p.o0 = p.in ? g.psi_off[slot0] + so : zero_slot(g);
+  p.o1 = p.in ? g.psi_off[slot1] + so : zero_slot(g);
   return p;
 }
 
-// u = ik w + c (psi~ + w) = w (ik + c) + c psi~ of one curl term at cell
-// (i,j,k), psi~ read only on the term's stripe.
+// psi~ <- b (psi~ + w) where the cell lies on the stripe
 template <typename T>
-__device__ __forceinline__ T term_u(const Geo &g, const T *__restrict__ apsi, int slot, int a,
-                                    const AxisP<T> &p, T w, int i, int j, int k) {
-  T u = w * p.s;
-  if (p.in) u += p.c * apsi[g.psi_off[slot] + stripe_off(g, a, i, j, k)];
-  return u;
+__device__ __forceinline__ void settle_at(T *__restrict__ apsi, bool in, int o, T b, T old, T w) {
+  if (in) apsi[o] = b * (old + w);
 }
 
 // Pass 1: settle psi~_H of the previous reverse step, then the transposed E
@@ -130,48 +139,61 @@ __device__ __forceinline__ T term_u(const Geo &g, const T *__restrict__ apsi, in
 // [m_a <= N-2] u(m + e_a).  In the interior (c = 0, ik = 1/dl) this is the
 // forward H stencil with -inv_dl in place of dtm_dl.
 template <typename T>
-__global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass1(Geo g, T dtm, T *__restrict__ adj,
-                                                             const T *__restrict__ prof,
-                                                             T *__restrict__ apsi) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * kAdjRows + threadIdx.y;
-  const int i = blockIdx.z;
-  if (k >= g.nz || j >= g.ny) return;
+__global__ __launch_bounds__(kFlatBlock) void k_adj_pass1(Geo g, T dtm, T *__restrict__ adj,
+                                                          const T *__restrict__ prof,
+                                                          T *__restrict__ apsi, int bpp,
+                                                          int xcd, int run) {
+  const int L = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
+  const int i = L / bpp;
+  const int q = (L - i * bpp) * kFlatBlock + threadIdx.x;
+  if (q >= g.ny * g.nz) return;
+  const int j = q / g.nz, k = q - j * g.nz;
   const size_t idx = cell(g, i, j, k);
   const size_t sxy = (size_t)g.ny * g.nz;
   const T *Ex = adj, *Ey = adj + g.vs, *Ez = adj + 2 * g.vs;
-  T hx = adj[3 * g.vs + idx], hy = adj[4 * g.vs + idx], hz = adj[5 * g.vs + idx];
-  // H terms: Hx (+ y, - z), Hy (+ z, - x), Hz (+ x, - y); w = -s dtm H~
-  settle(g, prof, apsi, 1, 1, PSI_HX_Y, i, j, k, -dtm * hx);
-  settle(g, prof, apsi, 1, 2, PSI_HX_Z, i, j, k, dtm * hx);
-  settle(g, prof, apsi, 1, 2, PSI_HY_Z, i, j, k, -dtm * hy);
-  settle(g, prof, apsi, 1, 0, PSI_HY_X, i, j, k, dtm * hy);
-  settle(g, prof, apsi, 1, 0, PSI_HZ_X, i, j, k, -dtm * hz);
-  settle(g, prof, apsi, 1, 1, PSI_HZ_Y, i, j, k, dtm * hz);
   const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
-  const AxisP<T> x0 = axis_p(g, prof, 0, 0, i), x1 = axis_p(g, prof, 0, 0, i + 1);
-  const AxisP<T> y0 = axis_p(g, prof, 0, 1, j), y1 = axis_p(g, prof, 0, 1, j + 1);
-  const AxisP<T> z0 = axis_p(g, prof, 0, 2, k), z1 = axis_p(g, prof, 0, 2, k + 1);
-  const T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
-  const T ex_j1 = j1 ? Ex[idx + g.nz] : T(0), ex_k1 = k1 ? Ex[idx + 1] : T(0);
-  const T ey_i1 = i1 ? Ey[idx + sxy] : T(0), ey_k1 = k1 ? Ey[idx + 1] : T(0);
-  const T ez_i1 = i1 ? Ez[idx + sxy] : T(0), ez_j1 = j1 ? Ez[idx + g.nz] : T(0);
+  const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
   const T Z = T(0);
+  // fields: own cell, +x/+y/+z neighbours (the cell itself past the edge)
+  T hx = adj[3 * g.vs + idx], hy = adj[4 * g.vs + idx], hz = adj[5 * g.vs + idx];
+  const T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
+  const T ex_j1 = Ex[j1 ? idx + g.nz : idx], ex_k1 = Ex[k1 ? idx + 1 : idx];
+  const T ey_i1 = Ey[i1 ? idx + sxy : idx], ey_k1 = Ey[k1 ? idx + 1 : idx];
+  const T ez_i1 = Ez[i1 ? idx + sxy : idx], ez_j1 = Ez[j1 ? idx + g.nz : idx];
+  // H-side psi~ of this cell (settled below) and E-side psi~ at the cell and
+  // its +1 neighbours
+  const AxisT<T> hxa = axis_t(g, prof, 1, 0, PSI_HY_X, PSI_HZ_X, i, j, k);
+  const AxisT<T> hya = axis_t(g, prof, 1, 1, PSI_HX_Y, PSI_HZ_Y, i, j, k);
+  const AxisT<T> hza = axis_t(g, prof, 1, 2, PSI_HX_Z, PSI_HY_Z, i, j, k);
+  const AxisT<T> x0 = axis_t(g, prof, 0, 0, PSI_EY_X, PSI_EZ_X, i, j, k);
+  const AxisT<T> x1 = axis_t(g, prof, 0, 0, PSI_EY_X, PSI_EZ_X, i + 1, j, k);
+  const AxisT<T> y0 = axis_t(g, prof, 0, 1, PSI_EX_Y, PSI_EZ_Y, i, j, k);
+  const AxisT<T> y1 = axis_t(g, prof, 0, 1, PSI_EX_Y, PSI_EZ_Y, i, j + 1, k);
+  const AxisT<T> z0 = axis_t(g, prof, 0, 2, PSI_EX_Z, PSI_EY_Z, i, j, k);
+  const AxisT<T> z1 = axis_t(g, prof, 0, 2, PSI_EX_Z, PSI_EY_Z, i, j, k + 1);
+  const T q_hyx = apsi[hxa.o0], q_hzx = apsi[hxa.o1];
+  const T q_hxy = apsi[hya.o0], q_hzy = apsi[hya.o1];
+  const T q_hxz = apsi[hza.o0], q_hyz = apsi[hza.o1];
+  const T q_eyx0 = apsi[x0.o0], q_ezx0 = apsi[x0.o1], q_eyx1 = apsi[x1.o0], q_ezx1 = apsi[x1.o1];
+  const T q_exy0 = apsi[y0.o0], q_ezy0 = apsi[y0.o1], q_exy1 = apsi[y1.o0], q_ezy1 = apsi[y1.o1];
+  const T q_exz0 = apsi[z0.o0], q_eyz0 = apsi[z0.o1], q_exz1 = apsi[z1.o0], q_eyz1 = apsi[z1.o1];
+  // settle: H terms Hx (+ y, - z), Hy (+ z, - x), Hz (+ x, - y); w = -s dtm H~
+  settle_at(apsi, hxa.in, hxa.o0, hxa.b, q_hyx, dtm * hy);
+  settle_at(apsi, hxa.in, hxa.o1, hxa.b, q_hzx, -dtm * hz);
+  settle_at(apsi, hya.in, hya.o0, hya.b, q_hxy, -dtm * hx);
+  settle_at(apsi, hya.in, hya.o1, hya.b, q_hzy, dtm * hz);
+  settle_at(apsi, hza.in, hza.o0, hza.b, q_hxz, dtm * hx);
+  settle_at(apsi, hza.in, hza.o1, hza.b, q_hyz, -dtm * hy);
+  // u = w (ik + c) + c psi~ for each E term at the cell (0) and neighbour (1)
   // Hx: Ey (+ z), Ez (- y)
-  hx += (k >= 1 ? term_u(g, apsi, PSI_EY_Z, 2, z0, ey, i, j, k) : Z) -
-        (k1 ? term_u(g, apsi, PSI_EY_Z, 2, z1, ey_k1, i, j, k + 1) : Z) +
-        (j >= 1 ? term_u(g, apsi, PSI_EZ_Y, 1, y0, -ez, i, j, k) : Z) -
-        (j1 ? term_u(g, apsi, PSI_EZ_Y, 1, y1, -ez_j1, i, j + 1, k) : Z);
+  hx += (k0 ? ey * z0.s + z0.c * q_eyz0 : Z) - (k1 ? ey_k1 * z1.s + z1.c * q_eyz1 : Z) +
+        (j0 ? -ez * y0.s + y0.c * q_ezy0 : Z) - (j1 ? -ez_j1 * y1.s + y1.c * q_ezy1 : Z);
   // Hy: Ex (- z), Ez (+ x)
-  hy += (k >= 1 ? term_u(g, apsi, PSI_EX_Z, 2, z0, -ex, i, j, k) : Z) -
-        (k1 ? term_u(g, apsi, PSI_EX_Z, 2, z1, -ex_k1, i, j, k + 1) : Z) +
-        (i >= 1 ? term_u(g, apsi, PSI_EZ_X, 0, x0, ez, i, j, k) : Z) -
-        (i1 ? term_u(g, apsi, PSI_EZ_X, 0, x1, ez_i1, i + 1, j, k) : Z);
+  hy += (k0 ? -ex * z0.s + z0.c * q_exz0 : Z) - (k1 ? -ex_k1 * z1.s + z1.c * q_exz1 : Z) +
+        (i0 ? ez * x0.s + x0.c * q_ezx0 : Z) - (i1 ? ez_i1 * x1.s + x1.c * q_ezx1 : Z);
   // Hz: Ex (+ y), Ey (- x)
-  hz += (j >= 1 ? term_u(g, apsi, PSI_EX_Y, 1, y0, ex, i, j, k) : Z) -
-        (j1 ? term_u(g, apsi, PSI_EX_Y, 1, y1, ex_j1, i, j + 1, k) : Z) +
-        (i >= 1 ? term_u(g, apsi, PSI_EY_X, 0, x0, -ey, i, j, k) : Z) -
-        (i1 ? term_u(g, apsi, PSI_EY_X, 0, x1, -ey_i1, i + 1, j, k) : Z);
+  hz += (j0 ? ex * y0.s + y0.c * q_exy0 : Z) - (j1 ? ex_j1 * y1.s + y1.c * q_exy1 : Z) +
+        (i0 ? -ey * x0.s + x0.c * q_eyx0 : Z) - (i1 ? -ey_i1 * x1.s + x1.c * q_eyx1 : Z);
   adj[3 * g.vs + idx] = hx;
   adj[4 * g.vs + idx] = hy;
   adj[5 * g.vs + idx] = hz;
@@ -183,53 +205,64 @@ __global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass1(Geo g, T dtm, T *__
 // and E~ += ce ([m_a >= 1] u(m - e_a) - [m_a <= N-2] u(m)).  In the interior
 // this is the forward E stencil with -ce dtm_dl in place of ce inv_dl.
 template <typename T>
-__global__ __launch_bounds__(64 * kAdjRows) void k_adj_pass2(Geo g, T dtm, T *__restrict__ adj,
-                                                             const T *__restrict__ coef,
-                                                             const T *__restrict__ prof,
-                                                             T *__restrict__ apsi) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * kAdjRows + threadIdx.y;
-  const int i = blockIdx.z;
-  if (k >= g.nz || j >= g.ny) return;
+__global__ __launch_bounds__(kFlatBlock) void k_adj_pass2(Geo g, T dtm, T *__restrict__ adj,
+                                                          const T *__restrict__ coef,
+                                                          const T *__restrict__ prof,
+                                                          T *__restrict__ apsi, int bpp,
+                                                          int xcd, int run) {
+  const int L = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
+  const int i = L / bpp;
+  const int q = (L - i * bpp) * kFlatBlock + threadIdx.x;
+  if (q >= g.ny * g.nz) return;
+  const int j = q / g.nz, k = q - j * g.nz;
   const size_t idx = cell(g, i, j, k);
   const size_t sxy = (size_t)g.ny * g.nz;
   const T *Hx = adj + 3 * g.vs, *Hy = adj + 4 * g.vs, *Hz = adj + 5 * g.vs;
-  const T ex = adj[idx], ey = adj[g.vs + idx], ez = adj[2 * g.vs + idx];
-  // E terms: Ex (+ y, - z), Ey (+ z, - x), Ez (+ x, - y); w = s ce E~
-  settle(g, prof, apsi, 0, 1, PSI_EX_Y, i, j, k, ex);
-  settle(g, prof, apsi, 0, 2, PSI_EX_Z, i, j, k, -ex);
-  settle(g, prof, apsi, 0, 2, PSI_EY_Z, i, j, k, ey);
-  settle(g, prof, apsi, 0, 0, PSI_EY_X, i, j, k, -ey);
-  settle(g, prof, apsi, 0, 0, PSI_EZ_X, i, j, k, ez);
-  settle(g, prof, apsi, 0, 1, PSI_EZ_Y, i, j, k, -ez);
   const bool i0 = i >= 1, j0 = j >= 1, k0 = k >= 1;
   const bool i1 = i + 1 < g.nx, j1 = j + 1 < g.ny, k1 = k + 1 < g.nz;
-  const AxisP<T> x0 = axis_p(g, prof, 1, 0, i), xm = axis_p(g, prof, 1, 0, i - 1);
-  const AxisP<T> y0 = axis_p(g, prof, 1, 1, j), ym = axis_p(g, prof, 1, 1, j - 1);
-  const AxisP<T> z0 = axis_p(g, prof, 1, 2, k), zm = axis_p(g, prof, 1, 2, k - 1);
-  const T hx = -dtm * Hx[idx], hy = -dtm * Hy[idx], hz = -dtm * Hz[idx];
-  const T hx_jm = j0 ? -dtm * Hx[idx - g.nz] : T(0), hx_km = k0 ? -dtm * Hx[idx - 1] : T(0);
-  const T hy_im = i0 ? -dtm * Hy[idx - sxy] : T(0), hy_km = k0 ? -dtm * Hy[idx - 1] : T(0);
-  const T hz_im = i0 ? -dtm * Hz[idx - sxy] : T(0), hz_jm = j0 ? -dtm * Hz[idx - g.nz] : T(0);
   const T Z = T(0);
+  const T ex = adj[idx], ey = adj[g.vs + idx], ez = adj[2 * g.vs + idx];
+  const T cx = coef[idx], cy = coef[g.vs + idx], cz = coef[2 * g.vs + idx];
+  // w = -dtm H~ at the cell and its -x/-y/-z neighbours (the cell itself at
+  // the low edge); the sign s of each term is applied below
+  const T hx = -dtm * Hx[idx], hy = -dtm * Hy[idx], hz = -dtm * Hz[idx];
+  const T hx_jm = -dtm * Hx[j0 ? idx - g.nz : idx], hx_km = -dtm * Hx[k0 ? idx - 1 : idx];
+  const T hy_im = -dtm * Hy[i0 ? idx - sxy : idx], hy_km = -dtm * Hy[k0 ? idx - 1 : idx];
+  const T hz_im = -dtm * Hz[i0 ? idx - sxy : idx], hz_jm = -dtm * Hz[j0 ? idx - g.nz : idx];
+  const AxisT<T> exa = axis_t(g, prof, 0, 0, PSI_EY_X, PSI_EZ_X, i, j, k);
+  const AxisT<T> eya = axis_t(g, prof, 0, 1, PSI_EX_Y, PSI_EZ_Y, i, j, k);
+  const AxisT<T> eza = axis_t(g, prof, 0, 2, PSI_EX_Z, PSI_EY_Z, i, j, k);
+  const AxisT<T> x0 = axis_t(g, prof, 1, 0, PSI_HY_X, PSI_HZ_X, i, j, k);
+  const AxisT<T> xm = axis_t(g, prof, 1, 0, PSI_HY_X, PSI_HZ_X, i - 1, j, k);
+  const AxisT<T> y0 = axis_t(g, prof, 1, 1, PSI_HX_Y, PSI_HZ_Y, i, j, k);
+  const AxisT<T> ym = axis_t(g, prof, 1, 1, PSI_HX_Y, PSI_HZ_Y, i, j - 1, k);
+  const AxisT<T> z0 = axis_t(g, prof, 1, 2, PSI_HX_Z, PSI_HY_Z, i, j, k);
+  const AxisT<T> zm = axis_t(g, prof, 1, 2, PSI_HX_Z, PSI_HY_Z, i, j, k - 1);
+  const T q_eyx = apsi[exa.o0], q_ezx = apsi[exa.o1];
+  const T q_exy = apsi[eya.o0], q_ezy = apsi[eya.o1];
+  const T q_exz = apsi[eza.o0], q_eyz = apsi[eza.o1];
+  const T q_hyx0 = apsi[x0.o0], q_hzx0 = apsi[x0.o1], q_hyxm = apsi[xm.o0], q_hzxm = apsi[xm.o1];
+  const T q_hxy0 = apsi[y0.o0], q_hzy0 = apsi[y0.o1], q_hxym = apsi[ym.o0], q_hzym = apsi[ym.o1];
+  const T q_hxz0 = apsi[z0.o0], q_hyz0 = apsi[z0.o1], q_hxzm = apsi[zm.o0], q_hyzm = apsi[zm.o1];
+  // settle: E terms Ex (+ y, - z), Ey (+ z, - x), Ez (+ x, - y); w = s ce E~
+  settle_at(apsi, exa.in, exa.o0, exa.b, q_eyx, -ey);
+  settle_at(apsi, exa.in, exa.o1, exa.b, q_ezx, ez);
+  settle_at(apsi, eya.in, eya.o0, eya.b, q_exy, ex);
+  settle_at(apsi, eya.in, eya.o1, eya.b, q_ezy, -ez);
+  settle_at(apsi, eza.in, eza.o0, eza.b, q_exz, -ex);
+  settle_at(apsi, eza.in, eza.o1, eza.b, q_eyz, ey);
   // Ex: Hy (+ z), Hz (- y)
-  const T ax = (k0 ? term_u(g, apsi, PSI_HY_Z, 2, zm, hy_km, i, j, k - 1) : Z) -
-               (k1 ? term_u(g, apsi, PSI_HY_Z, 2, z0, hy, i, j, k) : Z) +
-               (j0 ? term_u(g, apsi, PSI_HZ_Y, 1, ym, -hz_jm, i, j - 1, k) : Z) -
-               (j1 ? term_u(g, apsi, PSI_HZ_Y, 1, y0, -hz, i, j, k) : Z);
+  const T ax = (k0 ? hy_km * zm.s + zm.c * q_hyzm : Z) - (k1 ? hy * z0.s + z0.c * q_hyz0 : Z) +
+               (j0 ? -hz_jm * ym.s + ym.c * q_hzym : Z) - (j1 ? -hz * y0.s + y0.c * q_hzy0 : Z);
   // Ey: Hx (- z), Hz (+ x)
-  const T ay = (k0 ? term_u(g, apsi, PSI_HX_Z, 2, zm, -hx_km, i, j, k - 1) : Z) -
-               (k1 ? term_u(g, apsi, PSI_HX_Z, 2, z0, -hx, i, j, k) : Z) +
-               (i0 ? term_u(g, apsi, PSI_HZ_X, 0, xm, hz_im, i - 1, j, k) : Z) -
-               (i1 ? term_u(g, apsi, PSI_HZ_X, 0, x0, hz, i, j, k) : Z);
+  const T ay = (k0 ? -hx_km * zm.s + zm.c * q_hxzm : Z) - (k1 ? -hx * z0.s + z0.c * q_hxz0 : Z) +
+               (i0 ? hz_im * xm.s + xm.c * q_hzxm : Z) - (i1 ? hz * x0.s + x0.c * q_hzx0 : Z);
   // Ez: Hx (+ y), Hy (- x)
-  const T az = (j0 ? term_u(g, apsi, PSI_HX_Y, 1, ym, hx_jm, i, j - 1, k) : Z) -
-               (j1 ? term_u(g, apsi, PSI_HX_Y, 1, y0, hx, i, j, k) : Z) +
-               (i0 ? term_u(g, apsi, PSI_HY_X, 0, xm, -hy_im, i - 1, j, k) : Z) -
-               (i1 ? term_u(g, apsi, PSI_HY_X, 0, x0, -hy, i, j, k) : Z);
-  adj[idx] = ex + coef[idx] * ax;
-  adj[g.vs + idx] = ey + coef[g.vs + idx] * ay;
-  adj[2 * g.vs + idx] = ez + coef[2 * g.vs + idx] * az;
+  const T az = (j0 ? hx_jm * ym.s + ym.c * q_hxym : Z) - (j1 ? hx * y0.s + y0.c * q_hxy0 : Z) +
+               (i0 ? -hy_im * xm.s + xm.c * q_hyxm : Z) - (i1 ? -hy * x0.s + x0.c * q_hyx0 : Z);
+  adj[idx] = ex + cx * ax;
+  adj[g.vs + idx] = ey + cy * ay;
+  adj[2 * g.vs + idx] = ez + cz * az;
 }
 
 __device__ __forceinline__ size_t box_cell(const Geo &g, const Box &b, long long q) {
@@ -425,12 +458,14 @@ int reverse_step_t(hfdtd_adjoint *a, long long n, int slot) {
   hipLaunchKernelGGL(k_adj_grad<T>, dim3(ceil_div(a->box.n, 256)), dim3(256), 0, e->stream, g,
                      a->box, (const T *)a->adj, (const T *)e->coef, h1, h0, a->G);
   HF_HIP(hipGetLastError());
-  const dim3 grid(ceil_div(e->nz, 64), ceil_div(e->ny, kAdjRows), e->nx);
-  hipLaunchKernelGGL(k_adj_pass1<T>, grid, dim3(64, kAdjRows), 0, e->stream, g, T(e->dtm),
-                     (T *)a->adj, (const T *)e->prof, (T *)a->apsi);
+  const unsigned bpp = ceil_div((size_t)e->ny * e->nz, kFlatBlock);
+  const dim3 grid(bpp * (unsigned)e->nx);
+  hipLaunchKernelGGL(k_adj_pass1<T>, grid, dim3(kFlatBlock), 0, e->stream, g, T(e->dtm),
+                     (T *)a->adj, (const T *)e->prof, (T *)a->apsi, (int)bpp, e->xcd, e->split_run);
   HF_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_adj_pass2<T>, grid, dim3(64, kAdjRows), 0, e->stream, g, T(e->dtm),
-                     (T *)a->adj, (const T *)e->coef, (const T *)e->prof, (T *)a->apsi);
+  hipLaunchKernelGGL(k_adj_pass2<T>, grid, dim3(kFlatBlock), 0, e->stream, g, T(e->dtm),
+                     (T *)a->adj, (const T *)e->coef, (const T *)e->prof, (T *)a->apsi,
+                     (int)bpp, e->xcd, e->split_run);
   HF_HIP(hipGetLastError());
   if (n - 1 >= 1) return inject_t<T>(a, n - 1);
   return HFDTD_OK;
@@ -612,16 +647,19 @@ int hfdtd_adjoint_record(hfdtd_adjoint *a, int slot) {
 }
 
 int hfdtd_adjoint_forward(hfdtd_adjoint *a, const hfdtd_source *src, int nsrc,
-                          const double *amp, int n_steps, int64_t step0, int slot0, int probe) {
+                          const double *amp, int n_steps, int64_t step0, int slot0, int probe,
+                          int path) {
   if (!a || n_steps < 0 || nsrc < 0 || nsrc > kMaxSources || (nsrc > 0 && (!src || !amp)))
     return fail(HFDTD_EINVAL, "bad forward arguments (at most 16 sources)");
+  if (path != HFDTD_PATH_FUSED && path != HFDTD_PATH_SPLIT)
+    return fail(HFDTD_EINVAL, "unknown step path");
   if (slot0 >= 0 && slot0 + n_steps >= a->hist_slots)
     return fail(HFDTD_EINVAL, "recorded steps exceed the history slots");
   if (probe && !a->np) return fail(HFDTD_EINVAL, "no probes set");
   Engine *e = a->e;
   HF_HIP(hipSetDevice(e->device));
   std::vector<hfdtd_source> s(src, src + nsrc);
-  const bool fused = e->fused_fits();
+  const bool fused = path == HFDTD_PATH_FUSED && e->fused_fits();
   for (int q = 0; q < n_steps; ++q) {
     for (int r = 0; r < nsrc; ++r) s[r].amp = amp[(size_t)q * nsrc + r];
     int rc;

@@ -71,6 +71,8 @@ struct BatchArgs {
   int nfreq, comp;
   double *probe_acc;  // n x nfreq x 2
   int xcd;            // XCD count for the work-order remap (1 = off)
+  int run;            // split path: consecutive items per XCD run
+  int n_sims;
 };
 
 template <typename T>
@@ -145,6 +147,86 @@ __global__ __launch_bounds__(256) void k_batch_fixup(BatchArgs<T> a) {
   }
 }
 
+// ── batched split sweeps ──────────────────────────────────────────────────
+//
+// Work item n is a block of 256 consecutive cells of one i-plane of one
+// engine (tiles_k holds the blocks per plane), so engines of any extents
+// pack the wavefronts without idle lanes.  With one block per item the items
+// are remapped so every XCD runs runs of consecutive items (neighbour rows
+// share that XCD's L2).  The state is updated in place, in
+// the reference's order: H everywhere, sources, E everywhere, probes.
+
+template <typename T>
+__global__ __launch_bounds__(kFlatBlock) void k_batch_split_H(BatchArgs<T> a) {
+  for (int m = blockIdx.x; m < a.n_tiles; m += gridDim.x) {
+    const int n = gridDim.x == a.n_tiles ? xcd_runs(m, a.n_tiles, a.xcd, a.run) : m;
+    const SimDesc<T> &d = a.sims[a.tile_sim[n]];
+    const Geo g = d.g;
+    const int lt = n - d.tile_begin;
+    const int i = lt / d.tiles_k;
+    const int q = (lt - i * d.tiles_k) * kFlatBlock + threadIdx.x;
+    if (q >= g.ny * g.nz) continue;
+    const int j = q / g.nz;
+    split_h_cell(g, a.dtm_dl, a.dtm, d.st[0], d.prof, d.psi[0], i, j, q - j * g.nz);
+  }
+}
+
+template <typename T, bool LOSSY>
+__global__ __launch_bounds__(kFlatBlock) void k_batch_split_E(BatchArgs<T> a) {
+  for (int m = blockIdx.x; m < a.n_tiles; m += gridDim.x) {
+    const int n = gridDim.x == a.n_tiles ? xcd_runs(m, a.n_tiles, a.xcd, a.run) : m;
+    const SimDesc<T> &d = a.sims[a.tile_sim[n]];
+    const Geo g = d.g;
+    const int lt = n - d.tile_begin;
+    const int i = lt / d.tiles_k;
+    const int q = (lt - i * d.tiles_k) * kFlatBlock + threadIdx.x;
+    if (q >= g.ny * g.nz) continue;
+    const int j = q / g.nz;
+    split_e_cell<T, LOSSY>(g, a.inv_dl, d.st[0], d.coef, d.prof, d.psi[0], i, j, q - j * g.nz);
+  }
+}
+
+// Between the halves: one block per engine adds its source to E
+// (sources.cl:7-49 via src_term, as the single-engine split path does).
+template <typename T>
+__global__ __launch_bounds__(256) void k_batch_split_src(BatchArgs<T> a) {
+  const SimDesc<T> &d = a.sims[blockIdx.x];
+  if (!d.has_src) return;
+  const Geo g = d.g;
+  const Src<T> src = d.src;
+  const T amp = T(a.amp[(size_t)blockIdx.x * a.n_table + *a.step]);
+  const int ni = src.ci1 - src.ci0, nj = src.cj1 - src.cj0, nk = src.ck1 - src.ck0;
+  const long long total = (long long)ni * nj * nk;
+  T *F = d.st[0] + src.comp * g.vs;
+  const T *ce = d.coef + src.comp * g.vs;
+  for (long long t = threadIdx.x; t < total; t += blockDim.x) {
+    const int k = src.ck0 + (int)(t % nk);
+    const int j = src.cj0 + (int)((t / nk) % nj);
+    const int i = src.ci0 + (int)(t / ((long long)nk * nj));
+    const size_t idx = ((size_t)i * g.ny + j) * g.nz + k;
+    F[idx] += src_term<T>(src, i, j, k, ce[idx], amp);
+  }
+}
+
+// After the E half: one thread per engine records its probe.
+template <typename T>
+__global__ void k_batch_split_probe(BatchArgs<T> a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n_sims) return;
+  const SimDesc<T> &d = a.sims[s];
+  if (!d.has_probe) return;
+  const Geo g = d.g;
+  const size_t row = (size_t)s * a.n_table + *a.step;
+  const size_t idx = ((size_t)d.pi * g.ny + d.pj) * g.nz + d.pk;
+  const double v = (double)d.st[0][a.comp * g.vs + idx];
+  const double *ph = a.phase + row * a.nfreq * 2;
+  double *acc = a.probe_acc + (size_t)s * a.nfreq * 2;
+  for (int f = 0; f < a.nfreq; ++f) {
+    acc[2 * f] += v * ph[2 * f];
+    acc[2 * f + 1] += v * ph[2 * f + 1];
+  }
+}
+
 __global__ void k_step_advance(int *step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(step, 1);
 }
@@ -170,6 +252,7 @@ struct hfdtd_batch {
   int n_phase = 0;
   double *d_probe_acc = nullptr;
   int blocks = 0, rows = 4, planes = 16;
+  int path = HFDTD_PATH_FUSED;  // fused tiles, or the flat split sweeps
   bool lossy = false;
   hipGraphExec_t gexec = nullptr;
 };
@@ -198,10 +281,12 @@ static int batch_build(hfdtd_batch *b) {
   for (int s = 0; s < n; ++s) {
     Engine *e = b->eng[s];
     b->lossy = b->lossy || !e->lossless;
-    if (!e->fused_fits())
-      return fail(HFDTD_EUNSUPPORTED, "batched engines need field and psi slabs below 4 GB");
-    int rc = ensure_alt(e);
-    if (rc) return rc;
+    if (b->path == HFDTD_PATH_FUSED) {
+      if (!e->fused_fits())
+        return fail(HFDTD_EUNSUPPORTED, "batched engines need field and psi slabs below 4 GB");
+      int rc = ensure_alt(e);
+      if (rc) return rc;
+    }
     SimDesc<T> &d = desc[s];
     d.g.nx = e->nx;
     d.g.ny = e->ny;
@@ -217,11 +302,17 @@ static int batch_build(hfdtd_batch *b) {
     d.psi[1] = (T *)e->psi[e->cur ^ 1];
     d.coef = (const T *)e->coef;
     d.prof = (const T *)e->prof;
-    d.tiles_k = (int)ceil_div(e->nz, 64);
-    d.tiles_j = (int)ceil_div(e->ny, TJ);
-    const int tiles_i = (int)ceil_div(e->nx, P);
     d.tile_begin = begin;
-    const int nt = d.tiles_k * d.tiles_j * tiles_i;
+    int nt;
+    if (b->path == HFDTD_PATH_SPLIT) {  // blocks of 256 cells per plane
+      d.tiles_k = (int)ceil_div((size_t)e->ny * e->nz, kFlatBlock);
+      d.tiles_j = 1;
+      nt = d.tiles_k * e->nx;
+    } else {
+      d.tiles_k = (int)ceil_div(e->nz, 64);
+      d.tiles_j = (int)ceil_div(e->ny, TJ);
+      nt = d.tiles_k * d.tiles_j * (int)ceil_div(e->nx, P);
+    }
     tile_sim.insert(tile_sim.end(), nt, s);
     begin += nt;
     d.lossy = e->lossless ? 0 : 1;
@@ -287,11 +378,38 @@ static BatchArgs<T> batch_args(const hfdtd_batch *b) {
   a.comp = b->comp;
   a.probe_acc = b->d_probe_acc;
   a.xcd = e->xcd;
+  a.run = e->split_run;
+  a.n_sims = (int)b->eng.size();
   return a;
 }
 
 template <typename T>
+static int batch_launch_split(hfdtd_batch *b, const BatchArgs<T> &a) {
+  const int grid = b->blocks > 0 ? std::min(b->blocks, b->n_tiles) : b->n_tiles;
+  const int n = (int)b->eng.size();
+  hipLaunchKernelGGL(k_batch_split_H<T>, dim3(grid), dim3(kFlatBlock), 0, b->stream, a);
+  HF_HIP(hipGetLastError());
+  if (a.amp) {
+    hipLaunchKernelGGL(k_batch_split_src<T>, dim3(n), dim3(256), 0, b->stream, a);
+    HF_HIP(hipGetLastError());
+  }
+  if (b->lossy)
+    hipLaunchKernelGGL((k_batch_split_E<T, true>), dim3(grid), dim3(kFlatBlock), 0, b->stream, a);
+  else
+    hipLaunchKernelGGL((k_batch_split_E<T, false>), dim3(grid), dim3(kFlatBlock), 0, b->stream, a);
+  HF_HIP(hipGetLastError());
+  if (a.probe_acc) {
+    hipLaunchKernelGGL(k_batch_split_probe<T>, dim3(ceil_div(n, 64)), dim3(64), 0, b->stream, a);
+    HF_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, b->stream, b->d_step);
+  HF_HIP(hipGetLastError());
+  return HFDTD_OK;
+}
+
+template <typename T>
 static int batch_launch_step(hfdtd_batch *b, const BatchArgs<T> &a) {
+  if (b->path == HFDTD_PATH_SPLIT) return batch_launch_split<T>(b, a);
   const int grid = b->blocks > 0 ? std::min(b->blocks, b->n_tiles) : b->n_tiles;
 #define HF_BATCH(TJ)                                                                          \
   if (b->lossy)                                                                               \
@@ -350,7 +468,7 @@ static int batch_run_t(hfdtd_batch *b, int n_steps, int graph_steps) {
     if ((rc = batch_launch_step<T>(b, a))) return rc;
   HF_HIP(hipStreamSynchronize(b->stream));
   b->step_host += n_steps;
-  if (n_steps & 1)
+  if (b->path == HFDTD_PATH_FUSED && (n_steps & 1))
     for (Engine *e : b->eng) e->cur ^= 1;
   return HFDTD_OK;
 }
@@ -557,6 +675,13 @@ int hfdtd_batch_read_probes(hfdtd_batch *b, double *out) {
                         b->stream));
   HF_HIP(hipStreamSynchronize(b->stream));
   return HFDTD_OK;
+}
+
+int hfdtd_batch_set_path(hfdtd_batch *b, int path) {
+  if (!b || (path != HFDTD_PATH_FUSED && path != HFDTD_PATH_SPLIT))
+    return fail(HFDTD_EINVAL, "path must be HFDTD_PATH_FUSED or HFDTD_PATH_SPLIT");
+  b->path = path;
+  return batch_free_graph(b);
 }
 
 int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i) {

@@ -64,6 +64,7 @@ struct Engine {
   int tile_rows = 8;     // fused sweep: j rows per block
   int tile_planes = 0;   // fused sweep: i planes per block (0 = chosen from the grid)
   int xcd = 8;           // XCDs for the work-order remap (1 = hardware order)
+  int split_run = 128;   // split sweeps: consecutive blocks per XCD run (1 = round robin)
   int cus = 256;         // compute units of the device
 
   uint64_t bytes = 0;
@@ -87,7 +88,7 @@ struct Engine {
   size_t prof_len() const { return (size_t)nx + ny + nz; }
   // Resident fused blocks per CU, by rows (index) and lossy (0/1); 0 = not
   // yet queried.
-  int fused_occ[2][9] = {};
+  int fused_occ[2][13] = {};
   // i planes per fused block: the set value, or the count P in [8, 64] that
   // minimises the modelled sweep time (waves of resident blocks) x (P + 2):
   // each wave lasts as long as its blocks' P planes plus the two halo planes

@@ -538,11 +538,19 @@ int hfdtd_timing_read(hfdtd_engine *h, double *total_ms, int *launches) {
   return HFDTD_OK;
 }
 
+int hfdtd_set_split_schedule(hfdtd_engine *h, int run_blocks) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  if (run_blocks < 0 || run_blocks > 4096) return fail(HFDTD_EINVAL, "run_blocks must be 0..4096");
+  if (run_blocks) h->s.split_run = run_blocks;
+  return HFDTD_OK;
+}
+
 int hfdtd_set_fused_tiling(hfdtd_engine *h, int rows_j, int planes_i, int xcd_groups) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
-  if ((rows_j != 0 && rows_j != 1 && rows_j != 2 && rows_j != 4 && rows_j != 8 &&
-       rows_j != 6) || planes_i < -1)
-    return fail(HFDTD_EINVAL, "rows_j must be 1, 2, 4, 6 or 8 and planes_i >= -1");
+  if ((rows_j != 0 && rows_j != 1 && rows_j != 2 && rows_j != 4 && rows_j != 6 &&
+       rows_j != 8 && rows_j != 10 && rows_j != 12) ||
+      planes_i < -1)
+    return fail(HFDTD_EINVAL, "rows_j must be 1, 2, 4, 6, 8, 10 or 12 and planes_i >= -1");
   if (xcd_groups < 0 || xcd_groups > 64) return fail(HFDTD_EINVAL, "xcd_groups must be 0..64");
   if (rows_j) h->s.tile_rows = rows_j;
   if (planes_i) h->s.tile_planes = planes_i < 0 ? 0 : planes_i;

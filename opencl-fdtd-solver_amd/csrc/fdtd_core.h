@@ -224,6 +224,62 @@ __device__ __forceinline__ void e_math(const Geo &g, int i, int j, int k, const 
 
 constexpr int kMaxSources = 16;
 
+// ── split sweeps: one cell of the H or the E half ─────────────────────────
+
+constexpr int kFlatBlock = 256;  // threads per split-sweep block
+
+// H(i,j,k) and its psi from E (reference yee_update.cl:11-118), in place.
+template <typename T>
+__device__ __forceinline__ void split_h_cell(const Geo &g, T dtm_dl, T dtm, T *__restrict__ st,
+                                             const T *__restrict__ prof, T *__restrict__ psi,
+                                             int i, int j, int k) {
+  const T *Ex = st, *Ey = st + g.vs, *Ez = st + 2 * g.vs;
+  T *Hx = st + 3 * g.vs, *Hy = st + 4 * g.vs, *Hz = st + 5 * g.vs;
+  const size_t sxy = (size_t)g.ny * g.nz;
+  const size_t idx = (size_t)i * sxy + (size_t)j * g.nz + k;
+  const bool jn = j + 1 < g.ny, kn = k + 1 < g.nz, in = i + 1 < g.nx;
+  const StripeIdx s = stripe_idx(g, i, j, k);
+  T p[6] = {0, 0, 0, 0, 0, 0};
+  psi_load(g, psi, 0, s, p);
+  T hx = Hx[idx], hy = Hy[idx], hz = Hz[idx];
+  h_math(g, i, j, k, s, prof1(g, prof, 1, 0, i), prof1(g, prof, 1, 1, j), prof1(g, prof, 1, 2, k),
+         Ex[idx], Ey[idx], Ez[idx], jn ? Ex[idx + g.nz] : T(0), jn ? Ez[idx + g.nz] : T(0),
+         kn ? Ex[idx + 1] : T(0), kn ? Ey[idx + 1] : T(0), in ? Ey[idx + sxy] : T(0),
+         in ? Ez[idx + sxy] : T(0), hx, hy, hz, p, dtm_dl, dtm);
+  Hx[idx] = hx;
+  Hy[idx] = hy;
+  Hz[idx] = hz;
+  psi_store(g, psi, 0, s, p);
+}
+
+// E(i,j,k) and its psi from H (reference yee_update.cl:120-238), in place.
+template <typename T, bool LOSSY>
+__device__ __forceinline__ void split_e_cell(const Geo &g, T inv_dl, T *__restrict__ st,
+                                             const T *__restrict__ coef,
+                                             const T *__restrict__ prof, T *__restrict__ psi,
+                                             int i, int j, int k) {
+  T *Ex = st, *Ey = st + g.vs, *Ez = st + 2 * g.vs;
+  const T *Hx = st + 3 * g.vs, *Hy = st + 4 * g.vs, *Hz = st + 5 * g.vs;
+  const size_t sxy = (size_t)g.ny * g.nz;
+  const size_t idx = (size_t)i * sxy + (size_t)j * g.nz + k;
+  const StripeIdx s = stripe_idx(g, i, j, k);
+  T p[6] = {0, 0, 0, 0, 0, 0};
+  psi_load(g, psi, 6, s, p);
+  T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
+  const T ax = LOSSY ? coef[3 * g.vs + idx] : T(1);
+  const T ay = LOSSY ? coef[4 * g.vs + idx] : T(1);
+  const T az = LOSSY ? coef[5 * g.vs + idx] : T(1);
+  e_math(g, i, j, k, s, prof1(g, prof, 0, 0, i), prof1(g, prof, 0, 1, j), prof1(g, prof, 0, 2, k),
+         Hx[idx], Hy[idx], Hz[idx], j > 0 ? Hx[idx - g.nz] : T(0), j > 0 ? Hz[idx - g.nz] : T(0),
+         k > 0 ? Hx[idx - 1] : T(0), k > 0 ? Hy[idx - 1] : T(0), i > 0 ? Hy[idx - sxy] : T(0),
+         i > 0 ? Hz[idx - sxy] : T(0), ex, ey, ez, ax, ay, az, coef[idx], coef[g.vs + idx],
+         coef[2 * g.vs + idx], p, inv_dl);
+  Ex[idx] = ex;
+  Ey[idx] = ey;
+  Ez[idx] = ez;
+  psi_store(g, psi, 6, s, p);
+}
+
 template <typename T>
 struct Src {
   int comp, mode, taper;
@@ -305,6 +361,19 @@ __device__ __forceinline__ int xcd_remap(int n, int N, int X) {
   const int q = N / X, r = N % X;
   const int x = n % X, m = n / X;
   return x * q + min(x, r) + m;
+}
+
+// Hardware block n runs on XCD n % X.  xcd_runs hands XCD x runs of R
+// consecutive logical blocks: within each group of X*R logical blocks, XCD x
+// takes blocks x*R .. x*R+R-1, so neighbouring rows stay in one XCD's L2
+// while all XCDs work in the same address region.  A bijection on [0, N):
+// a last partial group keeps the identity.  R = 1 is the identity.
+__device__ __forceinline__ int xcd_runs(int n, int N, int X, int R) {
+  if (X <= 1 || R <= 1) return n;
+  const int G = X * R;
+  if (n >= N / G * G) return n;
+  const int x = n % X, m = n / X;
+  return (m / R) * G + x * R + m % R;
 }
 
 __device__ __forceinline__ void lds_barrier() {

@@ -21,67 +21,41 @@
 namespace hfdtd {
 
 // ── split sweeps (reference order: all of H, then sources, then all of E) ──
-
-constexpr int kSplitRows = 4;
+//
+// One thread per cell over the flattened (j, k) plane: a block of 256 threads
+// covers 256 consecutive cells of one i-plane, so rows of any length pack the
+// wavefronts without idle lanes (64-lane k-rows would leave up to 63 lanes
+// idle per row).  Logical block L works on plane L / bpp, bpp = blocks per
+// plane; xcd_runs hands every XCD runs of consecutive L, so most j+-1
+// neighbour rows a cell reads come through the same XCD's L2, while all
+// XCDs stream the same region of the grid.
 
 template <typename T>
-__global__ __launch_bounds__(64 * kSplitRows) void k_update_H(Geo g, T dtm_dl, T dtm,
-                                                              T *__restrict__ st,
-                                                              const T *__restrict__ prof,
-                                                              T *__restrict__ psi) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * kSplitRows + threadIdx.y;
-  const int i = blockIdx.z;
-  if (k >= g.nz || j >= g.ny) return;
-  const T *Ex = st, *Ey = st + g.vs, *Ez = st + 2 * g.vs;
-  T *Hx = st + 3 * g.vs, *Hy = st + 4 * g.vs, *Hz = st + 5 * g.vs;
-  const size_t sxy = (size_t)g.ny * g.nz;
-  const size_t idx = (size_t)i * sxy + (size_t)j * g.nz + k;
-  const bool jn = j + 1 < g.ny, kn = k + 1 < g.nz, in = i + 1 < g.nx;
-  const StripeIdx s = stripe_idx(g, i, j, k);
-  T p[6] = {0, 0, 0, 0, 0, 0};
-  psi_load(g, psi, 0, s, p);
-  T hx = Hx[idx], hy = Hy[idx], hz = Hz[idx];
-  h_math(g, i, j, k, s, prof1(g, prof, 1, 0, i), prof1(g, prof, 1, 1, j), prof1(g, prof, 1, 2, k),
-         Ex[idx], Ey[idx], Ez[idx], jn ? Ex[idx + g.nz] : T(0),
-         jn ? Ez[idx + g.nz] : T(0), kn ? Ex[idx + 1] : T(0), kn ? Ey[idx + 1] : T(0),
-         in ? Ey[idx + sxy] : T(0), in ? Ez[idx + sxy] : T(0), hx, hy, hz, p, dtm_dl, dtm);
-  Hx[idx] = hx;
-  Hy[idx] = hy;
-  Hz[idx] = hz;
-  psi_store(g, psi, 0, s, p);
+__global__ __launch_bounds__(kFlatBlock) void k_update_H(Geo g, T dtm_dl, T dtm,
+                                                          T *__restrict__ st,
+                                                          const T *__restrict__ prof,
+                                                          T *__restrict__ psi, int bpp,
+                                                          int xcd, int run) {
+  const int L = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
+  const int i = L / bpp;
+  const int q = (L - i * bpp) * kFlatBlock + threadIdx.x;
+  if (q >= g.ny * g.nz) return;
+  const int j = q / g.nz;
+  split_h_cell(g, dtm_dl, dtm, st, prof, psi, i, j, q - j * g.nz);
 }
 
 template <typename T, bool LOSSY>
-__global__ __launch_bounds__(64 * kSplitRows) void k_update_E(Geo g, T inv_dl,
-                                                              T *__restrict__ st,
-                                                              const T *__restrict__ coef,
-                                                              const T *__restrict__ prof,
-                                                              T *__restrict__ psi) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * kSplitRows + threadIdx.y;
-  const int i = blockIdx.z;
-  if (k >= g.nz || j >= g.ny) return;
-  T *Ex = st, *Ey = st + g.vs, *Ez = st + 2 * g.vs;
-  const T *Hx = st + 3 * g.vs, *Hy = st + 4 * g.vs, *Hz = st + 5 * g.vs;
-  const size_t sxy = (size_t)g.ny * g.nz;
-  const size_t idx = (size_t)i * sxy + (size_t)j * g.nz + k;
-  const StripeIdx s = stripe_idx(g, i, j, k);
-  T p[6] = {0, 0, 0, 0, 0, 0};
-  psi_load(g, psi, 6, s, p);
-  T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
-  const T ax = LOSSY ? coef[3 * g.vs + idx] : T(1);
-  const T ay = LOSSY ? coef[4 * g.vs + idx] : T(1);
-  const T az = LOSSY ? coef[5 * g.vs + idx] : T(1);
-  e_math(g, i, j, k, s, prof1(g, prof, 0, 0, i), prof1(g, prof, 0, 1, j), prof1(g, prof, 0, 2, k),
-         Hx[idx], Hy[idx], Hz[idx], j > 0 ? Hx[idx - g.nz] : T(0),
-         j > 0 ? Hz[idx - g.nz] : T(0), k > 0 ? Hx[idx - 1] : T(0), k > 0 ? Hy[idx - 1] : T(0),
-         i > 0 ? Hy[idx - sxy] : T(0), i > 0 ? Hz[idx - sxy] : T(0), ex, ey, ez, ax, ay, az,
-         coef[idx], coef[g.vs + idx], coef[2 * g.vs + idx], p, inv_dl);
-  Ex[idx] = ex;
-  Ey[idx] = ey;
-  Ez[idx] = ez;
-  psi_store(g, psi, 6, s, p);
+__global__ __launch_bounds__(kFlatBlock) void k_update_E(Geo g, T inv_dl, T *__restrict__ st,
+                                                          const T *__restrict__ coef,
+                                                          const T *__restrict__ prof,
+                                                          T *__restrict__ psi, int bpp,
+                                                          int xcd, int run) {
+  const int L = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
+  const int i = L / bpp;
+  const int q = (L - i * bpp) * kFlatBlock + threadIdx.x;
+  if (q >= g.ny * g.nz) return;
+  const int j = q / g.nz;
+  split_e_cell<T, LOSSY>(g, inv_dl, st, coef, prof, psi, i, j, q - j * g.nz);
 }
 
 // ── sources ──────────────────────────────────────────────────────────────
@@ -161,25 +135,25 @@ __global__ __launch_bounds__(256) void k_source_fixup(Geo g, T inv_dl, const T *
 
 template <typename T>
 static int update_H_t(Engine *e) {
-  dim3 grid(ceil_div(e->nz, 64), ceil_div(e->ny, kSplitRows), e->nx);
+  const unsigned bpp = ceil_div((size_t)e->ny * e->nz, kFlatBlock);
   int rc = timing_mark(e);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_update_H<T>, grid, dim3(64, kSplitRows), 0, e->stream, geo_of(e),
-                     T(e->dtm_dl), T(e->dtm), (T *)e->state[e->cur], (const T *)e->prof,
-                     (T *)e->psi[e->cur]);
+  hipLaunchKernelGGL(k_update_H<T>, dim3(bpp * (unsigned)e->nx), dim3(kFlatBlock), 0, e->stream,
+                     geo_of(e), T(e->dtm_dl), T(e->dtm), (T *)e->state[e->cur],
+                     (const T *)e->prof, (T *)e->psi[e->cur], (int)bpp, e->xcd, e->split_run);
   HF_HIP(hipGetLastError());
   return timing_mark(e);
 }
 
 template <typename T>
 static int update_E_t(Engine *e) {
-  dim3 grid(ceil_div(e->nz, 64), ceil_div(e->ny, kSplitRows), e->nx);
+  const unsigned bpp = ceil_div((size_t)e->ny * e->nz, kFlatBlock);
   auto kern = e->lossless ? k_update_E<T, false> : k_update_E<T, true>;
   int rc = timing_mark(e);
   if (rc) return rc;
-  hipLaunchKernelGGL(kern, grid, dim3(64, kSplitRows), 0, e->stream, geo_of(e), T(e->inv_dl),
-                     (T *)e->state[e->cur], (const T *)e->coef, (const T *)e->prof,
-                     (T *)e->psi[e->cur]);
+  hipLaunchKernelGGL(kern, dim3(bpp * (unsigned)e->nx), dim3(kFlatBlock), 0, e->stream,
+                     geo_of(e), T(e->inv_dl), (T *)e->state[e->cur], (const T *)e->coef,
+                     (const T *)e->prof, (T *)e->psi[e->cur], (int)bpp, e->xcd, e->split_run);
   HF_HIP(hipGetLastError());
   return timing_mark(e);
 }
@@ -198,10 +172,10 @@ static int sources_t(Engine *e, const hfdtd_source *src, int n) {
   return HFDTD_OK;
 }
 
-template <typename T, int TJ>
+template <typename T, int TJ, bool LOSSY>
 static int fused_tj(Engine *e, const SrcList<T> &sl) {
-  auto kern = e->lossless ? k_fused<T, TJ, false> : k_fused<T, TJ, true>;
-  int &occ = e->fused_occ[e->lossless ? 0 : 1][TJ];
+  auto kern = k_fused<T, TJ, LOSSY>;
+  int &occ = e->fused_occ[LOSSY ? 1 : 0][TJ];
   if (occ == 0) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * (TJ + 1), 0) !=
@@ -228,7 +202,7 @@ static int fused_tj(Engine *e, const SrcList<T> &sl) {
     const Src<T> &s = sl.s[q];
     most = std::max(most, (long long)(s.ci1 - s.ci0) * (s.cj1 - s.cj0) * (s.ck1 - s.ck0));
   }
-  auto fix = e->lossless ? k_source_fixup<T, false> : k_source_fixup<T, true>;
+  auto fix = k_source_fixup<T, LOSSY>;
   hipLaunchKernelGGL(fix, dim3((unsigned)ceil_div(most, 256), sl.n), dim3(256), 0, e->stream,
                      geo_of(e), T(e->inv_dl), (const T *)e->state[e->cur], (T *)e->state[o],
                      (const T *)e->coef, (const T *)e->prof, (const T *)e->psi[e->cur],
@@ -252,20 +226,24 @@ static int fused_t(Engine *e, const hfdtd_source *src, int n) {
                                     "use the split sweeps for this grid");
   int rc = ensure_alt(e);
   if (rc) return rc;
-  // FP64 slots are twice as large: tiles of at most 4 rows fit the LDS
+  // FP64 slots are twice as large: tiles of at most 4 rows fit the LDS.
+  // FP32 tiles of 12 rows fit only without the ca rows (lossy grids use 10).
+  const bool ll = e->lossless;
   if constexpr (sizeof(T) == 8) {
     switch (e->tile_rows) {
-      case 1: rc = fused_tj<T, 1>(e, sl); break;
-      case 2: rc = fused_tj<T, 2>(e, sl); break;
-      default: rc = fused_tj<T, 4>(e, sl); break;
+      case 1: rc = ll ? fused_tj<T, 1, false>(e, sl) : fused_tj<T, 1, true>(e, sl); break;
+      case 2: rc = ll ? fused_tj<T, 2, false>(e, sl) : fused_tj<T, 2, true>(e, sl); break;
+      default: rc = ll ? fused_tj<T, 4, false>(e, sl) : fused_tj<T, 4, true>(e, sl); break;
     }
   } else {
     switch (e->tile_rows) {
-      case 1: rc = fused_tj<T, 1>(e, sl); break;
-      case 2: rc = fused_tj<T, 2>(e, sl); break;
-      case 6: rc = fused_tj<T, 6>(e, sl); break;
-      case 8: rc = fused_tj<T, 8>(e, sl); break;
-      default: rc = fused_tj<T, 4>(e, sl); break;
+      case 1: rc = ll ? fused_tj<T, 1, false>(e, sl) : fused_tj<T, 1, true>(e, sl); break;
+      case 2: rc = ll ? fused_tj<T, 2, false>(e, sl) : fused_tj<T, 2, true>(e, sl); break;
+      case 6: rc = ll ? fused_tj<T, 6, false>(e, sl) : fused_tj<T, 6, true>(e, sl); break;
+      case 8: rc = ll ? fused_tj<T, 8, false>(e, sl) : fused_tj<T, 8, true>(e, sl); break;
+      case 10: rc = ll ? fused_tj<T, 10, false>(e, sl) : fused_tj<T, 10, true>(e, sl); break;
+      case 12: rc = ll ? fused_tj<T, 12, false>(e, sl) : fused_tj<T, 10, true>(e, sl); break;
+      default: rc = ll ? fused_tj<T, 4, false>(e, sl) : fused_tj<T, 4, true>(e, sl); break;
     }
   }
   if (rc) return rc;

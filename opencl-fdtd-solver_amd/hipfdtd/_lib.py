@@ -82,6 +82,7 @@ EXPORTS = {
     "hfdtd_step": (C.c_int, [C.c_void_p, C.POINTER(Source), C.c_int, C.c_int]),
     "hfdtd_synchronize": (C.c_int, [C.c_void_p]),
     "hfdtd_set_fused_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "hfdtd_set_split_schedule": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "hfdtd_n2f_create": (
@@ -138,6 +139,7 @@ EXPORTS = {
     "hfdtd_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "hfdtd_batch_read_probes": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "hfdtd_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "hfdtd_batch_set_path": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_adjoint_create": (
         C.c_int,
         [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_void_p)],
@@ -158,7 +160,7 @@ EXPORTS = {
     "hfdtd_adjoint_forward": (
         C.c_int,
         [C.c_void_p, C.POINTER(Source), C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int64,
-         C.c_int, C.c_int],
+         C.c_int, C.c_int, C.c_int],
     ),  # fmt: skip
     "hfdtd_adjoint_begin": (C.c_int, [C.c_void_p, C.c_int64]),
     "hfdtd_adjoint_reverse": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int]),

@@ -79,11 +79,13 @@ class ProbeDFTAdjoint:
     :class:`CurrentSource` (or (component, box, waveform) tuples, at most 16);
     probes: iterable of (component, i, j, k) with component Ex/Ey/Ez;
     freqs [Hz]; steps: run length N; checkpoint: segment length K;
-    eps_background: εr outside the design box (default 1).
+    eps_background: εr outside the design box (default 1); fused: the
+    forward path (``HIPFDTD``'s ``fused``, default per dtype).
     """
 
     def __init__(self, shape, dl, npml, design_box, sources, probes, freqs, steps, *,
-                 eps_background=None, checkpoint=100, dtype=np.float32, device=0):  # fmt: skip
+                 eps_background=None, checkpoint=100, dtype=np.float32, device=0,
+                 fused=None):  # fmt: skip
         self.shape = tuple(int(s) for s in shape)
         self.steps = int(steps)
         if self.steps < 1:
@@ -93,7 +95,7 @@ class ProbeDFTAdjoint:
         self.design_box = tuple(int(v) for v in design_box)
         if len(self.design_box) != 6:
             raise ValueError("design_box is (i0, i1, j0, j1, k0, k1)")
-        self.fdtd = HIPFDTD(self.shape, dl, npml=npml, dtype=dtype, device=device)
+        self.fdtd = HIPFDTD(self.shape, dl, npml=npml, dtype=dtype, device=device, fused=fused)
         self._lib = self.fdtd._lib
         if eps_background is None:
             eps_background = np.ones(self.shape)
@@ -146,8 +148,9 @@ class ProbeDFTAdjoint:
     def _run(self, n0, n1, slot0, probe):
         amp = self._amp[n0:n1]
         ptr = amp.ctypes.data_as(C.POINTER(C.c_double)) if self._nsrc else None
+        path = _lib.PATH_FUSED if self.fdtd.fused else _lib.PATH_SPLIT
         _lib.check(self._lib.hfdtd_adjoint_forward(self._handle, self._src, self._nsrc, ptr,
-                                                   n1 - n0, n0, slot0, int(probe)),
+                                                   n1 - n0, n0, slot0, int(probe), path),
                    "adjoint forward")  # fmt: skip
 
     def _sync(self):

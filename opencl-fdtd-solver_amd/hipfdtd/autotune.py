@@ -7,10 +7,11 @@ how many blocks a grid yields and how often the two halo planes are re-read;
 a persistent batched launch also picks its grid size.  The best choice
 depends on the grid extents, the dtype, lossy or not, and the device.
 
-``tune_engine(sim)`` times the fused sweep (HIP events on the engine stream)
-for every candidate on a scratch engine of the same shape and applies the
-fastest; ``tune_batch(batch)`` does the same for a ``BatchedFDTD`` before its
-tables are set.  All stepping happens on zero fields with no sources, which
+``tune_engine(sim)`` times the split sweeps and every fused-sweep candidate
+(HIP events on the engine stream) on a scratch engine of the same shape and
+applies the fastest; ``tune_batch(batch)`` does the same for a
+``BatchedFDTD`` (fused tiles or flat split sweeps) before its tables are
+set.  All stepping happens on zero fields with no sources, which
 leaves the fields exactly zero, so tuning never changes a simulation.
 Results are cached in a JSON file keyed by (kind, shapes, dtype, npml, lossy,
 device, library ABI); ``HIPFDTD_TUNE_CACHE`` names the file (default
@@ -31,7 +32,7 @@ import numpy as np
 
 from . import _lib
 
-FP32_ROWS = (1, 2, 4, 6, 8)
+FP32_ROWS = (2, 4, 6, 8, 10, 12)
 FP64_ROWS = (1, 2, 4)
 PLANES = (-1, 8, 16, 24, 32, 48, 64)
 BATCH_ROWS_FP32 = (2, 4, 6, 8)
@@ -97,55 +98,69 @@ def _time_engine(sim, steps, warmup):
     return ms / max(n, 1) * 1e3
 
 
-def tune_engine(sim, *, rows=None, planes=None, steps=20, warmup=3, use_cache=True,
-                write_cache=True, cache_file=None) -> dict:  # fmt: skip
-    """Pick (rows_j, planes_i) of the fused sweep for ``sim`` and apply it.
+def tune_engine(sim, *, rows=None, planes=None, paths=("fused", "split"), steps=20, warmup=3,
+                use_cache=True, write_cache=True, cache_file=None) -> dict:  # fmt: skip
+    """Pick the step path and, for the fused sweep, (rows_j, planes_i) for
+    ``sim``, and apply them.
 
-    Returns {"rows": r, "planes": p, "us": fastest µs per sweep, "table":
-    [[r, p, µs], ...], "cached": bool}.  ``planes`` -1 is the grid-derived
-    default.  Engines that run the split sweeps are left unchanged.
+    Times the fused sweep for rows x planes candidates and the split sweeps
+    (one candidate) on a scratch engine of the same shape.  Returns {"path":
+    "fused" | "split", "rows", "planes", "us": fastest µs of sweep kernels per
+    step, "table": [[path, rows, planes, µs], ...], "cached": bool};
+    ``planes`` -1 is the grid-derived default.  The fused candidates are
+    skipped for grids beyond the fused sweep's 4 GB slab limit.
     """
     from .engine import HIPFDTD
 
-    if not sim.fused:
-        return {"rows": None, "planes": None, "us": None, "table": [], "cached": False}
     shape = (sim.Nx, sim.Ny, sim.Nz)
-    key = cache_key("fused", [shape], sim.dtype, sim.npml, sim.lossy, sim.device_name)
+    key = cache_key("engine", [shape], sim.dtype, sim.npml, sim.lossy, sim.device_name)
     if use_cache:
         hit = load_cache(cache_file).get(key)
         if hit:
-            sim.set_fused_tiling(hit["rows"], hit["planes"], 0)
+            sim.set_path(hit["path"] == "fused")
+            if hit["path"] == "fused":
+                sim.set_fused_tiling(hit["rows"], hit["planes"], 0)
             return dict(hit, cached=True)
     fp64 = sim.dtype == np.dtype(np.float64)
     rows = rows or (FP64_ROWS if fp64 else FP32_ROWS)
     planes = planes or PLANES
-    scratch = HIPFDTD(shape, sim.dl, npml=sim.npml, dtype=sim.dtype, device=sim.device)
+    scratch = HIPFDTD(shape, sim.dl, npml=sim.npml, dtype=sim.dtype, device=sim.device,
+                      fused=False)  # fmt: skip
+    table = []
     try:
         if sim.lossy:
             scratch.set_epsilon(np.ones(shape, np.float32), np.zeros(shape, np.float32))
-        table = []
-        for r in rows:
-            for p in planes:
-                scratch.set_fused_tiling(r, p, 0)
-                table.append([r, p, round(_time_engine(scratch, steps, warmup), 3)])
+        if "split" in paths:
+            table.append(["split", None, None, round(_time_engine(scratch, steps, warmup), 3)])
+        scratch.set_path(True)
+        if "fused" in paths and scratch.fused:
+            for r in rows:
+                for p in planes:
+                    scratch.set_fused_tiling(r, p, 0)
+                    table.append(["fused", r, p, round(_time_engine(scratch, steps, warmup), 3)])
     finally:
         scratch.close()
-    r, p, us = min(table, key=lambda e: e[2])
-    sim.set_fused_tiling(r, p, 0)
-    entry = {"rows": r, "planes": p, "us": us, "table": table,
+    path, r, p, us = min(table, key=lambda e: e[3])
+    sim.set_path(path == "fused")
+    if path == "fused":
+        sim.set_fused_tiling(r, p, 0)
+    entry = {"path": path, "rows": r, "planes": p, "us": us, "table": table,
              "when": time.strftime("%Y-%m-%dT%H:%M:%S")}  # fmt: skip
     if write_cache:
         save_cache({key: entry}, cache_file)
     return dict(entry, cached=False)
 
 
-def tune_batch(batch, *, rows=None, planes=None, blocks=(0,), steps=10, warmup=2,
-               use_cache=True, write_cache=True, cache_file=None) -> dict:  # fmt: skip
-    """Pick (blocks, rows_j, planes_i) of a ``BatchedFDTD`` and apply it.
+def tune_batch(batch, *, rows=None, planes=None, blocks=(0,), paths=("fused", "split"),
+               steps=10, warmup=2, use_cache=True, write_cache=True, cache_file=None) -> dict:  # fmt: skip
+    """Pick the path ('fused' tiles or 'split' flat sweeps) and (blocks,
+    rows_j, planes_i) of a ``BatchedFDTD`` and apply them.
 
     Must run before the batch's source / probe tables are set, while every
     simulation is still at step 0 with zero fields (stepping them then leaves
-    them unchanged).  Returns the choice, its ms per step and the table.
+    them unchanged).  Returns the choice, its ms per step and the table
+    [[path, blocks, rows, planes, ms], ...]; rows and planes do not apply to
+    the split path.
     """
     sims = batch.sims
     if batch._n_table or any(s.step_num for s in sims):
@@ -156,24 +171,32 @@ def tune_batch(batch, *, rows=None, planes=None, blocks=(0,), steps=10, warmup=2
     if use_cache:
         hit = load_cache(cache_file).get(key)
         if hit:
-            batch.set_launch(hit["blocks"], hit["rows"], hit["planes"])
+            batch.set_path(hit["path"])
+            batch.set_launch(hit["blocks"], hit["rows"] or 0, hit["planes"] or 0)
             return dict(hit, cached=True)
     fp64 = s0.dtype == np.dtype(np.float64)
     rows = rows or (BATCH_ROWS_FP64 if fp64 else BATCH_ROWS_FP32)
     planes = planes or BATCH_PLANES
     lib = batch._lib
+
+    def timed(path, b, r, p):
+        batch.set_path(path)
+        batch.set_launch(b, r or 0, p or 0)
+        _lib.check(lib.hfdtd_batch_run(batch._handle, warmup, 0), "batch tune")
+        t0 = time.perf_counter()
+        _lib.check(lib.hfdtd_batch_run(batch._handle, steps, 0), "batch tune")
+        return [path, b, r, p, round((time.perf_counter() - t0) / steps * 1e3, 4)]
+
     table = []
     for b in blocks:
-        for r in rows:
-            for p in planes:
-                batch.set_launch(b, r, p)
-                _lib.check(lib.hfdtd_batch_run(batch._handle, warmup, 0), "batch tune")
-                t0 = time.perf_counter()
-                _lib.check(lib.hfdtd_batch_run(batch._handle, steps, 0), "batch tune")
-                table.append([b, r, p, round((time.perf_counter() - t0) / steps * 1e3, 4)])
-    b, r, p, ms = min(table, key=lambda e: e[3])
-    batch.set_launch(b, r, p)
-    entry = {"blocks": b, "rows": r, "planes": p, "ms": ms, "table": table,
+        if "fused" in paths:
+            table += [timed("fused", b, r, p) for r in rows for p in planes]
+        if "split" in paths:
+            table.append(timed("split", b, None, None))
+    path, b, r, p, ms = min(table, key=lambda e: e[4])
+    batch.set_path(path)
+    batch.set_launch(b, r or 0, p or 0)
+    entry = {"path": path, "blocks": b, "rows": r, "planes": p, "ms": ms, "table": table,
              "when": time.strftime("%Y-%m-%dT%H:%M:%S")}  # fmt: skip
     if write_cache:
         save_cache({key: entry}, cache_file)

@@ -34,6 +34,7 @@ class BatchedFDTD:
         self._n_table = 0
         self._step = 0
         self._nfreq = 0
+        self.path = "fused"
 
     def close(self):
         if self._handle is not None:
@@ -97,6 +98,16 @@ class BatchedFDTD:
     def set_launch(self, blocks: int = 0, rows_j: int = 0, planes_i: int = 0):
         _lib.check(self._lib.hfdtd_batch_set_launch(self._handle, int(blocks), int(rows_j),
                                                     int(planes_i)))  # fmt: skip
+
+    def set_path(self, path: str) -> None:
+        """'fused' (one persistent fused-tile launch per step, the default) or
+        'split' (persistent split sweeps over flattened planes: H, sources,
+        E, probes; no idle lanes on extents that are not multiples of 64)."""
+        if path not in ("fused", "split"):
+            raise ValueError("path must be 'fused' or 'split'")
+        code = _lib.PATH_FUSED if path == "fused" else _lib.PATH_SPLIT
+        _lib.check(self._lib.hfdtd_batch_set_path(self._handle, code), "batch_set_path")
+        self.path = path
 
     def run(self, n_steps: int, graph_steps: int = 0):
         """Advance every simulation ``n_steps`` steps (one launch per step, or

@@ -6,17 +6,25 @@ C ABI of ``include/hipfdtd.h``.  All device memory is owned by the library.
 
 Step execution.  The reference runs ``update_H``, then the source callbacks
 (at ``t + dt/2``), then ``update_E``, then the monitors
-(``engine.py:724-737``).  ``HIPFDTD`` keeps that order but runs H, sources
-and E as one fused sweep when it can: during the source phase,
-``add_source_Jx`` / ``add_source_Ex`` / ``add_source_J`` calls are recorded
-instead of launched; the fused sweep advances H and E everywhere, then a
-small fix-up kernel recomputes E on the source cells with the recorded
-sources added first, which is the reference's order cell by cell.  A
-callback that touches field data (``Ex``, ``read_point``, a field setter ...)
-first forces the H half to run on its own, so it sees exactly the state the
-reference would show it; the step then finishes with the split E sweep.
-``fused=False`` always uses the split sweeps, and so do grids whose field
-slab (six volumes) reaches 4 GB, beyond the fused sweep's 32-bit offsets.
+(``engine.py:724-737``).  ``HIPFDTD`` keeps that order on two paths:
+
+* split sweeps (``fused=False``): ``update_H``, the source callbacks (each
+  ``add_source_*`` launches at once), ``update_E``; one thread per cell over
+  the flattened (j, k) planes;
+* fused sweep (``fused=True``): during the source phase, ``add_source_*``
+  calls are recorded instead of launched; one sweep advances H and E
+  everywhere, then a small fix-up kernel recomputes E on the source cells
+  with the recorded sources added first, which is the reference's order
+  cell by cell.  A callback that touches field data (``Ex``, ``read_point``,
+  a field setter ...) first forces the H half to run on its own, so it sees
+  exactly the state the reference would show it; the step then finishes
+  with the split E sweep.  Grids whose field slab (six volumes) reaches 4 GB,
+  beyond the fused sweep's 32-bit offsets, use the split sweeps.
+
+``fused=None`` (the default) picks the path measured faster on MI355X: the
+split sweeps for FP32 (they stream at close to copy bandwidth and leave no
+lanes idle on extents that are not multiples of 64), the fused sweep for
+FP64.  ``hipfdtd.autotune.tune_engine`` measures both for a given grid.
 """
 
 from __future__ import annotations
@@ -50,7 +58,7 @@ class HIPFDTD(SourceMonitorMixin):
     MEMORY_HEADROOM_BYTES = 512 * 1024 * 1024
 
     def __init__(self, shape, dl=None, npml=20, dtype=np.float32, ctx=None, queue=None,
-                 accum_fp64=False, *, device=0, dx=None, fused=True):  # fmt: skip
+                 accum_fp64=False, *, device=0, dx=None, fused=None):  # fmt: skip
         if dl is None:
             dl = dx
         if dl is None:
@@ -84,7 +92,10 @@ class HIPFDTD(SourceMonitorMixin):
         item = np.dtype(self.dtype).itemsize
         vstride = (self.Nx * self.Ny * self.Nz + 64 + 63) // 64 * 64
         psi_total = 4 * (self.psi_x_size + self.psi_y_size + self.psi_z_size) + 64
-        self.fused = bool(fused) and max(6 * vstride, psi_total) * item < 2**32
+        self._fused_fits = max(6 * vstride, psi_total) * item < 2**32
+        if fused is None:
+            fused = self.default_fused(dtype)
+        self.fused = bool(fused) and self._fused_fits
         self._sources = []
         self._monitors = []
         self.lossy = False  # σ or ca volumes set: the sweeps read ca
@@ -132,9 +143,20 @@ class HIPFDTD(SourceMonitorMixin):
     # ── memory budget (engine.py:320-363) ──
 
     @staticmethod
-    def estimate_device_memory_bytes(shape, npml, dtype=np.float32, fused=True):
+    def default_fused(dtype) -> bool:
+        """Path chosen when ``fused=None``: fused sweep for FP64, split for FP32."""
+        return np.dtype(dtype) == np.dtype(np.float64)
+
+    def set_path(self, fused: bool) -> None:
+        """Switch between the fused sweep (if the grid allows it) and the split
+        sweeps; takes effect at the next step, the state carries over."""
+        self.fused = bool(fused) and self._fused_fits
+
+    @staticmethod
+    def estimate_device_memory_bytes(shape, npml, dtype=np.float32, fused=None):
         """Device bytes for the 12 volume arrays, the face psi stripes and,
-        with ``fused``, the ping-pong copies of fields and psi.
+        with ``fused`` (default: the path ``fused=None`` picks for ``dtype``),
+        the ping-pong copies of fields and psi.
 
         The reference counts 9 volume arrays (``engine.py:325``) and leaves out
         the three ``ca`` arrays it allocates; this estimate counts all twelve.
@@ -146,6 +168,8 @@ class HIPFDTD(SourceMonitorMixin):
         if npml > 0:
             psi = 4 * (2 * npml * ny * nz + nx * 2 * npml * nz + nx * ny * 2 * npml) * item
         total = 12 * vol + psi
+        if fused is None:
+            fused = HIPFDTD.default_fused(dtype)
         if fused:
             total += 6 * vol + psi
         return total
@@ -326,6 +350,11 @@ class HIPFDTD(SourceMonitorMixin):
         planes_i=-1 returns to the grid-derived plane count)."""
         _lib.check(self._lib.hfdtd_set_fused_tiling(self._handle, int(rows_j), int(planes_i),
                                                     int(xcd_groups)))  # fmt: skip
+
+    def set_split_schedule(self, run_blocks: int = 0) -> None:
+        """Work order of the split sweeps: runs of ``run_blocks`` consecutive
+        256-cell blocks per XCD (1 = hardware round robin, 0 keeps)."""
+        _lib.check(self._lib.hfdtd_set_split_schedule(self._handle, int(run_blocks)))
 
     def kernel_timing(self, enable: bool) -> None:
         """Start (True) or stop (False) HIP-event timing of the sweep kernels."""

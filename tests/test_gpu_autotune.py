@@ -1,9 +1,10 @@
 """GPU: the autotuner picks a launch shape without changing results.
 
 * ``tune_engine`` on a fresh engine leaves its fields zero, applies a
-  candidate from its table, is served from the cache the second time, and
-  the tuned engine steps bit-identically to an untuned one (the tile shape
-  only reorders independent cell updates);
+  candidate (the split sweeps or a fused tiling) from its table, is served
+  from the cache the second time, and the tuned engine steps bit-identically
+  to an untuned one on the same path (the tile shape only reorders
+  independent cell updates);
 * ``tune_batch`` on a fresh batch applies a schedule and refuses a batch
   whose tables are set.
 """
@@ -32,14 +33,16 @@ def test_tune_engine_applies_and_caches(tmp_path):
     cache = tmp_path / "tune.json"
     sim = hipfdtd.HIPFDTD(SHAPE, DL, npml=6)
     res = tune_engine(sim, planes=(-1, 8, 24), steps=5, warmup=1, cache_file=cache)
-    print("[autotune] engine", res["rows"], res["planes"], res["us"], flush=True)
+    print("[autotune] engine", res["path"], res["rows"], res["planes"], res["us"], flush=True)
     assert not res["cached"]
-    assert [res["rows"], res["planes"], res["us"]] in res["table"]
+    assert [res["path"], res["rows"], res["planes"], res["us"]] in res["table"]
+    assert len(res["table"]) == 1 + 6 * 3
+    assert sim.fused == (res["path"] == "fused")
     assert cache.exists()
     again = tune_engine(sim, cache_file=cache)
-    assert again["cached"] and again["rows"] == res["rows"] and again["planes"] == res["planes"]
+    assert again["cached"] and again["path"] == res["path"] and again["planes"] == res["planes"]
     assert not np.any(sim.read_volume("Ex"))
-    ref = hipfdtd.HIPFDTD(SHAPE, DL, npml=6)
+    ref = hipfdtd.HIPFDTD(SHAPE, DL, npml=6, fused=res["path"] == "fused")
     for s in (sim, ref):
         s.add_source(src)
         s.run(25)
@@ -56,8 +59,12 @@ def test_tune_batch_applies_schedule(tmp_path):
     b = BatchedFDTD(sims)
     res = tune_batch(b, rows=(2, 4), planes=(8, 16), steps=3, warmup=1,
                      cache_file=tmp_path / "tune.json")  # fmt: skip
-    print("[autotune] batch", res["blocks"], res["rows"], res["planes"], res["ms"], flush=True)
-    assert (res["rows"], res["planes"]) in {(2, 8), (2, 16), (4, 8), (4, 16)}
+    print("[autotune] batch", res["path"], res["blocks"], res["rows"], res["planes"], res["ms"],
+          flush=True)  # fmt: skip
+    assert len(res["table"]) == 5
+    assert res["path"] == "split" or (res["rows"], res["planes"]) in {(2, 8), (2, 16), (4, 8),
+                                                                      (4, 16)}  # fmt: skip
+    assert b.path == res["path"]
     for s in sims:
         assert not np.any(s.read_volume("Ez"))
     b.set_sources([("z", (10, 11, 10, 11, 10, 11))] * 3, [lambda t: 1.0] * 3, 4)

@@ -72,11 +72,15 @@ def test_point_probe_matches_host_dft():
     assert np.max(np.abs(got - acc)) / np.max(np.abs(acc)) < 1e-5
 
 
-@pytest.mark.parametrize("graph_steps", [0, 10])
-def test_batch_matches_serial_engines(graph_steps):
+@pytest.mark.parametrize("graph_steps,path", [(0, "fused"), (10, "fused"), (0, "split"),
+                                              (10, "split")])  # fmt: skip
+def test_batch_matches_serial_engines(graph_steps, path):
+    """Batched fused tiles match the serial fused engines to 1e-6 (the sweep
+    differs from the serial engines' launch shape only); the batched split
+    sweeps match serial split engines bit for bit."""
     rng = np.random.default_rng(4)
     shapes = [tuple(int(v) for v in rng.integers(20, 72, size=3)) for _ in range(6)]
-    print(f"[batch] shapes={shapes} graph_steps={graph_steps}", flush=True)
+    print(f"[batch] shapes={shapes} graph_steps={graph_steps} path={path}", flush=True)
     n_steps = 60
     cases = [sphere_case(rng, s) for s in shapes]
     nf = 4
@@ -91,7 +95,7 @@ def test_batch_matches_serial_engines(graph_steps):
     freqs = [list(np.linspace(0.7, 1.3, nf) * f) for _, f in cases]
     serial = []
     for shape, (eps, f), fr in zip(shapes, cases, freqs):
-        sim = hip().HIPFDTD(shape, 1e-3, npml=8)
+        sim = hip().HIPFDTD(shape, 1e-3, npml=8, fused=path == "fused")
         sim.set_epsilon(eps)
         wf = waveform(f)
         sim.add_source(lambda s, wf=wf, b=src_box(shape): s.add_source_J("z", wf(s.t), b))
@@ -104,6 +108,7 @@ def test_batch_matches_serial_engines(graph_steps):
         sim.set_epsilon(eps)
         sims.append(sim)
     batch = hip().batch.BatchedFDTD(sims)
+    batch.set_path(path)
     batch.set_sources([("z", src_box(s)) for s in shapes], [waveform(f) for _, f in cases],
                       n_steps)  # fmt: skip
     batch.set_probes([probe_cell(s) for s in shapes], "Ez", freqs, n_steps)
@@ -112,6 +117,10 @@ def test_batch_matches_serial_engines(graph_steps):
     dfts = batch.probe_dfts()
     for q, sim in enumerate(sims):
         ref_f, ref_p = serial[q]
+        if path == "split":
+            for name in ref_f:
+                np.testing.assert_array_equal(sim.fields()[name], ref_f[name], err_msg=name)
+            np.testing.assert_array_equal(dfts[q], ref_p)
         assert group_error(sim.fields(), ref_f) < 1e-6, q
         assert np.max(np.abs(dfts[q] - ref_p)) / np.max(np.abs(ref_p)) < 1e-6, q
         assert sim.step_num == n_steps and math.isclose(sim.t, n_steps * sim.dt, rel_tol=1e-12)

@@ -126,15 +126,16 @@ def test_fused_matches_split_and_oracle(shape, npml, lossy):
     assert group_error(out[False], ora.fields()) < 1e-5
 
 
-@pytest.mark.parametrize("rows,planes,xcd", [(1, 1, 8), (2, 5, 1), (4, 16, 8), (8, 7, 3),
-                                             (6, 1000, 8)])  # fmt: skip
-def test_fused_tilings_are_bit_identical(rows, planes, xcd):
+@pytest.mark.parametrize("rows,planes,xcd,lossy", [
+    (1, 1, 8, True), (2, 5, 1, True), (4, 16, 8, True), (8, 7, 3, True), (6, 1000, 8, True),
+    (10, 9, 8, True), (12, 13, 8, True), (12, 11, 8, False), (10, -1, 2, False)])  # fmt: skip
+def test_fused_tilings_are_bit_identical(rows, planes, xcd, lossy):
     shape = (45, 35, 100)
-    eps, sig = _random_media(shape, 3, True)
-    ref = hip().HIPFDTD(shape, 1e-3, npml=6)
+    eps, sig = _random_media(shape, 3, lossy)
+    ref = hip().HIPFDTD(shape, 1e-3, npml=6, fused=True)
     ref.set_epsilon(eps, sig)
     _drive(ref, shape, 12)
-    sim = hip().HIPFDTD(shape, 1e-3, npml=6)
+    sim = hip().HIPFDTD(shape, 1e-3, npml=6, fused=True)
     sim.set_fused_tiling(rows, planes, xcd)
     sim.set_epsilon(eps, sig)
     _drive(sim, shape, 12)
@@ -199,7 +200,8 @@ def test_field_access_read_point_and_shapes():
     assert sim.step_num == 1 and sim.t == pytest.approx(sim.dt)
 
 
-def test_callback_that_edits_fields_matches_soft_source():
+@pytest.mark.parametrize("fused", [True, False])
+def test_callback_that_edits_fields_matches_soft_source(fused):
     """The harness pattern `ex = f.Ex; ex[...] += amp; upload` (cases.py:64-75)
     equals add_source_Ex; the callback sees E before and H after the H update."""
     shape = (24, 22, 26)
@@ -211,9 +213,9 @@ def test_callback_that_edits_fields_matches_soft_source():
         ex[10:15, 9:14, 12] += np.float32(fo.gaussian_sine(f.t, 3e10, 1e10))
         f.Ex = ex
 
-    a = hip().HIPFDTD(shape, 1e-3, npml=4)
+    a = hip().HIPFDTD(shape, 1e-3, npml=4, fused=fused)
     a.add_source(edit)
-    b = hip().HIPFDTD(shape, 1e-3, npml=4)
+    b = hip().HIPFDTD(shape, 1e-3, npml=4, fused=fused)
     b.add_source(lambda f: f.add_source_Ex(12, fo.gaussian_sine(f.t, 3e10, 1e10), 10, 15, 9, 14))
     for _ in range(30):
         a.step()
@@ -222,9 +224,10 @@ def test_callback_that_edits_fields_matches_soft_source():
     assert seen[1][1] == pytest.approx(1.5 * a.dt)
 
 
-def test_npml_zero_runs_and_matches_oracle():
+@pytest.mark.parametrize("fused", [True, False])
+def test_npml_zero_runs_and_matches_oracle(fused):
     shape = (20, 20, 20)
-    sim = hip().HIPFDTD(shape, 1e-3, npml=0)
+    sim = hip().HIPFDTD(shape, 1e-3, npml=0, fused=fused)
     ora = fo.OracleFDTD(shape, 1e-3, npml=0)
     for s in (sim, ora):
         s.add_source(lambda f: f.add_source_Ex(10, 0.1))

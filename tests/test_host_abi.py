@@ -98,8 +98,11 @@ def test_memory_estimate_counts_all_arrays():
     psi = (4 * (2 * npml * 100 * 100) + 4 * (100 * 2 * npml * 100) + 4 * (100 * 100 * 2 * npml)) * 4
     split = HIPFDTD.estimate_device_memory_bytes(shape, npml, fused=False)
     assert split == 12 * 100**3 * 4 + psi
-    fused = HIPFDTD.estimate_device_memory_bytes(shape, npml)
+    fused = HIPFDTD.estimate_device_memory_bytes(shape, npml, fused=True)
     assert fused == split + 6 * 100**3 * 4 + psi
+    # default path: split sweeps for FP32, fused sweep for FP64
+    assert HIPFDTD.estimate_device_memory_bytes(shape, npml) == split
+    assert HIPFDTD.estimate_device_memory_bytes(shape, npml, np.float64) == 2 * fused
     assert HIPFDTD.estimate_device_memory_bytes((40, 50, 60), 0, fused=False) == 12 * 40 * 50 * 60 * 4
 
 
