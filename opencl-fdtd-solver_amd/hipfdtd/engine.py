@@ -334,6 +334,34 @@ class HIPFDTD(SourceMonitorMixin):
             if progress_every and i % progress_every == 0:
                 _LOG.info("  step %d/%d  t=%.3e s", i, n_steps, self.t)
 
+    def run_until_converged(self, monitor, tol=1e-3, check_every=100, max_steps=100_000,
+                            min_steps=0, freq_index=0):  # fmt: skip
+        """Step until a DFT monitor's sums settle.
+
+        Every ``check_every`` steps the monitor's relative change against its
+        last snapshot, ``dft_relative_change`` (reference ``monitors.py:498-569``,
+        reduced on the device: one scalar comes back per check), is recorded
+        and a new snapshot is taken; the run stops at the first check with a
+        change below ``tol`` once ``min_steps`` steps are done, or after
+        ``max_steps``.  Between checks nothing synchronises with the host.
+        Returns (steps run, [(step, relative change), ...]).
+        """
+        if check_every < 1:
+            raise ValueError("check_every must be >= 1")
+        history = []
+        done = 0
+        monitor.snapshot_dft()
+        while done < max_steps:
+            n = min(int(check_every), int(max_steps) - done)
+            self.run(n)
+            done += n
+            rel = monitor.dft_relative_change(freq_index)
+            monitor.snapshot_dft()
+            history.append((self.step_num, rel))
+            if done >= min_steps and rel < tol:
+                break
+        return done, history
+
     def synchronize(self) -> None:
         """Wait for all queued device work (``queue.finish()``)."""
         _lib.check(self._lib.hfdtd_synchronize(self._handle), "synchronize")

@@ -4,7 +4,9 @@ Run once in a container that has the reference checkout at ``/root/reference``
 (``python tests/golden/make_fixtures.py``).  Only data files are copied: the
 ``ocl_*.npy`` outputs and ``angles_deg.npy`` that the reference's own golden
 tests compare against (``tests/test_meep_plane_baselines.py:99-121``,
-``tests/test_meep_farfield_baselines.py:103-144``).  They are loaded with
+``tests/test_meep_farfield_baselines.py:103-144``), the ``meep_*.npy`` arrays
+its discrepancy reports compare them with, and those two committed reports
+(``baselines/discrepancy_report{,_farfield}.json``).  They are loaded with
 ``numpy.load(allow_pickle=False)``.  The inputs that produced them are restated
 in ``cases.json`` from the case definitions in
 ``tests/meep_validation/plane_cases.py:62-146`` and
@@ -89,15 +91,26 @@ CASES = {
 }
 
 
+REPORTS = {
+    # the reference's committed OpenCL-vs-Meep discrepancy reports (expected
+    # outputs of its metric functions on the ocl_* / meep_* arrays)
+    "discrepancy_report.json": "reference_discrepancy_report.json",
+    "discrepancy_report_farfield.json": "reference_discrepancy_report_farfield.json",
+}
+
+
 def main() -> None:
     for name in PLANE_CASES + FARFIELD_CASES:
         dst = HERE / name
         dst.mkdir(exist_ok=True)
         for src in sorted((REF / name).glob("*.npy")):
-            if not (src.name.startswith("ocl_") or src.name == "angles_deg.npy"):
+            if not (src.name.startswith(("ocl_", "meep_")) or src.name == "angles_deg.npy"):
                 continue
             np.load(src, allow_pickle=False)  # data only; refuse anything pickled
             shutil.copyfile(src, dst / src.name)
+    for src, dst in REPORTS.items():
+        json.loads((REF / src).read_text())  # plain JSON data
+        shutil.copyfile(REF / src, HERE / dst)
     (HERE / "cases.json").write_text(json.dumps(CASES, indent=2, sort_keys=True) + "\n")
 
 
