@@ -8,10 +8,11 @@ at f0 = c0/(20 dl), 50 % bandwidth; a Huygens box [32, 223]^3 recording the
 DFT at 8 frequencies; after the run, the near-to-far transform at
 181 θ x 2 φ angles, R = 1 m.  Inputs are synthetic (generated here).
 
-A "step" is one full Yee time step on the engine's default FP32 path: the
-split H and E sweeps with the sheet source between them (``--path fused``:
-the fused H + E sweep and the source fix-up), and the 8-frequency face-DFT
-accumulation.  ``value`` is Mcell-updates/s over the K timed steps.
+A "step" is one full Yee time step on the FP32 step path the engine's
+autotuner picks for this grid before anything is timed (``--path`` forces
+one: ``split``, the H and E sweeps with the sheet source between them;
+``fused`` or ``march``, one H + E sweep and the source fix-up), and the
+8-frequency face-DFT accumulation.  ``value`` is Mcell-updates/s over the K timed steps.
 ``roofline`` is the step's algorithmic HBM traffic (72 B/cell + 8 B per psi
 entry) divided by the time of the step's sweep kernels, taken with HIP
 events on the engine's stream around every sweep launch of the timed steps.  ``cpu_baseline``
@@ -19,8 +20,9 @@ times this repository's NumPy restatement of the reference's face-CPML engine
 (one core) on a bounded sample of steps of the same grid.
 
 ``roofline.traffic`` is the measured HBM traffic of the step's sweep
-kernels, taken from ``profiles/split_sweeps_256cube_hbm_traffic.txt`` (or
-``profiles/k_fused_256cube_hbm_traffic.txt`` for the fused path): two
+kernels, taken from ``profiles/split_sweeps_256cube_hbm_traffic.txt``
+(``k_fused_256cube_hbm_traffic.txt``, ``k_march_256cube_hbm_traffic.txt``
+for the one-launch paths): two
 rocprofv3 PMC passes, FETCH_SIZE and WRITE_SIZE, each scaled by what the same
 counter reads for a plain device copy of known size (``--copy-calibration``
 runs that copy).
@@ -128,8 +130,19 @@ class Workload:
 def run_gpu(args, wl: Workload):
     import hipfdtd
 
-    fused = {"auto": None, "fused": True, "split": False}[args.path]
-    sim = hipfdtd.HIPFDTD(wl.shape, wl.dl, npml=wl.npml, dtype=np.float32, fused=fused)
+    path = None if args.path == "auto" else args.path
+    sim = hipfdtd.HIPFDTD(wl.shape, wl.dl, npml=wl.npml, dtype=np.float32, path=path)
+    tuned = None
+    if args.path == "auto":
+        # the engine's autotuner picks the step path and launch shape (on a
+        # scratch engine, before anything is timed)
+        from hipfdtd.autotune import tune_engine
+
+        tuned = tune_engine(sim, rows=(8,), planes=(-1,), march_rows=(1, 2),
+                            march_planes=(-1, 24), steps=10, warmup=2, use_cache=False,
+                            write_cache=False)  # fmt: skip
+        print(f"[bench] autotuned path {tuned['path']} rows={tuned['rows']} "
+              f"planes={tuned['planes']}: {tuned['table']}", file=sys.stderr, flush=True)
     sim.set_epsilon(wl.eps)
     sim.add_source(wl.source)
     mon = hipfdtd.HIPNear2FarMonitor(sim, wl.center, wl.size, freqs=wl.freqs)
@@ -152,7 +165,8 @@ def run_gpu(args, wl: Workload):
     t3 = time.perf_counter()
     finite = bool(np.all(np.isfinite(eh)))
     return dict(wall_s=t1 - t0, kern_ms=kern_ms, launches=launches, n2f_ms=(t3 - t2) * 1e3,
-                finite=finite, device=sim.device_name, fused=sim.fused)  # fmt: skip
+                finite=finite, device=sim.device_name, path=sim.path,
+                tuned=None if tuned is None else tuned["table"])  # fmt: skip
 
 
 def run_cpu_baseline(wl: Workload, steps: int):
@@ -457,15 +471,20 @@ def run_batch(args):
     }))  # fmt: skip
 
 
-TRAFFIC_FILES = {True: ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt",
-                 False: ROOT / "profiles" / "split_sweeps_256cube_hbm_traffic.txt"}  # fmt: skip
+TRAFFIC_FILES = {"fused": ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt",
+                 "split": ROOT / "profiles" / "split_sweeps_256cube_hbm_traffic.txt",
+                 "march": ROOT / "profiles" / "k_march_256cube_hbm_traffic.txt"}  # fmt: skip
+
+KERNELS = {"fused": "k_fused (fused H + E sweep; sources by k_source_fixup)",
+           "split": "k_update_H + k_update_E (split sweeps, one step; sources by k_add_source)",
+           "march": "k_march (register-pipelined H + E sweep; sources by k_source_fixup)"}
 
 
-def measured_traffic(fused):
+def measured_traffic(path):
     """HBM bytes per step of the sweep kernels recorded in the profile
     summary for this path, or None."""
     try:
-        for line in TRAFFIC_FILES[fused].read_text().splitlines():
+        for line in TRAFFIC_FILES[path].read_text().splitlines():
             if line.startswith("traffic_bytes_per_step:"):
                 return float(line.split(":", 1)[1])
     except OSError:
@@ -498,8 +517,8 @@ def main():
                     choices=["yee-256", "plane-dft", "adjoint", "batch"])
     ap.add_argument("--graph-steps", type=int, default=100, help="batch: steps per HIP graph")
     ap.add_argument("--repeat", type=int, default=2, help="adjoint: timed forward+backward runs")
-    ap.add_argument("--path", default="auto", choices=["auto", "fused", "split"],
-                    help="256^3 workload: engine step path (auto = the engine's default)")
+    ap.add_argument("--path", default="auto", choices=["auto", "fused", "split", "march"],
+                    help="256^3 workload: engine step path (auto = picked by the autotuner)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -534,8 +553,7 @@ def main():
     # sweep kernels of one step: k_fused, or k_update_H + k_update_E
     step_kern_s = res["kern_ms"] / args.steps / 1e3
     achieved = b_alg / step_kern_s / 1e9
-    kernel = ("k_fused (fused H + E sweep; sources by k_source_fixup)" if res["fused"] else
-              "k_update_H + k_update_E (split sweeps, one step; sources by k_add_source)")  # fmt: skip
+    kernel = KERNELS[res["path"]]
     out = {
         "metric": "Mcell-updates/s",
         "value": round(mcells, 1),
@@ -555,6 +573,7 @@ def main():
             "grid": list(wl.shape),
             "npml": wl.npml,
             "n2f_angles": "181 theta x 2 phi, R=1 m",
+            "step_path": res["path"],
         },
         "roofline": {
             "bound": "hbm",
@@ -563,7 +582,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(res["fused"]),
+            "traffic": measured_traffic(res["path"]),
             "bytes_alg_per_step": b_alg,
             "sweep_us_per_step": round(step_kern_s * 1e6, 2),
             "launches_per_step": res["launches"] // args.steps,

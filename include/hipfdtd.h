@@ -62,6 +62,7 @@ extern "C" {
 /* step-path selection for hfdtd_step */
 #define HFDTD_PATH_SPLIT 0 /* update_H, sources, update_E: two sweeps        */
 #define HFDTD_PATH_FUSED 1 /* one fused H+sources+E sweep (ping-pong state)  */
+#define HFDTD_PATH_MARCH 2 /* one register-pipelined fused sweep (ping-pong) */
 
 typedef struct hfdtd_engine hfdtd_engine;
 typedef struct hfdtd_n2f hfdtd_n2f;
@@ -138,7 +139,8 @@ int hfdtd_update_E(hfdtd_engine *e);
 /* replaces add_source_Ex / add_source_Jx (engine.py:448-538) for n sources */
 int hfdtd_add_sources(hfdtd_engine *e, const hfdtd_source *src, int n);
 /* one Yee step H -> sources -> E (engine.py:724-735 minus the callbacks);
- * path = HFDTD_PATH_SPLIT or HFDTD_PATH_FUSED.  n <= 16 sources. */
+ * path = HFDTD_PATH_SPLIT, HFDTD_PATH_FUSED (LDS-DMA tiles) or
+ * HFDTD_PATH_MARCH (register-pipelined tiles).  n <= 16 sources. */
 int hfdtd_step(hfdtd_engine *e, const hfdtd_source *src, int n, int path);
 /* queue.finish() */
 int hfdtd_synchronize(hfdtd_engine *e);
@@ -153,6 +155,14 @@ int hfdtd_set_fused_tiling(hfdtd_engine *e, int rows_j, int planes_i, int xcd_gr
  * 256-cell blocks (1 = hardware round robin; 0 keeps the current value).
  * Used by the autotuner. */
 int hfdtd_set_split_schedule(hfdtd_engine *e, int run_blocks);
+/* March-sweep launch shape (HFDTD_PATH_MARCH): j rows per wavefront (1 or 2;
+ * a block is 4 wavefronts of 64 k; FP64 engines use 1), planes of i per block
+ * (rounded up to even; -1 = chosen from the grid), consecutive blocks per XCD
+ * run, and non-temporal field streams (1 on, 0 off, -1 when the slabs exceed
+ * the last-level cache).  0 keeps rows / planes / run, -2 keeps nontemporal.
+ * Used by the autotuner; no counterpart in the reference. */
+int hfdtd_set_march_tiling(hfdtd_engine *e, int rows_per_wave, int planes_i, int run_blocks,
+                           int nontemporal);
 
 /* HIP-event timing of the Yee sweep kernels on the engine's stream: while
  * enabled, every update_H / update_E / fused sweep launch is bracketed by a

@@ -66,6 +66,15 @@ struct Engine {
   int xcd = 8;           // XCDs for the work-order remap (1 = hardware order)
   int split_run = 128;   // split sweeps: consecutive blocks per XCD run (1 = round robin)
   int cus = 256;         // compute units of the device
+  // march sweep (register-pipelined fused sweep): rows per wave (1 or 2;
+  // 4 rows per wave-row group), i planes per block (0 = chosen from the
+  // grid), consecutive blocks per XCD run, non-temporal streams (-1 = when
+  // the slabs exceed the last-level cache, 0 = never, 1 = always)
+  int march_rows = 2;
+  int march_planes = 0;
+  int march_run = 16;
+  int march_nt = -1;
+  int march_occ[2][3] = {};  // resident march blocks per CU by [lossy][rows]
 
   uint64_t bytes = 0;
 
@@ -94,6 +103,36 @@ struct Engine {
   // each wave lasts as long as its blocks' P planes plus the two halo planes
   // every block re-reads; slots = resident blocks per CU x CUs.  Ties go to
   // more planes per block.
+  // i planes per march block (even): the set value, or the even P in
+  // [8, 64] minimising (waves of resident blocks) x (P + 1).  With PML the
+  // x-stripes form chunks of their own and P chunks the planes between.
+  int march_planes_for(int rows_per_wave, int per_cu) const {
+    if (march_planes > 0) return (march_planes + 1) / 2 * 2;
+    const long long slots = (long long)(per_cu > 0 ? per_cu : 1) * cus;
+    const long long tiles =
+        (long long)((nz + 63) / 64) * ((ny + 4 * rows_per_wave - 1) / (4 * rows_per_wave));
+    int best = 64;
+    long long best_t = -1;
+    const bool split = npml > 0 && nx > 2 * npml;  // x-stripes in chunks of their own
+    const int inner = split ? nx - 2 * npml : nx;
+    for (int P = 64; P >= 8; P -= 2) {
+      const long long blocks = tiles * ((inner + P - 1) / P + (split ? 2 : 0));
+      const long long t = (blocks + slots - 1) / slots * (P + 1);
+      if (best_t < 0 || t < best_t) {
+        best_t = t;
+        best = P;
+      }
+    }
+    return best;
+  }
+  // Non-temporal march streams pay off once the slabs no longer fit the
+  // device's last-level cache (256 MB on MI355X): the stores then stop
+  // evicting the halo rows other blocks still need.
+  bool march_streaming() const {
+    if (march_nt >= 0) return march_nt > 0;
+    const double slabs = (12.0 + (lossless ? 3.0 : 6.0)) * (double)vstride * elem;
+    return slabs > 256.0 * 1024 * 1024;
+  }
   int fused_planes(int rows, int per_cu) const {
     if (tile_planes > 0) return tile_planes;
     const long long slots = (long long)(per_cu > 0 ? per_cu : 1) * cus;
@@ -120,6 +159,7 @@ int launch_update_H(Engine *e);
 int launch_update_E(Engine *e);
 int launch_sources(Engine *e, const hfdtd_source *src, int n);
 int launch_fused_step(Engine *e, const hfdtd_source *src, int n);
+int launch_march_step(Engine *e, const hfdtd_source *src, int n);
 // ensure the ping-pong partners exist (engine.hip)
 int ensure_alt(Engine *e);
 

@@ -498,6 +498,7 @@ int hfdtd_step(hfdtd_engine *h, const hfdtd_source *src, int n, int path) {
   if (!h || (n > 0 && !src) || n < 0) return fail(HFDTD_EINVAL, "bad source list");
   Engine *e = &h->s;
   if (path == HFDTD_PATH_FUSED) return launch_fused_step(e, src, n);
+  if (path == HFDTD_PATH_MARCH) return launch_march_step(e, src, n);
   if (path != HFDTD_PATH_SPLIT) return fail(HFDTD_EINVAL, "unknown step path");
   int rc = launch_update_H(e);
   if (rc) return rc;
@@ -542,6 +543,21 @@ int hfdtd_set_split_schedule(hfdtd_engine *h, int run_blocks) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
   if (run_blocks < 0 || run_blocks > 4096) return fail(HFDTD_EINVAL, "run_blocks must be 0..4096");
   if (run_blocks) h->s.split_run = run_blocks;
+  return HFDTD_OK;
+}
+
+int hfdtd_set_march_tiling(hfdtd_engine *h, int rows_per_wave, int planes_i, int run_blocks,
+                           int nontemporal) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  if (rows_per_wave < 0 || rows_per_wave > 2) return fail(HFDTD_EINVAL, "rows_per_wave must be 0..2");
+  if (planes_i < -1 || planes_i > 4096) return fail(HFDTD_EINVAL, "planes_i must be -1..4096");
+  if (run_blocks < 0 || run_blocks > 4096) return fail(HFDTD_EINVAL, "run_blocks must be 0..4096");
+  if (nontemporal < -2 || nontemporal > 1) return fail(HFDTD_EINVAL, "nontemporal must be -2..1");
+  Engine *e = &h->s;
+  if (rows_per_wave) e->march_rows = rows_per_wave;
+  if (planes_i) e->march_planes = planes_i < 0 ? 0 : planes_i;
+  if (run_blocks) e->march_run = run_blocks;
+  if (nontemporal != -2) e->march_nt = nontemporal;
   return HFDTD_OK;
 }
 

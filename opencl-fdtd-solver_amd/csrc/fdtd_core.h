@@ -738,12 +738,30 @@ __device__ __forceinline__ int issue_psi_halo(const Geo &g, const TileDma<T> &m,
 // New psi of a row-wave cell, stored for every lane inside wave-uniform
 // branches (lanes outside the grid or the stripe write to the slab padding).
 // Returns the number of stores issued.
+// Whether a psi slab is small enough (below 2 GB with its padding) for its
+// store resource to carry the true size, so out-of-range stores are dropped.
+template <typename T>
+__device__ __forceinline__ bool psi_drop_ok(const Geo &g) {
+  return ((unsigned long long)psi_slot(g, PSI_COUNT) + 64ull) * sizeof(T) < 0x80000000ull;
+}
+
+template <typename T>
+__device__ __forceinline__ Rsrc psi_store_rsrc(const Geo &g, T *q1) {
+  const int bytes = (int)((psi_slot(g, PSI_COUNT) + 64u) * (unsigned)sizeof(T));
+  return psi_drop_ok<T>(g) ? __builtin_amdgcn_make_buffer_rsrc(q1, (short)0, bytes, 0x00020000)
+                           : rsrc_of(q1);
+}
+
 template <typename T>
 __device__ __forceinline__ int store_psi_row(const Geo &g, const TileDma<T> &m, int base,
                                              const StripeIdx &so, bool xst, const RowWave<T> &r,
                                              int tk, const T p[6]) {
   const unsigned es = (unsigned)sizeof(T);
-  const unsigned junk = (psi_slot(g, PSI_COUNT) + (unsigned)tk) * es;
+  // lanes that must not store: past the end of the psi resource (sized when
+  // the slab is below 2 GB: the hardware drops the store), else this lane's
+  // slot in the slab padding; one shared padding address for every wave of
+  // the grid would pile all their stores onto one line
+  const unsigned junk = psi_drop_ok<T>(g) ? 0x80000000u : (psi_slot(g, PSI_COUNT) + (unsigned)tk) * es;
   int c = 0;
   if (xst) {
     buf_store(m.wps, p[0], r.act ? (psi_slot(g, base) + so.qx) * es : junk, 0u);
@@ -971,7 +989,7 @@ __device__ __forceinline__ void fused_tile(const Geo &g, int planes, T dtm_dl, T
   a.m.cf = rsrc_of(coef);
   a.m.ps = rsrc_of(q0);
   a.m.wst = rsrc_of(s1);
-  a.m.wps = rsrc_of(q1);
+  a.m.wps = psi_store_rsrc<T>(g, q1);
   a.m.ok = lane_off<T, kRowK>(g, tk, k0, j0);
   a.m.oz = lane_off<T, kRowZ>(g, tk, k0, j0);
   a.m.oc[0] = col_off<T>(g, tk, k0, j0, 0);

@@ -1,0 +1,548 @@
+// Register-pipelined fused sweep ("march"): H and E of one Yee step in one
+// launch, every value of the grid read and written once.
+//
+// A block of four wavefronts owns a 64(k) x 4*RW(j) column tile and marches
+// along i over `planes` planes, H then E on each.  Wave w owns rows
+// j0 + RW*w ... j0 + RW*w + RW - 1 (lane = k - k0), one "slot" per row.  A
+// further halo slot per wave recomputes the H values the tile's E needs from
+// outside the tile: lanes 0-15 of wave w take row j0 - 1 at columns
+// k0 + 16w ... k0 + 16w + 15, lanes 16 ... 16 + RW - 1 take column k0 - 1 of
+// the wave's own rows.  New H goes to LDS (double-buffered by plane parity,
+// one barrier per plane); E then takes its j-1 and k-1 H neighbours from
+// LDS and its i-1 neighbour from registers (H of the previous plane).
+//
+// Loads are software-pipelined through registers: while plane i is
+// computed, the loads of plane i+1 are in flight (the own cell's old H, ce,
+// ca and psi_z, E at i+2, and the E neighbours at j+1 and k+1).  E of a
+// plane is loaded once, as the i+1 neighbour of the plane before, and kept.
+// Every field load and store of the loop is unconditional and goes through
+// a buffer resource: the per-lane byte offset of a slot is fixed for the
+// whole block, the plane and volume parts are wave-uniform, so issuing one
+// costs no vector ALU work and the compiler's counted waits stay exact from
+// one plane to the next.  Lanes and planes outside the grid load clamped
+// addresses and store into the volume padding (psi into the stripe slab's
+// padding).  psi of the x and y stripes, uniform per plane and per row, is
+// loaded at the top of the plane that uses it, before the prefetch, so
+// waiting for it never waits for the prefetch.
+//
+// The per-cell arithmetic is h_math / e_math, shared with every other
+// sweep.  State is ping-ponged like the LDS-DMA fused sweep (reads slab 0,
+// writes slab 1) and sources are redone on their cells by the fix-up
+// kernel after the sweep.  Offsets are 32-bit: slabs stay below 4 GB
+// (checked on the host).
+#pragma once
+
+#include "fdtd_core.h"
+
+namespace hfdtd {
+
+constexpr int kMarchThreads = 256;  // four wavefronts
+// A buffer offset no slab reaches whose resource carries its size (psi
+// slabs stay below 2 GB, checked on the host): stores there fall outside the
+// resource's range and are dropped, loads return 0, neither touches memory.
+constexpr unsigned kDropOffset = 0x80000000u;
+
+template <typename T, int RW>
+struct MarchLds {
+  T h[2][3][4 * RW + 2][65];  // [plane parity][Hx Hy Hz][row j0-1 .. j0+4RW, trash][col k0-1 ..]
+};
+
+// Plane-i data of one slot, loaded one plane ahead in two parts: the H part
+// at the top of plane i-1, the E part after its H half, so each is in
+// flight for a full half-plane while the registers of the part it replaces
+// are still being read.
+template <typename T>
+struct MarchH {
+  T ex, ey, ez;          // E(i+1) at the cell (becomes E of the next plane)
+  T hx, hy, hz;          // old H(i)
+  T exj, ezj, exk, eyk;  // E(i) at j+1 and at k+1
+  T p[6];                // old psi of the H slots (PML tiles)
+};
+
+template <typename T>
+struct MarchE {
+  T cx, cy, cz;  // ce(i)
+  T ax, ay, az;  // ca(i) (lossy grids)
+  T p[6];        // old psi of the E slots (PML tiles)
+};
+
+// Fixed data of one slot.  The cell is clamped into the grid: values
+// computed for cells outside it are never stored nor used.
+struct MarchSlot {
+  int j, k;
+  unsigned o4;   // byte offset of (j, k) inside a plane
+  unsigned qy4;  // byte offset inside a y-stripe plane (kDropOffset outside the stripe)
+  unsigned qz4;  // byte offset inside the z-stripe rows of a plane (kDropOffset outside)
+  bool iny, inz;
+  bool own;      // an owned cell inside the grid
+  int li;        // LDS element index (row * 65 + column)
+};
+
+// Raw buffer loads and stores; NT marks them non-temporal (streamed once).
+template <typename T, bool NT>
+__device__ __forceinline__ T bld(Rsrc r, unsigned voff, unsigned soff) {
+  constexpr int aux = NT ? 2 : 0;
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, aux));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, aux));
+  }
+}
+
+template <typename T, bool NT>
+__device__ __forceinline__ void bst(Rsrc r, T v, unsigned voff, unsigned soff) {
+  constexpr int aux = NT ? 2 : 0;
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, aux);
+  } else {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, aux);
+  }
+}
+
+// Bytes of a six-volume field slab.
+template <typename T>
+__device__ __forceinline__ unsigned long long slab_bytes(const Geo &g) {
+  return 6ull * (unsigned long long)g.vs * sizeof(T);
+}
+
+// psi addressing of one plane: whether it lies in an x-stripe, and the
+// plane parts (in elements) of the x-, y- and z-stripe offsets.
+struct PsiPlane {
+  bool inx;
+  unsigned xs, ys, zs;
+};
+
+template <typename T, int RW, bool LOSSY, bool XY, bool Z, bool NT>
+struct March {
+  static constexpr int S = RW + 1;  // RW owned rows + the halo slot
+  static constexpr int TJ = 4 * RW;
+  static constexpr unsigned E = sizeof(T);
+
+  Geo g;
+  T dtm_dl, dtm, inv_dl;
+  Rsrc r0, r1, rc, rq0, rq1;  // slab 0, slab 1, coefficients, psi 0, psi 1
+  const T *__restrict__ prof;
+  MarchSlot sl[S];
+  Prof1<T> pyh[S], pye[S], pzh[S], pze[S];
+  T hyc[RW], hzc[RW];  // new H(i-1) at the owned cells
+  T oe[S][3];          // E(i) at every slot (copied from the loads of plane i-1)
+  unsigned padb;       // store offset of lanes and planes outside the grid
+  int iend;            // first plane past this block's chunk
+
+  __device__ __forceinline__ unsigned vsb() const { return (unsigned)g.vs * E; }
+  __device__ __forceinline__ unsigned plane_b(int i) const {
+    return (unsigned)min(i, g.nx - 1) * (unsigned)(g.ny * g.nz) * E;
+  }
+  // byte offset of psi slot s plus a part given in elements
+  __device__ __forceinline__ unsigned psib(int s, unsigned elems) const {
+    return (psi_slot(g, s) + elems) * E;
+  }
+  __device__ __forceinline__ PsiPlane psi_plane(int i) const {
+    const int n = g.npml, ic = min(i, g.nx - 1);
+    PsiPlane pp;
+    pp.inx = n > 0 && (ic < n || ic >= g.nx - n);
+    pp.xs = pp.inx ? (unsigned)stripe_local(ic, n, g.nx) * (unsigned)(g.ny * g.nz) : 0u;
+    pp.ys = (unsigned)ic * 2u * (unsigned)n * (unsigned)g.nz;
+    pp.zs = (unsigned)ic * (unsigned)g.ny * 2u * (unsigned)n;
+    return pp;
+  }
+
+  // H part of plane i.  psi of the x and y stripes is loaded on every plane
+  // of an XY block, psi_z on every plane of a Z block; lanes outside a
+  // stripe load from past the end of the psi slab's resource, which returns
+  // 0 without touching memory (h_math ignores the value).  Pointing them all
+  // at one valid element instead would send every wave of the grid to the
+  // same cache line.
+  __device__ __forceinline__ void load_h(MarchH<T> &s, const MarchSlot &m, unsigned pl,
+                                         unsigned pl1, const PsiPlane &pp) const {
+    const unsigned vs = vsb(), nzb = (unsigned)g.nz * E;
+    s.ex = bld<T, false>(r0, m.o4, pl1);
+    s.ey = bld<T, false>(r0, m.o4, vs + pl1);
+    s.ez = bld<T, false>(r0, m.o4, 2 * vs + pl1);
+    s.hx = bld<T, false>(r0, m.o4, 3 * vs + pl);
+    s.hy = bld<T, false>(r0, m.o4, 4 * vs + pl);
+    s.hz = bld<T, false>(r0, m.o4, 5 * vs + pl);
+    // j+1 and k+1 neighbours; past the last row / column they read the next
+    // row / plane or the padding, inside the slab, and h_math ignores them
+    s.exj = bld<T, false>(r0, m.o4, pl + nzb);
+    s.ezj = bld<T, false>(r0, m.o4, 2 * vs + pl + nzb);
+    s.exk = bld<T, false>(r0, m.o4 + E, pl);
+    s.eyk = bld<T, false>(r0, m.o4 + E, vs + pl);
+    if constexpr (XY) {
+      const unsigned vx = pp.inx ? m.o4 : kDropOffset;
+      s.p[0] = bld<T, false>(rq0, vx, psib(0, pp.xs));
+      s.p[1] = bld<T, false>(rq0, vx, psib(1, pp.xs));
+      s.p[2] = bld<T, false>(rq0, m.qy4, psib(2, pp.ys));
+      s.p[3] = bld<T, false>(rq0, m.qy4, psib(3, pp.ys));
+    }
+    if constexpr (Z) {
+      s.p[4] = bld<T, false>(rq0, m.qz4, psib(4, pp.zs));
+      s.p[5] = bld<T, false>(rq0, m.qz4, psib(5, pp.zs));
+    }
+  }
+
+  __device__ __forceinline__ void load_e(MarchE<T> &s, const MarchSlot &m, unsigned pl,
+                                         const PsiPlane &pp) const {
+    const unsigned vs = vsb();
+    s.cx = bld<T, NT>(rc, m.o4, pl);
+    s.cy = bld<T, NT>(rc, m.o4, vs + pl);
+    s.cz = bld<T, NT>(rc, m.o4, 2 * vs + pl);
+    if constexpr (LOSSY) {
+      s.ax = bld<T, NT>(rc, m.o4, 3 * vs + pl);
+      s.ay = bld<T, NT>(rc, m.o4, 4 * vs + pl);
+      s.az = bld<T, NT>(rc, m.o4, 5 * vs + pl);
+    }
+    if constexpr (XY) {
+      const unsigned vx = pp.inx ? m.o4 : kDropOffset;
+      s.p[0] = bld<T, false>(rq0, vx, psib(6, pp.xs));
+      s.p[1] = bld<T, false>(rq0, vx, psib(7, pp.xs));
+      s.p[2] = bld<T, false>(rq0, m.qy4, psib(8, pp.ys));
+      s.p[3] = bld<T, false>(rq0, m.qy4, psib(9, pp.ys));
+    }
+    if constexpr (Z) {
+      s.p[4] = bld<T, false>(rq0, m.qz4, psib(10, pp.zs));
+      s.p[5] = bld<T, false>(rq0, m.qz4, psib(11, pp.zs));
+    }
+  }
+
+  __device__ __forceinline__ void load_h_all(MarchH<T> (&s)[S], int i) const {
+    const unsigned pl = plane_b(i), pl1 = plane_b(i + 1);
+    const PsiPlane pp = psi_plane(i);
+#pragma unroll
+    for (int q = 0; q < S; ++q) load_h(s[q], sl[q], pl, pl1, pp);
+  }
+
+  __device__ __forceinline__ void load_e_all(MarchE<T> (&s)[RW], int i) const {
+    const unsigned pl = plane_b(i);
+    const PsiPlane pp = psi_plane(i);
+#pragma unroll
+    for (int q = 0; q < RW; ++q) load_e(s[q], sl[q], pl, pp);
+  }
+
+  // psi store of an owned cell to stripe slot s at byte offset a (slot base
+  // excluded, the wave-uniform part).  Lanes that must not store get an
+  // offset past the end of the psi slab, whose resource carries its true
+  // size, and the hardware drops their store; every store stays
+  // unconditional, so the compiler's counted waits stay exact.
+  __device__ __forceinline__ void psi_st(T v, int s, bool ok, unsigned a) const {
+    bst<T, false>(rq1, v, ok ? a : kDropOffset, psi_slot(g, s) * E);
+  }
+
+  __device__ __forceinline__ StripeIdx stripes(const MarchSlot &m, bool inx) const {
+    StripeIdx st;
+    st.inx = XY && inx;
+    st.iny = m.iny;
+    st.inz = m.inz;
+    st.qx = st.qy = st.qz = 0;
+    return st;
+  }
+
+  // Plane i: H on every slot, barrier, E on the owned slots.  hc / ec hold
+  // the loads of plane i; hn / en, no longer read, receive those of plane
+  // i+1.  E of the plane is carried in oe, so no value of a set stays live
+  // while its next loads are in flight: the sets keep their registers from
+  // one plane to the next and the loop needs no copies of in-flight values.
+  __device__ __forceinline__ void step(MarchLds<T, RW> &L, int buf, int i, MarchH<T> (&hc)[S],
+                                       MarchH<T> (&hn)[S], MarchE<T> (&ec)[RW],
+                                       MarchE<T> (&en)[RW]) {
+    const unsigned sxy = (unsigned)(g.ny * g.nz);
+    const unsigned vs = vsb();
+    const int ic = min(i, g.nx - 1);
+    const bool live = i < iend;
+    // keep these loads out of the previous plane's code, where the old
+    // values of hn are still live
+    __builtin_amdgcn_sched_barrier(0);
+    load_h_all(hn, i + 1);
+    __builtin_amdgcn_sched_barrier(0);
+
+    // ── H(i) ──
+    const PsiPlane pp = psi_plane(ic);
+    const Prof1<T> pxh = prof1(g, prof, 1, 0, ic), pxe = prof1(g, prof, 0, 0, ic);
+    const unsigned pl = (unsigned)ic * sxy * E;
+    T h[S][3];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      const MarchSlot &m = sl[q];
+      const StripeIdx st = stripes(m, pp.inx);
+      T p[6] = {0, 0, 0, 0, 0, 0};
+      if constexpr (XY) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) p[t] = hc[q].p[t];
+      }
+      if constexpr (Z) {
+        p[4] = hc[q].p[4];
+        p[5] = hc[q].p[5];
+      }
+      h[q][0] = hc[q].hx;
+      h[q][1] = hc[q].hy;
+      h[q][2] = hc[q].hz;
+      h_math(g, ic, m.j, m.k, st, pxh, pyh[q], pzh[q], oe[q][0], oe[q][1], oe[q][2], hc[q].exj,
+             hc[q].ezj, hc[q].exk, hc[q].eyk, hc[q].ey, hc[q].ez, h[q][0], h[q][1], h[q][2], p,
+             dtm_dl, dtm);
+      (&L.h[buf][0][0][0])[m.li] = h[q][0];
+      (&L.h[buf][1][0][0])[m.li] = h[q][1];
+      (&L.h[buf][2][0][0])[m.li] = h[q][2];
+      if (q < RW) {
+        const bool ok = m.own && live;
+        const unsigned v = ok ? m.o4 + pl : padb;
+        bst<T, NT>(r1, h[q][0], v, 3 * vs);
+        bst<T, NT>(r1, h[q][1], v, 4 * vs);
+        bst<T, NT>(r1, h[q][2], v, 5 * vs);
+        if constexpr (XY) {
+          psi_st(p[0], 0, ok && pp.inx, m.o4 + pp.xs * E);
+          psi_st(p[1], 1, ok && pp.inx, m.o4 + pp.xs * E);
+          psi_st(p[2], 2, ok && m.iny, m.qy4 + pp.ys * E);
+          psi_st(p[3], 3, ok && m.iny, m.qy4 + pp.ys * E);
+        }
+        if constexpr (Z) {
+          psi_st(p[4], 4, ok && m.inz, m.qz4 + pp.zs * E);
+          psi_st(p[5], 5, ok && m.inz, m.qz4 + pp.zs * E);
+        }
+      }
+    }
+    // the E part of plane i+1, in flight across the barrier and the E half
+    __builtin_amdgcn_sched_barrier(0);
+    load_e_all(en, i + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    lds_barrier();
+
+    // ── E(i) ──
+#pragma unroll
+    for (int q = 0; q < RW; ++q) {
+      const MarchSlot &m = sl[q];
+      const T hz_jm = (&L.h[buf][2][0][0])[m.li - 65], hx_jm = (&L.h[buf][0][0][0])[m.li - 65];
+      const T hy_km = (&L.h[buf][1][0][0])[m.li - 1], hx_km = (&L.h[buf][0][0][0])[m.li - 1];
+      const StripeIdx st = stripes(m, pp.inx);
+      T p[6] = {0, 0, 0, 0, 0, 0};
+      if constexpr (XY) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) p[t] = ec[q].p[t];
+      }
+      if constexpr (Z) {
+        p[4] = ec[q].p[4];
+        p[5] = ec[q].p[5];
+      }
+      T ex = oe[q][0], ey = oe[q][1], ez = oe[q][2];
+      const T ax = LOSSY ? ec[q].ax : T(1), ay = LOSSY ? ec[q].ay : T(1);
+      const T az = LOSSY ? ec[q].az : T(1);
+      e_math(g, ic, m.j, m.k, st, pxe, pye[q], pze[q], h[q][0], h[q][1], h[q][2], hx_jm, hz_jm,
+             hx_km, hy_km, hyc[q], hzc[q], ex, ey, ez, ax, ay, az, ec[q].cx, ec[q].cy, ec[q].cz,
+             p, inv_dl);
+      const bool ok = m.own && live;
+      const unsigned v = ok ? m.o4 + pl : padb;
+      bst<T, NT>(r1, ex, v, 0u);
+      bst<T, NT>(r1, ey, v, vs);
+      bst<T, NT>(r1, ez, v, 2 * vs);
+      if constexpr (XY) {
+        psi_st(p[0], 6, ok && pp.inx, m.o4 + pp.xs * E);
+        psi_st(p[1], 7, ok && pp.inx, m.o4 + pp.xs * E);
+        psi_st(p[2], 8, ok && m.iny, m.qy4 + pp.ys * E);
+        psi_st(p[3], 9, ok && m.iny, m.qy4 + pp.ys * E);
+      }
+      if constexpr (Z) {
+        psi_st(p[4], 10, ok && m.inz, m.qz4 + pp.zs * E);
+        psi_st(p[5], 11, ok && m.inz, m.qz4 + pp.zs * E);
+      }
+      hyc[q] = h[q][1];
+      hzc[q] = h[q][2];
+    }
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      oe[q][0] = hc[q].ex;
+      oe[q][1] = hc[q].ey;
+      oe[q][2] = hc[q].ez;
+    }
+  }
+
+  // Slot set-up, the carry H(i0-1) and the first loads, then the planes.
+  __device__ __forceinline__ void run(MarchLds<T, RW> &L, int planes, int k0, int j0, int i0,
+                                      int i1) {
+    iend = i1;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int n = g.npml;
+#pragma unroll
+    for (int q = 0; q <= RW; ++q) {
+      int j, k, lr, lc;
+      bool own = false;
+      if (q < RW) {
+        j = j0 + RW * w + q;
+        k = k0 + lane;
+        lr = RW * w + q + 1;
+        lc = lane + 1;
+        own = j < g.ny && k < g.nz;
+      } else if (lane < 16) {
+        j = j0 - 1;
+        k = k0 + 16 * w + lane;
+        lr = 0;
+        lc = 16 * w + lane + 1;
+      } else if (lane < 16 + RW) {
+        j = j0 + RW * w + lane - 16;
+        k = k0 - 1;
+        lr = RW * w + lane - 16 + 1;
+        lc = 0;
+      } else {
+        j = j0;
+        k = k0;
+        lr = TJ + 1;  // trash row
+        lc = lane;
+      }
+      MarchSlot &m = sl[q];
+      m.j = min(max(j, 0), g.ny - 1);
+      m.k = min(max(k, 0), g.nz - 1);
+      m.o4 = (unsigned)(m.j * g.nz + m.k) * E;
+      m.iny = XY && n > 0 && (m.j < n || m.j >= g.ny - n);
+      m.inz = Z && n > 0 && (m.k < n || m.k >= g.nz - n);
+      m.qy4 = m.iny ? (unsigned)(stripe_local(m.j, n, g.ny) * g.nz + m.k) * E : kDropOffset;
+      m.qz4 = m.inz ? (unsigned)(m.j * 2 * n + stripe_local(m.k, n, g.nz)) * E : kDropOffset;
+      m.li = lr * 65 + lc;
+      m.own = own;
+      pyh[q] = prof1(g, prof, 1, 1, m.j);
+      pye[q] = prof1(g, prof, 0, 1, m.j);
+      pzh[q] = prof1(g, prof, 1, 2, m.k);
+      pze[q] = prof1(g, prof, 0, 2, m.k);
+    }
+    // dropped by the hardware when the slab resource carries its size (slabs
+    // below 2 GB), else this lane's slot in the volume padding
+    if (slab_bytes<T>(g) < 0x80000000ull) padb = kDropOffset;
+    else padb = ((unsigned)((size_t)g.nx * g.ny * g.nz) + (unsigned)lane) * E;
+
+    const unsigned vs = vsb();
+    MarchH<T> A[S], B[S];
+    MarchE<T> EA[RW], EB[RW];
+    // E(i0) of every slot
+    {
+      const unsigned p0 = plane_b(i0);
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        oe[q][0] = bld<T, false>(r0, sl[q].o4, p0);
+        oe[q][1] = bld<T, false>(r0, sl[q].o4, vs + p0);
+        oe[q][2] = bld<T, false>(r0, sl[q].o4, 2 * vs + p0);
+      }
+    }
+    // new H(i0-1) at the owned cells (unused when i0 == 0: e_math drops the
+    // i-1 term on plane 0)
+    {
+      const int im = max(i0 - 1, 0);
+      const unsigned pm = plane_b(im), nzb = (unsigned)g.nz * E;
+      const PsiPlane pp = psi_plane(im);
+      const Prof1<T> pxh = prof1(g, prof, 1, 0, im);
+#pragma unroll
+      for (int q = 0; q < RW; ++q) {
+        const MarchSlot &m = sl[q];
+        StripeIdx st = stripes(m, pp.inx);
+        st.inx = pp.inx;  // the plane before the chunk may lie in an x-stripe in any variant
+        T p[6] = {0, 0, 0, 0, 0, 0};
+        if (st.inx) {
+          p[0] = bld<T, false>(rq0, m.o4, psib(0, pp.xs));
+          p[1] = bld<T, false>(rq0, m.o4, psib(1, pp.xs));
+        }
+        if (st.iny) {
+          p[2] = bld<T, false>(rq0, m.qy4, psib(2, pp.ys));
+          p[3] = bld<T, false>(rq0, m.qy4, psib(3, pp.ys));
+        }
+        if (st.inz) {
+          p[4] = bld<T, false>(rq0, m.qz4, psib(4, pp.zs));
+          p[5] = bld<T, false>(rq0, m.qz4, psib(5, pp.zs));
+        }
+        T hx = bld<T, false>(r0, m.o4, 3 * vs + pm), hy = bld<T, false>(r0, m.o4, 4 * vs + pm);
+        T hz = bld<T, false>(r0, m.o4, 5 * vs + pm);
+        h_math(g, im, m.j, m.k, st, pxh, pyh[q], pzh[q], bld<T, false>(r0, m.o4, pm),
+               bld<T, false>(r0, m.o4, vs + pm), bld<T, false>(r0, m.o4, 2 * vs + pm),
+               bld<T, false>(r0, m.o4, pm + nzb), bld<T, false>(r0, m.o4, 2 * vs + pm + nzb),
+               bld<T, false>(r0, m.o4 + E, pm), bld<T, false>(r0, m.o4 + E, vs + pm), oe[q][1],
+               oe[q][2], hx, hy, hz, p, dtm_dl, dtm);
+        hyc[q] = hy;
+        hzc[q] = hz;
+      }
+    }
+    load_h_all(A, i0);
+    load_e_all(EA, i0);
+    // the last plane's loads read the (clamped) plane after the chunk and go
+    // unused; a peeled last pair makes the register allocator shuffle the
+    // sets at the loop head, which waits for every load in flight
+    int t = 0;  // planes >= 2 and even
+#pragma unroll 1
+    do {
+      step(L, 0, i0 + t, A, B, EA, EB);
+      step(L, 1, i0 + t + 1, B, A, EB, EA);
+      t += 2;
+    } while (t < planes);
+  }
+};
+
+// Chunks of i planes: with PML, the x-stripes get chunks of their own
+// ([0, n) and [nx - n, nx)) and the planes between them chunks of `planes`
+// (the last one shorter), so only those chunks carry x-stripe psi.
+struct MarchChunks {
+  int split;   // 1: x-stripe chunks of their own
+  int inner;   // chunks between the x-stripes (all chunks when split == 0)
+  __device__ __forceinline__ void range(const Geo &g, int planes, int c, int &i0, int &i1) const {
+    const int n = g.npml;
+    if (!split) {
+      i0 = c * planes;
+      i1 = min(i0 + planes, g.nx);
+    } else if (c == 0) {
+      i0 = 0;
+      i1 = n;
+    } else if (c == inner + 1) {
+      i0 = g.nx - n;
+      i1 = g.nx;
+    } else {
+      i0 = n + (c - 1) * planes;
+      i1 = min(i0 + planes, g.nx - n);
+    }
+  }
+};
+
+// One block per (k tile, j tile, i chunk).  A block takes the variant with
+// the psi it needs: XY when its chunk or its rows (with the halo row) meet
+// an x- or y-stripe, Z when its columns (with the halo column) meet a
+// z-stripe.  (The plane before the chunk, recomputed once, reads its psi in
+// any variant.)
+template <typename T, int RW, bool LOSSY, bool NT>
+__global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dtm_dl, T dtm,
+                                                         T inv_dl, const T *__restrict__ s0,
+                                                         T *__restrict__ s1,
+                                                         const T *__restrict__ coef,
+                                                         const T *__restrict__ prof,
+                                                         const T *__restrict__ q0,
+                                                         T *__restrict__ q1, int tiles_k,
+                                                         int tiles_j, MarchChunks ch, int xcd,
+                                                         int run) {
+  __shared__ MarchLds<T, RW> L;
+  const int B = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
+  const int bk = B % tiles_k, bj = (B / tiles_k) % tiles_j, bc = B / (tiles_k * tiles_j);
+  const int k0 = bk * 64, j0 = bj * 4 * RW;
+  int i0, i1;
+  ch.range(g, planes, bc, i0, i1);
+  const int np = (i1 - i0 + 1) / 2 * 2;  // even; planes past i1 are masked
+  const int n = g.npml;
+  const bool xy = n > 0 && (i0 < n || i1 > g.nx - n || j0 - 1 < n ||
+                            min(j0 + 4 * RW - 1, g.ny - 1) >= g.ny - n);
+  const bool z = n > 0 && (k0 - 1 < n || min(k0 + 63, g.nz - 1) >= g.nz - n);
+  const unsigned long long sb = slab_bytes<T>(g);
+  const Rsrc r0 = rsrc_of(s0), rc = rsrc_of(coef);
+  const Rsrc r1 = __builtin_amdgcn_make_buffer_rsrc(s1, (short)0, sb < 0x80000000ull ? (int)sb : -1,
+                                                    0x00020000);
+  const int psi_bytes = (int)((psi_slot(g, PSI_COUNT) + 64u) * (unsigned)sizeof(T));
+  const Rsrc rq0 = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(q0), (short)0, psi_bytes,
+                                                     0x00020000);
+  const Rsrc rq1 = __builtin_amdgcn_make_buffer_rsrc(q1, (short)0, psi_bytes, 0x00020000);
+  if (xy && z) {
+    March<T, RW, LOSSY, true, true, NT> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, prof};
+    m.run(L, np, k0, j0, i0, i1);
+  } else if (xy) {
+    March<T, RW, LOSSY, true, false, NT> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, prof};
+    m.run(L, np, k0, j0, i0, i1);
+  } else if (z) {
+    March<T, RW, LOSSY, false, true, NT> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, prof};
+    m.run(L, np, k0, j0, i0, i1);
+  } else {
+    March<T, RW, LOSSY, false, false, NT> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, prof};
+    m.run(L, np, k0, j0, i0, i1);
+  }
+}
+
+}  // namespace hfdtd

@@ -17,6 +17,7 @@
 #include <algorithm>
 
 #include "fdtd_core.h"
+#include "march.h"
 
 namespace hfdtd {
 
@@ -172,6 +173,25 @@ static int sources_t(Engine *e, const hfdtd_source *src, int n) {
   return HFDTD_OK;
 }
 
+// E on the source cells of a ping-pong sweep, redone with the sources added.
+template <typename T, bool LOSSY>
+static int source_fixup(Engine *e, const SrcList<T> &sl) {
+  if (sl.n == 0) return HFDTD_OK;
+  const int o = e->cur ^ 1;
+  long long most = 0;
+  for (int q = 0; q < sl.n; ++q) {
+    const Src<T> &s = sl.s[q];
+    most = std::max(most, (long long)(s.ci1 - s.ci0) * (s.cj1 - s.cj0) * (s.ck1 - s.ck0));
+  }
+  auto fix = k_source_fixup<T, LOSSY>;
+  hipLaunchKernelGGL(fix, dim3((unsigned)ceil_div(most, 256), sl.n), dim3(256), 0, e->stream,
+                     geo_of(e), T(e->inv_dl), (const T *)e->state[e->cur], (T *)e->state[o],
+                     (const T *)e->coef, (const T *)e->prof, (const T *)e->psi[e->cur],
+                     (T *)e->psi[o], sl);
+  HF_HIP(hipGetLastError());
+  return HFDTD_OK;
+}
+
 template <typename T, int TJ, bool LOSSY>
 static int fused_tj(Engine *e, const SrcList<T> &sl) {
   auto kern = k_fused<T, TJ, LOSSY>;
@@ -196,36 +216,88 @@ static int fused_tj(Engine *e, const SrcList<T> &sl) {
                      (T *)e->psi[o], tiles_k, tiles_j, e->xcd);
   HF_HIP(hipGetLastError());
   rc = timing_mark(e);
-  if (rc || sl.n == 0) return rc;
-  long long most = 0;
-  for (int q = 0; q < sl.n; ++q) {
-    const Src<T> &s = sl.s[q];
-    most = std::max(most, (long long)(s.ci1 - s.ci0) * (s.cj1 - s.cj0) * (s.ck1 - s.ck0));
+  if (rc) return rc;
+  return source_fixup<T, LOSSY>(e, sl);
+}
+
+template <typename T, int RW, bool LOSSY, bool NT>
+static int march_rw(Engine *e, const SrcList<T> &sl) {
+  auto kern = k_march<T, RW, LOSSY, NT>;
+  int &occ = e->march_occ[LOSSY ? 1 : 0][RW];
+  if (occ == 0) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kMarchThreads, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    occ = per_cu;
   }
-  auto fix = k_source_fixup<T, LOSSY>;
-  hipLaunchKernelGGL(fix, dim3((unsigned)ceil_div(most, 256), sl.n), dim3(256), 0, e->stream,
-                     geo_of(e), T(e->inv_dl), (const T *)e->state[e->cur], (T *)e->state[o],
-                     (const T *)e->coef, (const T *)e->prof, (const T *)e->psi[e->cur],
-                     (T *)e->psi[o], sl);
+  const int planes = e->march_planes_for(RW, occ);
+  const int tiles_k = (int)ceil_div(e->nz, 64), tiles_j = (int)ceil_div(e->ny, 4 * RW);
+  MarchChunks ch;
+  ch.split = e->npml > 0 && e->nx > 2 * e->npml ? 1 : 0;
+  ch.inner = (int)ceil_div(ch.split ? e->nx - 2 * e->npml : e->nx, planes);
+  const dim3 grid(tiles_k * tiles_j * (ch.inner + 2 * ch.split));
+  int rc = timing_mark(e);
+  if (rc) return rc;
+  const int o = e->cur ^ 1;
+  hipLaunchKernelGGL(kern, grid, dim3(kMarchThreads), 0, e->stream, geo_of(e), planes,
+                     T(e->dtm_dl), T(e->dtm), T(e->inv_dl), (const T *)e->state[e->cur],
+                     (T *)e->state[o], (const T *)e->coef, (const T *)e->prof,
+                     (const T *)e->psi[e->cur], (T *)e->psi[o], tiles_k, tiles_j, ch, e->xcd,
+                     e->march_run);
   HF_HIP(hipGetLastError());
-  return HFDTD_OK;
+  rc = timing_mark(e);
+  if (rc) return rc;
+  return source_fixup<T, LOSSY>(e, sl);
 }
 
 template <typename T>
-static int fused_t(Engine *e, const hfdtd_source *src, int n) {
+static int build_srclist(Engine *e, const hfdtd_source *src, int n, SrcList<T> &sl) {
   if (n > kMaxSources) return fail(HFDTD_EINVAL, "at most 16 sources per fused step");
-  SrcList<T> sl;
   sl.n = 0;
   for (int q = 0; q < n; ++q) {
     Src<T> s;
     if (!make_src<T>(e, src[q], s)) return fail(HFDTD_EINVAL, "invalid source descriptor");
     if (s.ci1 > s.ci0 && s.cj1 > s.cj0 && s.ck1 > s.ck0) sl.s[sl.n++] = s;
   }
+  return HFDTD_OK;
+}
+
+template <typename T>
+static int march_t(Engine *e, const hfdtd_source *src, int n) {
+  SrcList<T> sl;
+  int rc = build_srclist<T>(e, src, n, sl);
+  if (rc) return rc;
+  if (!e->fused_fits() || ((double)e->psi_off[PSI_COUNT] + 64.0) * e->elem >= 2147483648.0)
+    return fail(HFDTD_EUNSUPPORTED, "the march sweep needs field slabs below 4 GB and psi "
+                                    "slabs below 2 GB; use the split sweeps for this grid");
+  if ((rc = ensure_alt(e))) return rc;
+  const bool ll = e->lossless, nt = e->march_streaming();
+  if constexpr (sizeof(T) == 8) {
+    rc = ll ? (nt ? march_rw<T, 1, false, true>(e, sl) : march_rw<T, 1, false, false>(e, sl))
+            : (nt ? march_rw<T, 1, true, true>(e, sl) : march_rw<T, 1, true, false>(e, sl));
+  } else if (e->march_rows == 1) {
+    rc = ll ? (nt ? march_rw<T, 1, false, true>(e, sl) : march_rw<T, 1, false, false>(e, sl))
+            : (nt ? march_rw<T, 1, true, true>(e, sl) : march_rw<T, 1, true, false>(e, sl));
+  } else {
+    rc = ll ? (nt ? march_rw<T, 2, false, true>(e, sl) : march_rw<T, 2, false, false>(e, sl))
+            : (nt ? march_rw<T, 2, true, true>(e, sl) : march_rw<T, 2, true, false>(e, sl));
+  }
+  if (rc) return rc;
+  e->cur ^= 1;
+  return HFDTD_OK;
+}
+
+template <typename T>
+static int fused_t(Engine *e, const hfdtd_source *src, int n) {
+  SrcList<T> sl;
+  int rc = build_srclist<T>(e, src, n, sl);
+  if (rc) return rc;
   if (!e->fused_fits())
     return fail(HFDTD_EUNSUPPORTED, "the fused sweep needs field and psi slabs below 4 GB; "
                                     "use the split sweeps for this grid");
-  int rc = ensure_alt(e);
-  if (rc) return rc;
+  if ((rc = ensure_alt(e))) return rc;
   // FP64 slots are twice as large: tiles of at most 4 rows fit the LDS.
   // FP32 tiles of 12 rows fit only without the ca rows (lossy grids use 10).
   const bool ll = e->lossless;
@@ -249,6 +321,11 @@ static int fused_t(Engine *e, const hfdtd_source *src, int n) {
   if (rc) return rc;
   e->cur ^= 1;
   return HFDTD_OK;
+}
+
+int launch_march_step(Engine *e, const hfdtd_source *src, int n) {
+  HF_HIP(hipSetDevice(e->device));
+  return e->fp64 ? march_t<double>(e, src, n) : march_t<float>(e, src, n);
 }
 
 int launch_update_H(Engine *e) {

@@ -32,6 +32,8 @@ ACC_FP64 = 1
 ACC_KAHAN = 2
 PATH_SPLIT = 0
 PATH_FUSED = 1
+PATH_MARCH = 2
+PATH_CODES = {"split": PATH_SPLIT, "fused": PATH_FUSED, "march": PATH_MARCH}
 DFT_DIRECT = 0
 DFT_BUFFERED = 1
 
@@ -83,6 +85,7 @@ EXPORTS = {
     "hfdtd_synchronize": (C.c_int, [C.c_void_p]),
     "hfdtd_set_fused_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "hfdtd_set_split_schedule": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfdtd_set_march_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "hfdtd_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "hfdtd_n2f_create": (

@@ -7,9 +7,9 @@ how many blocks a grid yields and how often the two halo planes are re-read;
 a persistent batched launch also picks its grid size.  The best choice
 depends on the grid extents, the dtype, lossy or not, and the device.
 
-``tune_engine(sim)`` times the split sweeps and every fused-sweep candidate
-(HIP events on the engine stream) on a scratch engine of the same shape and
-applies the fastest; ``tune_batch(batch)`` does the same for a
+``tune_engine(sim)`` times the split sweeps and every fused-sweep and
+march-sweep candidate (HIP events on the engine stream) on a scratch engine
+of the same shape and applies the fastest; ``tune_batch(batch)`` does the same for a
 ``BatchedFDTD`` (fused tiles or flat split sweeps) before its tables are
 set.  All stepping happens on zero fields with no sources, which
 leaves the fields exactly zero, so tuning never changes a simulation.
@@ -35,6 +35,9 @@ from . import _lib
 FP32_ROWS = (2, 4, 6, 8, 10, 12)
 FP64_ROWS = (1, 2, 4)
 PLANES = (-1, 8, 16, 24, 32, 48, 64)
+MARCH_ROWS_FP32 = (1, 2)
+MARCH_ROWS_FP64 = (1,)
+MARCH_PLANES = (-1, 16, 24, 32)
 BATCH_ROWS_FP32 = (2, 4, 6, 8)
 BATCH_ROWS_FP64 = (2, 4)
 BATCH_PLANES = (8, 16, 32, 64)
@@ -87,28 +90,39 @@ def cache_key(kind, shapes, dtype, npml, lossy, device_name) -> str:
 
 
 def _time_engine(sim, steps, warmup):
+    """µs of sweep kernels per step (the split path launches two per step)."""
     for _ in range(warmup):
         sim.step()
     sim.synchronize()
     sim.kernel_timing(True)
     for _ in range(steps):
         sim.step()
-    ms, n = sim.kernel_timing_read()
+    ms, _ = sim.kernel_timing_read()
     sim.kernel_timing(False)
-    return ms / max(n, 1) * 1e3
+    return ms / max(steps, 1) * 1e3
 
 
-def tune_engine(sim, *, rows=None, planes=None, paths=("fused", "split"), steps=20, warmup=3,
-                use_cache=True, write_cache=True, cache_file=None) -> dict:  # fmt: skip
-    """Pick the step path and, for the fused sweep, (rows_j, planes_i) for
-    ``sim``, and apply them.
+def _apply(sim, path, r, p):
+    sim.set_path(path)
+    if path == "fused":
+        sim.set_fused_tiling(r, p, 0)
+    elif path == "march":
+        sim.set_march_tiling(r, p, 0, -2)
 
-    Times the fused sweep for rows x planes candidates and the split sweeps
-    (one candidate) on a scratch engine of the same shape.  Returns {"path":
-    "fused" | "split", "rows", "planes", "us": fastest µs of sweep kernels per
-    step, "table": [[path, rows, planes, µs], ...], "cached": bool};
-    ``planes`` -1 is the grid-derived default.  The fused candidates are
-    skipped for grids beyond the fused sweep's 4 GB slab limit.
+
+def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=None,
+                paths=("fused", "split", "march"), steps=20, warmup=3, use_cache=True,
+                write_cache=True, cache_file=None) -> dict:  # fmt: skip
+    """Pick the step path and its launch shape for ``sim``, and apply them.
+
+    Times, on a scratch engine of the same shape, the split sweeps (one
+    candidate), the LDS-DMA fused sweep for ``rows`` (j rows per block) x
+    ``planes`` candidates and the march sweep for ``march_rows`` (j rows per
+    wavefront) x ``march_planes`` candidates.  Returns {"path": "split" |
+    "fused" | "march", "rows", "planes", "us": fastest µs of sweep kernels
+    per step, "table": [[path, rows, planes, µs], ...], "cached": bool};
+    ``planes`` -1 is the grid-derived default.  One-launch candidates are
+    skipped for grids beyond their 32-bit slab limit.
     """
     from .engine import HIPFDTD
 
@@ -117,13 +131,13 @@ def tune_engine(sim, *, rows=None, planes=None, paths=("fused", "split"), steps=
     if use_cache:
         hit = load_cache(cache_file).get(key)
         if hit:
-            sim.set_path(hit["path"] == "fused")
-            if hit["path"] == "fused":
-                sim.set_fused_tiling(hit["rows"], hit["planes"], 0)
+            _apply(sim, hit["path"], hit["rows"], hit["planes"])
             return dict(hit, cached=True)
     fp64 = sim.dtype == np.dtype(np.float64)
     rows = rows or (FP64_ROWS if fp64 else FP32_ROWS)
     planes = planes or PLANES
+    march_rows = march_rows or (MARCH_ROWS_FP64 if fp64 else MARCH_ROWS_FP32)
+    march_planes = march_planes or MARCH_PLANES
     scratch = HIPFDTD(shape, sim.dl, npml=sim.npml, dtype=sim.dtype, device=sim.device,
                       fused=False)  # fmt: skip
     table = []
@@ -132,18 +146,22 @@ def tune_engine(sim, *, rows=None, planes=None, paths=("fused", "split"), steps=
             scratch.set_epsilon(np.ones(shape, np.float32), np.zeros(shape, np.float32))
         if "split" in paths:
             table.append(["split", None, None, round(_time_engine(scratch, steps, warmup), 3)])
-        scratch.set_path(True)
+        scratch.set_path("fused")
         if "fused" in paths and scratch.fused:
             for r in rows:
                 for p in planes:
                     scratch.set_fused_tiling(r, p, 0)
                     table.append(["fused", r, p, round(_time_engine(scratch, steps, warmup), 3)])
+        scratch.set_path("march")
+        if "march" in paths and scratch.path == "march":
+            for r in march_rows:
+                for p in march_planes:
+                    scratch.set_march_tiling(r, p, 0, -2)
+                    table.append(["march", r, p, round(_time_engine(scratch, steps, warmup), 3)])
     finally:
         scratch.close()
     path, r, p, us = min(table, key=lambda e: e[3])
-    sim.set_path(path == "fused")
-    if path == "fused":
-        sim.set_fused_tiling(r, p, 0)
+    _apply(sim, path, r, p)
     entry = {"path": path, "rows": r, "planes": p, "us": us, "table": table,
              "when": time.strftime("%Y-%m-%dT%H:%M:%S")}  # fmt: skip
     if write_cache:

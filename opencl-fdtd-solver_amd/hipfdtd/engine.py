@@ -58,7 +58,7 @@ class HIPFDTD(SourceMonitorMixin):
     MEMORY_HEADROOM_BYTES = 512 * 1024 * 1024
 
     def __init__(self, shape, dl=None, npml=20, dtype=np.float32, ctx=None, queue=None,
-                 accum_fp64=False, *, device=0, dx=None, fused=None):  # fmt: skip
+                 accum_fp64=False, *, device=0, dx=None, fused=None, path=None):  # fmt: skip
         if dl is None:
             dl = dx
         if dl is None:
@@ -93,9 +93,9 @@ class HIPFDTD(SourceMonitorMixin):
         vstride = (self.Nx * self.Ny * self.Nz + 64 + 63) // 64 * 64
         psi_total = 4 * (self.psi_x_size + self.psi_y_size + self.psi_z_size) + 64
         self._fused_fits = max(6 * vstride, psi_total) * item < 2**32
-        if fused is None:
-            fused = self.default_fused(dtype)
-        self.fused = bool(fused) and self._fused_fits
+        self.path = "split"
+        self.fused = False
+        self.set_path(self._resolve_path(fused, path, dtype))
         self._sources = []
         self._monitors = []
         self.lossy = False  # σ or ca volumes set: the sweeps read ca
@@ -142,15 +142,43 @@ class HIPFDTD(SourceMonitorMixin):
 
     # ── memory budget (engine.py:320-363) ──
 
-    @staticmethod
-    def default_fused(dtype) -> bool:
-        """Path chosen when ``fused=None``: fused sweep for FP64, split for FP32."""
-        return np.dtype(dtype) == np.dtype(np.float64)
+    PATHS = ("split", "fused", "march")
 
-    def set_path(self, fused: bool) -> None:
-        """Switch between the fused sweep (if the grid allows it) and the split
-        sweeps; takes effect at the next step, the state carries over."""
-        self.fused = bool(fused) and self._fused_fits
+    @staticmethod
+    def default_path(dtype) -> str:
+        """Path chosen when neither ``fused`` nor ``path`` is given: the LDS-DMA
+        fused sweep for FP64, the split sweeps for FP32."""
+        return "fused" if np.dtype(dtype) == np.dtype(np.float64) else "split"
+
+    @classmethod
+    def default_fused(cls, dtype) -> bool:
+        """Whether the default path is a one-launch (ping-pong) sweep."""
+        return cls.default_path(dtype) != "split"
+
+    @classmethod
+    def _resolve_path(cls, fused, path, dtype) -> str:
+        if path is not None:
+            if path not in cls.PATHS:
+                raise ValueError(f"path must be one of {cls.PATHS}; got {path!r}")
+            return path
+        if fused is None:
+            return cls.default_path(dtype)
+        return "fused" if fused else "split"
+
+    def set_path(self, path) -> None:
+        """Step path: 'split' (two sweeps, in place), 'fused' (LDS-DMA tiles)
+        or 'march' (register-pipelined tiles); ``True`` / ``False`` mean
+        'fused' / 'split'.  One-launch paths fall back to 'split' on grids
+        beyond their 32-bit offsets.  Takes effect at the next step; the state
+        carries over."""
+        if isinstance(path, (bool, np.bool_)):
+            path = "fused" if path else "split"
+        if path not in self.PATHS:
+            raise ValueError(f"path must be one of {self.PATHS}; got {path!r}")
+        if path != "split" and not self._fused_fits:
+            path = "split"
+        self.path = path
+        self.fused = path != "split"
 
     @staticmethod
     def estimate_device_memory_bytes(shape, npml, dtype=np.float32, fused=None):
@@ -320,7 +348,8 @@ class HIPFDTD(SourceMonitorMixin):
         else:
             n = len(self._pending)
             arr = (_lib.Source * max(n, 1))(*self._pending)
-            _lib.check(self._lib.hfdtd_step(self._handle, arr, n, _lib.PATH_FUSED), "step")
+            _lib.check(self._lib.hfdtd_step(self._handle, arr, n, _lib.PATH_CODES[self.path]),
+                       "step")  # fmt: skip
         self._pending = []
         self.t += self.dt
         self.step_num += 1
@@ -371,6 +400,16 @@ class HIPFDTD(SourceMonitorMixin):
         _lib.check(self._lib.hfdtd_reset_fields(self._handle), "reset_fields")
         self.t = 0.0
         self.step_num = 0
+
+    def set_march_tiling(self, rows_per_wave: int = 0, planes_i: int = 0, run_blocks: int = 0,
+                         nontemporal: int = -2) -> None:  # fmt: skip
+        """Launch shape of the march sweep: j rows per wavefront (1 or 2),
+        planes of i per block (-1 = from the grid), consecutive blocks per XCD
+        run, non-temporal streams (1 / 0 / -1 = when the slabs exceed the
+        last-level cache); 0 (-2 for ``nontemporal``) keeps a setting."""
+        _lib.check(self._lib.hfdtd_set_march_tiling(self._handle, int(rows_per_wave),
+                                                    int(planes_i), int(run_blocks),
+                                                    int(nontemporal)), "set_march_tiling")  # fmt: skip
 
     def set_fused_tiling(self, rows_j: int = 0, planes_i: int = 0, xcd_groups: int = 0) -> None:
         """Launch shape of the fused sweep: rows of j and planes of i per block,

@@ -1,7 +1,7 @@
 """GPU: the autotuner picks a launch shape without changing results.
 
 * ``tune_engine`` on a fresh engine leaves its fields zero, applies a
-  candidate (the split sweeps or a fused tiling) from its table, is served
+  candidate (the split sweeps, a fused or a march tiling) from its table, is served
   from the cache the second time, and the tuned engine steps bit-identically
   to an untuned one on the same path (the tile shape only reorders
   independent cell updates);
@@ -32,17 +32,18 @@ def test_tune_engine_applies_and_caches(tmp_path):
 
     cache = tmp_path / "tune.json"
     sim = hipfdtd.HIPFDTD(SHAPE, DL, npml=6)
-    res = tune_engine(sim, planes=(-1, 8, 24), steps=5, warmup=1, cache_file=cache)
+    res = tune_engine(sim, planes=(-1, 8, 24), march_planes=(-1, 4), steps=5, warmup=1,
+                      cache_file=cache)  # fmt: skip
     print("[autotune] engine", res["path"], res["rows"], res["planes"], res["us"], flush=True)
     assert not res["cached"]
     assert [res["path"], res["rows"], res["planes"], res["us"]] in res["table"]
-    assert len(res["table"]) == 1 + 6 * 3
-    assert sim.fused == (res["path"] == "fused")
+    assert len(res["table"]) == 1 + 6 * 3 + 2 * 2
+    assert sim.path == res["path"]
     assert cache.exists()
     again = tune_engine(sim, cache_file=cache)
     assert again["cached"] and again["path"] == res["path"] and again["planes"] == res["planes"]
     assert not np.any(sim.read_volume("Ex"))
-    ref = hipfdtd.HIPFDTD(SHAPE, DL, npml=6, fused=res["path"] == "fused")
+    ref = hipfdtd.HIPFDTD(SHAPE, DL, npml=6, path=res["path"])
     for s in (sim, ref):
         s.add_source(src)
         s.run(25)

@@ -4,7 +4,8 @@ Tolerances (peak-normalised, max|a-b|/max|b|):
 * FP32 HIP vs POCL FP32 fixtures: 1e-4 (target stated for the path);
 * FP32 HIP vs the FP32 oracle on the same inputs: 1e-5;
 * FP64 HIP vs the FP64 oracle: 1e-12;
-* fused sweep vs split sweeps: 1e-6; fused tilings among themselves: bit-exact.
+* fused sweep vs split sweeps: 1e-6; fused tilings among themselves: bit-exact;
+* march sweep vs fused sweep, and march tilings among themselves: bit-exact.
 """
 
 from __future__ import annotations
@@ -57,11 +58,11 @@ def run_plane(sim, case):
 
 
 @pytest.mark.parametrize("name", ["vacuum_sheet", "dielectric_block"])
-@pytest.mark.parametrize("fused", [True, False])
-def test_planes_match_fixtures_and_oracle(cases, name, fused):
+@pytest.mark.parametrize("path", ["fused", "split", "march"])
+def test_planes_match_fixtures_and_oracle(cases, name, path):
     case = cases[name]
-    print(f"[plane] {name} fused={fused}", flush=True)
-    sim = hip().HIPFDTD(tuple(case["shape"]), case["dl"], npml=case["npml"], fused=fused)
+    print(f"[plane] {name} path={path}", flush=True)
+    sim = hip().HIPFDTD(tuple(case["shape"]), case["dl"], npml=case["npml"], path=path)
     got = run_plane(sim, case)
     ora = run_plane(fo.OracleFDTD(tuple(case["shape"]), case["dl"], npml=case["npml"]), case)
     for s, plane in got.items():
@@ -113,17 +114,19 @@ def test_fused_matches_split_and_oracle(shape, npml, lossy):
     print(f"[fused-vs-split] shape={shape} npml={npml} lossy={lossy}", flush=True)
     eps, sig = _random_media(shape, 11, lossy)
     out = {}
-    for fused in (True, False):
-        sim = hip().HIPFDTD(shape, 1e-3, npml=npml, fused=fused)
+    for path in ("fused", "split", "march"):
+        sim = hip().HIPFDTD(shape, 1e-3, npml=npml, path=path)
         sim.set_epsilon(eps, sig)
         _drive(sim, shape, 40)
-        out[fused] = sim.fields()
+        out[path] = sim.fields()
     ora = fo.OracleFDTD(shape, 1e-3, npml=npml)
     ora.set_epsilon(eps, sig)
     _drive(ora, shape, 40)
-    assert group_error(out[True], out[False]) < 1e-6
-    assert group_error(out[True], ora.fields()) < 1e-5
-    assert group_error(out[False], ora.fields()) < 1e-5
+    assert group_error(out["fused"], out["split"]) < 1e-6
+    for path in ("fused", "split", "march"):
+        assert group_error(out[path], ora.fields()) < 1e-5, path
+    for n in FIELDS:
+        np.testing.assert_array_equal(out["march"][n], out["fused"][n], err_msg=n)
 
 
 @pytest.mark.parametrize("rows,planes,xcd,lossy", [
@@ -137,6 +140,26 @@ def test_fused_tilings_are_bit_identical(rows, planes, xcd, lossy):
     _drive(ref, shape, 12)
     sim = hip().HIPFDTD(shape, 1e-3, npml=6, fused=True)
     sim.set_fused_tiling(rows, planes, xcd)
+    sim.set_epsilon(eps, sig)
+    _drive(sim, shape, 12)
+    a, b = ref.fields(), sim.fields()
+    for n in FIELDS:
+        np.testing.assert_array_equal(a[n], b[n], err_msg=n)
+
+
+@pytest.mark.parametrize("rows,planes,run,nt,lossy", [
+    (2, -1, 16, -1, False), (1, 2, 1, 0, True), (2, 4, 7, 1, True), (1, 1000, 3, 1, False),
+    (2, 6, 1, 0, False), (1, -1, 64, 1, True)])  # fmt: skip
+def test_march_tilings_are_bit_identical(rows, planes, run, nt, lossy):
+    print(f"[march-tiling] rows={rows} planes={planes} run={run} nt={nt} lossy={lossy}",
+          flush=True)  # fmt: skip
+    shape = (45, 35, 100)
+    eps, sig = _random_media(shape, 3, lossy)
+    ref = hip().HIPFDTD(shape, 1e-3, npml=6, path="fused")
+    ref.set_epsilon(eps, sig)
+    _drive(ref, shape, 12)
+    sim = hip().HIPFDTD(shape, 1e-3, npml=6, path="march")
+    sim.set_march_tiling(rows, planes, run, nt)
     sim.set_epsilon(eps, sig)
     _drive(sim, shape, 12)
     a, b = ref.fields(), sim.fields()
@@ -161,11 +184,11 @@ def test_set_epsilon_coefficients_match_oracle_bitwise():
         sim.set_epsilon(eps, np.ones((3, 3, 3), np.float32))
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_jx_closed_form_and_rim(fused):
+@pytest.mark.parametrize("path", ["fused", "split", "march"])
+def test_jx_closed_form_and_rim(path):
     shape, npml, z = (16, 16, 16), 2, 8
     p = npml
-    sim = hip().HIPFDTD(shape, 1e-3, npml=npml, fused=fused)
+    sim = hip().HIPFDTD(shape, 1e-3, npml=npml, path=path)
     eps = np.ones(shape, np.float32)
     eps[8, 8, z] = 4.0
     sim.set_epsilon(eps)
@@ -175,7 +198,7 @@ def test_jx_closed_form_and_rim(fused):
     assert ex[8, 8, z] == pytest.approx(-sim.dt / (fo.EPS0 * 2.5) * 0.25, rel=1e-6)
     assert ex[p, p, z] == pytest.approx(-sim.dt / fo.EPS0 * 0.25, rel=1e-6)
     assert ex[0, 0, z] == 0.0
-    sim2 = hip().HIPFDTD(shape, 1e-3, npml=npml, fused=fused)
+    sim2 = hip().HIPFDTD(shape, 1e-3, npml=npml, path=path)
     sim2.add_source(lambda f: f.add_source_Jx(z, 1.0, i0=p, i1=14, j0=p, j1=14, rim_taper=True,
                                               rim_edge=0.5, rim_renorm=False))  # fmt: skip
     sim2.step()
@@ -200,8 +223,8 @@ def test_field_access_read_point_and_shapes():
     assert sim.step_num == 1 and sim.t == pytest.approx(sim.dt)
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_callback_that_edits_fields_matches_soft_source(fused):
+@pytest.mark.parametrize("path", ["fused", "split", "march"])
+def test_callback_that_edits_fields_matches_soft_source(path):
     """The harness pattern `ex = f.Ex; ex[...] += amp; upload` (cases.py:64-75)
     equals add_source_Ex; the callback sees E before and H after the H update."""
     shape = (24, 22, 26)
@@ -213,9 +236,9 @@ def test_callback_that_edits_fields_matches_soft_source(fused):
         ex[10:15, 9:14, 12] += np.float32(fo.gaussian_sine(f.t, 3e10, 1e10))
         f.Ex = ex
 
-    a = hip().HIPFDTD(shape, 1e-3, npml=4, fused=fused)
+    a = hip().HIPFDTD(shape, 1e-3, npml=4, path=path)
     a.add_source(edit)
-    b = hip().HIPFDTD(shape, 1e-3, npml=4, fused=fused)
+    b = hip().HIPFDTD(shape, 1e-3, npml=4, path=path)
     b.add_source(lambda f: f.add_source_Ex(12, fo.gaussian_sine(f.t, 3e10, 1e10), 10, 15, 9, 14))
     for _ in range(30):
         a.step()
@@ -224,10 +247,10 @@ def test_callback_that_edits_fields_matches_soft_source(fused):
     assert seen[1][1] == pytest.approx(1.5 * a.dt)
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_npml_zero_runs_and_matches_oracle(fused):
+@pytest.mark.parametrize("path", ["fused", "split", "march"])
+def test_npml_zero_runs_and_matches_oracle(path):
     shape = (20, 20, 20)
-    sim = hip().HIPFDTD(shape, 1e-3, npml=0, fused=fused)
+    sim = hip().HIPFDTD(shape, 1e-3, npml=0, path=path)
     ora = fo.OracleFDTD(shape, 1e-3, npml=0)
     for s in (sim, ora):
         s.add_source(lambda f: f.add_source_Ex(10, 0.1))
