@@ -42,9 +42,16 @@ constexpr int kMarchThreads = 256;  // four wavefronts
 // resource's range and are dropped, loads return 0, neither touches memory.
 constexpr unsigned kDropOffset = 0x80000000u;
 
+// LDS of a block: new H and E of the planes, double-buffered by plane
+// parity, on a grid of rows j0-1 ... j0+4RW (then a trash row and one spare
+// row, so neighbour reads from the trash row stay inside the array) by
+// columns k0-1 ... k0+64.
+constexpr int kMarchCols = 66;
 template <typename T, int RW>
 struct MarchLds {
-  T h[2][3][4 * RW + 2][65];  // [plane parity][Hx Hy Hz][row j0-1 .. j0+4RW, trash][col k0-1 ..]
+  static constexpr int R = 4 * RW + 4;
+  T h[2][3][R][kMarchCols];  // new H(i): [parity][Hx Hy Hz]
+  T e[2][3][R][kMarchCols];  // E(i) (old), the j+1 / k+1 neighbours of H
 };
 
 // Plane-i data of one slot, loaded one plane ahead in two parts: the H part
@@ -53,10 +60,9 @@ struct MarchLds {
 // are still being read.
 template <typename T>
 struct MarchH {
-  T ex, ey, ez;          // E(i+1) at the cell (becomes E of the next plane)
-  T hx, hy, hz;          // old H(i)
-  T exj, ezj, exk, eyk;  // E(i) at j+1 and at k+1
-  T p[6];                // old psi of the H slots (PML tiles)
+  T ex, ey, ez;  // E(i+1) at the cell (becomes E of the next plane)
+  T hx, hy, hz;  // old H(i)
+  T p[6];        // old psi of the H slots (PML tiles)
 };
 
 template <typename T>
@@ -75,7 +81,7 @@ struct MarchSlot {
   unsigned qz4;  // byte offset inside the z-stripe rows of a plane (kDropOffset outside)
   bool iny, inz;
   bool own;      // an owned cell inside the grid
-  int li;        // LDS element index (row * 65 + column)
+  int lh, le;    // LDS element index (row * kMarchCols + column) of its H and E
 };
 
 // Raw buffer loads and stores; NT marks them non-temporal (streamed once).
@@ -111,9 +117,10 @@ __device__ __forceinline__ unsigned long long slab_bytes(const Geo &g) {
 struct PsiPlane {
   bool inx;
   unsigned xs, ys, zs;
+  bool dead;  // a plane past the chunk: its loads are dropped
 };
 
-template <typename T, int RW, bool LOSSY, bool XY, bool Z, bool NT>
+template <typename T, int RW, bool LOSSY, bool X, bool Y, bool Z, bool NT>
 struct March {
   static constexpr int S = RW + 1;  // RW owned rows + the halo slot
   static constexpr int TJ = 4 * RW;
@@ -129,6 +136,7 @@ struct March {
   T oe[S][3];          // E(i) at every slot (copied from the loads of plane i-1)
   unsigned padb;       // store offset of lanes and planes outside the grid
   int iend;            // first plane past this block's chunk
+  bool sized;          // slab resources carry their sizes (slabs below 2 GB)
 
   __device__ __forceinline__ unsigned vsb() const { return (unsigned)g.vs * E; }
   __device__ __forceinline__ unsigned plane_b(int i) const {
@@ -145,40 +153,45 @@ struct March {
     pp.xs = pp.inx ? (unsigned)stripe_local(ic, n, g.nx) * (unsigned)(g.ny * g.nz) : 0u;
     pp.ys = (unsigned)ic * 2u * (unsigned)n * (unsigned)g.nz;
     pp.zs = (unsigned)ic * (unsigned)g.ny * 2u * (unsigned)n;
+    pp.dead = sized && i >= iend;
     return pp;
+  }
+  // plane part of a field load: past the resource's end for a plane after
+  // the chunk (the load is dropped and returns 0), else the clamped plane
+  __device__ __forceinline__ unsigned load_plane(int i, bool dead) const {
+    return dead ? kDropOffset : plane_b(i);
   }
 
   // H part of plane i.  psi of the x and y stripes is loaded on every plane
-  // of an XY block, psi_z on every plane of a Z block; lanes outside a
+  // of an X block, psi_y on every plane of a Y block, psi_z on every plane
+  // of a Z block; lanes outside a
   // stripe load from past the end of the psi slab's resource, which returns
   // 0 without touching memory (h_math ignores the value).  Pointing them all
   // at one valid element instead would send every wave of the grid to the
   // same cache line.
   __device__ __forceinline__ void load_h(MarchH<T> &s, const MarchSlot &m, unsigned pl,
                                          unsigned pl1, const PsiPlane &pp) const {
-    const unsigned vs = vsb(), nzb = (unsigned)g.nz * E;
+    const unsigned vs = vsb();
     s.ex = bld<T, false>(r0, m.o4, pl1);
     s.ey = bld<T, false>(r0, m.o4, vs + pl1);
     s.ez = bld<T, false>(r0, m.o4, 2 * vs + pl1);
     s.hx = bld<T, false>(r0, m.o4, 3 * vs + pl);
     s.hy = bld<T, false>(r0, m.o4, 4 * vs + pl);
     s.hz = bld<T, false>(r0, m.o4, 5 * vs + pl);
-    // j+1 and k+1 neighbours; past the last row / column they read the next
-    // row / plane or the padding, inside the slab, and h_math ignores them
-    s.exj = bld<T, false>(r0, m.o4, pl + nzb);
-    s.ezj = bld<T, false>(r0, m.o4, 2 * vs + pl + nzb);
-    s.exk = bld<T, false>(r0, m.o4 + E, pl);
-    s.eyk = bld<T, false>(r0, m.o4 + E, vs + pl);
-    if constexpr (XY) {
-      const unsigned vx = pp.inx ? m.o4 : kDropOffset;
+    if constexpr (X) {
+      const unsigned vx = pp.inx && !pp.dead ? m.o4 : kDropOffset;
       s.p[0] = bld<T, false>(rq0, vx, psib(0, pp.xs));
       s.p[1] = bld<T, false>(rq0, vx, psib(1, pp.xs));
-      s.p[2] = bld<T, false>(rq0, m.qy4, psib(2, pp.ys));
-      s.p[3] = bld<T, false>(rq0, m.qy4, psib(3, pp.ys));
+    }
+    if constexpr (Y) {
+      const unsigned vy = pp.dead ? kDropOffset : m.qy4;
+      s.p[2] = bld<T, false>(rq0, vy, psib(2, pp.ys));
+      s.p[3] = bld<T, false>(rq0, vy, psib(3, pp.ys));
     }
     if constexpr (Z) {
-      s.p[4] = bld<T, false>(rq0, m.qz4, psib(4, pp.zs));
-      s.p[5] = bld<T, false>(rq0, m.qz4, psib(5, pp.zs));
+      const unsigned vz = pp.dead ? kDropOffset : m.qz4;
+      s.p[4] = bld<T, false>(rq0, vz, psib(4, pp.zs));
+      s.p[5] = bld<T, false>(rq0, vz, psib(5, pp.zs));
     }
   }
 
@@ -193,29 +206,33 @@ struct March {
       s.ay = bld<T, NT>(rc, m.o4, 4 * vs + pl);
       s.az = bld<T, NT>(rc, m.o4, 5 * vs + pl);
     }
-    if constexpr (XY) {
-      const unsigned vx = pp.inx ? m.o4 : kDropOffset;
+    if constexpr (X) {
+      const unsigned vx = pp.inx && !pp.dead ? m.o4 : kDropOffset;
       s.p[0] = bld<T, false>(rq0, vx, psib(6, pp.xs));
       s.p[1] = bld<T, false>(rq0, vx, psib(7, pp.xs));
-      s.p[2] = bld<T, false>(rq0, m.qy4, psib(8, pp.ys));
-      s.p[3] = bld<T, false>(rq0, m.qy4, psib(9, pp.ys));
+    }
+    if constexpr (Y) {
+      const unsigned vy = pp.dead ? kDropOffset : m.qy4;
+      s.p[2] = bld<T, false>(rq0, vy, psib(8, pp.ys));
+      s.p[3] = bld<T, false>(rq0, vy, psib(9, pp.ys));
     }
     if constexpr (Z) {
-      s.p[4] = bld<T, false>(rq0, m.qz4, psib(10, pp.zs));
-      s.p[5] = bld<T, false>(rq0, m.qz4, psib(11, pp.zs));
+      const unsigned vz = pp.dead ? kDropOffset : m.qz4;
+      s.p[4] = bld<T, false>(rq0, vz, psib(10, pp.zs));
+      s.p[5] = bld<T, false>(rq0, vz, psib(11, pp.zs));
     }
   }
 
   __device__ __forceinline__ void load_h_all(MarchH<T> (&s)[S], int i) const {
-    const unsigned pl = plane_b(i), pl1 = plane_b(i + 1);
     const PsiPlane pp = psi_plane(i);
+    const unsigned pl = load_plane(i, pp.dead), pl1 = load_plane(i + 1, pp.dead);
 #pragma unroll
     for (int q = 0; q < S; ++q) load_h(s[q], sl[q], pl, pl1, pp);
   }
 
   __device__ __forceinline__ void load_e_all(MarchE<T> (&s)[RW], int i) const {
-    const unsigned pl = plane_b(i);
     const PsiPlane pp = psi_plane(i);
+    const unsigned pl = load_plane(i, pp.dead);
 #pragma unroll
     for (int q = 0; q < RW; ++q) load_e(s[q], sl[q], pl, pp);
   }
@@ -231,7 +248,7 @@ struct March {
 
   __device__ __forceinline__ StripeIdx stripes(const MarchSlot &m, bool inx) const {
     StripeIdx st;
-    st.inx = XY && inx;
+    st.inx = X && inx;
     st.iny = m.iny;
     st.inz = m.inz;
     st.qx = st.qy = st.qz = 0;
@@ -266,9 +283,13 @@ struct March {
       const MarchSlot &m = sl[q];
       const StripeIdx st = stripes(m, pp.inx);
       T p[6] = {0, 0, 0, 0, 0, 0};
-      if constexpr (XY) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) p[t] = hc[q].p[t];
+      if constexpr (X) {
+        p[0] = hc[q].p[0];
+        p[1] = hc[q].p[1];
+      }
+      if constexpr (Y) {
+        p[2] = hc[q].p[2];
+        p[3] = hc[q].p[3];
       }
       if constexpr (Z) {
         p[4] = hc[q].p[4];
@@ -277,21 +298,28 @@ struct March {
       h[q][0] = hc[q].hx;
       h[q][1] = hc[q].hy;
       h[q][2] = hc[q].hz;
-      h_math(g, ic, m.j, m.k, st, pxh, pyh[q], pzh[q], oe[q][0], oe[q][1], oe[q][2], hc[q].exj,
-             hc[q].ezj, hc[q].exk, hc[q].eyk, hc[q].ey, hc[q].ez, h[q][0], h[q][1], h[q][2], p,
-             dtm_dl, dtm);
-      (&L.h[buf][0][0][0])[m.li] = h[q][0];
-      (&L.h[buf][1][0][0])[m.li] = h[q][1];
-      (&L.h[buf][2][0][0])[m.li] = h[q][2];
+      // E(i) at j+1 and k+1 (written to LDS during the previous plane); past
+      // the grid's last row / column they are clamped copies h_math ignores
+      const T exj = (&L.e[buf][0][0][0])[m.le + kMarchCols];
+      const T ezj = (&L.e[buf][2][0][0])[m.le + kMarchCols];
+      const T exk = (&L.e[buf][0][0][0])[m.le + 1];
+      const T eyk = (&L.e[buf][1][0][0])[m.le + 1];
+      h_math(g, ic, m.j, m.k, st, pxh, pyh[q], pzh[q], oe[q][0], oe[q][1], oe[q][2], exj, ezj,
+             exk, eyk, hc[q].ey, hc[q].ez, h[q][0], h[q][1], h[q][2], p, dtm_dl, dtm);
+      (&L.h[buf][0][0][0])[m.lh] = h[q][0];
+      (&L.h[buf][1][0][0])[m.lh] = h[q][1];
+      (&L.h[buf][2][0][0])[m.lh] = h[q][2];
       if (q < RW) {
         const bool ok = m.own && live;
         const unsigned v = ok ? m.o4 + pl : padb;
         bst<T, NT>(r1, h[q][0], v, 3 * vs);
         bst<T, NT>(r1, h[q][1], v, 4 * vs);
         bst<T, NT>(r1, h[q][2], v, 5 * vs);
-        if constexpr (XY) {
+        if constexpr (X) {
           psi_st(p[0], 0, ok && pp.inx, m.o4 + pp.xs * E);
           psi_st(p[1], 1, ok && pp.inx, m.o4 + pp.xs * E);
+        }
+        if constexpr (Y) {
           psi_st(p[2], 2, ok && m.iny, m.qy4 + pp.ys * E);
           psi_st(p[3], 3, ok && m.iny, m.qy4 + pp.ys * E);
         }
@@ -300,6 +328,13 @@ struct March {
           psi_st(p[5], 5, ok && m.inz, m.qz4 + pp.zs * E);
         }
       }
+    }
+    // E(i+1) of every slot for the next plane's H neighbours
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      (&L.e[buf ^ 1][0][0][0])[sl[q].le] = hc[q].ex;
+      (&L.e[buf ^ 1][1][0][0])[sl[q].le] = hc[q].ey;
+      (&L.e[buf ^ 1][2][0][0])[sl[q].le] = hc[q].ez;
     }
     // the E part of plane i+1, in flight across the barrier and the E half
     __builtin_amdgcn_sched_barrier(0);
@@ -311,13 +346,18 @@ struct March {
 #pragma unroll
     for (int q = 0; q < RW; ++q) {
       const MarchSlot &m = sl[q];
-      const T hz_jm = (&L.h[buf][2][0][0])[m.li - 65], hx_jm = (&L.h[buf][0][0][0])[m.li - 65];
-      const T hy_km = (&L.h[buf][1][0][0])[m.li - 1], hx_km = (&L.h[buf][0][0][0])[m.li - 1];
+      const T hz_jm = (&L.h[buf][2][0][0])[m.lh - kMarchCols];
+      const T hx_jm = (&L.h[buf][0][0][0])[m.lh - kMarchCols];
+      const T hy_km = (&L.h[buf][1][0][0])[m.lh - 1], hx_km = (&L.h[buf][0][0][0])[m.lh - 1];
       const StripeIdx st = stripes(m, pp.inx);
       T p[6] = {0, 0, 0, 0, 0, 0};
-      if constexpr (XY) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) p[t] = ec[q].p[t];
+      if constexpr (X) {
+        p[0] = ec[q].p[0];
+        p[1] = ec[q].p[1];
+      }
+      if constexpr (Y) {
+        p[2] = ec[q].p[2];
+        p[3] = ec[q].p[3];
       }
       if constexpr (Z) {
         p[4] = ec[q].p[4];
@@ -334,9 +374,11 @@ struct March {
       bst<T, NT>(r1, ex, v, 0u);
       bst<T, NT>(r1, ey, v, vs);
       bst<T, NT>(r1, ez, v, 2 * vs);
-      if constexpr (XY) {
+      if constexpr (X) {
         psi_st(p[0], 6, ok && pp.inx, m.o4 + pp.xs * E);
         psi_st(p[1], 7, ok && pp.inx, m.o4 + pp.xs * E);
+      }
+      if constexpr (Y) {
         psi_st(p[2], 8, ok && m.iny, m.qy4 + pp.ys * E);
         psi_st(p[3], 9, ok && m.iny, m.qy4 + pp.ys * E);
       }
@@ -359,44 +401,73 @@ struct March {
   __device__ __forceinline__ void run(MarchLds<T, RW> &L, int planes, int k0, int j0, int i0,
                                       int i1) {
     iend = i1;
+    sized = slab_bytes<T>(g) < 0x80000000ull;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int n = g.npml;
+    // halo slot lanes of wave w: 0-15 H on row j0-1 (columns k0+16w ...);
+    // 16 .. 16+RW-1 H on column k0-1 (the wave's rows); 32-47 E only on
+    // row j0+TJ (columns k0+16w ...); 48-50 E only on column k0+64 (rows
+    // j0-1+3w ...); lane 51 of wave 0 E only at (j0+TJ, k0-1); the rest idle.
+    // H of E-only lanes and both of idle lanes go to the trash row.
+    constexpr int trash_row = TJ + 2;
 #pragma unroll
     for (int q = 0; q <= RW; ++q) {
-      int j, k, lr, lc;
+      int j, k, hr, hc, er, ec;
       bool own = false;
       if (q < RW) {
         j = j0 + RW * w + q;
         k = k0 + lane;
-        lr = RW * w + q + 1;
-        lc = lane + 1;
+        hr = er = RW * w + q + 1;
+        hc = ec = lane + 1;
         own = j < g.ny && k < g.nz;
       } else if (lane < 16) {
         j = j0 - 1;
         k = k0 + 16 * w + lane;
-        lr = 0;
-        lc = 16 * w + lane + 1;
+        hr = er = 0;
+        hc = ec = 16 * w + lane + 1;
       } else if (lane < 16 + RW) {
         j = j0 + RW * w + lane - 16;
         k = k0 - 1;
-        lr = RW * w + lane - 16 + 1;
-        lc = 0;
+        hr = er = RW * w + lane - 16 + 1;
+        hc = ec = 0;
+      } else if (lane >= 32 && lane < 48) {
+        j = j0 + TJ;
+        k = k0 + 16 * w + lane - 32;
+        hr = trash_row;
+        hc = lane;
+        er = TJ + 1;
+        ec = 16 * w + lane - 32 + 1;
+      } else if (lane >= 48 && lane < 51 && 3 * w + lane - 48 <= TJ) {
+        j = j0 - 1 + 3 * w + lane - 48;
+        k = k0 + 64;
+        hr = trash_row;
+        hc = lane;
+        er = 3 * w + lane - 48;
+        ec = 65;
+      } else if (lane == 51 && w == 0) {
+        j = j0 + TJ;
+        k = k0 - 1;
+        hr = trash_row;
+        hc = lane;
+        er = TJ + 1;
+        ec = 0;
       } else {
         j = j0;
         k = k0;
-        lr = TJ + 1;  // trash row
-        lc = lane;
+        hr = er = trash_row;
+        hc = ec = lane;
       }
       MarchSlot &m = sl[q];
       m.j = min(max(j, 0), g.ny - 1);
       m.k = min(max(k, 0), g.nz - 1);
       m.o4 = (unsigned)(m.j * g.nz + m.k) * E;
-      m.iny = XY && n > 0 && (m.j < n || m.j >= g.ny - n);
+      m.iny = Y && n > 0 && (m.j < n || m.j >= g.ny - n);
       m.inz = Z && n > 0 && (m.k < n || m.k >= g.nz - n);
       m.qy4 = m.iny ? (unsigned)(stripe_local(m.j, n, g.ny) * g.nz + m.k) * E : kDropOffset;
       m.qz4 = m.inz ? (unsigned)(m.j * 2 * n + stripe_local(m.k, n, g.nz)) * E : kDropOffset;
-      m.li = lr * 65 + lc;
+      m.lh = hr * kMarchCols + hc;
+      m.le = er * kMarchCols + ec;
       m.own = own;
       pyh[q] = prof1(g, prof, 1, 1, m.j);
       pye[q] = prof1(g, prof, 0, 1, m.j);
@@ -405,7 +476,7 @@ struct March {
     }
     // dropped by the hardware when the slab resource carries its size (slabs
     // below 2 GB), else this lane's slot in the volume padding
-    if (slab_bytes<T>(g) < 0x80000000ull) padb = kDropOffset;
+    if (sized) padb = kDropOffset;
     else padb = ((unsigned)((size_t)g.nx * g.ny * g.nz) + (unsigned)lane) * E;
 
     const unsigned vs = vsb();
@@ -457,8 +528,15 @@ struct March {
         hzc[q] = hz;
       }
     }
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      (&L.e[0][0][0][0])[sl[q].le] = oe[q][0];
+      (&L.e[0][1][0][0])[sl[q].le] = oe[q][1];
+      (&L.e[0][2][0][0])[sl[q].le] = oe[q][2];
+    }
     load_h_all(A, i0);
     load_e_all(EA, i0);
+    lds_barrier();
     // the last plane's loads read the (clamped) plane after the chunk and go
     // unused; a peeled last pair makes the register allocator shuffle the
     // sets at the loop head, which waits for every load in flight
@@ -497,10 +575,11 @@ struct MarchChunks {
 };
 
 // One block per (k tile, j tile, i chunk).  A block takes the variant with
-// the psi it needs: XY when its chunk or its rows (with the halo row) meet
-// an x- or y-stripe, Z when its columns (with the halo column) meet a
-// z-stripe.  (The plane before the chunk, recomputed once, reads its psi in
-// any variant.)
+// the psi it needs: X when its chunk meets an x-stripe, Y when its rows
+// (with the halo row) meet a y-stripe, Z when its columns (with the halo
+// column) meet a z-stripe.  (The plane before the chunk, recomputed once,
+// reads its psi in any variant.)  Fewer psi operations per plane also keep
+// a plane's loads and stores within the 63 a wavefront can count.
 template <typename T, int RW, bool LOSSY, bool NT>
 __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dtm_dl, T dtm,
                                                          T inv_dl, const T *__restrict__ s0,
@@ -519,30 +598,38 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
   ch.range(g, planes, bc, i0, i1);
   const int np = (i1 - i0 + 1) / 2 * 2;  // even; planes past i1 are masked
   const int n = g.npml;
-  const bool xy = n > 0 && (i0 < n || i1 > g.nx - n || j0 - 1 < n ||
-                            min(j0 + 4 * RW - 1, g.ny - 1) >= g.ny - n);
+  const bool x = n > 0 && (i0 < n || i1 > g.nx - n);
+  const bool y = n > 0 && (j0 - 1 < n || min(j0 + 4 * RW - 1, g.ny - 1) >= g.ny - n);
   const bool z = n > 0 && (k0 - 1 < n || min(k0 + 63, g.nz - 1) >= g.nz - n);
   const unsigned long long sb = slab_bytes<T>(g);
-  const Rsrc r0 = rsrc_of(s0), rc = rsrc_of(coef);
-  const Rsrc r1 = __builtin_amdgcn_make_buffer_rsrc(s1, (short)0, sb < 0x80000000ull ? (int)sb : -1,
+  const int sbr = sb < 0x80000000ull ? (int)sb : -1;  // the coefficient slab has the same size
+  const Rsrc r0 = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(s0), (short)0, sbr, 0x00020000);
+  const Rsrc rc = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(coef), (short)0, sbr,
                                                     0x00020000);
+  const Rsrc r1 = __builtin_amdgcn_make_buffer_rsrc(s1, (short)0, sbr, 0x00020000);
   const int psi_bytes = (int)((psi_slot(g, PSI_COUNT) + 64u) * (unsigned)sizeof(T));
   const Rsrc rq0 = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(q0), (short)0, psi_bytes,
                                                      0x00020000);
   const Rsrc rq1 = __builtin_amdgcn_make_buffer_rsrc(q1, (short)0, psi_bytes, 0x00020000);
-  if (xy && z) {
-    March<T, RW, LOSSY, true, true, NT> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, prof};
-    m.run(L, np, k0, j0, i0, i1);
-  } else if (xy) {
-    March<T, RW, LOSSY, true, false, NT> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, prof};
-    m.run(L, np, k0, j0, i0, i1);
-  } else if (z) {
-    March<T, RW, LOSSY, false, true, NT> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, prof};
-    m.run(L, np, k0, j0, i0, i1);
-  } else {
-    March<T, RW, LOSSY, false, false, NT> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, prof};
-    m.run(L, np, k0, j0, i0, i1);
+#define HF_MARCH_RUN(X, Y, Z)                                                                  \
+  {                                                                                           \
+    March<T, RW, LOSSY, X, Y, Z, NT> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, prof};   \
+    m.run(L, np, k0, j0, i0, i1);                                                             \
   }
+  if (x) {
+    if (y) {
+      if (z) HF_MARCH_RUN(true, true, true) else HF_MARCH_RUN(true, true, false)
+    } else {
+      if (z) HF_MARCH_RUN(true, false, true) else HF_MARCH_RUN(true, false, false)
+    }
+  } else {
+    if (y) {
+      if (z) HF_MARCH_RUN(false, true, true) else HF_MARCH_RUN(false, true, false)
+    } else {
+      if (z) HF_MARCH_RUN(false, false, true) else HF_MARCH_RUN(false, false, false)
+    }
+  }
+#undef HF_MARCH_RUN
 }
 
 }  // namespace hfdtd
