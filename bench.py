@@ -133,6 +133,9 @@ def run_gpu(args, wl: Workload):
     path = None if args.path == "auto" else args.path
     sim = hipfdtd.HIPFDTD(wl.shape, wl.dl, npml=wl.npml, dtype=np.float32, path=path)
     tuned = None
+    if args.path == "march" and args.march_tiling:
+        r, p = (int(v) for v in args.march_tiling.split(","))
+        sim.set_march_tiling(r, p, 0, -2)
     if args.path == "auto":
         # the engine's autotuner picks the step path and launch shape (on a
         # scratch engine, before anything is timed)
@@ -517,6 +520,8 @@ def main():
                     choices=["yee-256", "plane-dft", "adjoint", "batch"])
     ap.add_argument("--graph-steps", type=int, default=100, help="batch: steps per HIP graph")
     ap.add_argument("--repeat", type=int, default=2, help="adjoint: timed forward+backward runs")
+    ap.add_argument("--march-tiling", default=None, metavar="ROWS,PLANES",
+                    help="with --path march: rows per wavefront and planes per block")
     ap.add_argument("--path", default="auto", choices=["auto", "fused", "split", "march"],
                     help="256^3 workload: engine step path (auto = picked by the autotuner)")
     args = ap.parse_args()

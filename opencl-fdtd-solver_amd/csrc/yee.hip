@@ -225,11 +225,18 @@ static int march_rw(Engine *e, const SrcList<T> &sl) {
   auto kern = k_march<T, RW, LOSSY, NT>;
   int &occ = e->march_occ[LOSSY ? 1 : 0][RW];
   if (occ == 0) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kMarchThreads, 0) !=
-            hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
+    // resident blocks per CU from the kernel's own resource use: a block puts
+    // one wavefront on each of the four SIMDs, each SIMD holds 512 VGPRs per
+    // lane (allocated in granules of 8), a CU has 160 KB of LDS
+    hipFuncAttributes fa;
+    int per_cu = 1;
+    if (hipFuncGetAttributes(&fa, (const void *)kern) == hipSuccess && fa.numRegs > 0) {
+      const int vg = (fa.numRegs + 7) / 8 * 8;
+      per_cu = 512 / vg;
+      if (fa.sharedSizeBytes > 0)
+        per_cu = std::min(per_cu, (int)(160 * 1024 / fa.sharedSizeBytes));
+      per_cu = std::max(per_cu, 1);
+    }
     occ = per_cu;
   }
   const int planes = e->march_planes_for(RW, occ);
