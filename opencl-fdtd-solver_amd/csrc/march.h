@@ -27,16 +27,65 @@
 //
 // The per-cell arithmetic is h_math / e_math, shared with every other
 // sweep.  State is ping-ponged like the LDS-DMA fused sweep (reads slab 0,
-// writes slab 1) and sources are redone on their cells by the fix-up
-// kernel after the sweep.  Offsets are 32-bit: slabs stay below 4 GB
+// writes slab 1); up to four sources of the step are added to E on their
+// cells right before its update, in the order the fix-up kernel of the
+// LDS-DMA sweep adds them (steps with more sources use that kernel).  Offsets are 32-bit: slabs stay below 4 GB
 // (checked on the host).
 #pragma once
+
+#include <type_traits>
 
 #include "fdtd_core.h"
 
 namespace hfdtd {
 
 constexpr int kMarchThreads = 256;  // four wavefronts
+constexpr int kMarchSources = 4;    // sources applied inside the sweep (more: fix-up kernel)
+
+// Sources of one step, passed to the sweep by value.  The block copies them
+// to LDS once and reads them there with volatile loads, so they occupy no
+// registers across the plane loop.
+template <typename T>
+struct MarchSources {
+  int n;
+  int i0, i1;  // planes covered by some source (their union's bounds)
+  Src<T> s[kMarchSources];
+};
+
+// a if c0, else b if c1, else c, by masking the bit patterns.
+template <typename T>
+__device__ __forceinline__ T pick3(bool c0, bool c1, T a, T b, T c) {
+  typedef typename std::conditional<sizeof(T) == 4, unsigned, unsigned long long>::type U;
+  const U m0 = c0 ? ~U(0) : U(0), m1 = !c0 && c1 ? ~U(0) : U(0), m2 = ~(m0 | m1);
+  return __builtin_bit_cast(T, (__builtin_bit_cast(U, a) & m0) | (__builtin_bit_cast(U, b) & m1) |
+                                   (__builtin_bit_cast(U, c) & m2));
+}
+
+// Source t of the LDS copy, read field by field through volatile loads (so
+// the compiler re-reads them where they are used instead of holding them).
+template <typename T>
+__device__ __forceinline__ Src<T> lds_src(const Src<T> *base, int t) {
+  const volatile Src<T> *p = base + t;
+  Src<T> s;
+  s.comp = p->comp;
+  s.mode = p->mode;
+  s.taper = p->taper;
+  s.i0 = p->i0;
+  s.i1 = p->i1;
+  s.j0 = p->j0;
+  s.j1 = p->j1;
+  s.k0 = p->k0;
+  s.k1 = p->k1;
+  s.ci0 = p->ci0;
+  s.ci1 = p->ci1;
+  s.cj0 = p->cj0;
+  s.cj1 = p->cj1;
+  s.ck0 = p->ck0;
+  s.ck1 = p->ck1;
+  s.amp = p->amp;
+  s.re = p->re;
+  return s;
+}
 // A buffer offset no slab reaches whose resource carries its size (psi
 // slabs stay below 2 GB, checked on the host): stores there fall outside the
 // resource's range and are dropped, loads return 0, neither touches memory.
@@ -130,6 +179,9 @@ struct March {
   T dtm_dl, dtm, inv_dl;
   Rsrc r0, r1, rc, rq0, rq1;  // slab 0, slab 1, coefficients, psi 0, psi 1
   const T *__restrict__ prof;
+  const Src<T> *lsrc;  // sources of the step (LDS copy)
+  int nsrc;
+  int src_i0, src_i1;  // planes some source covers
   MarchSlot sl[S];
   Prof1<T> pyh[S], pye[S], pzh[S], pze[S];
   T hyc[RW], hzc[RW];  // new H(i-1) at the owned cells
@@ -364,6 +416,25 @@ struct March {
         p[5] = ec[q].p[5];
       }
       T ex = oe[q][0], ey = oe[q][1], ez = oe[q][2];
+      // sources of this step on the cell, added before the update in list
+      // order, as the reference adds them between its H and E kernels; the
+      // plane and row tests are wave-uniform
+      for (int t = 0; t < (ic >= src_i0 && ic < src_i1 ? nsrc : 0); ++t) {
+        const volatile Src<T> *vp = lsrc + t;
+        if (ic < vp->ci0 || ic >= vp->ci1 || m.j < vp->cj0 || m.j >= vp->cj1) continue;
+        const Src<T> sq = lds_src(lsrc, t);
+        if (m.k >= sq.ck0 && m.k < sq.ck1) {
+          const int c = sq.comp;
+          const bool c0 = c == 0, c1 = c == 1;
+          // ce of the component by masking bits: a select of the three values
+          // is turned into an indexed load from a scratch copy of them
+          const T ce = pick3(c0, c1, ec[q].cx, ec[q].cy, ec[q].cz);
+          const T v = src_term<T>(sq, ic, m.j, m.k, ce, sq.amp);
+          ex = c0 ? ex + v : ex;
+          ey = c1 ? ey + v : ey;
+          ez = !c0 && !c1 ? ez + v : ez;
+        }
+      }
       const T ax = LOSSY ? ec[q].ax : T(1), ay = LOSSY ? ec[q].ay : T(1);
       const T az = LOSSY ? ec[q].az : T(1);
       e_math(g, ic, m.j, m.k, st, pxe, pye[q], pze[q], h[q][0], h[q][1], h[q][2], hx_jm, hz_jm,
@@ -589,8 +660,13 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
                                                          const T *__restrict__ q0,
                                                          T *__restrict__ q1, int tiles_k,
                                                          int tiles_j, MarchChunks ch, int xcd,
-                                                         int run) {
+                                                         int run, const MarchSources<T> src) {
   __shared__ MarchLds<T, RW> L;
+  __shared__ Src<T> lsrc[kMarchSources];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int t = 0; t < kMarchSources; ++t) lsrc[t] = src.s[t];
+  }  // visible after the barrier that ends the prologue
   const int B = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
   const int bk = B % tiles_k, bj = (B / tiles_k) % tiles_j, bc = B / (tiles_k * tiles_j);
   const int k0 = bk * 64, j0 = bj * 4 * RW;
@@ -613,7 +689,8 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
   const Rsrc rq1 = __builtin_amdgcn_make_buffer_rsrc(q1, (short)0, psi_bytes, 0x00020000);
 #define HF_MARCH_RUN(X, Y, Z)                                                                  \
   {                                                                                           \
-    March<T, RW, LOSSY, X, Y, Z, NT> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, prof};   \
+    March<T, RW, LOSSY, X, Y, Z, NT> m{g,   dtm_dl, dtm, inv_dl, r0,   r1,   rc,                    \
+                                       rq0, rq1,    prof, lsrc,   src.n, src.i0, src.i1};       \
     m.run(L, np, k0, j0, i0, i1);                                                             \
   }
   if (x) {

@@ -245,6 +245,16 @@ static int march_rw(Engine *e, const SrcList<T> &sl) {
   ch.split = e->npml > 0 && e->nx > 2 * e->npml ? 1 : 0;
   ch.inner = (int)ceil_div(ch.split ? e->nx - 2 * e->npml : e->nx, planes);
   const dim3 grid(tiles_k * tiles_j * (ch.inner + 2 * ch.split));
+  MarchSources<T> ms;
+  const bool inside = sl.n <= kMarchSources;
+  ms.n = inside ? sl.n : 0;
+  ms.i0 = e->nx;
+  ms.i1 = 0;
+  for (int q = 0; q < ms.n; ++q) {
+    ms.s[q] = sl.s[q];
+    ms.i0 = std::min(ms.i0, sl.s[q].ci0);
+    ms.i1 = std::max(ms.i1, sl.s[q].ci1);
+  }
   int rc = timing_mark(e);
   if (rc) return rc;
   const int o = e->cur ^ 1;
@@ -252,10 +262,10 @@ static int march_rw(Engine *e, const SrcList<T> &sl) {
                      T(e->dtm_dl), T(e->dtm), T(e->inv_dl), (const T *)e->state[e->cur],
                      (T *)e->state[o], (const T *)e->coef, (const T *)e->prof,
                      (const T *)e->psi[e->cur], (T *)e->psi[o], tiles_k, tiles_j, ch, e->xcd,
-                     e->march_run);
+                     e->march_run, ms);
   HF_HIP(hipGetLastError());
   rc = timing_mark(e);
-  if (rc) return rc;
+  if (rc || inside) return rc;
   return source_fixup<T, LOSSY>(e, sl);
 }
 
