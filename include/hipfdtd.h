@@ -308,8 +308,9 @@ int hfdtd_adjoint_save(hfdtd_adjoint *a, int c);
 int hfdtd_adjoint_restore(hfdtd_adjoint *a, int c);
 /* E over the gradient box into history slot */
 int hfdtd_adjoint_record(hfdtd_adjoint *a, int slot);
-/* n_steps forward steps of the engine (path HFDTD_PATH_FUSED when the grid
- * allows it, else the split sweeps), step q using src[r].amp =
+/* n_steps forward steps of the engine (path HFDTD_PATH_FUSED or
+ * HFDTD_PATH_MARCH when the grid allows it, else the split sweeps), step q
+ * using src[r].amp =
  * amp[q*nsrc + r]; the step taking the grid to t_n, n = step0 + q + 1,
  * accumulates the probe DFT if probe != 0 and records E into slot
  * slot0 + q + 1 if slot0 >= 0 */

@@ -651,7 +651,7 @@ int hfdtd_adjoint_forward(hfdtd_adjoint *a, const hfdtd_source *src, int nsrc,
                           int path) {
   if (!a || n_steps < 0 || nsrc < 0 || nsrc > kMaxSources || (nsrc > 0 && (!src || !amp)))
     return fail(HFDTD_EINVAL, "bad forward arguments (at most 16 sources)");
-  if (path != HFDTD_PATH_FUSED && path != HFDTD_PATH_SPLIT)
+  if (path != HFDTD_PATH_FUSED && path != HFDTD_PATH_SPLIT && path != HFDTD_PATH_MARCH)
     return fail(HFDTD_EINVAL, "unknown step path");
   if (slot0 >= 0 && slot0 + n_steps >= a->hist_slots)
     return fail(HFDTD_EINVAL, "recorded steps exceed the history slots");
@@ -659,12 +659,13 @@ int hfdtd_adjoint_forward(hfdtd_adjoint *a, const hfdtd_source *src, int nsrc,
   Engine *e = a->e;
   HF_HIP(hipSetDevice(e->device));
   std::vector<hfdtd_source> s(src, src + nsrc);
-  const bool fused = path == HFDTD_PATH_FUSED && e->fused_fits();
+  const bool one = path != HFDTD_PATH_SPLIT && e->fused_fits();
   for (int q = 0; q < n_steps; ++q) {
     for (int r = 0; r < nsrc; ++r) s[r].amp = amp[(size_t)q * nsrc + r];
     int rc;
-    if (fused) {
-      rc = launch_fused_step(e, s.data(), nsrc);
+    if (one) {
+      rc = path == HFDTD_PATH_MARCH ? launch_march_step(e, s.data(), nsrc)
+                                    : launch_fused_step(e, s.data(), nsrc);
     } else {
       rc = launch_update_H(e);
       if (!rc && nsrc) rc = launch_sources(e, s.data(), nsrc);

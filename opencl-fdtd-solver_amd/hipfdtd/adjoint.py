@@ -80,7 +80,7 @@ class ProbeDFTAdjoint:
     probes: iterable of (component, i, j, k) with component Ex/Ey/Ez;
     freqs [Hz]; steps: run length N; checkpoint: segment length K;
     eps_background: εr outside the design box (default 1); fused: the
-    forward path (``HIPFDTD``'s ``fused``, default per dtype).
+    forward path (``HIPFDTD``'s ``fused``, default per dtype and grid size).
     """
 
     def __init__(self, shape, dl, npml, design_box, sources, probes, freqs, steps, *,
@@ -148,7 +148,7 @@ class ProbeDFTAdjoint:
     def _run(self, n0, n1, slot0, probe):
         amp = self._amp[n0:n1]
         ptr = amp.ctypes.data_as(C.POINTER(C.c_double)) if self._nsrc else None
-        path = _lib.PATH_FUSED if self.fdtd.fused else _lib.PATH_SPLIT
+        path = _lib.PATH_CODES[self.fdtd.path]
         _lib.check(self._lib.hfdtd_adjoint_forward(self._handle, self._src, self._nsrc, ptr,
                                                    n1 - n0, n0, slot0, int(probe), path),
                    "adjoint forward")  # fmt: skip

@@ -95,7 +95,7 @@ class HIPFDTD(SourceMonitorMixin):
         self._fused_fits = max(6 * vstride, psi_total) * item < 2**32
         self.path = "split"
         self.fused = False
-        self.set_path(self._resolve_path(fused, path, dtype))
+        self.set_path(self._resolve_path(fused, path, dtype, (self.Nx, self.Ny, self.Nz)))
         self._sources = []
         self._monitors = []
         self.lossy = False  # σ or ca volumes set: the sweeps read ca
@@ -143,26 +143,35 @@ class HIPFDTD(SourceMonitorMixin):
     # ── memory budget (engine.py:320-363) ──
 
     PATHS = ("split", "fused", "march")
+    # grids of at least this many cells default to the march sweep
+    MARCH_DEFAULT_CELLS = 200**3
 
-    @staticmethod
-    def default_path(dtype) -> str:
-        """Path chosen when neither ``fused`` nor ``path`` is given: the LDS-DMA
-        fused sweep for FP64, the split sweeps for FP32."""
+    @classmethod
+    def default_path(cls, dtype, shape=None) -> str:
+        """Path chosen when neither ``fused`` nor ``path`` is given, from
+        MI355X measurements: the march sweep on grids of 200^3 cells and more
+        (256^3 FP32: 264 µs against 303 µs split; FP64 528 against 651 µs
+        fused; 512^3 FP32 1.80 against 2.23 ms split), below that the split
+        sweeps for FP32 (160^3: 66 µs against 88 µs march) and the LDS-DMA
+        fused sweep for FP64.  ``hipfdtd.autotune.tune_engine`` measures all
+        of them for a given grid."""
+        if shape is not None and int(np.prod([int(v) for v in shape])) >= cls.MARCH_DEFAULT_CELLS:
+            return "march"
         return "fused" if np.dtype(dtype) == np.dtype(np.float64) else "split"
 
     @classmethod
-    def default_fused(cls, dtype) -> bool:
+    def default_fused(cls, dtype, shape=None) -> bool:
         """Whether the default path is a one-launch (ping-pong) sweep."""
-        return cls.default_path(dtype) != "split"
+        return cls.default_path(dtype, shape) != "split"
 
     @classmethod
-    def _resolve_path(cls, fused, path, dtype) -> str:
+    def _resolve_path(cls, fused, path, dtype, shape=None) -> str:
         if path is not None:
             if path not in cls.PATHS:
                 raise ValueError(f"path must be one of {cls.PATHS}; got {path!r}")
             return path
         if fused is None:
-            return cls.default_path(dtype)
+            return cls.default_path(dtype, shape)
         return "fused" if fused else "split"
 
     def set_path(self, path) -> None:
@@ -183,8 +192,9 @@ class HIPFDTD(SourceMonitorMixin):
     @staticmethod
     def estimate_device_memory_bytes(shape, npml, dtype=np.float32, fused=None):
         """Device bytes for the 12 volume arrays, the face psi stripes and,
-        with ``fused`` (default: the path ``fused=None`` picks for ``dtype``),
-        the ping-pong copies of fields and psi.
+        with ``fused`` (default: whether the path picked for ``dtype`` and
+        ``shape`` is a one-launch sweep), the ping-pong copies of fields and
+        psi.
 
         The reference counts 9 volume arrays (``engine.py:325``) and leaves out
         the three ``ca`` arrays it allocates; this estimate counts all twelve.
@@ -197,7 +207,7 @@ class HIPFDTD(SourceMonitorMixin):
             psi = 4 * (2 * npml * ny * nz + nx * 2 * npml * nz + nx * ny * 2 * npml) * item
         total = 12 * vol + psi
         if fused is None:
-            fused = HIPFDTD.default_fused(dtype)
+            fused = HIPFDTD.default_fused(dtype, shape)
         if fused:
             total += 6 * vol + psi
         return total

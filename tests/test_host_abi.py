@@ -100,9 +100,16 @@ def test_memory_estimate_counts_all_arrays():
     assert split == 12 * 100**3 * 4 + psi
     fused = HIPFDTD.estimate_device_memory_bytes(shape, npml, fused=True)
     assert fused == split + 6 * 100**3 * 4 + psi
-    # default path: split sweeps for FP32, fused sweep for FP64
+    # default path: split sweeps for FP32, fused sweep for FP64; the march sweep
+    # (ping-pong state too) from 200^3 cells on
     assert HIPFDTD.estimate_device_memory_bytes(shape, npml) == split
     assert HIPFDTD.estimate_device_memory_bytes(shape, npml, np.float64) == 2 * fused
+    assert HIPFDTD.default_path(np.float32, shape) == "split"
+    assert HIPFDTD.default_path(np.float64, shape) == "fused"
+    big = (256, 256, 256)
+    assert HIPFDTD.default_path(np.float32, big) == "march"
+    assert HIPFDTD.estimate_device_memory_bytes(big, npml) == HIPFDTD.estimate_device_memory_bytes(
+        big, npml, fused=True)  # fmt: skip
     assert HIPFDTD.estimate_device_memory_bytes((40, 50, 60), 0, fused=False) == 12 * 40 * 50 * 60 * 4
 
 
