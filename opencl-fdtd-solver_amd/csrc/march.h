@@ -668,7 +668,11 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
     for (int t = 0; t < kMarchSources; ++t) lsrc[t] = src.s[t];
   }  // visible after the barrier that ends the prologue
   const int B = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
-  const int bk = B % tiles_k, bj = (B / tiles_k) % tiles_j, bc = B / (tiles_k * tiles_j);
+  const int bk = B % tiles_k, bc = B / (tiles_k * tiles_j);
+  // XCD x runs the same j-tiles of every chunk unless the tile order rotates
+  // from chunk to chunk: then the CPML rows at both ends of j land on a
+  // different XCD in each chunk instead of loading the same two XCDs
+  const int bj = ((B / tiles_k) % tiles_j + bc * max(1, run / tiles_k)) % tiles_j;
   const int k0 = bk * 64, j0 = bj * 4 * RW;
   int i0, i1;
   ch.range(g, planes, bc, i0, i1);
