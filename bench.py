@@ -480,7 +480,7 @@ TRAFFIC_FILES = {"fused": ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt",
 
 KERNELS = {"fused": "k_fused (fused H + E sweep; sources by k_source_fixup)",
            "split": "k_update_H + k_update_E (split sweeps, one step; sources by k_add_source)",
-           "march": "k_march (register-pipelined H + E sweep; sources by k_source_fixup)"}
+           "march": "k_march (register-pipelined H + E sweep, sources applied in the sweep)"}
 
 
 def measured_traffic(path):
