@@ -553,7 +553,8 @@ struct March {
     const unsigned vs = vsb();
     MarchH<T> A[S], B[S];
     MarchE<T> EA[RW], EB[RW];
-    // E(i0) of every slot
+    // E(i0) of every slot, then the loads of the first plane: they are in
+    // flight while the carry below waits for its own loads
     {
       const unsigned p0 = plane_b(i0);
 #pragma unroll
@@ -563,6 +564,8 @@ struct March {
         oe[q][2] = bld<T, false>(r0, sl[q].o4, 2 * vs + p0);
       }
     }
+    load_h_all(A, i0);
+    load_e_all(EA, i0);
     // new H(i0-1) at the owned cells (unused when i0 == 0: e_math drops the
     // i-1 term on plane 0)
     {
@@ -605,8 +608,6 @@ struct March {
       (&L.e[0][1][0][0])[sl[q].le] = oe[q][1];
       (&L.e[0][2][0][0])[sl[q].le] = oe[q][2];
     }
-    load_h_all(A, i0);
-    load_e_all(EA, i0);
     lds_barrier();
     // the last plane's loads read the (clamped) plane after the chunk and go
     // unused; a peeled last pair makes the register allocator shuffle the
