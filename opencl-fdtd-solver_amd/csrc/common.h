@@ -71,6 +71,14 @@ struct Engine {
   // grid), consecutive blocks per XCD run, non-temporal streams (-1 = when
   // the slabs exceed the last-level cache, 0 = never, 1 = always)
   int march_rows = 2;
+  // pair sweep (two steps per launch): i planes per block; the watched
+  // boxes whose state between the two steps goes to the shadow slab, and
+  // whether field reads see that slab
+  int pair_planes = 24;
+  int watch_n = 0;
+  int watch_b[8][6] = {};
+  void *shadow = nullptr;
+  bool view_shadow = false;
   int march_planes = 0;
   int march_run = 16;
   int march_nt = -1;
@@ -87,7 +95,10 @@ struct Engine {
   char *vol(int slab_parity, int c) const {
     return (char *)state[slab_parity] + (size_t)c * vstride * elem;
   }
-  void *field(int c) const { return vol(cur, c); }
+  void *field(int c) const {
+    if (view_shadow && shadow) return (char *)shadow + (size_t)c * vstride * elem;
+    return vol(cur, c);
+  }
   void *coef_vol(int c) const { return (char *)coef + (size_t)c * vstride * elem; }
   // The fused sweeps address slabs with 32-bit byte offsets.
   bool fused_fits() const {
@@ -160,6 +171,7 @@ int launch_update_E(Engine *e);
 int launch_sources(Engine *e, const hfdtd_source *src, int n);
 int launch_fused_step(Engine *e, const hfdtd_source *src, int n);
 int launch_march_step(Engine *e, const hfdtd_source *src, int n);
+int launch_pair_steps(Engine *e, const hfdtd_source *sa, int na, const hfdtd_source *sb, int nb);
 // ensure the ping-pong partners exist (engine.hip)
 int ensure_alt(Engine *e);
 

@@ -174,7 +174,7 @@ static int direct_t(hfdtd_dft *d, const double *phase) {
   }
   Engine *e = d->e;
   hipLaunchKernelGGL((k_dft_direct<T, A, KAHAN>), dim3((unsigned)ceil_div(d->P, 256)), dim3(256),
-                     0, e->stream, (const T *)e->state[e->cur], (long long)e->vstride, d->pl,
+                     0, e->stream, (const T *)e->field(0), (long long)e->vstride, d->pl,
                      d->P, d->F, ph, (A *)d->acc, (A *)d->cmp);
   HF_HIP(hipGetLastError());
   return HFDTD_OK;
@@ -273,7 +273,7 @@ int hfdtd_dft_accumulate(hfdtd_dft *d, const double *phase) {
   PhaseRow ph;
   for (int j = 0; j < 2 * kDftMaxFreq; ++j) ph.v[j] = j < 2 * d->F ? (float)phase[j] : 0.0f;
   hipLaunchKernelGGL(k_dft_gather<float>, dim3((unsigned)std::max<long long>(1, ceil_div(d->P, 256))),
-                     dim3(256), 0, e->stream, (const float *)e->state[e->cur],
+                     dim3(256), 0, e->stream, (const float *)e->field(0),
                      (long long)e->vstride, d->pl, d->P, d->buf + (size_t)d->k * d->Ppad, ph,
                      d->W + (size_t)d->k * 32);
   HF_HIP(hipGetLastError());

@@ -167,6 +167,7 @@ static void release(Engine *e) {
   fr(e->psi[1]);
   fr(e->coef);
   fr(e->prof);
+  fr(e->shadow);
   for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
   e->events.clear();
   e->events_used = 0;
@@ -504,6 +505,47 @@ int hfdtd_step(hfdtd_engine *h, const hfdtd_source *src, int n, int path) {
   if (rc) return rc;
   if (n > 0 && (rc = launch_sources(e, src, n))) return rc;
   return launch_update_E(e);
+}
+
+int hfdtd_step_pair(hfdtd_engine *h, const hfdtd_source *src_a, int na, const hfdtd_source *src_b,
+                    int nb) {
+  if (!h || na < 0 || nb < 0 || (na > 0 && !src_a) || (nb > 0 && !src_b))
+    return fail(HFDTD_EINVAL, "bad source list");
+  return launch_pair_steps(&h->s, src_a, na, src_b, nb);
+}
+
+int hfdtd_set_watch(hfdtd_engine *h, const int *boxes, int n) {
+  if (!h || (n > 0 && !boxes)) return fail(HFDTD_EINVAL, "NULL argument");
+  if (n < 0 || n > 8) return fail(HFDTD_EINVAL, "at most 8 watched boxes");
+  Engine *e = &h->s;
+  HF_HIP(hipSetDevice(e->device));
+  const int hi[3] = {e->nx, e->ny, e->nz};
+  for (int b = 0; b < n; ++b)
+    for (int a = 0; a < 3; ++a) {
+      const int lo = std::min(std::max(boxes[6 * b + 2 * a], 0), hi[a]);
+      e->watch_b[b][2 * a] = lo;
+      e->watch_b[b][2 * a + 1] = std::min(std::max(boxes[6 * b + 2 * a + 1], lo), hi[a]);
+    }
+  if (n > 0 && !e->shadow) {
+    int rc = dmalloc(e, &e->shadow, 6 * e->vstride * e->elem);
+    if (rc) return rc;
+  }
+  e->watch_n = n;
+  return HFDTD_OK;
+}
+
+int hfdtd_set_view(hfdtd_engine *h, int shadow) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  if (shadow && !h->s.shadow) return fail(HFDTD_EINVAL, "no watched boxes: no shadow slab");
+  h->s.view_shadow = shadow != 0;
+  return HFDTD_OK;
+}
+
+int hfdtd_set_pair_planes(hfdtd_engine *h, int planes_i) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  if (planes_i < 0 || planes_i > 4096) return fail(HFDTD_EINVAL, "planes_i must be 0..4096");
+  h->s.pair_planes = planes_i ? planes_i : 24;
+  return HFDTD_OK;
 }
 
 int hfdtd_synchronize(hfdtd_engine *h) {

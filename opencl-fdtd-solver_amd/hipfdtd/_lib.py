@@ -33,7 +33,10 @@ ACC_KAHAN = 2
 PATH_SPLIT = 0
 PATH_FUSED = 1
 PATH_MARCH = 2
-PATH_CODES = {"split": PATH_SPLIT, "fused": PATH_FUSED, "march": PATH_MARCH}
+# single steps of the "pair" path (two steps per launch) take the march sweep
+PATH_CODES = {"split": PATH_SPLIT, "fused": PATH_FUSED, "march": PATH_MARCH, "pair": PATH_MARCH}
+PAIR_MAX_SOURCES = 4  # sources per step the pair sweep applies itself
+PAIR_MAX_WATCH = 8    # watched boxes
 DFT_DIRECT = 0
 DFT_BUFFERED = 1
 
@@ -85,6 +88,11 @@ EXPORTS = {
     "hfdtd_synchronize": (C.c_int, [C.c_void_p]),
     "hfdtd_set_fused_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "hfdtd_set_split_schedule": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfdtd_step_pair": (C.c_int, [C.c_void_p, C.POINTER(Source), C.c_int, C.POINTER(Source),
+                                  C.c_int]),
+    "hfdtd_set_watch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int]),
+    "hfdtd_set_view": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfdtd_set_pair_planes": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_set_march_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "hfdtd_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),

@@ -86,6 +86,13 @@ class HIPPlaneDFTMonitor:
         _lib.check(self._lib.hfdtd_dft_accumulate(
             self._handle, self._buf.ctypes.data_as(C.POINTER(C.c_double))), "dft")  # fmt: skip
 
+    def watch_boxes(self):
+        """The recorded plane, for the pair path."""
+        box = [0, self.fdtd.Nx, 0, self.fdtd.Ny, 0, self.fdtd.Nz]
+        a = _AXES[self.axis]
+        box[2 * a], box[2 * a + 1] = self.index, self.index + 1
+        return [tuple(box)]
+
     def dft(self) -> np.ndarray:
         """Sums so far (a partial block is folded in first)."""
         ncomp, n0, n1, nf = self.shape

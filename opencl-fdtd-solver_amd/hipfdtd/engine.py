@@ -93,6 +93,7 @@ class HIPFDTD(SourceMonitorMixin):
         vstride = (self.Nx * self.Ny * self.Nz + 64 + 63) // 64 * 64
         psi_total = 4 * (self.psi_x_size + self.psi_y_size + self.psi_z_size) + 64
         self._fused_fits = max(6 * vstride, psi_total) * item < 2**32
+        self._pair_fits = max(6 * vstride, psi_total) * item < 2**31 and item == 4
         self.path = "split"
         self.fused = False
         self.set_path(self._resolve_path(fused, path, dtype, (self.Nx, self.Ny, self.Nz)))
@@ -101,6 +102,8 @@ class HIPFDTD(SourceMonitorMixin):
         self.lossy = False  # σ or ca volumes set: the sweeps read ca
         self._handle = None
         self._deferring = False
+        self._pairing = False
+        self._watch_set = None
         self._h_done = False
         self._pending: list[_lib.Source] = []
 
@@ -142,7 +145,7 @@ class HIPFDTD(SourceMonitorMixin):
 
     # ── memory budget (engine.py:320-363) ──
 
-    PATHS = ("split", "fused", "march")
+    PATHS = ("split", "fused", "march", "pair")
     # grids of at least this many cells default to the march sweep
     MARCH_DEFAULT_CELLS = 200**3
 
@@ -175,15 +178,19 @@ class HIPFDTD(SourceMonitorMixin):
         return "fused" if fused else "split"
 
     def set_path(self, path) -> None:
-        """Step path: 'split' (two sweeps, in place), 'fused' (LDS-DMA tiles)
-        or 'march' (register-pipelined tiles); ``True`` / ``False`` mean
-        'fused' / 'split'.  One-launch paths fall back to 'split' on grids
-        beyond their 32-bit offsets.  Takes effect at the next step; the state
-        carries over."""
+        """Step path: 'split' (two sweeps, in place), 'fused' (LDS-DMA tiles),
+        'march' (register-pipelined tiles) or 'pair' (``run`` advances two
+        steps per launch, single steps take the march sweep); ``True`` /
+        ``False`` mean 'fused' / 'split'.  One-launch paths fall back to
+        'split' on grids beyond their 32-bit offsets, 'pair' to 'march' for
+        FP64 or slabs of 2 GB and more.  Takes effect at the next step; the
+        state carries over."""
         if isinstance(path, (bool, np.bool_)):
             path = "fused" if path else "split"
         if path not in self.PATHS:
             raise ValueError(f"path must be one of {self.PATHS}; got {path!r}")
+        if path == "pair" and not self._pair_fits:
+            path = "march"
         if path != "split" and not self._fused_fits:
             path = "split"
         self.path = path
@@ -330,6 +337,10 @@ class HIPFDTD(SourceMonitorMixin):
 
     def _materialize(self) -> None:
         """Inside the source phase: run the H half now and flush recorded sources."""
+        if self._pairing:
+            raise RuntimeError("a source callback read the fields while the pair path evaluated "
+                               "the sources of two steps; use set_path('march') for such "
+                               "callbacks")
         if self._deferring and not self._h_done:
             _lib.check(self._lib.hfdtd_update_H(self._handle), "update_H")
             self._h_done = True
@@ -367,11 +378,97 @@ class HIPFDTD(SourceMonitorMixin):
             mon(self)
 
     def run(self, n_steps, progress_every=0):
-        """Run ``n_steps`` steps."""
-        for i in range(int(n_steps)):
+        """Run ``n_steps`` steps (on the 'pair' path two per launch where it can)."""
+        n = int(n_steps)
+        i = 0
+        while i < n:
+            if self.path == "pair" and n - i >= 2 and self._step_pair():
+                i += 2
+                if progress_every and (i - 2) // progress_every != i // progress_every:
+                    _LOG.info("  step %d/%d  t=%.3e s", i, n_steps, self.t)
+                continue
             self.step()
             if progress_every and i % progress_every == 0:
                 _LOG.info("  step %d/%d  t=%.3e s", i, n_steps, self.t)
+            i += 1
+
+    def _pair_watch(self):
+        """Cells the monitors read, as boxes (i0, i1, j0, j1, k0, k1), or None
+        when a monitor does not name them (``watch_boxes``) or there are more
+        boxes than the pair sweep records."""
+        boxes = []
+        for mon in self._monitors:
+            fn = getattr(mon, "watch_boxes", None)
+            if fn is None:
+                return None
+            boxes.extend(tuple(int(v) for v in b) for b in fn())
+        if len(boxes) > _lib.PAIR_MAX_WATCH:
+            return None
+        return tuple(boxes)
+
+    def _single(self, pending) -> None:
+        """One step on the march sweep with an already evaluated source list."""
+        n = len(pending)
+        arr = (_lib.Source * max(n, 1))(*pending)
+        _lib.check(self._lib.hfdtd_step(self._handle, arr, n, _lib.PATH_MARCH), "step")
+        self.t += self.dt
+        self.step_num += 1
+        for mon in self._monitors:
+            mon(self)
+
+    def _step_pair(self) -> bool:
+        """Two steps in one launch of the pair sweep: the source callbacks of
+        both steps are evaluated first (at t + dt/2 and t + 3dt/2), the
+        monitors then run after each step, for the first one on the state
+        between the two steps, which the sweep wrote for the cells they
+        watch.  False (nothing done) when a monitor cannot name its cells."""
+        watch = self._pair_watch()
+        if watch is None:
+            return False
+        lists = []
+        t0 = self.t
+        try:
+            self._deferring = self._pairing = True
+            for s in range(2):
+                self._pending = []
+                self.t = t0 + (s + 0.5) * self.dt
+                for src in self._sources:
+                    src(self)
+                lists.append(self._pending)
+        finally:
+            self._deferring = self._pairing = False
+            self._pending = []
+            self.t = t0
+        if max(len(v) for v in lists) > _lib.PAIR_MAX_SOURCES:
+            for v in lists:
+                self._single(v)
+            return True
+        if watch != self._watch_set:
+            flat = [v for b in watch for v in b]
+            arr = (C.c_int * max(len(flat), 1))(*flat)
+            _lib.check(self._lib.hfdtd_set_watch(self._handle, arr, len(watch)), "set_watch")
+            self._watch_set = watch
+        a = (_lib.Source * max(len(lists[0]), 1))(*lists[0])
+        b = (_lib.Source * max(len(lists[1]), 1))(*lists[1])
+        _lib.check(self._lib.hfdtd_step_pair(self._handle, a, len(lists[0]), b, len(lists[1])),
+                   "step_pair")  # fmt: skip
+        self.t = t0 + self.dt
+        self.step_num += 1
+        if self._monitors:
+            view = len(watch) > 0
+            if view:
+                _lib.check(self._lib.hfdtd_set_view(self._handle, 1), "set_view")
+            try:
+                for mon in self._monitors:
+                    mon(self)
+            finally:
+                if view:
+                    _lib.check(self._lib.hfdtd_set_view(self._handle, 0), "set_view")
+        self.t = t0 + 2 * self.dt
+        self.step_num += 1
+        for mon in self._monitors:
+            mon(self)
+        return True
 
     def run_until_converged(self, monitor, tol=1e-3, check_every=100, max_steps=100_000,
                             min_steps=0, freq_index=0):  # fmt: skip
@@ -420,6 +517,10 @@ class HIPFDTD(SourceMonitorMixin):
         _lib.check(self._lib.hfdtd_set_march_tiling(self._handle, int(rows_per_wave),
                                                     int(planes_i), int(run_blocks),
                                                     int(nontemporal)), "set_march_tiling")  # fmt: skip
+
+    def set_pair_planes(self, planes_i: int = 0) -> None:
+        """Planes of i per block of the pair sweep (0 = 24)."""
+        _lib.check(self._lib.hfdtd_set_pair_planes(self._handle, int(planes_i)), "set_pair_planes")
 
     def set_fused_tiling(self, rows_j: int = 0, planes_i: int = 0, xcd_groups: int = 0) -> None:
         """Launch shape of the fused sweep: rows of j and planes of i per block,

@@ -60,6 +60,11 @@ class HIPPointProbe:
         _lib.check(self._lib.hfdtd_probe_accumulate(
             self._handle, self._buf.ctypes.data_as(C.POINTER(C.c_double))), "probe")  # fmt: skip
 
+    def watch_boxes(self):
+        """The probed cell, for the pair path."""
+        i, j, k = self.ijk
+        return [(i, i + 1, j, j + 1, k, k + 1)]
+
     def dft(self) -> np.ndarray:
         out = np.empty(2 * len(self.freqs), dtype=np.float64)
         _lib.check(self._lib.hfdtd_probe_read(self._handle,
