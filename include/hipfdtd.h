@@ -173,15 +173,17 @@ int hfdtd_set_march_tiling(hfdtd_engine *e, int rows_per_wave, int planes_i, int
 int hfdtd_step_pair(hfdtd_engine *e, const hfdtd_source *src_a, int na,
                     const hfdtd_source *src_b, int nb);
 /* Cells whose state between the two steps of hfdtd_step_pair is kept for
- * observers: n <= 8 boxes, 6 ints each (i0, i1, j0, j1, k0, k1, half-open;
- * clipped to the grid).  n > 0 allocates a shadow field slab on first use;
- * n = 0 records nothing.  Extension, no counterpart in the reference. */
+ * observers: n <= 8 boxes, 7 ints each (i0, i1, j0, j1, k0, k1, half-open
+ * and clipped to the grid, then a mask of the components kept, bit c for
+ * Ex Ey Ez Hx Hy Hz, 0 = all).  n > 0 allocates a shadow field slab on
+ * first use; n = 0 records nothing.  Extension, no counterpart in the
+ * reference. */
 int hfdtd_set_watch(hfdtd_engine *e, const int *boxes, int n);
 /* 1: field reads (volumes, monitors, probes) see the shadow slab, i.e. the
  * state after the first step of the last hfdtd_step_pair at the watched
  * cells (other cells are undefined); 0: the current state.  Extension. */
 int hfdtd_set_view(hfdtd_engine *e, int shadow);
-/* i planes per block of hfdtd_step_pair (0 = 24).  Extension. */
+/* i planes per block of hfdtd_step_pair (0 = 24, at most 1024).  Extension. */
 int hfdtd_set_pair_planes(hfdtd_engine *e, int planes_i);
 
 /* HIP-event timing of the Yee sweep kernels on the engine's stream: while

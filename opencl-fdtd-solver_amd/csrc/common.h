@@ -77,6 +77,7 @@ struct Engine {
   int pair_planes = 24;
   int watch_n = 0;
   int watch_b[8][6] = {};
+  int watch_c[8] = {};  // component masks
   void *shadow = nullptr;
   bool view_shadow = false;
   int march_planes = 0;

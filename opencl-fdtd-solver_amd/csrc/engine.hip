@@ -520,12 +520,15 @@ int hfdtd_set_watch(hfdtd_engine *h, const int *boxes, int n) {
   Engine *e = &h->s;
   HF_HIP(hipSetDevice(e->device));
   const int hi[3] = {e->nx, e->ny, e->nz};
-  for (int b = 0; b < n; ++b)
+  for (int b = 0; b < n; ++b) {
     for (int a = 0; a < 3; ++a) {
-      const int lo = std::min(std::max(boxes[6 * b + 2 * a], 0), hi[a]);
+      const int lo = std::min(std::max(boxes[7 * b + 2 * a], 0), hi[a]);
       e->watch_b[b][2 * a] = lo;
-      e->watch_b[b][2 * a + 1] = std::min(std::max(boxes[6 * b + 2 * a + 1], lo), hi[a]);
+      e->watch_b[b][2 * a + 1] = std::min(std::max(boxes[7 * b + 2 * a + 1], lo), hi[a]);
     }
+    const int c = boxes[7 * b + 6];
+    e->watch_c[b] = c ? c & 63 : 63;
+  }
   if (n > 0 && !e->shadow) {
     int rc = dmalloc(e, &e->shadow, 6 * e->vstride * e->elem);
     if (rc) return rc;
@@ -543,7 +546,7 @@ int hfdtd_set_view(hfdtd_engine *h, int shadow) {
 
 int hfdtd_set_pair_planes(hfdtd_engine *h, int planes_i) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
-  if (planes_i < 0 || planes_i > 4096) return fail(HFDTD_EINVAL, "planes_i must be 0..4096");
+  if (planes_i < 0 || planes_i > 1024) return fail(HFDTD_EINVAL, "planes_i must be 0..1024");
   h->s.pair_planes = planes_i ? planes_i : 24;
   return HFDTD_OK;
 }

@@ -128,22 +128,45 @@ __device__ __forceinline__ Prof1<T> prof1(const Geo &g, const T *__restrict__ pr
 // p[] holds the old psi of the H slots (Hy_x Hz_x Hx_y Hz_y Hx_z Hy_z) for
 // the stripes the cell lies in (other entries are not read) and receives the
 // new ones (0 outside the stripes).
+// Interior form of h_math (no PML, every neighbour inside the grid).
+template <typename T>
+__device__ __forceinline__ void h_interior(T ex, T ey, T ez, T ex_j1, T ez_j1, T ex_k1, T ey_k1,
+                                           T ey_i1, T ez_i1, T &hx, T &hy, T &hz, T dtm_dl) {
+  const T dEz_dy = ez_j1 - ez;  // yee_update.cl:11-40
+  const T dEy_dz = ey_k1 - ey;
+  const T dEx_dz = ex_k1 - ex;
+  const T dEz_dx = ez_i1 - ez;
+  const T dEy_dx = ey_i1 - ey;
+  const T dEx_dy = ex_j1 - ex;
+  hx -= dtm_dl * (dEz_dy - dEy_dz);
+  hy -= dtm_dl * (dEx_dz - dEz_dx);
+  hz -= dtm_dl * (dEy_dx - dEx_dy);
+}
+
+// Interior form of e_math.
+template <typename T>
+__device__ __forceinline__ void e_interior(T hx, T hy, T hz, T hx_jm, T hz_jm, T hx_km, T hy_km,
+                                           T hy_im, T hz_im, T &ex, T &ey, T &ez, T cax, T cay,
+                                           T caz, T cex, T cey, T cez, T inv_dl) {
+  const T dHz_dy = hz - hz_jm;  // yee_update.cl:120-155
+  const T dHy_dz = hy - hy_km;
+  const T dHx_dz = hx - hx_km;
+  const T dHz_dx = hz - hz_im;
+  const T dHy_dx = hy - hy_im;
+  const T dHx_dy = hx - hx_jm;
+  ex = cax * ex + cex * inv_dl * (dHz_dy - dHy_dz);
+  ey = cay * ey + cey * inv_dl * (dHx_dz - dHz_dx);
+  ez = caz * ez + cez * inv_dl * (dHy_dx - dHx_dy);
+}
+
 template <typename T>
 __device__ __forceinline__ void h_math(const Geo &g, int i, int j, int k, const StripeIdx &s,
                                        const Prof1<T> &px, const Prof1<T> &py,
                                        const Prof1<T> &pz, T ex, T ey, T ez, T ex_j1, T ez_j1,
                                        T ex_k1, T ey_k1, T ey_i1, T ez_i1, T &hx, T &hy, T &hz,
                                        T p[6], T dtm_dl, T dtm) {
-  if (is_interior(g, i, j, k)) {  // yee_update.cl:11-40
-    const T dEz_dy = ez_j1 - ez;
-    const T dEy_dz = ey_k1 - ey;
-    const T dEx_dz = ex_k1 - ex;
-    const T dEz_dx = ez_i1 - ez;
-    const T dEy_dx = ey_i1 - ey;
-    const T dEx_dy = ex_j1 - ex;
-    hx -= dtm_dl * (dEz_dy - dEy_dz);
-    hy -= dtm_dl * (dEx_dz - dEz_dx);
-    hz -= dtm_dl * (dEy_dx - dEx_dy);
+  if (is_interior(g, i, j, k)) {
+    h_interior(ex, ey, ez, ex_j1, ez_j1, ex_k1, ey_k1, ey_i1, ez_i1, hx, hy, hz, dtm_dl);
     return;
   }
   // PML shell, or npml == 0 (yee_update.cl:42-118)
@@ -183,16 +206,9 @@ __device__ __forceinline__ void e_math(const Geo &g, int i, int j, int k, const 
                                        T hx_km, T hy_km, T hy_im, T hz_im, T &ex, T &ey, T &ez,
                                        T cax, T cay, T caz, T cex, T cey, T cez, T p[6],
                                        T inv_dl) {
-  if (is_interior(g, i, j, k)) {  // yee_update.cl:120-155
-    const T dHz_dy = hz - hz_jm;
-    const T dHy_dz = hy - hy_km;
-    const T dHx_dz = hx - hx_km;
-    const T dHz_dx = hz - hz_im;
-    const T dHy_dx = hy - hy_im;
-    const T dHx_dy = hx - hx_jm;
-    ex = cax * ex + cex * inv_dl * (dHz_dy - dHy_dz);
-    ey = cay * ey + cey * inv_dl * (dHx_dz - dHz_dx);
-    ez = caz * ez + cez * inv_dl * (dHy_dx - dHx_dy);
+  if (is_interior(g, i, j, k)) {
+    e_interior(hx, hy, hz, hx_jm, hz_jm, hx_km, hy_km, hy_im, hz_im, ex, ey, ez, cax, cay, caz,
+               cex, cey, cez, inv_dl);
     return;
   }
   const T dHz_dy = j > 0 ? hz - hz_jm : T(0);  // yee_update.cl:157-238

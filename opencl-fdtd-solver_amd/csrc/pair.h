@@ -38,6 +38,7 @@ namespace hfdtd {
 
 constexpr int kPairCols = 60;   // core columns of a tile: 64 lanes less two halo lanes a side
 constexpr int kPairWatch = 8;   // watched boxes
+constexpr int kPairMaxPlanes = 1024;  // i planes per block at most
 
 template <typename T>
 struct PairLoad {
@@ -61,6 +62,7 @@ struct PairSources {
 struct PairWatch {
   int n;
   int b[kPairWatch][6];  // i0, i1, j0, j1, k0, k1 (half-open)
+  int c[kPairWatch];     // components recorded (bit c: Ex Ey Ez Hx Hy Hz)
 };
 
 // Rows exchanged between wavefronts; wave w writes slot w + 1, reads its
@@ -73,7 +75,10 @@ struct PairLds {
   T e1[2][NW + 2][2][64];  // E1 (Ex, Ez), first row, by front parity
   T h2[NW + 2][2][64];     // H2 (Hx, Hz), last row
   Src<T> src[2][kMarchSources];
-  int wb[kPairWatch][6];  // watched boxes
+  // per plane of the chunk, byte c: the watched boxes holding the plane
+  // that record component c
+  unsigned long long pmask[kPairMaxPlanes];
+  int wb[kPairWatch][4];  // j0, j1, k0, k1 of the watched boxes
   int wn;
 };
 
@@ -102,7 +107,7 @@ __device__ __forceinline__ T lane_prev(T v) {
   return lane_shift<0x138>(v);  // wave_shr:1
 }
 
-template <typename T, int NW, int RW, bool LOSSY, bool X, bool Y, bool Z>
+template <typename T, int NW, int RW, bool LOSSY, bool INT, bool X, bool Y, bool Z>
 struct Pair {
   static constexpr unsigned E = sizeof(T);
   static constexpr int TJ = NW * RW - 4;  // core rows
@@ -237,33 +242,37 @@ struct Pair {
     }
   }
 
-  // watched boxes whose i range holds plane i, one bit each (wave-uniform)
-  __device__ __forceinline__ unsigned watch_plane(const PairLds<T, NW> &L, int i) const {
-    unsigned m = 0;
-    for (int b = 0; b < L.wn; ++b)
-      if (i >= L.wb[b][0] && i < L.wb[b][1]) m |= 1u << b;
-    return __builtin_amdgcn_readfirstlane(m);
-  }
 
   // Front plane p: loads of plane p+1, H1(p) and E2(p-2), barrier, E1(p)
   // and H2(p-1), barrier.  cur holds the loads of plane p, nxt receives
-  // those of plane p+1.
+  // those of plane p+1.  A sub-update runs only on the rows and fronts
+  // whose results some stored cell depends on (wave-uniform tests): H1 on
+  // rows 0 .. R-2 and fronts up to i1, E1 on rows 1 .. R-2 and fronts
+  // i0-1 .. i1, H2 on rows 1 .. R-3 and fronts i0 .. i1, E2 on rows
+  // 2 .. R-3 and fronts i0+2 .. i1+1 (R rows, chunk [i0, i1)).
   __device__ __forceinline__ void step(PairLds<T, NW> &L, int par, int p, PairLoad<T> (&cur)[RW],
                                        PairLoad<T> (&nxt)[RW], int nsrc0, int nsrc1, int sa0,
                                        int sa1, int sb0, int sb1, bool wplane) {
+    constexpr int R = NW * RW;
     __builtin_amdgcn_sched_barrier(0);
     load(nxt, p + 1);
     __builtin_amdgcn_sched_barrier(0);
     const unsigned vs = vsb();
     const int ia = clampx(p), ib = clampx(p - 1), ie = clampx(p - 2);
     const PsiPlane ppa = psi_plane(p), ppb = psi_plane(p - 1), ppe = psi_plane(p - 2);
-    const Prof1<T> pxh_a = prof1(g, prof, 1, 0, ia), pxe_a = prof1(g, prof, 0, 0, ia);
-    const Prof1<T> pxh_b = prof1(g, prof, 1, 0, ib), pxe_e = prof1(g, prof, 0, 0, ie);
+    const bool fh1 = p <= i1, fe1 = p >= i0 - 1 && p <= i1, fh2 = p >= i0 && p <= i1;
+    const bool fe2 = p >= i0 + 2 && p <= i1 + 1;
 
     // ── H1(p) ──
     T h1[RW][3], ph1[RW][6];
 #pragma unroll
     for (int q = 0; q < RW; ++q) {
+      const int r = RW * w + q;
+      h1[q][0] = cur[q].hx;
+      h1[q][1] = cur[q].hy;
+      h1[q][2] = cur[q].hz;
+#pragma unroll
+      for (int s = 0; s < 6; ++s) ph1[q][s] = (s < 2 ? X : s < 4 ? Y : Z) ? cur[q].ph[s] : T(0);
       T exj, ezj;
       if (q + 1 < RW) {
         exj = e0[q + 1][0];
@@ -273,14 +282,17 @@ struct Pair {
         ezj = L.e0[w + 2][1][lane];
       }
       const T exk = lane_next(e0[q][0]), eyk = lane_next(e0[q][1]);
-      h1[q][0] = cur[q].hx;
-      h1[q][1] = cur[q].hy;
-      h1[q][2] = cur[q].hz;
-#pragma unroll
-      for (int s = 0; s < 6; ++s) ph1[q][s] = (s < 2 ? X : s < 4 ? Y : Z) ? cur[q].ph[s] : T(0);
-      h_math(g, ia, jr[q], kc, stripes(q, ppa.inx), pxh_a, pyh[q], pzh, e0[q][0], e0[q][1],
-             e0[q][2], exj, ezj, exk, eyk, cur[q].ey, cur[q].ez, h1[q][0], h1[q][1], h1[q][2],
-             ph1[q], dtm_dl, dtm);
+      if (fh1 && r <= R - 2) {
+        if constexpr (INT) {
+          h_interior(e0[q][0], e0[q][1], e0[q][2], exj, ezj, exk, eyk, cur[q].ey, cur[q].ez,
+                     h1[q][0], h1[q][1], h1[q][2], dtm_dl);
+        } else {
+          const Prof1<T> pxh = prof1(g, prof, 1, 0, ia);
+          h_math(g, ia, jr[q], kc, stripes(q, ppa.inx), pxh, pyh[q], pzh, e0[q][0], e0[q][1],
+                 e0[q][2], exj, ezj, exk, eyk, cur[q].ey, cur[q].ez, h1[q][0], h1[q][1],
+                 h1[q][2], ph1[q], dtm_dl, dtm);
+        }
+      }
     }
     // ── E2(p-2) ──
     {
@@ -288,6 +300,7 @@ struct Pair {
       const unsigned pl = plane_b(p - 2);
 #pragma unroll
       for (int q = 0; q < RW; ++q) {
+        const int r = RW * w + q;
         T hx_jm, hz_jm;
         if (q > 0) {
           hx_jm = h2m[q - 1][0];
@@ -297,33 +310,41 @@ struct Pair {
           hz_jm = L.h2[w][1][lane];
         }
         const T hx_km = lane_prev(h2m[q][0]), hy_km = lane_prev(h2m[q][1]);
-        T ex = e1mm[q][0], ey = e1mm[q][1], ez = e1mm[q][2];
-        add_sources(L.src[1], nsrc1, sb0, sb1, p - 2, q, cmm[q][0], cmm[q][1], cmm[q][2], ex,
-                    ey, ez);
-        const T ax = LOSSY ? amm[q][0] : T(1), ay = LOSSY ? amm[q][1] : T(1);
-        const T az = LOSSY ? amm[q][2] : T(1);
-        T ps[6];
+        if (fe2 && r >= 2 && r <= R - 3) {
+          T ex = e1mm[q][0], ey = e1mm[q][1], ez = e1mm[q][2];
+          add_sources(L.src[1], nsrc1, sb0, sb1, p - 2, q, cmm[q][0], cmm[q][1], cmm[q][2], ex,
+                      ey, ez);
+          const T ax = LOSSY ? amm[q][0] : T(1), ay = LOSSY ? amm[q][1] : T(1);
+          const T az = LOSSY ? amm[q][2] : T(1);
+          T ps[6];
 #pragma unroll
-        for (int s = 0; s < 6; ++s) ps[s] = pe1mm[q][s];
-        e_math(g, ie, jr[q], kc, stripes(q, ppe.inx), pxe_e, pye[q], pze, h2m[q][0], h2m[q][1],
-               h2m[q][2], hx_jm, hz_jm, hx_km, hy_km, h2y[q], h2z[q], ex, ey, ez, ax, ay, az,
-               cmm[q][0], cmm[q][1], cmm[q][2], ps, inv_dl);
-        const bool ok = own[q] && live;
-        const unsigned v = ok ? o4[q] + pl : kDropOffset;
-        bst<T, false>(r1, ex, v, 0u);
-        bst<T, false>(r1, ey, v, vs);
-        bst<T, false>(r1, ez, v, 2 * vs);
-        if constexpr (X) {
-          psi_st(ps[0], 6, ok && ppe.inx, o4[q] + ppe.xs * E);
-          psi_st(ps[1], 7, ok && ppe.inx, o4[q] + ppe.xs * E);
-        }
-        if constexpr (Y) {
-          psi_st(ps[2], 8, ok && iny[q], qy4[q] + ppe.ys * E);
-          psi_st(ps[3], 9, ok && iny[q], qy4[q] + ppe.ys * E);
-        }
-        if constexpr (Z) {
-          psi_st(ps[4], 10, ok && inz, qz4[q] + ppe.zs * E);
-          psi_st(ps[5], 11, ok && inz, qz4[q] + ppe.zs * E);
+          for (int s = 0; s < 6; ++s) ps[s] = pe1mm[q][s];
+          if constexpr (INT) {
+            e_interior(h2m[q][0], h2m[q][1], h2m[q][2], hx_jm, hz_jm, hx_km, hy_km, h2y[q],
+                       h2z[q], ex, ey, ez, ax, ay, az, cmm[q][0], cmm[q][1], cmm[q][2], inv_dl);
+          } else {
+            const Prof1<T> pxe = prof1(g, prof, 0, 0, ie);
+            e_math(g, ie, jr[q], kc, stripes(q, ppe.inx), pxe, pye[q], pze, h2m[q][0],
+                   h2m[q][1], h2m[q][2], hx_jm, hz_jm, hx_km, hy_km, h2y[q], h2z[q], ex, ey, ez,
+                   ax, ay, az, cmm[q][0], cmm[q][1], cmm[q][2], ps, inv_dl);
+          }
+          const bool ok = own[q] && live;
+          const unsigned v = ok ? o4[q] + pl : kDropOffset;
+          bst<T, false>(r1, ex, v, 0u);
+          bst<T, false>(r1, ey, v, vs);
+          bst<T, false>(r1, ez, v, 2 * vs);
+          if constexpr (X) {
+            psi_st(ps[0], 6, ok && ppe.inx, o4[q] + ppe.xs * E);
+            psi_st(ps[1], 7, ok && ppe.inx, o4[q] + ppe.xs * E);
+          }
+          if constexpr (Y) {
+            psi_st(ps[2], 8, ok && iny[q], qy4[q] + ppe.ys * E);
+            psi_st(ps[3], 9, ok && iny[q], qy4[q] + ppe.ys * E);
+          }
+          if constexpr (Z) {
+            psi_st(ps[4], 10, ok && inz, qz4[q] + ppe.zs * E);
+            psi_st(ps[5], 11, ok && inz, qz4[q] + ppe.zs * E);
+          }
         }
       }
     }
@@ -335,6 +356,12 @@ struct Pair {
     T e1[RW][3], pe1[RW][6];
 #pragma unroll
     for (int q = 0; q < RW; ++q) {
+      const int r = RW * w + q;
+      e1[q][0] = e0[q][0];
+      e1[q][1] = e0[q][1];
+      e1[q][2] = e0[q][2];
+#pragma unroll
+      for (int s = 0; s < 6; ++s) pe1[q][s] = (s < 2 ? X : s < 4 ? Y : Z) ? cur[q].pe[s] : T(0);
       T hx_jm, hz_jm;
       if (q > 0) {
         hx_jm = h1[q - 1][0];
@@ -344,34 +371,35 @@ struct Pair {
         hz_jm = L.h1[w][1][lane];
       }
       const T hx_km = lane_prev(h1[q][0]), hy_km = lane_prev(h1[q][1]);
-      e1[q][0] = e0[q][0];
-      e1[q][1] = e0[q][1];
-      e1[q][2] = e0[q][2];
-      add_sources(L.src[0], nsrc0, sa0, sa1, p, q, cur[q].cx, cur[q].cy, cur[q].cz, e1[q][0],
-                  e1[q][1], e1[q][2]);
-#pragma unroll
-      for (int s = 0; s < 6; ++s) pe1[q][s] = (s < 2 ? X : s < 4 ? Y : Z) ? cur[q].pe[s] : T(0);
-      const T ax = LOSSY ? cur[q].ax : T(1), ay = LOSSY ? cur[q].ay : T(1);
-      const T az = LOSSY ? cur[q].az : T(1);
-      e_math(g, ia, jr[q], kc, stripes(q, ppa.inx), pxe_a, pye[q], pze, h1[q][0], h1[q][1],
-             h1[q][2], hx_jm, hz_jm, hx_km, hy_km, h1m[q][1], h1m[q][2], e1[q][0], e1[q][1],
-             e1[q][2], ax, ay, az, cur[q].cx, cur[q].cy, cur[q].cz, pe1[q], inv_dl);
+      if (fe1 && r >= 1 && r <= R - 2) {
+        add_sources(L.src[0], nsrc0, sa0, sa1, p, q, cur[q].cx, cur[q].cy, cur[q].cz, e1[q][0],
+                    e1[q][1], e1[q][2]);
+        const T ax = LOSSY ? cur[q].ax : T(1), ay = LOSSY ? cur[q].ay : T(1);
+        const T az = LOSSY ? cur[q].az : T(1);
+        if constexpr (INT) {
+          e_interior(h1[q][0], h1[q][1], h1[q][2], hx_jm, hz_jm, hx_km, hy_km, h1m[q][1],
+                     h1m[q][2], e1[q][0], e1[q][1], e1[q][2], ax, ay, az, cur[q].cx, cur[q].cy,
+                     cur[q].cz, inv_dl);
+        } else {
+          const Prof1<T> pxe = prof1(g, prof, 0, 0, ia);
+          e_math(g, ia, jr[q], kc, stripes(q, ppa.inx), pxe, pye[q], pze, h1[q][0], h1[q][1],
+                 h1[q][2], hx_jm, hz_jm, hx_km, hy_km, h1m[q][1], h1m[q][2], e1[q][0], e1[q][1],
+                 e1[q][2], ax, ay, az, cur[q].cx, cur[q].cy, cur[q].cz, pe1[q], inv_dl);
+        }
+      }
     }
-    // the state between the two steps, for the watched cells (plane and
-    // wave tests uniform; rarely taken)
-    const unsigned pm = wplane ? watch_plane(L, p) : 0u;
-    if (pm & wmw) {
+    // the state between the two steps, for the watched cells and
+    // components (plane, wave and component tests uniform; rarely taken)
+    const unsigned long long pm = wplane ? L.pmask[p - i0] : 0ull;
+    if (wplane && pm) {
       const unsigned pl = plane_b(p);
 #pragma unroll
-      for (int q = 0; q < RW; ++q) {
-        if (wm[q] & pm) {
-          const unsigned v = o4[q] + pl;
-          bst<T, false>(rs, e1[q][0], v, 0u);
-          bst<T, false>(rs, e1[q][1], v, vs);
-          bst<T, false>(rs, e1[q][2], v, 2 * vs);
-          bst<T, false>(rs, h1[q][0], v, 3 * vs);
-          bst<T, false>(rs, h1[q][1], v, 4 * vs);
-          bst<T, false>(rs, h1[q][2], v, 5 * vs);
+      for (int c = 0; c < 6; ++c) {
+        const unsigned pc = __builtin_amdgcn_readfirstlane((unsigned)(pm >> (8 * c)) & 0xFFu);
+        if (pc & wmw) {
+#pragma unroll
+          for (int q = 0; q < RW; ++q)
+            if (wm[q] & pc) bst<T, false>(rs, c < 3 ? e1[q][c] : h1[q][c - 3], o4[q] + pl, c * vs);
         }
       }
     }
@@ -382,6 +410,10 @@ struct Pair {
       const unsigned pl = plane_b(p - 1);
 #pragma unroll
       for (int q = 0; q < RW; ++q) {
+        const int r = RW * w + q;
+        h2[q][0] = h1m[q][0];
+        h2[q][1] = h1m[q][1];
+        h2[q][2] = h1m[q][2];
         T exj, ezj;
         if (q + 1 < RW) {
           exj = e1m[q + 1][0];
@@ -391,31 +423,36 @@ struct Pair {
           ezj = L.e1[par ^ 1][w + 2][1][lane];
         }
         const T exk = lane_next(e1m[q][0]), eyk = lane_next(e1m[q][1]);
-        h2[q][0] = h1m[q][0];
-        h2[q][1] = h1m[q][1];
-        h2[q][2] = h1m[q][2];
-        T ps[6];
+        if (fh2 && r >= 1 && r <= R - 3) {
+          T ps[6];
 #pragma unroll
-        for (int s = 0; s < 6; ++s) ps[s] = ph1m[q][s];
-        h_math(g, ib, jr[q], kc, stripes(q, ppb.inx), pxh_b, pyh[q], pzh, e1m[q][0], e1m[q][1],
-               e1m[q][2], exj, ezj, exk, eyk, e1[q][1], e1[q][2], h2[q][0], h2[q][1], h2[q][2],
-               ps, dtm_dl, dtm);
-        const bool ok = own[q] && live;
-        const unsigned v = ok ? o4[q] + pl : kDropOffset;
-        bst<T, false>(r1, h2[q][0], v, 3 * vs);
-        bst<T, false>(r1, h2[q][1], v, 4 * vs);
-        bst<T, false>(r1, h2[q][2], v, 5 * vs);
-        if constexpr (X) {
-          psi_st(ps[0], 0, ok && ppb.inx, o4[q] + ppb.xs * E);
-          psi_st(ps[1], 1, ok && ppb.inx, o4[q] + ppb.xs * E);
-        }
-        if constexpr (Y) {
-          psi_st(ps[2], 2, ok && iny[q], qy4[q] + ppb.ys * E);
-          psi_st(ps[3], 3, ok && iny[q], qy4[q] + ppb.ys * E);
-        }
-        if constexpr (Z) {
-          psi_st(ps[4], 4, ok && inz, qz4[q] + ppb.zs * E);
-          psi_st(ps[5], 5, ok && inz, qz4[q] + ppb.zs * E);
+          for (int s = 0; s < 6; ++s) ps[s] = ph1m[q][s];
+          if constexpr (INT) {
+            h_interior(e1m[q][0], e1m[q][1], e1m[q][2], exj, ezj, exk, eyk, e1[q][1], e1[q][2],
+                       h2[q][0], h2[q][1], h2[q][2], dtm_dl);
+          } else {
+            const Prof1<T> pxh = prof1(g, prof, 1, 0, ib);
+            h_math(g, ib, jr[q], kc, stripes(q, ppb.inx), pxh, pyh[q], pzh, e1m[q][0], e1m[q][1],
+                   e1m[q][2], exj, ezj, exk, eyk, e1[q][1], e1[q][2], h2[q][0], h2[q][1],
+                   h2[q][2], ps, dtm_dl, dtm);
+          }
+          const bool ok = own[q] && live;
+          const unsigned v = ok ? o4[q] + pl : kDropOffset;
+          bst<T, false>(r1, h2[q][0], v, 3 * vs);
+          bst<T, false>(r1, h2[q][1], v, 4 * vs);
+          bst<T, false>(r1, h2[q][2], v, 5 * vs);
+          if constexpr (X) {
+            psi_st(ps[0], 0, ok && ppb.inx, o4[q] + ppb.xs * E);
+            psi_st(ps[1], 1, ok && ppb.inx, o4[q] + ppb.xs * E);
+          }
+          if constexpr (Y) {
+            psi_st(ps[2], 2, ok && iny[q], qy4[q] + ppb.ys * E);
+            psi_st(ps[3], 3, ok && iny[q], qy4[q] + ppb.ys * E);
+          }
+          if constexpr (Z) {
+            psi_st(ps[4], 4, ok && inz, qz4[q] + ppb.zs * E);
+            psi_st(ps[5], 5, ok && inz, qz4[q] + ppb.zs * E);
+          }
         }
       }
     }
@@ -510,35 +547,39 @@ struct Pair {
 #pragma unroll
       for (int b = 0; b < kPairWatch; ++b)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) L.wb[b][c] = watch.b[b][c];
+        for (int c = 0; c < 4; ++c) L.wb[b][c] = watch.b[b][2 + c];
       L.wn = watch.n;
+    }
+    // per plane of the chunk and component, the watched boxes holding it
+    for (int t = threadIdx.x; t < i1 - i0; t += NW * 64) {
+      unsigned long long m = 0;
+#pragma unroll
+      for (int b = 0; b < kPairWatch; ++b)
+        if (b < watch.n && i0 + t >= watch.b[b][0] && i0 + t < watch.b[b][1])
+#pragma unroll
+          for (int c = 0; c < 6; ++c)
+            if ((watch.c[b] >> c) & 1) m |= 1ull << (8 * c + b);
+      L.pmask[t] = m;
     }
     L.e0[w + 1][0][lane] = e0[0][0];
     L.e0[w + 1][1][lane] = e0[0][2];
     lds_barrier();
     const int ns0 = src.n[0], ns1 = src.n[1];
     const int sa0 = src.i0[0], sa1 = src.i1[0], sb0 = src.i0[1], sb1 = src.i1[1];
-    // planes with watched cells: the union of the boxes' i ranges, within
-    // the chunk (a halo plane's step-1 values are not all exact, and another
-    // block owns the plane)
-    int wi0 = g.nx, wi1 = 0;
-    for (int b = 0; b < L.wn; ++b) {
-      wi0 = min(wi0, L.wb[b][0]);
-      wi1 = max(wi1, L.wb[b][1]);
-    }
-    wi0 = max(wi0, i0);
-    wi1 = min(wi1, i1);
+    // watched planes: those of the chunk only (a halo plane's step-1 values
+    // are not all exact, and another block owns the plane)
     unsigned any = 0;
 #pragma unroll
     for (int q = 0; q < RW; ++q) {
       wm[q] = 0;
       for (int b = 0; b < L.wn; ++b) {
         const int *x = L.wb[b];
-        if (jr[q] >= x[2] && jr[q] < x[3] && kc >= x[4] && kc < x[5]) wm[q] |= 1u << b;
+        if (jr[q] >= x[0] && jr[q] < x[1] && kc >= x[2] && kc < x[3]) wm[q] |= 1u << b;
       }
       if (!own[q]) wm[q] = 0;
       any |= wm[q];
     }
+
     wmw = 0;
     for (int b = 0; b < kPairWatch; ++b)
       if (__ballot((any >> b) & 1u)) wmw |= 1u << b;
@@ -549,8 +590,8 @@ struct Pair {
 #pragma unroll 1
     do {
       const int p = i0 - 2 + t;
-      step(L, 0, p, A, B, ns0, ns1, sa0, sa1, sb0, sb1, p >= wi0 && p < wi1);
-      step(L, 1, p + 1, B, A, ns0, ns1, sa0, sa1, sb0, sb1, p + 1 >= wi0 && p + 1 < wi1);
+      step(L, 0, p, A, B, ns0, ns1, sa0, sa1, sb0, sb1, wmw && p >= i0 && p < i1);
+      step(L, 1, p + 1, B, A, ns0, ns1, sa0, sa1, sb0, sb1, wmw && p + 1 >= i0 && p + 1 < i1);
       t += 2;
     } while (t < nf);
   }
@@ -589,22 +630,25 @@ __global__ __launch_bounds__(NW * 64) void k_pair(Geo g, int planes, T dtm_dl, T
   const Rsrc rq0 = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(q0), (short)0, psi_bytes,
                                                      0x00020000);
   const Rsrc rq1 = __builtin_amdgcn_make_buffer_rsrc(q1, (short)0, psi_bytes, 0x00020000);
-#define HF_PAIR_RUN(X, Y, Z)                                                                   \
+#define HF_PAIR_RUN(I, X, Y, Z)                                                                \
   {                                                                                            \
-    Pair<T, NW, RW, LOSSY, X, Y, Z> m{g, dtm_dl, dtm, inv_dl, r0, r1, rc, rq0, rq1, rs, prof}; \
+    Pair<T, NW, RW, LOSSY, I, X, Y, Z> m{g,   dtm_dl, dtm, inv_dl, r0,  r1,                      \
+                                         rc,  rq0,    rq1, rs,     prof};                        \
     m.run(L, k0, j0, i0, i1, src, watch);                                                      \
   }
-  if (x) {
+  if (n > 0 && !x && !y && !z) {
+    HF_PAIR_RUN(true, false, false, false)  // every cell of the tile interior
+  } else if (x) {
     if (y) {
-      if (z) HF_PAIR_RUN(true, true, true) else HF_PAIR_RUN(true, true, false)
+      if (z) HF_PAIR_RUN(false, true, true, true) else HF_PAIR_RUN(false, true, true, false)
     } else {
-      if (z) HF_PAIR_RUN(true, false, true) else HF_PAIR_RUN(true, false, false)
+      if (z) HF_PAIR_RUN(false, true, false, true) else HF_PAIR_RUN(false, true, false, false)
     }
   } else {
     if (y) {
-      if (z) HF_PAIR_RUN(false, true, true) else HF_PAIR_RUN(false, true, false)
+      if (z) HF_PAIR_RUN(false, false, true, true) else HF_PAIR_RUN(false, false, true, false)
     } else {
-      if (z) HF_PAIR_RUN(false, false, true) else HF_PAIR_RUN(false, false, false)
+      if (z) HF_PAIR_RUN(false, false, false, true) else HF_PAIR_RUN(false, false, false, false)
     }
   }
 #undef HF_PAIR_RUN

@@ -318,8 +318,10 @@ static int pair_launch(Engine *e, const PairSources<T> &ps) {
   const dim3 grid(tiles_k * tiles_j * chunks);
   PairWatch pw{};
   pw.n = e->shadow ? e->watch_n : 0;
-  for (int b = 0; b < pw.n; ++b)
+  for (int b = 0; b < pw.n; ++b) {
     for (int c = 0; c < 6; ++c) pw.b[b][c] = e->watch_b[b][c];
+    pw.c[b] = e->watch_c[b];
+  }
   int rc = timing_mark(e);
   if (rc) return rc;
   const int o = e->cur ^ 1;

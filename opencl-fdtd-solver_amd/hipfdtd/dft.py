@@ -91,7 +91,8 @@ class HIPPlaneDFTMonitor:
         box = [0, self.fdtd.Nx, 0, self.fdtd.Ny, 0, self.fdtd.Nz]
         a = _AXES[self.axis]
         box[2 * a], box[2 * a + 1] = self.index, self.index + 1
-        return [tuple(box)]
+        mask = sum(1 << COMPONENTS.index(c) for c in self.components)
+        return [tuple(box) + (mask,)]
 
     def dft(self) -> np.ndarray:
         """Sums so far (a partial block is folded in first)."""

@@ -393,15 +393,17 @@ class HIPFDTD(SourceMonitorMixin):
             i += 1
 
     def _pair_watch(self):
-        """Cells the monitors read, as boxes (i0, i1, j0, j1, k0, k1), or None
-        when a monitor does not name them (``watch_boxes``) or there are more
-        boxes than the pair sweep records."""
+        """Cells the monitors read, as boxes (i0, i1, j0, j1, k0, k1[, mask])
+        with an optional mask of the components read (bit c for Ex .. Hz,
+        0 = all), or None when a monitor does not name them
+        (``watch_boxes``) or there are more boxes than the pair sweep
+        records."""
         boxes = []
         for mon in self._monitors:
             fn = getattr(mon, "watch_boxes", None)
             if fn is None:
                 return None
-            boxes.extend(tuple(int(v) for v in b) for b in fn())
+            boxes.extend(tuple(int(v) for v in (tuple(b) + (0,))[:7]) for b in fn())
         if len(boxes) > _lib.PAIR_MAX_WATCH:
             return None
         return tuple(boxes)

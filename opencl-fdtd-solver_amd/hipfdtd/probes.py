@@ -63,7 +63,7 @@ class HIPPointProbe:
     def watch_boxes(self):
         """The probed cell, for the pair path."""
         i, j, k = self.ijk
-        return [(i, i + 1, j, j + 1, k, k + 1)]
+        return [(i, i + 1, j, j + 1, k, k + 1, 1 << COMPONENTS.index(self.comp))]
 
     def dft(self) -> np.ndarray:
         out = np.empty(2 * len(self.freqs), dtype=np.float64)

@@ -163,9 +163,10 @@ def test_pair_entry_points_reject_bad_arguments():
     sim = hip().HIPFDTD((16, 16, 16), 1e-3, npml=2, path="pair")
     lib, h = sim._lib, sim._handle
     assert lib.hfdtd_set_view(h, 1) == _lib.EINVAL  # no watched boxes yet
-    boxes = (C.c_int * 54)(*([0, 16] * 27))
+    boxes = (C.c_int * 63)(*([0, 16, 0, 16, 0, 16, 0] * 9))
     assert lib.hfdtd_set_watch(h, boxes, 9) == _lib.EINVAL
     assert lib.hfdtd_set_pair_planes(h, -1) == _lib.EINVAL
+    assert lib.hfdtd_set_pair_planes(h, 1025) == _lib.EINVAL
     src = (_lib.Source * 5)()
     for s in src:
         s.comp, s.mode, s.i0, s.i1, s.j0, s.j1, s.k0, s.k1 = 0, 1, 4, 5, 4, 5, 4, 5
