@@ -11,7 +11,9 @@ DFT at 8 frequencies; after the run, the near-to-far transform at
 A "step" is one full Yee time step on the FP32 step path the engine's
 autotuner picks for this grid before anything is timed (``--path`` forces
 one: ``split``, the H and E sweeps with the sheet source between them;
-``fused`` or ``march``, one H + E sweep and the source fix-up), and the
+``fused`` or ``march``, one H + E sweep and the source fix-up; ``pair``,
+one sweep for every two steps, the face DFT of the step between them read
+from the cells the sweep records), and the
 8-frequency face-DFT accumulation.  ``value`` is Mcell-updates/s over the K timed steps.
 ``roofline`` is the step's algorithmic HBM traffic (72 B/cell + 8 B per psi
 entry) divided by the time of the step's sweep kernels, taken with HIP
@@ -21,8 +23,8 @@ times this repository's NumPy restatement of the reference's face-CPML engine
 
 ``roofline.traffic`` is the measured HBM traffic of the step's sweep
 kernels, taken from ``profiles/split_sweeps_256cube_hbm_traffic.txt``
-(``k_fused_256cube_hbm_traffic.txt``, ``k_march_256cube_hbm_traffic.txt``
-for the one-launch paths): two
+(``k_fused_256cube_hbm_traffic.txt``, ``k_march_256cube_hbm_traffic.txt``,
+``k_pair_256cube_hbm_traffic.txt`` for the one-launch paths): two
 rocprofv3 PMC passes, FETCH_SIZE and WRITE_SIZE, each scaled by what the same
 counter reads for a plain device copy of known size (``--copy-calibration``
 runs that copy).
@@ -150,13 +152,11 @@ def run_gpu(args, wl: Workload):
     sim.add_source(wl.source)
     mon = hipfdtd.HIPNear2FarMonitor(sim, wl.center, wl.size, freqs=wl.freqs)
     assert (mon.ix0, mon.ix1) == wl.box, (mon.ix0, mon.ix1)
-    for _ in range(args.warmup):
-        sim.step()
+    sim.run(args.warmup)  # the pair path takes two steps per launch in run()
     sim.synchronize()
     sim.kernel_timing(True)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sim.step()
+    sim.run(args.steps)
     sim.synchronize()
     t1 = time.perf_counter()
     kern_ms, launches = sim.kernel_timing_read()
@@ -476,11 +476,13 @@ def run_batch(args):
 
 TRAFFIC_FILES = {"fused": ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt",
                  "split": ROOT / "profiles" / "split_sweeps_256cube_hbm_traffic.txt",
-                 "march": ROOT / "profiles" / "k_march_256cube_hbm_traffic.txt"}  # fmt: skip
+                 "march": ROOT / "profiles" / "k_march_256cube_hbm_traffic.txt",
+                 "pair": ROOT / "profiles" / "k_pair_256cube_hbm_traffic.txt"}  # fmt: skip
 
 KERNELS = {"fused": "k_fused (fused H + E sweep; sources by k_source_fixup)",
            "split": "k_update_H + k_update_E (split sweeps, one step; sources by k_add_source)",
-           "march": "k_march (register-pipelined H + E sweep, sources applied in the sweep)"}
+           "march": "k_march (register-pipelined H + E sweep, sources applied in the sweep)",
+           "pair": "k_pair (two H + E steps per launch, sources applied in the sweep)"}
 
 
 def measured_traffic(path):
@@ -490,7 +492,7 @@ def measured_traffic(path):
         for line in TRAFFIC_FILES[path].read_text().splitlines():
             if line.startswith("traffic_bytes_per_step:"):
                 return float(line.split(":", 1)[1])
-    except OSError:
+    except (OSError, KeyError):
         pass
     return None
 
@@ -522,7 +524,7 @@ def main():
     ap.add_argument("--repeat", type=int, default=2, help="adjoint: timed forward+backward runs")
     ap.add_argument("--march-tiling", default=None, metavar="ROWS,PLANES",
                     help="with --path march: rows per wavefront and planes per block")
-    ap.add_argument("--path", default="auto", choices=["auto", "fused", "split", "march"],
+    ap.add_argument("--path", default="auto", choices=["auto", "fused", "split", "march", "pair"],
                     help="256^3 workload: engine step path (auto = picked by the autotuner)")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -555,7 +557,8 @@ def main():
     mcells = cells * args.steps / res["wall_s"] / 1e6
     ms_step = res["wall_s"] / args.steps * 1e3
     b_alg = 72.0 * cells + 8.0 * wl.psi_entries()
-    # sweep kernels of one step: k_fused, or k_update_H + k_update_E
+    # sweep kernels per step: k_fused, k_march, k_update_H + k_update_E, or
+    # half a k_pair launch
     step_kern_s = res["kern_ms"] / args.steps / 1e3
     achieved = b_alg / step_kern_s / 1e9
     kernel = KERNELS[res["path"]]
@@ -590,7 +593,7 @@ def main():
             "traffic": measured_traffic(res["path"]),
             "bytes_alg_per_step": b_alg,
             "sweep_us_per_step": round(step_kern_s * 1e6, 2),
-            "launches_per_step": res["launches"] // args.steps,
+            "launches_per_step": round(res["launches"] / args.steps, 3),
         },
         "n2f_ms": round(res["n2f_ms"], 2),
         "farfield_finite": res["finite"],
