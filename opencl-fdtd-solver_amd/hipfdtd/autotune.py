@@ -38,6 +38,8 @@ PLANES = (-1, 8, 16, 24, 32, 48, 64)
 MARCH_ROWS_FP32 = (1, 2)
 MARCH_ROWS_FP64 = (1,)
 MARCH_PLANES = (-1, 16, 24, 32)
+PAIR_PLANES = (16, 24, 32)
+DEFAULT_PATHS = ("fused", "split", "march")
 BATCH_ROWS_FP32 = (2, 4, 6, 8)
 BATCH_ROWS_FP64 = (2, 4)
 BATCH_PLANES = (8, 16, 32, 64)
@@ -90,13 +92,12 @@ def cache_key(kind, shapes, dtype, npml, lossy, device_name) -> str:
 
 
 def _time_engine(sim, steps, warmup):
-    """µs of sweep kernels per step (the split path launches two per step)."""
-    for _ in range(warmup):
-        sim.step()
+    """µs of sweep kernels per step (the split path launches two per step,
+    the pair path one per two steps)."""
+    sim.run(warmup)
     sim.synchronize()
     sim.kernel_timing(True)
-    for _ in range(steps):
-        sim.step()
+    sim.run(steps)
     ms, _ = sim.kernel_timing_read()
     sim.kernel_timing(False)
     return ms / max(steps, 1) * 1e3
@@ -108,26 +109,32 @@ def _apply(sim, path, r, p):
         sim.set_fused_tiling(r, p, 0)
     elif path == "march":
         sim.set_march_tiling(r, p, 0, -2)
+    elif path == "pair":
+        sim.set_pair_planes(p)
 
 
 def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=None,
-                paths=("fused", "split", "march"), steps=20, warmup=3, use_cache=True,
+                pair_planes=None, paths=DEFAULT_PATHS, steps=20, warmup=3, use_cache=True,
                 write_cache=True, cache_file=None) -> dict:  # fmt: skip
     """Pick the step path and its launch shape for ``sim``, and apply them.
 
     Times, on a scratch engine of the same shape, the split sweeps (one
     candidate), the LDS-DMA fused sweep for ``rows`` (j rows per block) x
     ``planes`` candidates and the march sweep for ``march_rows`` (j rows per
-    wavefront) x ``march_planes`` candidates.  Returns {"path": "split" |
-    "fused" | "march", "rows", "planes", "us": fastest µs of sweep kernels
+    wavefront) x ``march_planes`` candidates; with "pair" in ``paths``
+    (opt-in) also the two-steps-per-launch sweep for ``pair_planes``.  The
+    timing runs without monitors: the pair sweep's record of watched cells
+    for monitors is not included.  Returns {"path": "split" | "fused" |
+    "march" | "pair", "rows", "planes", "us": fastest µs of sweep kernels
     per step, "table": [[path, rows, planes, µs], ...], "cached": bool};
     ``planes`` -1 is the grid-derived default.  One-launch candidates are
-    skipped for grids beyond their 32-bit slab limit.
+    skipped for grids beyond their slab limits.
     """
     from .engine import HIPFDTD
 
     shape = (sim.Nx, sim.Ny, sim.Nz)
-    key = cache_key("engine", [shape], sim.dtype, sim.npml, sim.lossy, sim.device_name)
+    kind = "engine" if tuple(paths) == DEFAULT_PATHS else "engine:" + "+".join(sorted(paths))
+    key = cache_key(kind, [shape], sim.dtype, sim.npml, sim.lossy, sim.device_name)
     if use_cache:
         hit = load_cache(cache_file).get(key)
         if hit:
@@ -138,6 +145,7 @@ def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=No
     planes = planes or PLANES
     march_rows = march_rows or (MARCH_ROWS_FP64 if fp64 else MARCH_ROWS_FP32)
     march_planes = march_planes or MARCH_PLANES
+    pair_planes = pair_planes or PAIR_PLANES
     scratch = HIPFDTD(shape, sim.dl, npml=sim.npml, dtype=sim.dtype, device=sim.device,
                       fused=False)  # fmt: skip
     table = []
@@ -158,6 +166,11 @@ def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=No
                 for p in march_planes:
                     scratch.set_march_tiling(r, p, 0, -2)
                     table.append(["march", r, p, round(_time_engine(scratch, steps, warmup), 3)])
+        scratch.set_path("pair")
+        if "pair" in paths and scratch.path == "pair":
+            for p in pair_planes:
+                scratch.set_pair_planes(p)
+                table.append(["pair", None, p, round(_time_engine(scratch, steps, warmup), 3)])
     finally:
         scratch.close()
     path, r, p, us = min(table, key=lambda e: e[3])

@@ -51,6 +51,32 @@ def test_tune_engine_applies_and_caches(tmp_path):
         assert np.array_equal(sim.read_volume(name), ref.read_volume(name)), name
 
 
+def test_tune_engine_with_the_pair_path(tmp_path):
+    """The pair sweep is an opt-in candidate with a cache entry of its own;
+    whichever path wins, the tuned engine steps like a plain march engine
+    (the pair sweep equals two march steps bit for bit)."""
+    import hipfdtd
+    from hipfdtd.autotune import tune_engine
+
+    cache = tmp_path / "tune.json"
+    sim = hipfdtd.HIPFDTD(SHAPE, DL, npml=6)
+    res = tune_engine(sim, march_rows=(2,), march_planes=(-1,), pair_planes=(8, 24),
+                      paths=("march", "pair"), steps=6, warmup=2, cache_file=cache)  # fmt: skip
+    print("[autotune] pair", res["path"], res["planes"], res["table"], flush=True)
+    assert [e[0] for e in res["table"]] == ["march", "pair", "pair"]
+    assert sim.path == res["path"] in ("march", "pair")
+    plain = tune_engine(hipfdtd.HIPFDTD(SHAPE, DL, npml=6), march_planes=(-1,), steps=4,
+                        warmup=1, cache_file=cache)  # fmt: skip
+    assert not plain["cached"] and plain["path"] != "pair"
+    assert not np.any(sim.read_volume("Ex"))
+    ref = hipfdtd.HIPFDTD(SHAPE, DL, npml=6, path="march")
+    for s in (sim, ref):
+        s.add_source(src)
+        s.run(26)
+    for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        assert np.array_equal(sim.read_volume(name), ref.read_volume(name)), name
+
+
 def test_tune_batch_applies_schedule(tmp_path):
     import hipfdtd
     from hipfdtd.autotune import tune_batch
