@@ -17,10 +17,13 @@
 // next / previous lane (DPP wavefront shifts); j from the same lane's next /
 // previous row, or, for a wave's first and last rows, through LDS (two
 // barriers per plane).  Every cell of the tile, halo included, goes through
-// h_math / e_math with its own indices, its psi and its sources, so a cell
+// h_math / e_math (h_interior / e_interior on tiles wholly inside the CPML-
+// free interior) with its own indices, its psi and its sources, so a cell
 // stored by this sweep equals the one two single steps produce, bit for
 // bit.  Values of halo cells outside the grid are never used: h_math and
-// e_math ignore neighbours past the grid's faces.
+// e_math ignore neighbours past the grid's faces.  Sub-updates whose results
+// no stored cell depends on (outer rows, the first and last fronts of a
+// chunk) are skipped.
 //
 // psi: the old psi of a cell is loaded with its plane, updated by step 1,
 // carried in registers to step 2 (H: one plane, E: two planes) and stored
@@ -28,8 +31,9 @@
 // stripes its tile (halo included) meets.
 //
 // Watched cells: the state between the two steps never reaches the field
-// slab.  Core cells inside up to kPairWatch boxes also write their step-1
-// H and E to a shadow slab (same layout), for observers of that step.
+// slab.  Core cells inside up to kPairWatch boxes also write the boxes'
+// components of their step-1 H and E to a shadow slab (same layout), for
+// observers of that step.
 #pragma once
 
 #include "march.h"
