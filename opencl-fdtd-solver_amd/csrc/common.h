@@ -39,6 +39,8 @@ enum PsiSlot {
 // Axis of the stripe each slot lives in (0 = x, 1 = y, 2 = z).
 constexpr int kPsiAxis[PSI_COUNT] = {0, 0, 1, 1, 2, 2, 0, 0, 1, 1, 2, 2};
 
+constexpr int kMatMax = 48;  // materials of an indexed grid at most (march sweep)
+
 struct Engine {
   int nx = 0, ny = 0, nz = 0, npml = 0;
   int fp64 = 0;
@@ -84,6 +86,15 @@ struct Engine {
   int march_run = 16;
   int march_nt = -1;
   int march_occ[2][3] = {};  // resident march blocks per CU by [lossy][rows]
+  int march_occ_mat[3] = {};  // the same for the material-indexed sweep by rows
+  // Lossless FP32 grids with at most kMatMax distinct eps_r values: a byte of
+  // material per cell and the table of ce for every pair of materials (an
+  // edge's ce depends only on its two cells), which the march sweep reads
+  // instead of the three ce volumes.  mat_ok while they match the ce volumes.
+  void *mat = nullptr;
+  float *mtab = nullptr;
+  int nmat = 0;
+  bool mat_ok = false;
 
   uint64_t bytes = 0;
 

@@ -5,6 +5,7 @@
 // state lives in device memory owned by the engine; the host only borrows
 // pointers for the duration of a call.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -168,6 +169,13 @@ static void release(Engine *e) {
   fr(e->coef);
   fr(e->prof);
   fr(e->shadow);
+  fr(e->mat);
+  {
+    void *t = e->mtab;
+    fr(t);
+    e->mtab = nullptr;
+  }
+  e->mat_ok = false;
   for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
   e->events.clear();
   e->events_used = 0;
@@ -192,6 +200,66 @@ int ensure_alt(Engine *e) {
   int rc = dmalloc(e, &e->state[other], 6 * e->vstride * e->elem);
   if (rc) return rc;
   return dmalloc(e, &e->psi[other], (e->psi_off[PSI_COUNT] + 64) * e->elem);
+}
+
+// Material index of a lossless FP32 grid (march sweep): the distinct eps_r
+// values, at most kMatMax of them, a byte per cell naming its value, and
+// for every pair (a, b) the ce of an edge between cells of eps a and b,
+// computed as k_materials computes it (FP64, then rounded), so the table
+// holds exactly the values of the ce volumes.  Grids with more values, FP64
+// or lossy grids keep mat_ok false and the sweep reads the ce volumes.
+template <typename S>
+static int build_material_index(Engine *e, const S *eps) {
+  e->mat_ok = false;
+  std::vector<double> vals;  // sorted distinct values
+  double last = 0.0;
+  bool have = false;
+  for (size_t c = 0; c < e->ncell; ++c) {  // the distinct values (runs are common)
+    const double v = (double)eps[c];
+    if (have && v == last) continue;
+    if (v != v) return HFDTD_OK;  // NaN: no exact index
+    auto it = std::lower_bound(vals.begin(), vals.end(), v);
+    if (it == vals.end() || *it != v) {
+      if ((int)vals.size() == kMatMax) return HFDTD_OK;  // too many materials
+      vals.insert(it, v);
+    }
+    last = v;
+    have = true;
+  }
+  std::vector<unsigned char> idx(e->ncell);
+  int last_i = 0;
+  have = false;
+  for (size_t c = 0; c < e->ncell; ++c) {
+    const double v = (double)eps[c];
+    if (!have || v != last) {
+      last_i = (int)(std::lower_bound(vals.begin(), vals.end(), v) - vals.begin());
+      last = v;
+      have = true;
+    }
+    idx[c] = (unsigned char)last_i;
+  }
+  const int m = (int)vals.size();
+  std::vector<float> tab((size_t)m * m);
+  const double scale = e->dt / kEPS0;
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b < m; ++b) tab[(size_t)a * m + b] = (float)(scale / (0.5 * (vals[a] + vals[b])));
+  if (!e->mat) {
+    int rc = dmalloc(e, &e->mat, e->ncell + 64);
+    if (rc) return rc;
+  }
+  if (!e->mtab) {
+    void *t = nullptr;
+    int rc = dmalloc(e, &t, (size_t)kMatMax * kMatMax * sizeof(float));
+    if (rc) return rc;
+    e->mtab = (float *)t;
+  }
+  HF_HIP(hipMemcpyAsync(e->mat, idx.data(), e->ncell, hipMemcpyHostToDevice, e->stream));
+  HF_HIP(hipMemcpyAsync(e->mtab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice,
+                        e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  e->nmat = m;
+  e->mat_ok = true;
+  return HFDTD_OK;
 }
 
 }  // namespace hfdtd
@@ -420,6 +488,10 @@ int hfdtd_set_materials(hfdtd_engine *h, const void *eps_r, const void *sigma, i
   (void)hipFree(d_eps);
   if (d_sig) (void)hipFree(d_sig);
   e->lossless = sigma == nullptr;
+  e->mat_ok = false;
+  if (rc == HFDTD_OK && !e->fp64 && sigma == nullptr)
+    rc = src_fp64 ? build_material_index(e, (const double *)eps_r)
+                  : build_material_index(e, (const float *)eps_r);
   return rc;
 }
 
@@ -428,6 +500,7 @@ int hfdtd_write_volume(hfdtd_engine *h, int which, const void *host) {
   Engine *e = &h->s;
   void *slot = volume_ptr(e, which);
   if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
+  if (which >= HFDTD_CEX) e->mat_ok = false;  // coefficients no longer from the index
   HF_HIP(hipSetDevice(e->device));
   HF_HIP(hipMemcpyAsync(slot, host, e->ncell * e->elem, hipMemcpyHostToDevice, e->stream));
   HF_HIP(hipStreamSynchronize(e->stream));

@@ -117,6 +117,7 @@ struct MarchH {
 template <typename T>
 struct MarchE {
   T cx, cy, cz;  // ce(i)
+  unsigned mi1, mj1, mk1;  // indexed grids: material of the cells at i+1, j+1, k+1
   T ax, ay, az;  // ca(i) (lossy grids)
   T p[6];        // old psi of the E slots (PML tiles)
 };
@@ -128,6 +129,7 @@ struct MarchSlot {
   unsigned o4;   // byte offset of (j, k) inside a plane
   unsigned qy4;  // byte offset inside a y-stripe plane (kDropOffset outside the stripe)
   unsigned qz4;  // byte offset inside the z-stripe rows of a plane (kDropOffset outside)
+  unsigned o1, o1j, o1k;  // material-index offsets of (j, k), (j+1, k), (j, k+1), clamped
   bool iny, inz;
   bool own;      // an owned cell inside the grid
   int lh, le;    // LDS element index (row * kMarchCols + column) of its H and E
@@ -169,7 +171,7 @@ struct PsiPlane {
   bool dead;  // a plane past the chunk: its loads are dropped
 };
 
-template <typename T, int RW, bool LOSSY, bool X, bool Y, bool Z, bool NT>
+template <typename T, int RW, bool LOSSY, bool X, bool Y, bool Z, bool NT, bool MAT = false>
 struct March {
   static constexpr int S = RW + 1;  // RW owned rows + the halo slot
   static constexpr int TJ = 4 * RW;
@@ -182,6 +184,12 @@ struct March {
   const Src<T> *lsrc;  // sources of the step (LDS copy)
   int nsrc;
   int src_i0, src_i1;  // planes some source covers
+  // indexed grids: one byte of material per cell, ce of an edge from the
+  // table of its two cells' materials (LDS copy)
+  Rsrc rmat;
+  const float *mtab;
+  int nmat;
+  unsigned mc[RW];  // material of the owned cells on the current plane
   MarchSlot sl[S];
   Prof1<T> pyh[S], pye[S], pzh[S], pze[S];
   T hyc[RW], hzc[RW];  // new H(i-1) at the owned cells
@@ -247,12 +255,22 @@ struct March {
     }
   }
 
+  __device__ __forceinline__ unsigned mat_plane(int i, bool dead) const {
+    return dead ? kDropOffset : (unsigned)min(i, g.nx - 1) * (unsigned)(g.ny * g.nz);
+  }
+
   __device__ __forceinline__ void load_e(MarchE<T> &s, const MarchSlot &m, unsigned pl,
-                                         const PsiPlane &pp) const {
+                                         unsigned pm, unsigned pm1, const PsiPlane &pp) const {
     const unsigned vs = vsb();
-    s.cx = bld<T, NT>(rc, m.o4, pl);
-    s.cy = bld<T, NT>(rc, m.o4, vs + pl);
-    s.cz = bld<T, NT>(rc, m.o4, 2 * vs + pl);
+    if constexpr (MAT) {
+      s.mi1 = __builtin_amdgcn_raw_buffer_load_b8(rmat, m.o1, pm1, 0);
+      s.mj1 = __builtin_amdgcn_raw_buffer_load_b8(rmat, m.o1j, pm, 0);
+      s.mk1 = __builtin_amdgcn_raw_buffer_load_b8(rmat, m.o1k, pm, 0);
+    } else {
+      s.cx = bld<T, NT>(rc, m.o4, pl);
+      s.cy = bld<T, NT>(rc, m.o4, vs + pl);
+      s.cz = bld<T, NT>(rc, m.o4, 2 * vs + pl);
+    }
     if constexpr (LOSSY) {
       s.ax = bld<T, NT>(rc, m.o4, 3 * vs + pl);
       s.ay = bld<T, NT>(rc, m.o4, 4 * vs + pl);
@@ -285,8 +303,9 @@ struct March {
   __device__ __forceinline__ void load_e_all(MarchE<T> (&s)[RW], int i) const {
     const PsiPlane pp = psi_plane(i);
     const unsigned pl = load_plane(i, pp.dead);
+    const unsigned pm = MAT ? mat_plane(i, pp.dead) : 0u, pm1 = MAT ? mat_plane(i + 1, pp.dead) : 0u;
 #pragma unroll
-    for (int q = 0; q < RW; ++q) load_e(s[q], sl[q], pl, pp);
+    for (int q = 0; q < RW; ++q) load_e(s[q], sl[q], pl, pm, pm1, pp);
   }
 
   // psi store of an owned cell to stripe slot s at byte offset a (slot base
@@ -416,6 +435,18 @@ struct March {
         p[5] = ec[q].p[5];
       }
       T ex = oe[q][0], ey = oe[q][1], ez = oe[q][2];
+      T cx, cy, cz;
+      if constexpr (MAT) {
+        const float *tb = mtab + mc[q] * (unsigned)nmat;
+        cx = tb[ec[q].mi1];
+        cy = tb[ec[q].mj1];
+        cz = tb[ec[q].mk1];
+        mc[q] = ec[q].mi1;  // the cell's material on the next plane
+      } else {
+        cx = ec[q].cx;
+        cy = ec[q].cy;
+        cz = ec[q].cz;
+      }
       // sources of this step on the cell, added before the update in list
       // order, as the reference adds them between its H and E kernels; the
       // plane and row tests are wave-uniform
@@ -428,7 +459,7 @@ struct March {
           const bool c0 = c == 0, c1 = c == 1;
           // ce of the component by masking bits: a select of the three values
           // is turned into an indexed load from a scratch copy of them
-          const T ce = pick3(c0, c1, ec[q].cx, ec[q].cy, ec[q].cz);
+          const T ce = pick3(c0, c1, cx, cy, cz);
           const T v = src_term<T>(sq, ic, m.j, m.k, ce, sq.amp);
           ex = c0 ? ex + v : ex;
           ey = c1 ? ey + v : ey;
@@ -438,8 +469,7 @@ struct March {
       const T ax = LOSSY ? ec[q].ax : T(1), ay = LOSSY ? ec[q].ay : T(1);
       const T az = LOSSY ? ec[q].az : T(1);
       e_math(g, ic, m.j, m.k, st, pxe, pye[q], pze[q], h[q][0], h[q][1], h[q][2], hx_jm, hz_jm,
-             hx_km, hy_km, hyc[q], hzc[q], ex, ey, ez, ax, ay, az, ec[q].cx, ec[q].cy, ec[q].cz,
-             p, inv_dl);
+             hx_km, hy_km, hyc[q], hzc[q], ex, ey, ez, ax, ay, az, cx, cy, cz, p, inv_dl);
       const bool ok = m.own && live;
       const unsigned v = ok ? m.o4 + pl : padb;
       bst<T, NT>(r1, ex, v, 0u);
@@ -533,6 +563,9 @@ struct March {
       m.j = min(max(j, 0), g.ny - 1);
       m.k = min(max(k, 0), g.nz - 1);
       m.o4 = (unsigned)(m.j * g.nz + m.k) * E;
+      m.o1 = (unsigned)(m.j * g.nz + m.k);
+      m.o1j = (unsigned)(min(m.j + 1, g.ny - 1) * g.nz + m.k);
+      m.o1k = (unsigned)(m.j * g.nz + min(m.k + 1, g.nz - 1));
       m.iny = Y && n > 0 && (m.j < n || m.j >= g.ny - n);
       m.inz = Z && n > 0 && (m.k < n || m.k >= g.nz - n);
       m.qy4 = m.iny ? (unsigned)(stripe_local(m.j, n, g.ny) * g.nz + m.k) * E : kDropOffset;
@@ -562,6 +595,11 @@ struct March {
         oe[q][0] = bld<T, false>(r0, sl[q].o4, p0);
         oe[q][1] = bld<T, false>(r0, sl[q].o4, vs + p0);
         oe[q][2] = bld<T, false>(r0, sl[q].o4, 2 * vs + p0);
+      }
+      if constexpr (MAT) {
+        const unsigned pm = mat_plane(i0, false);
+#pragma unroll
+        for (int q = 0; q < RW; ++q) mc[q] = __builtin_amdgcn_raw_buffer_load_b8(rmat, sl[q].o1, pm, 0);
       }
     }
     load_h_all(A, i0);
@@ -652,7 +690,7 @@ struct MarchChunks {
 // column) meet a z-stripe.  (The plane before the chunk, recomputed once,
 // reads its psi in any variant.)  Fewer psi operations per plane also keep
 // a plane's loads and stores within the 63 a wavefront can count.
-template <typename T, int RW, bool LOSSY, bool NT>
+template <typename T, int RW, bool LOSSY, bool NT, bool MAT>
 __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dtm_dl, T dtm,
                                                          T inv_dl, const T *__restrict__ s0,
                                                          T *__restrict__ s1,
@@ -661,13 +699,20 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
                                                          const T *__restrict__ q0,
                                                          T *__restrict__ q1, int tiles_k,
                                                          int tiles_j, MarchChunks ch, int xcd,
-                                                         int run, const MarchSources<T> src) {
+                                                         int run, const MarchSources<T> src,
+                                                         const unsigned char *__restrict__ mat,
+                                                         const float *__restrict__ mtab,
+                                                         int nmat) {
   __shared__ MarchLds<T, RW> L;
   __shared__ Src<T> lsrc[kMarchSources];
+  __shared__ float ltab[MAT ? kMatMax * kMatMax : 1];
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int t = 0; t < kMarchSources; ++t) lsrc[t] = src.s[t];
   }  // visible after the barrier that ends the prologue
+  if constexpr (MAT) {
+    for (int t = threadIdx.x; t < nmat * nmat; t += kMarchThreads) ltab[t] = mtab[t];
+  }
   const int B = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
   const int bk = B % tiles_k, bc = B / (tiles_k * tiles_j);
   // XCD x runs the same j-tiles of every chunk unless the tile order rotates
@@ -692,10 +737,15 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
   const Rsrc rq0 = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(q0), (short)0, psi_bytes,
                                                      0x00020000);
   const Rsrc rq1 = __builtin_amdgcn_make_buffer_rsrc(q1, (short)0, psi_bytes, 0x00020000);
+  // the material index: one byte per cell, sized so loads of dead planes drop
+  const Rsrc rmat = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<unsigned char *>(mat), (short)0,
+      MAT ? (int)min((unsigned long long)g.nx * g.ny * g.nz + 64ull, 0x7fffffffull) : 0, 0x00020000);
 #define HF_MARCH_RUN(X, Y, Z)                                                                  \
   {                                                                                           \
-    March<T, RW, LOSSY, X, Y, Z, NT> m{g,   dtm_dl, dtm, inv_dl, r0,   r1,   rc,                    \
-                                       rq0, rq1,    prof, lsrc,   src.n, src.i0, src.i1};       \
+    March<T, RW, LOSSY, X, Y, Z, NT, MAT> m{g,   dtm_dl, dtm,  inv_dl, r0,     r1,              \
+                                            rc,  rq0,    rq1,  prof,   lsrc,   src.n,           \
+                                            src.i0, src.i1, rmat, ltab, nmat};                  \
     m.run(L, np, k0, j0, i0, i1);                                                             \
   }
   if (x) {

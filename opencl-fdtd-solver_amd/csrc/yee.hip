@@ -221,10 +221,10 @@ static int fused_tj(Engine *e, const SrcList<T> &sl) {
   return source_fixup<T, LOSSY>(e, sl);
 }
 
-template <typename T, int RW, bool LOSSY, bool NT>
+template <typename T, int RW, bool LOSSY, bool NT, bool MAT = false>
 static int march_rw(Engine *e, const SrcList<T> &sl) {
-  auto kern = k_march<T, RW, LOSSY, NT>;
-  int &occ = e->march_occ[LOSSY ? 1 : 0][RW];
+  auto kern = k_march<T, RW, LOSSY, NT, MAT>;
+  int &occ = MAT ? e->march_occ_mat[RW] : e->march_occ[LOSSY ? 1 : 0][RW];
   if (occ == 0) {
     // resident blocks per CU from the kernel's own resource use: a block puts
     // one wavefront on each of the four SIMDs, each SIMD holds 512 VGPRs per
@@ -263,7 +263,8 @@ static int march_rw(Engine *e, const SrcList<T> &sl) {
                      T(e->dtm_dl), T(e->dtm), T(e->inv_dl), (const T *)e->state[e->cur],
                      (T *)e->state[o], (const T *)e->coef, (const T *)e->prof,
                      (const T *)e->psi[e->cur], (T *)e->psi[o], tiles_k, tiles_j, ch, e->xcd,
-                     e->march_run, ms);
+                     e->march_run, ms, MAT ? (const unsigned char *)e->mat : nullptr,
+                     MAT ? (const float *)e->mtab : nullptr, MAT ? e->nmat : 0);
   HF_HIP(hipGetLastError());
   rc = timing_mark(e);
   if (rc || inside) return rc;
@@ -298,6 +299,8 @@ static int march_t(Engine *e, const hfdtd_source *src, int n) {
   } else if (e->march_rows == 1) {
     rc = ll ? (nt ? march_rw<T, 1, false, true>(e, sl) : march_rw<T, 1, false, false>(e, sl))
             : (nt ? march_rw<T, 1, true, true>(e, sl) : march_rw<T, 1, true, false>(e, sl));
+  } else if (ll && e->mat_ok) {  // lossless with a material index: ce from its table
+    rc = nt ? march_rw<T, 2, false, true, true>(e, sl) : march_rw<T, 2, false, false, true>(e, sl);
   } else {
     rc = ll ? (nt ? march_rw<T, 2, false, true>(e, sl) : march_rw<T, 2, false, false>(e, sl))
             : (nt ? march_rw<T, 2, true, true>(e, sl) : march_rw<T, 2, true, false>(e, sl));

@@ -1,0 +1,105 @@
+"""Material-indexed march sweep (lossless FP32 grids with few distinct eps_r).
+
+An edge's ce depends only on the eps_r of its two cells, so on a grid with
+at most 48 distinct values the march sweep reads one byte of material per
+cell and takes ce from a table computed exactly as the ce volumes are.
+Results must equal the fused sweep (which reads the ce volumes) bit for bit;
+grids with more values, lossy grids and grids whose ce volumes were written
+by hand fall back to the volumes.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import fdtd_oracle as fo
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+DL = 1e-3
+
+
+def hip():
+    import hipfdtd
+
+    return hipfdtd
+
+
+def _boxes_eps(shape, nvals, seed, dtype=np.float32):
+    """Vacuum with random boxes of nvals - 1 distinct eps_r values."""
+    rng = np.random.default_rng(seed)
+    vals = rng.uniform(1.5, 9.0, size=nvals - 1)
+    eps = np.ones(shape, dtype=np.float64)
+    for q in range(3 * nvals):
+        lo = [int(rng.integers(0, n - 2)) for n in shape]
+        hi = [min(n, a + int(rng.integers(2, max(3, n // 3)))) for a, n in zip(lo, shape)]
+        eps[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = vals[q % (nvals - 1)]
+    flat = eps.reshape(-1)  # every value on at least one cell, vacuum too
+    for q in range(nvals - 1):
+        flat[(q * 7919 + 13) % flat.size] = vals[q]
+    flat[(nvals * 7919 + 13) % flat.size] = 1.0
+    return eps.astype(dtype)
+
+
+def _run(path, shape, npml, eps, nsrc, steps, poke=None):
+    sim = hip().HIPFDTD(shape, DL, npml=npml, path=path)
+    if path == "march":
+        sim.set_march_tiling(2, 0, 0, -2)
+    sim.set_epsilon(eps)
+    if poke is not None:
+        poke(sim)
+    nx, ny, nz = shape
+
+    def src(f):
+        a = fo.gaussian_sine(f.t, 3e10, 1.5e10)
+        f.add_source_Jx(nz // 2, a, 2, nx - 3, 3, ny - 2, rim_taper=True, rim_edge=0.7)
+        for q in range(nsrc - 1):
+            f.add_source_J("xyz"[q % 3], (0.4 - 0.1 * q) * a,
+                           (nx // 3 + q, nx // 3 + q + 1, ny // 2, ny // 2 + 1, 5 + q, 6 + q))  # fmt: skip
+
+    sim.add_source(src)
+    sim.run(steps)
+    return sim.fields()
+
+
+def _same(a, b):
+    for n in FIELDS:
+        np.testing.assert_array_equal(a[n], b[n], err_msg=n)
+
+
+@pytest.mark.parametrize("shape,npml,nvals,nsrc,dtype", [
+    ((45, 35, 100), 6, 5, 3, np.float32),
+    ((40, 52, 70), 8, 48, 4, np.float64),
+    ((40, 52, 70), 8, 49, 2, np.float32),
+    ((33, 40, 130), 0, 12, 6, np.float32)])  # fmt: skip
+def test_indexed_march_matches_fused(shape, npml, nvals, nsrc, dtype):
+    print(f"[mat] shape={shape} npml={npml} nvals={nvals} nsrc={nsrc} {np.dtype(dtype).name}",
+          flush=True)  # fmt: skip
+    eps = _boxes_eps(shape, nvals, 3, dtype)
+    assert len(np.unique(eps)) == nvals
+    _same(_run("fused", shape, npml, eps, nsrc, 16), _run("march", shape, npml, eps, nsrc, 16))
+
+
+def test_hand_written_coefficients_turn_the_index_off():
+    shape = (45, 35, 100)
+    eps = _boxes_eps(shape, 6, 4)
+    cex = None
+
+    def poke(sim):
+        nonlocal cex
+        if cex is None:
+            cex = sim.read_volume("ce_x") * np.float32(0.9)
+        sim.write_volume("ce_x", cex)
+
+    print(f"[mat-poke] shape={shape}", flush=True)
+    _same(_run("fused", shape, 6, eps, 2, 12, poke), _run("march", shape, 6, eps, 2, 12, poke))
+
+
+def test_indexed_march_full_size():
+    """The benchmark's kind of medium on its grid: 40 boxes on vacuum, 256^3."""
+    shape = (256, 256, 256)
+    eps = _boxes_eps(shape, 41, 2)
+    print(f"[mat-full] shape={shape} nvals={len(np.unique(eps))}", flush=True)
+    _same(_run("fused", shape, 12, eps, 1, 6), _run("march", shape, 12, eps, 1, 6))
