@@ -211,34 +211,45 @@ int ensure_alt(Engine *e) {
 template <typename S>
 static int build_material_index(Engine *e, const S *eps) {
   e->mat_ok = false;
-  std::vector<double> vals;  // sorted distinct values
-  double last = 0.0;
+  // values keyed by their bit patterns (+0 and -0 stay apart), NaN refused
+  auto bits = [](double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, sizeof u);
+    return u;
+  };
+  std::vector<uint64_t> keys;  // sorted distinct patterns
+  uint64_t last = 0;
   bool have = false;
   for (size_t c = 0; c < e->ncell; ++c) {  // the distinct values (runs are common)
     const double v = (double)eps[c];
-    if (have && v == last) continue;
-    if (v != v) return HFDTD_OK;  // NaN: no exact index
-    auto it = std::lower_bound(vals.begin(), vals.end(), v);
-    if (it == vals.end() || *it != v) {
-      if ((int)vals.size() == kMatMax) return HFDTD_OK;  // too many materials
-      vals.insert(it, v);
+    const uint64_t u = bits(v);
+    if (have && u == last) continue;
+    // NaN, or a value whose doubling overflows (the last plane's ce uses the
+    // cell value itself, the table 0.5 * (v + v)): no exact index
+    if (v != v || std::fabs(v) > 8.0e307) return HFDTD_OK;
+    auto it = std::lower_bound(keys.begin(), keys.end(), u);
+    if (it == keys.end() || *it != u) {
+      if ((int)keys.size() == kMatMax) return HFDTD_OK;  // too many materials
+      keys.insert(it, u);
     }
-    last = v;
+    last = u;
     have = true;
   }
   std::vector<unsigned char> idx(e->ncell);
   int last_i = 0;
   have = false;
   for (size_t c = 0; c < e->ncell; ++c) {
-    const double v = (double)eps[c];
-    if (!have || v != last) {
-      last_i = (int)(std::lower_bound(vals.begin(), vals.end(), v) - vals.begin());
-      last = v;
+    const uint64_t u = bits((double)eps[c]);
+    if (!have || u != last) {
+      last_i = (int)(std::lower_bound(keys.begin(), keys.end(), u) - keys.begin());
+      last = u;
       have = true;
     }
     idx[c] = (unsigned char)last_i;
   }
-  const int m = (int)vals.size();
+  const int m = (int)keys.size();
+  std::vector<double> vals(m);
+  for (int a = 0; a < m; ++a) std::memcpy(&vals[a], &keys[a], sizeof(double));
   std::vector<float> tab((size_t)m * m);
   const double scale = e->dt / kEPS0;
   for (int a = 0; a < m; ++a)
