@@ -138,17 +138,18 @@ def run_gpu(args, wl: Workload):
     if args.path == "march" and args.march_tiling:
         r, p = (int(v) for v in args.march_tiling.split(","))
         sim.set_march_tiling(r, p, 0, -2)
+    sim.set_epsilon(wl.eps)
     if args.path == "auto":
         # the engine's autotuner picks the step path and launch shape (on a
-        # scratch engine, before anything is timed)
+        # scratch engine with the same kind of coefficients, before anything
+        # is timed)
         from hipfdtd.autotune import tune_engine
 
         tuned = tune_engine(sim, rows=(8,), planes=(-1,), march_rows=(1, 2),
-                            march_planes=(-1, 24), steps=10, warmup=2, use_cache=False,
+                            march_planes=(-1, 24, 40), steps=10, warmup=2, use_cache=False,
                             write_cache=False)  # fmt: skip
         print(f"[bench] autotuned path {tuned['path']} rows={tuned['rows']} "
               f"planes={tuned['planes']}: {tuned['table']}", file=sys.stderr, flush=True)
-    sim.set_epsilon(wl.eps)
     sim.add_source(wl.source)
     mon = hipfdtd.HIPNear2FarMonitor(sim, wl.center, wl.size, freqs=wl.freqs)
     assert (mon.ix0, mon.ix1) == wl.box, (mon.ix0, mon.ix1)

@@ -628,6 +628,12 @@ int hfdtd_set_view(hfdtd_engine *h, int shadow) {
   return HFDTD_OK;
 }
 
+int hfdtd_material_count(const hfdtd_engine *h, int *nmat) {
+  if (!h || !nmat) return fail(HFDTD_EINVAL, "NULL argument");
+  *nmat = h->s.mat_ok ? h->s.nmat : 0;
+  return HFDTD_OK;
+}
+
 int hfdtd_set_pair_planes(hfdtd_engine *h, int planes_i) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
   if (planes_i < 0 || planes_i > 1024) return fail(HFDTD_EINVAL, "planes_i must be 0..1024");

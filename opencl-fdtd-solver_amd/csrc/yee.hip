@@ -297,8 +297,11 @@ static int march_t(Engine *e, const hfdtd_source *src, int n) {
     rc = ll ? (nt ? march_rw<T, 1, false, true>(e, sl) : march_rw<T, 1, false, false>(e, sl))
             : (nt ? march_rw<T, 1, true, true>(e, sl) : march_rw<T, 1, true, false>(e, sl));
   } else if (e->march_rows == 1) {
-    rc = ll ? (nt ? march_rw<T, 1, false, true>(e, sl) : march_rw<T, 1, false, false>(e, sl))
-            : (nt ? march_rw<T, 1, true, true>(e, sl) : march_rw<T, 1, true, false>(e, sl));
+    if (ll && e->mat_ok)
+      rc = nt ? march_rw<T, 1, false, true, true>(e, sl) : march_rw<T, 1, false, false, true>(e, sl);
+    else
+      rc = ll ? (nt ? march_rw<T, 1, false, true>(e, sl) : march_rw<T, 1, false, false>(e, sl))
+              : (nt ? march_rw<T, 1, true, true>(e, sl) : march_rw<T, 1, true, false>(e, sl));
   } else if (ll && e->mat_ok) {  // lossless with a material index: ce from its table
     rc = nt ? march_rw<T, 2, false, true, true>(e, sl) : march_rw<T, 2, false, false, true>(e, sl);
   } else {

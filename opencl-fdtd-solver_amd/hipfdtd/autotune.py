@@ -133,7 +133,11 @@ def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=No
     from .engine import HIPFDTD
 
     shape = (sim.Nx, sim.Ny, sim.Nz)
+    # a grid with a material index runs other march kernels: tune those
+    indexed = not sim.lossy and sim.material_count() > 0
     kind = "engine" if tuple(paths) == DEFAULT_PATHS else "engine:" + "+".join(sorted(paths))
+    if indexed:
+        kind += ":indexed"
     key = cache_key(kind, [shape], sim.dtype, sim.npml, sim.lossy, sim.device_name)
     if use_cache:
         hit = load_cache(cache_file).get(key)
@@ -152,6 +156,8 @@ def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=No
     try:
         if sim.lossy:
             scratch.set_epsilon(np.ones(shape, np.float32), np.zeros(shape, np.float32))
+        elif indexed:
+            scratch.set_epsilon(np.ones(shape, np.float32))  # one material: indexed
         if "split" in paths:
             table.append(["split", None, None, round(_time_engine(scratch, steps, warmup), 3)])
         scratch.set_path("fused")

@@ -520,6 +520,14 @@ class HIPFDTD(SourceMonitorMixin):
                                                     int(planes_i), int(run_blocks),
                                                     int(nontemporal)), "set_march_tiling")  # fmt: skip
 
+    def material_count(self) -> int:
+        """Materials of the march sweep's material index (0: the sweep reads
+        the ce volumes).  Lossless FP32 grids with at most 48 distinct εr
+        values get one at ``set_epsilon``."""
+        n = C.c_int(0)
+        _lib.check(self._lib.hfdtd_material_count(self._handle, C.byref(n)), "material_count")
+        return int(n.value)
+
     def set_pair_planes(self, planes_i: int = 0) -> None:
         """Planes of i per block of the pair sweep (0 = 24)."""
         _lib.check(self._lib.hfdtd_set_pair_planes(self._handle, int(planes_i)), "set_pair_planes")
