@@ -136,8 +136,8 @@ def run_gpu(args, wl: Workload):
     sim = hipfdtd.HIPFDTD(wl.shape, wl.dl, npml=wl.npml, dtype=np.float32, path=path)
     tuned = None
     if args.path == "march" and args.march_tiling:
-        r, p = (int(v) for v in args.march_tiling.split(","))
-        sim.set_march_tiling(r, p, 0, -2)
+        v = [int(x) for x in args.march_tiling.split(",")]
+        sim.set_march_tiling(v[0], v[1], v[2] if len(v) > 2 else 0, v[3] if len(v) > 3 else -2)
     sim.set_epsilon(wl.eps)
     if args.path == "auto":
         # the engine's autotuner picks the step path and launch shape (on a
@@ -523,8 +523,9 @@ def main():
                     choices=["yee-256", "plane-dft", "adjoint", "batch"])
     ap.add_argument("--graph-steps", type=int, default=100, help="batch: steps per HIP graph")
     ap.add_argument("--repeat", type=int, default=2, help="adjoint: timed forward+backward runs")
-    ap.add_argument("--march-tiling", default=None, metavar="ROWS,PLANES",
-                    help="with --path march: rows per wavefront and planes per block")
+    ap.add_argument("--march-tiling", default=None, metavar="ROWS,PLANES[,RUN[,NT]]",
+                    help="with --path march: rows per wavefront, planes per block, blocks per XCD "
+                         "run, non-temporal streams (1 / 0)")
     ap.add_argument("--path", default="auto", choices=["auto", "fused", "split", "march", "pair"],
                     help="256^3 workload: engine step path (auto = picked by the autotuner)")
     args = ap.parse_args()

@@ -156,7 +156,8 @@ int hfdtd_set_fused_tiling(hfdtd_engine *e, int rows_j, int planes_i, int xcd_gr
  * Used by the autotuner. */
 int hfdtd_set_split_schedule(hfdtd_engine *e, int run_blocks);
 /* March-sweep launch shape (HFDTD_PATH_MARCH): j rows per wavefront (1 or 2;
- * a block is 4 wavefronts of 64 k; FP64 engines use 1), planes of i per block
+ * a block is 4 wavefronts of 64 k; FP64 engines use 1; until set, one on
+ * grids with a material index and two otherwise), planes of i per block
  * (rounded up to even; -1 = chosen from the grid), consecutive blocks per XCD
  * run, and non-temporal field streams (1 on, 0 off, -1 when the slabs exceed
  * the last-level cache).  0 keeps rows / planes / run, -2 keeps nontemporal.

@@ -72,7 +72,7 @@ struct Engine {
   // 4 rows per wave-row group), i planes per block (0 = chosen from the
   // grid), consecutive blocks per XCD run, non-temporal streams (-1 = when
   // the slabs exceed the last-level cache, 0 = never, 1 = always)
-  int march_rows = 2;
+  int march_rows = 0;  // 0: one row on material-indexed grids, else two
   // pair sweep (two steps per launch): i planes per block; the watched
   // boxes whose state between the two steps goes to the shadow slab, and
   // whether field reads see that slab
@@ -127,8 +127,10 @@ struct Engine {
   // every block re-reads; slots = resident blocks per CU x CUs.  Ties go to
   // more planes per block.
   // i planes per march block (even): the set value, or the even P in
-  // [8, 64] minimising (waves of resident blocks) x (P + 1).  With PML the
-  // x-stripes form chunks of their own and P chunks the planes between.
+  // [8, 64] minimising (waves of resident blocks) x (P + 3), the 3 for the
+  // plane before the chunk and its prologue (measured at 256^3: short
+  // chunks cost more than one plane each).  With PML the x-stripes form
+  // chunks of their own and P chunks the planes between.
   int march_planes_for(int rows_per_wave, int per_cu) const {
     if (march_planes > 0) return (march_planes + 1) / 2 * 2;
     const long long slots = (long long)(per_cu > 0 ? per_cu : 1) * cus;
@@ -140,7 +142,7 @@ struct Engine {
     const int inner = split ? nx - 2 * npml : nx;
     for (int P = 64; P >= 8; P -= 2) {
       const long long blocks = tiles * ((inner + P - 1) / P + (split ? 2 : 0));
-      const long long t = (blocks + slots - 1) / slots * (P + 1);
+      const long long t = (blocks + slots - 1) / slots * (P + 3);
       if (best_t < 0 || t < best_t) {
         best_t = t;
         best = P;

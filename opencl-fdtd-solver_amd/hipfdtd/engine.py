@@ -512,7 +512,8 @@ class HIPFDTD(SourceMonitorMixin):
 
     def set_march_tiling(self, rows_per_wave: int = 0, planes_i: int = 0, run_blocks: int = 0,
                          nontemporal: int = -2) -> None:  # fmt: skip
-        """Launch shape of the march sweep: j rows per wavefront (1 or 2),
+        """Launch shape of the march sweep: j rows per wavefront (1 or 2;
+        until set, one on grids with a material index and two otherwise),
         planes of i per block (-1 = from the grid), consecutive blocks per XCD
         run, non-temporal streams (1 / 0 / -1 = when the slabs exceed the
         last-level cache); 0 (-2 for ``nontemporal``) keeps a setting."""
