@@ -264,8 +264,10 @@ def run_plane_dft(args):
     ref = [m.dft() for m in allm["direct_fp64"]]
     err = {}
     for name in ("buffered_k32", "direct_fp32", "direct_kahan"):
-        err[name] = max(float(np.max(np.abs(m.dft() - r)) / np.max(np.abs(r)))
-                        for m, r in zip(allm[name], ref))  # fmt: skip
+        # planes the pulse has not reached yet hold exact zeros in every mode
+        errs = [float(np.max(np.abs(m.dft() - r)) / np.max(np.abs(r)))
+                for m, r in zip(allm[name], ref) if np.max(np.abs(r)) > 0]  # fmt: skip
+        err[name] = max(errs) if errs else None
     print(json.dumps({
         "metric": "plane_dft_us_per_step", "unit": "us", "higher_is_better": False,
         "steps": args.steps, "dtype": "f32", "data": "synthetic",
