@@ -48,7 +48,6 @@ constexpr int kMarchSources = 4;    // sources applied inside the sweep (more: f
 template <typename T>
 struct MarchSources {
   int n;
-  int i0, i1;  // planes covered by some source (their union's bounds)
   Src<T> s[kMarchSources];
 };
 
@@ -183,7 +182,7 @@ struct March {
   const T *__restrict__ prof;
   const Src<T> *lsrc;  // sources of the step (LDS copy)
   int nsrc;
-  int src_i0, src_i1;  // planes some source covers
+  int src_i0, src_i1;  // planes on which some source can reach this tile
   // indexed grids: one byte of material per cell, ce of an edge from the
   // table of its two cells' materials (LDS copy)
   Rsrc rmat;
@@ -722,6 +721,19 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
   const int k0 = bk * 64, j0 = bj * 4 * RW;
   int i0, i1;
   ch.range(g, planes, bc, i0, i1);
+  // planes on which a source of the step can reach a slot of this tile: the
+  // sources whose box meets rows j0-1 ... j0+4RW and columns k0-1 ... k0+64
+  // (every slot's clamped cell lies there); the per-cell tests of the sweep
+  // still decide, so a tile no source touches skips the source loop
+  int si0 = g.nx, si1 = 0;
+#pragma unroll
+  for (int t = 0; t < kMarchSources; ++t) {
+    if (t < src.n && src.s[t].cj0 <= j0 + 4 * RW && src.s[t].cj1 > j0 - 1 &&
+        src.s[t].ck0 <= k0 + 64 && src.s[t].ck1 > k0 - 1) {
+      si0 = min(si0, src.s[t].ci0);
+      si1 = max(si1, src.s[t].ci1);
+    }
+  }
   const int np = (i1 - i0 + 1) / 2 * 2;  // even; planes past i1 are masked
   const int n = g.npml;
   const bool x = n > 0 && (i0 < n || i1 > g.nx - n);
@@ -745,7 +757,7 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
   {                                                                                           \
     March<T, RW, LOSSY, X, Y, Z, NT, MAT> m{g,   dtm_dl, dtm,  inv_dl, r0,     r1,              \
                                             rc,  rq0,    rq1,  prof,   lsrc,   src.n,           \
-                                            src.i0, src.i1, rmat, ltab, nmat};                  \
+                                            si0,    si1,  rmat, ltab, nmat};                    \
     m.run(L, np, k0, j0, i0, i1);                                                             \
   }
   if (x) {

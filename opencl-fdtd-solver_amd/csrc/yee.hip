@@ -249,13 +249,7 @@ static int march_rw(Engine *e, const SrcList<T> &sl) {
   MarchSources<T> ms;
   const bool inside = sl.n <= kMarchSources;
   ms.n = inside ? sl.n : 0;
-  ms.i0 = e->nx;
-  ms.i1 = 0;
-  for (int q = 0; q < ms.n; ++q) {
-    ms.s[q] = sl.s[q];
-    ms.i0 = std::min(ms.i0, sl.s[q].ci0);
-    ms.i1 = std::max(ms.i1, sl.s[q].ci1);
-  }
+  for (int q = 0; q < ms.n; ++q) ms.s[q] = sl.s[q];
   int rc = timing_mark(e);
   if (rc) return rc;
   const int o = e->cur ^ 1;
