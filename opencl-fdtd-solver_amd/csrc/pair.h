@@ -59,7 +59,6 @@ struct PairLoad {
 template <typename T>
 struct PairSources {
   int n[2];
-  int i0[2], i1[2];  // planes some source of the list covers
   Src<T> s[2][kMarchSources];
 };
 
@@ -569,7 +568,23 @@ struct Pair {
     L.e0[w + 1][1][lane] = e0[0][2];
     lds_barrier();
     const int ns0 = src.n[0], ns1 = src.n[1];
-    const int sa0 = src.i0[0], sa1 = src.i1[0], sb0 = src.i0[1], sb1 = src.i1[1];
+    // plane windows of the sources of each list that meet this tile's rows
+    // j0-2 ... j0+TJ+1 and columns k0-2 ... k0+61 (every slot's clamped cell
+    // lies there); the per-cell tests still decide, so a tile no source
+    // touches skips the source loops
+    int sa0 = g.nx, sa1 = 0, sb0 = g.nx, sb1 = 0;
+#pragma unroll
+    for (int t = 0; t < kMarchSources; ++t) {
+      const Src<T> &a = src.s[0][t], &b = src.s[1][t];
+      if (t < ns0 && a.cj0 <= j0 + TJ + 1 && a.cj1 > j0 - 2 && a.ck0 <= k0 + 61 && a.ck1 > k0 - 2) {
+        sa0 = min(sa0, a.ci0);
+        sa1 = max(sa1, a.ci1);
+      }
+      if (t < ns1 && b.cj0 <= j0 + TJ + 1 && b.cj1 > j0 - 2 && b.ck0 <= k0 + 61 && b.ck1 > k0 - 2) {
+        sb0 = min(sb0, b.ci0);
+        sb1 = max(sb1, b.ci1);
+      }
+    }
     // watched planes: those of the chunk only (a halo plane's step-1 values
     // are not all exact, and another block owns the plane)
     unsigned any = 0;

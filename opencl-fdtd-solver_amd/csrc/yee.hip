@@ -359,13 +359,7 @@ static int pair_t(Engine *e, const hfdtd_source *src_a, int na, const hfdtd_sour
   PairSources<T> ps{};
   for (int t = 0; t < 2; ++t) {
     ps.n[t] = l[t].n;
-    ps.i0[t] = e->nx;
-    ps.i1[t] = 0;
-    for (int q = 0; q < l[t].n; ++q) {
-      ps.s[t][q] = l[t].s[q];
-      ps.i0[t] = std::min(ps.i0[t], l[t].s[q].ci0);
-      ps.i1[t] = std::max(ps.i1[t], l[t].s[q].ci1);
-    }
+    for (int q = 0; q < l[t].n; ++q) ps.s[t][q] = l[t].s[q];
   }
   rc = e->lossless ? pair_shape<T, false>(e, ps) : pair_shape<T, true>(e, ps);
   if (rc) return rc;
