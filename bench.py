@@ -21,6 +21,15 @@ events on the engine's stream around every sweep launch of the timed steps.  ``c
 times this repository's NumPy restatement of the reference's face-CPML engine
 (one core) on a bounded sample of steps of the same grid.
 
+The timed window ends with the face-DFT buffer folded into the sums (the
+K = 32 matrix-core contractions of the window's samples all run inside it),
+and after it the near-to-far transform runs at all 8 frequencies
+(``n2f_ms``).  ``fp64_check`` runs the whole workload (``--accuracy-steps``,
+default the configuration's 2000 steps) once more on the same FP32 path and
+on the FP64 engine from the same inputs, and reports the peak-normalised
+error of every field, every box-DFT component and the far E and H against
+the FP64 run.
+
 ``roofline.traffic`` is the measured HBM traffic of the step's sweep
 kernels, taken from ``profiles/split_sweeps_256cube_hbm_traffic.txt``
 (``k_fused_256cube_hbm_traffic.txt``, ``k_march_256cube_hbm_traffic.txt``,
@@ -75,6 +84,7 @@ sys.path.insert(0, str(ROOT / "opencl-fdtd-solver_amd"))
 
 C0 = 299_792_458.0
 HBM_PEAK_GBS = 8000.0
+FIELDS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 
 
 def c2_epsilon(shape, npml, seed=2, n_boxes=40):
@@ -154,33 +164,94 @@ def run_gpu(args, wl: Workload):
     mon = hipfdtd.HIPNear2FarMonitor(sim, wl.center, wl.size, freqs=wl.freqs)
     assert (mon.ix0, mon.ix1) == wl.box, (mon.ix0, mon.ix1)
     sim.run(args.warmup)  # the pair path takes two steps per launch in run()
+    mon.flush()  # the window starts with an empty K-step buffer
     sim.synchronize()
     sim.kernel_timing(True)
     t0 = time.perf_counter()
     sim.run(args.steps)
+    # every face-DFT sample of the window is folded into the sums inside it:
+    # the K = 32 contractions of full blocks and one for the partial block
+    mon.flush()
     sim.synchronize()
     t1 = time.perf_counter()
     kern_ms, launches = sim.kernel_timing_read()
     sim.kernel_timing(False)
     print(f"[bench] timed {args.steps} steps in {t1 - t0:.3f} s", file=sys.stderr, flush=True)
 
+    obs = wl.obs_points()
     t2 = time.perf_counter()
-    eh = mon.get_farfields(wl.obs_points(), freq_index=wl.n_freq // 2)
+    eh = [mon.get_farfields(obs, freq_index=f) for f in range(wl.n_freq)]
     t3 = time.perf_counter()
     finite = bool(np.all(np.isfinite(eh)))
     return dict(wall_s=t1 - t0, kern_ms=kern_ms, launches=launches, n2f_ms=(t3 - t2) * 1e3,
                 finite=finite, device=sim.device_name, path=sim.path,
-                tuned=None if tuned is None else tuned["table"])  # fmt: skip
+                tuned=tuned)  # fmt: skip
 
 
-def run_cpu_baseline(wl: Workload, steps: int):
-    """NumPy face-CPML restatement (one core) on `steps` steps of the same grid."""
+def run_accuracy(wl: Workload, tuned, steps: int, path=None):
+    """The whole workload (``steps`` steps, box DFT, far field) on the FP32
+    engine the benchmark times (same path and launch shape) and on the FP64
+    engine, from the same inputs.  Peak-normalised errors of the FP32 run
+    against the FP64 one: the E and H fields after the last step (each group
+    normalised by its peak), the E and H components of the box DFT (worst of
+    the 8 frequencies), and the far E and H at 8 frequencies x 362 angles
+    (worst frequency); ``per_component`` normalises every field and DFT
+    component by its own peak instead."""
+    import hipfdtd
+    from hipfdtd.autotune import apply
+
+    obs = wl.obs_points()
+    out = {}
+    for dtype in (np.float32, np.float64):
+        sim = hipfdtd.HIPFDTD(wl.shape, wl.dl, npml=wl.npml, dtype=dtype,
+                              path=path if dtype == np.float32 else None)
+        sim.set_epsilon(wl.eps)
+        if dtype == np.float32 and tuned is not None:
+            apply(sim, tuned)
+        sim.add_source(wl.source)
+        mon = hipfdtd.HIPNear2FarMonitor(sim, wl.center, wl.size, freqs=wl.freqs)
+        sim.run(steps)
+        fields = sim.fields()
+        dft = {c: np.stack([mon.read_dft(c, f) for f in range(wl.n_freq)]) for c in FIELDS}
+        ff = np.stack([mon.get_farfields(obs, freq_index=f) for f in range(wl.n_freq)])
+        out[np.dtype(dtype).name] = (fields, dft, ff, sim.path)
+        sim.close()
+    (f32, d32, ff32, path32), (f64, d64, ff64, path64) = out["float32"], out["float64"]
+
+    def pn(a, b):
+        den = float(np.max(np.abs(b)))
+        return float(np.max(np.abs(a.astype(b.dtype) - b))) / den if den > 0 else 0.0
+
+    def pn_group(a, b, comps):
+        den = max(float(np.max(np.abs(b[c]))) for c in comps)
+        num = max(float(np.max(np.abs(a[c].astype(b[c].dtype) - b[c]))) for c in comps)
+        return num / den if den > 0 else 0.0
+
+    groups = {"E": FIELDS[:3], "H": FIELDS[3:]}
+    fields = {c: pn(f32[c], f64[c]) for c in FIELDS}
+    box = {c: max(pn(d32[c][f], d64[c][f]) for f in range(wl.n_freq)) for c in FIELDS}
+    fields_g = {g: pn_group(f32, f64, cs) for g, cs in groups.items()}
+    box_g = {g: max(pn_group({c: d32[c][f] for c in cs}, {c: d64[c][f] for c in cs}, cs)
+                    for f in range(wl.n_freq)) for g, cs in groups.items()}  # fmt: skip
+    far = {"E": max(pn(ff32[f][:, :3], ff64[f][:, :3]) for f in range(wl.n_freq)),
+           "H": max(pn(ff32[f][:, 3:], ff64[f][:, 3:]) for f in range(wl.n_freq))}  # fmt: skip
+    return {"steps": steps, "reference": "FP64 HIP engine, same inputs (path %s)" % path64,
+            "fp32_path": path32,
+            "per_component": {"fields": fields, "box_dft": box},
+            "fields": fields_g, "box_dft": box_g, "farfield": far,
+            "max": max(list(fields_g.values()) + list(box_g.values()) + list(far.values()))}  # fmt: skip
+
+
+def run_cpu_baseline(wl: Workload, steps: int, warmup: int = 2):
+    """NumPy face-CPML restatement (one core) on `steps` steps of the same
+    grid after `warmup` untimed ones."""
     from oracle import fdtd_oracle as fo
 
     sim = fo.OracleFDTD(wl.shape, wl.dl, npml=wl.npml)
     sim.set_epsilon(wl.eps)
     sim.add_source(wl.source)
-    sim.step()  # warm-up
+    for _ in range(warmup):
+        sim.step()
     t0 = time.perf_counter()
     for _ in range(steps):
         sim.step()
@@ -518,8 +589,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=2000,
+                    help="256^3 workload: steps of the FP32-vs-FP64 accuracy run (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", default="yee-256",
                     choices=["yee-256", "plane-dft", "adjoint", "batch"])
@@ -565,6 +639,7 @@ def main():
     # half a k_pair launch
     step_kern_s = res["kern_ms"] / args.steps / 1e3
     achieved = b_alg / step_kern_s / 1e9
+    traffic = measured_traffic(res["path"])
     kernel = KERNELS[res["path"]]
     out = {
         "metric": "Mcell-updates/s",
@@ -594,24 +669,36 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(res["path"]),
+            "traffic": traffic,
             "bytes_alg_per_step": b_alg,
             "sweep_us_per_step": round(step_kern_s * 1e6, 2),
             "launches_per_step": round(res["launches"] / args.steps, 3),
+            # achieved/frac count algorithmic bytes; these count the bytes
+            # the counters saw (traffic) over the same sweep time
+            "traffic_gbs": None if traffic is None else round(traffic / step_kern_s / 1e9, 1),
+            "traffic_frac": None if traffic is None
+            else round(traffic / step_kern_s / 1e9 / HBM_PEAK_GBS, 4),
         },
         "n2f_ms": round(res["n2f_ms"], 2),
+        "n2f_freqs": wl.n_freq,
         "farfield_finite": res["finite"],
         "device": res["device"],
     }
+    if args.accuracy_steps > 0:
+        out["fp64_check"] = run_accuracy(wl, res["tuned"], args.accuracy_steps,
+                                         None if args.path == "auto" else args.path)
+        print(f"[bench] FP32 vs FP64 after {args.accuracy_steps} steps: max peak-normalised "
+              f"error {out['fp64_check']['max']:.2e}", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        cpu_mcells, cpu_s = run_cpu_baseline(wl, args.cpu_steps)
+        cpu_mcells, cpu_s = run_cpu_baseline(wl, args.cpu_steps, args.cpu_warmup)
         out["cpu_baseline"] = {
             "value": round(cpu_mcells, 3),
             "unit": "Mcell/s",
             "cores": 1,
             "kind": "port",
-            "sample": f"{args.cpu_steps} steps of the same 256^3 workload (NumPy face-CPML "
-                      f"restatement, {cpu_s:.1f} s), host has {os.cpu_count()} CPUs",
+            "sample": f"{args.cpu_steps} steps after {args.cpu_warmup} warm-up steps of the same "
+                      f"256^3 workload (NumPy face-CPML restatement, {cpu_s:.1f} s), host has "
+                      f"{os.cpu_count()} CPUs",
         }
     print(json.dumps(out))
 

@@ -237,6 +237,10 @@ int hfdtd_n2f_farfield(hfdtd_n2f *m, int freq, const double *obs, int n_obs, dou
  * at most 16 frequencies).  Readers (snapshot, rel change, read, farfield)
  * fold a partial block in first. */
 int hfdtd_n2f_set_buffering(hfdtd_n2f *m, int K);
+/* folds the samples buffered so far into the sums now (asynchronous, on the
+ * engine's stream); the readers above do it themselves.  No counterpart in
+ * the reference, whose per-step sums need none. */
+int hfdtd_n2f_flush(hfdtd_n2f *m);
 
 /* ── point probe (device-side DFT of one field sample) ──────────────────── */
 

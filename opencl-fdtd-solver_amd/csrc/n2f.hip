@@ -673,6 +673,11 @@ int hfdtd_n2f_accumulate(hfdtd_n2f *m, const double *phase) {
   return accumulate_t<float, float, false>(m, phase);
 }
 
+int hfdtd_n2f_flush(hfdtd_n2f *m) {
+  if (!m) return fail(HFDTD_EINVAL, "NULL argument");
+  return n2f_flush(m);
+}
+
 int hfdtd_n2f_snapshot(hfdtd_n2f *m) {
   if (!m) return fail(HFDTD_EINVAL, "NULL argument");
   int rc = n2f_flush(m);

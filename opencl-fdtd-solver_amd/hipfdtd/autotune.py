@@ -103,6 +103,12 @@ def _time_engine(sim, steps, warmup):
     return ms / max(steps, 1) * 1e3
 
 
+def apply(sim, result) -> None:
+    """Apply a ``tune_engine`` result (path and launch shape) to another
+    engine of the same shape and medium kind."""
+    _apply(sim, result["path"], result["rows"], result["planes"])
+
+
 def _apply(sim, path, r, p):
     sim.set_path(path)
     if path == "fused":

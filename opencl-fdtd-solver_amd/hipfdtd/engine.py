@@ -428,12 +428,21 @@ class HIPFDTD(SourceMonitorMixin):
         if watch is None:
             return False
         lists = []
-        t0 = self.t
+        t0, n0 = self.t, self.step_num
+        # both steps' sources are recorded, never launched: whatever path ran
+        # the previous step, no H half of this launch has run yet
+        self._h_done = False
         try:
             self._deferring = self._pairing = True
+            t_s = t0
             for s in range(2):
                 self._pending = []
-                self.t = t0 + (s + 0.5) * self.dt
+                # the callbacks of step n0 + s see the time and step count
+                # step() shows them (t + dt/2, summed as step() sums it, and
+                # step_num n0 + s)
+                self.t = t_s + 0.5 * self.dt
+                self.step_num = n0 + s
+                t_s = t_s + self.dt
                 for src in self._sources:
                     src(self)
                 lists.append(self._pending)
@@ -441,6 +450,7 @@ class HIPFDTD(SourceMonitorMixin):
             self._deferring = self._pairing = False
             self._pending = []
             self.t = t0
+            self.step_num = n0
         if max(len(v) for v in lists) > _lib.PAIR_MAX_SOURCES:
             for v in lists:
                 self._single(v)
@@ -466,7 +476,7 @@ class HIPFDTD(SourceMonitorMixin):
             finally:
                 if view:
                     _lib.check(self._lib.hfdtd_set_view(self._handle, 0), "set_view")
-        self.t = t0 + 2 * self.dt
+        self.t = (t0 + self.dt) + self.dt
         self.step_num += 1
         for mon in self._monitors:
             mon(self)

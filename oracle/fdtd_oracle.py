@@ -29,10 +29,17 @@ How each part is pinned (see ``tests/test_oracle_*.py``):
   (``tests/test_unit_engine.py:183-243``);
 * face ψ ≡ volume ψ (an independent full-volume restatement below,
   ``numpy_engine.py:213-284``), σ = 0 ≡ lossless, and plane-wave energy
-  bounds with npml = 0.
-
-Lossy (σ > 0) time stepping is pinned only by invariants (σ = 0 identity,
-monotone attenuation): the reference holds no fixture for it.
+  bounds with npml = 0;
+* lossy (σ > 0) time stepping, for which the reference holds no fixture, by
+  closed forms (``tests/test_oracle_lossy.py``): a uniform field shrinks by
+  exactly ca per step and follows exp(-σt/ε) to the Crank–Nicolson error;
+  one E update from a hand-set H equals ca·E0 + cb·(CPML-weighted curl)
+  term by term in interior and PML cells; a plane wave in a lossy half-space
+  decays and turns as the discrete scheme's dispersion relation predicts.
+  Planted mistakes (ca dropped, ca on the curl term, the loss not halved,
+  the lossless ce for cb) fail these checks;
+* vacuum dispersion: a plane wave's phase per wavelength follows the Yee
+  relation sin(k dl/2) = sin(ω dt/2) dl/(c dt).
 """
 
 from __future__ import annotations

@@ -17,12 +17,21 @@ From-scratch NumPy restatement of the reference's Huygens-box monitor:
 The L-term sign is a parameter: the OpenCL kernel uses ``η0·N⊥ − r̂×L``
 (``dft_farfield.cl:617``), the NumPy monitor ``+ r̂×L`` (``monitors.py:171, 378``).
 Which one produced the committed far-field fixtures is settled by
-``tests/test_oracle_farfield.py`` (both are evaluated, only one matches).
+``tests/test_oracle_fixtures.py::test_oracle_matches_farfield_fixtures``: both
+are evaluated and only the "minus" form matches.
 
-Pinned by the reference's POCL fixtures ``tests/golden/*_farfield`` (19-angle
-|S| dB cut and the E/H vectors at +z and +x), by the phase recurrence against
-direct evaluation, and by the far-field of an analytic point-dipole current
-sheet (see the tests).
+Pinned by (all in ``tests/``):
+
+* the reference's POCL fixtures ``tests/golden/*_farfield`` (19-angle |S| dB
+  cut and the E/H vectors at +z and +x), ``test_oracle_fixtures.py``;
+* the phase recurrence against direct evaluation and the box snapping and
+  packing arithmetic, ``test_oracle_closed_forms.py``;
+* what ``l_sign = +1`` changes: with only H sampled on the faces both signs
+  agree, with only E sampled they are opposite,
+  ``test_oracle_closed_forms.py::test_l_sign_acts_on_the_magnetic_current_term_only``.
+
+The analytic far-field checks (dipole cos²θ cut, endfire null, Mie E-plane)
+run on the HIP engine (``tests/test_gpu_analytic.py``).
 """
 
 from __future__ import annotations

@@ -43,13 +43,19 @@ def _boxes_eps(shape, nvals, seed, dtype=np.float32):
     return eps.astype(dtype)
 
 
-def _run(path, shape, npml, eps, nsrc, steps, poke=None):
+def _run(path, shape, npml, eps, nsrc, steps, poke=None, rows=0, planes=0, materials=None):
+    """Fields after ``steps`` steps on ``path``; on the march path ``rows``
+    rows per wavefront (0: the engine's own choice, one row on indexed
+    grids) and ``planes`` planes per block (0: from the grid), and the
+    material index must hold ``materials`` values (0: off)."""
     sim = hip().HIPFDTD(shape, DL, npml=npml, path=path)
     if path == "march":
-        sim.set_march_tiling(2, 0, 0, -2)
+        sim.set_march_tiling(rows, planes, 0, -2)
     sim.set_epsilon(eps)
     if poke is not None:
         poke(sim)
+    if path == "march" and materials is not None:
+        assert sim.material_count() == materials, (sim.material_count(), materials)
     nx, ny, nz = shape
 
     def src(f):
@@ -69,17 +75,21 @@ def _same(a, b):
         np.testing.assert_array_equal(a[n], b[n], err_msg=n)
 
 
-@pytest.mark.parametrize("shape,npml,nvals,nsrc,dtype", [
-    ((45, 35, 100), 6, 5, 3, np.float32),
-    ((40, 52, 70), 8, 48, 4, np.float64),
-    ((40, 52, 70), 8, 49, 2, np.float32),
-    ((33, 40, 130), 0, 12, 6, np.float32)])  # fmt: skip
-def test_indexed_march_matches_fused(shape, npml, nvals, nsrc, dtype):
-    print(f"[mat] shape={shape} npml={npml} nvals={nvals} nsrc={nsrc} {np.dtype(dtype).name}",
-          flush=True)  # fmt: skip
+@pytest.mark.parametrize("rows", [0, 1, 2])
+@pytest.mark.parametrize("shape,npml,nvals,nsrc,dtype,indexed", [
+    ((45, 35, 100), 6, 5, 3, np.float32, True),
+    ((40, 52, 70), 8, 48, 4, np.float64, False),
+    ((40, 52, 70), 8, 48, 2, np.float32, True),
+    ((40, 52, 70), 8, 49, 2, np.float32, False),
+    ((33, 40, 130), 0, 12, 6, np.float32, True)])  # fmt: skip
+def test_indexed_march_matches_fused(shape, npml, nvals, nsrc, dtype, indexed, rows):
+    print(f"[mat] shape={shape} npml={npml} nvals={nvals} nsrc={nsrc} {np.dtype(dtype).name} "
+          f"rows={rows}", flush=True)  # fmt: skip
     eps = _boxes_eps(shape, nvals, 3, dtype)
     assert len(np.unique(eps)) == nvals
-    _same(_run("fused", shape, npml, eps, nsrc, 16), _run("march", shape, npml, eps, nsrc, 16))
+    want = nvals if indexed else 0
+    _same(_run("fused", shape, npml, eps, nsrc, 16),
+          _run("march", shape, npml, eps, nsrc, 16, rows=rows, materials=want))  # fmt: skip
 
 
 def test_hand_written_coefficients_turn_the_index_off():
@@ -94,12 +104,19 @@ def test_hand_written_coefficients_turn_the_index_off():
         sim.write_volume("ce_x", cex)
 
     print(f"[mat-poke] shape={shape}", flush=True)
-    _same(_run("fused", shape, 6, eps, 2, 12, poke), _run("march", shape, 6, eps, 2, 12, poke))
+    for rows in (0, 1, 2):
+        _same(_run("fused", shape, 6, eps, 2, 12, poke),
+              _run("march", shape, 6, eps, 2, 12, poke, rows=rows, materials=0))  # fmt: skip
 
 
-def test_indexed_march_full_size():
-    """The benchmark's kind of medium on its grid: 40 boxes on vacuum, 256^3."""
+@pytest.mark.parametrize("rows,planes", [(0, 0), (1, 24), (2, 24)])
+def test_indexed_march_full_size(rows, planes):
+    """The benchmark's kind of medium on its grid: 40 boxes on vacuum, 256^3,
+    with the engine's own tiling, the tiling the benchmark's autotuner picks
+    (one row, 24 planes) and two rows."""
     shape = (256, 256, 256)
     eps = _boxes_eps(shape, 41, 2)
-    print(f"[mat-full] shape={shape} nvals={len(np.unique(eps))}", flush=True)
-    _same(_run("fused", shape, 12, eps, 1, 6), _run("march", shape, 12, eps, 1, 6))
+    print(f"[mat-full] shape={shape} nvals={len(np.unique(eps))} rows={rows} planes={planes}",
+          flush=True)  # fmt: skip
+    _same(_run("fused", shape, 12, eps, 1, 6),
+          _run("march", shape, 12, eps, 1, 6, rows=rows, planes=planes, materials=41))  # fmt: skip

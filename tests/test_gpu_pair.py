@@ -183,3 +183,57 @@ def test_pair_full_size_matches_march():
     print(f"[pair-full] shape={shape}", flush=True)
     r = _pair_and_march(shape, 12, False, 2, 6)
     _assert_fields_equal(r["march"][0], r["pair"][0])
+
+
+@pytest.mark.parametrize("first", ["split", "march-read"])
+def test_switch_to_pair_after_other_steps_matches_march(first):
+    """Steps on the split path, or march steps whose callback reads a field,
+    leave the H half of their step done; after set_path('pair') both steps'
+    sources must still reach the pair sweep.  Both engines run the same
+    first steps, then one goes on with the march sweep and one with the
+    pair sweep; bitwise equal."""
+    shape, npml = (40, 36, 72), 6
+    print(f"[pair-switch] first={first} shape={shape}", flush=True)
+    eps, _ = _media(shape, 5, False)
+    out = []
+    for then in ("march", "pair"):
+        sim = hip().HIPFDTD(shape, 1e-3, npml=npml, path="march")
+        sim.set_epsilon(eps)
+
+        def src(f):
+            a = fo.gaussian_sine(f.t, 3e10, 1.5e10)
+            if first == "march-read" and f.step_num < 3:
+                f.read_point("Ex", 20, 18, 36)  # runs the H half before the sources
+            f.add_source_J("y", a, (20, 21, 18, 19, 30, 31))
+            f.add_source_Jx(36, 0.5 * a, 10, 30, 9, 27, rim_taper=True)
+
+        sim.add_source(src)
+        sim.set_path("split" if first == "split" else "march")
+        sim.run(3)
+        sim.set_path(then)
+        assert sim.path == then
+        sim.run(10)
+        out.append(sim)
+    _assert_fields_equal(out[0], out[1])
+
+
+def test_pair_callbacks_see_step_num():
+    """A source keyed on step_num gets the same inputs on the pair path as on
+    the march path (the second step of a pair sees step_num n + 1)."""
+    shape = (32, 30, 40)
+    print(f"[pair-step-num] shape={shape}", flush=True)
+    res, seen = {}, {}
+    for path in ("march", "pair"):
+        sim = hip().HIPFDTD(shape, 1e-3, npml=4, path=path)
+        log = []
+
+        def src(f, log=log):
+            log.append((f.step_num, f.t))
+            f.add_source_J("z", 1.0 + (f.step_num % 3), (16, 17, 15, 16, 20, 21))
+
+        sim.add_source(src)
+        sim.run(9)
+        res[path], seen[path] = sim, log
+    assert seen["pair"] == seen["march"]
+    assert [n for n, _ in seen["march"]] == list(range(9))
+    _assert_fields_equal(res["march"], res["pair"])

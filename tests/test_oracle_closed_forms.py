@@ -181,3 +181,23 @@ def test_box_current_source_closed_form():
     assert sim.Ey[3, 3, 2] == pytest.approx(base, rel=1e-6)          # interior of the box
     assert sim.Ey[2, 3, 2] == pytest.approx(0.5 * base, rel=1e-6)    # i edge
     assert sim.Ey[2, 3, 1] == pytest.approx(0.25 * base, rel=1e-6)   # i and k edges
+
+
+def test_l_sign_acts_on_the_magnetic_current_term_only():
+    """E = pref (η0 N⊥ + l r̂×L): with only H sampled on the faces L = 0 and
+    both signs agree; with only E sampled N = 0 and they are opposite.  The
+    POCL fixtures pin the l = -1 form; this pins what l = +1 changes."""
+    sim = fo.OracleFDTD((20, 20, 20), 1e-3, npml=3)
+    mon = OracleNear2Far(sim, (0.01, 0.01, 0.01), (0.008, 0.008, 0.008), 3e9)
+    rng = np.random.default_rng(4)
+    pts = rng.normal(size=(7, 3))
+    for c in mon.acc:
+        mon.acc[c][:] = rng.normal(size=mon.acc[c].shape)
+    full = {c: v.copy() for c, v in mon.acc.items()}
+    for keep, sign in (("H", 1.0), ("E", -1.0)):
+        for c in mon.acc:
+            mon.acc[c][:] = full[c] if c[0] == keep else 0.0
+        a = mon.farfields(pts, l_sign=+1.0)
+        b = mon.farfields(pts, l_sign=-1.0)
+        assert np.max(np.abs(a)) > 0
+        np.testing.assert_allclose(a, sign * b, rtol=1e-13, atol=1e-13 * np.max(np.abs(a)))
