@@ -76,16 +76,18 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("rows", [0, 1, 2])
-@pytest.mark.parametrize("shape,npml,nvals,nsrc,dtype,indexed", [
+@pytest.mark.parametrize("shape,npml,nvals,nsrc,eps_dtype,indexed", [
     ((45, 35, 100), 6, 5, 3, np.float32, True),
-    ((40, 52, 70), 8, 48, 4, np.float64, False),
+    ((40, 52, 70), 8, 48, 4, np.float64, True),
     ((40, 52, 70), 8, 48, 2, np.float32, True),
     ((40, 52, 70), 8, 49, 2, np.float32, False),
     ((33, 40, 130), 0, 12, 6, np.float32, True)])  # fmt: skip
-def test_indexed_march_matches_fused(shape, npml, nvals, nsrc, dtype, indexed, rows):
-    print(f"[mat] shape={shape} npml={npml} nvals={nvals} nsrc={nsrc} {np.dtype(dtype).name} "
-          f"rows={rows}", flush=True)  # fmt: skip
-    eps = _boxes_eps(shape, nvals, 3, dtype)
+def test_indexed_march_matches_fused(shape, npml, nvals, nsrc, eps_dtype, indexed, rows):
+    """FP32 engines; ``eps_dtype`` is the dtype of the εr array handed to
+    set_epsilon (both are indexed when at most 48 values occur)."""
+    print(f"[mat] shape={shape} npml={npml} nvals={nvals} nsrc={nsrc} "
+          f"eps {np.dtype(eps_dtype).name} rows={rows}", flush=True)  # fmt: skip
+    eps = _boxes_eps(shape, nvals, 3, eps_dtype)
     assert len(np.unique(eps)) == nvals
     want = nvals if indexed else 0
     _same(_run("fused", shape, npml, eps, nsrc, 16),
@@ -120,3 +122,14 @@ def test_indexed_march_full_size(rows, planes):
           flush=True)  # fmt: skip
     _same(_run("fused", shape, 12, eps, 1, 6),
           _run("march", shape, 12, eps, 1, 6, rows=rows, planes=planes, materials=41))  # fmt: skip
+
+
+def test_fp64_engine_has_no_index():
+    shape = (45, 35, 100)
+    eps = _boxes_eps(shape, 5, 3)
+    sim = hip().HIPFDTD(shape, DL, npml=6, path="march", dtype=np.float64)
+    sim.set_epsilon(eps)
+    assert sim.material_count() == 0
+    sim2 = hip().HIPFDTD(shape, DL, npml=6, path="march")
+    sim2.set_epsilon(eps, np.zeros(shape, np.float32))  # lossy: ca volumes, no index
+    assert sim2.material_count() == 0
