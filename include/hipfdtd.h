@@ -229,6 +229,20 @@ int hfdtd_n2f_read(hfdtd_n2f *m, int comp, int freq, double *host);
  * geom_fp64 selects double (1) or the field dtype (0) for r.r' phases. */
 int hfdtd_n2f_farfield(hfdtd_n2f *m, int freq, const double *obs, int n_obs, double k_wave,
                        double eta0, int l_sign, int geom_fp64, double *eh);
+/* No counterpart in the reference.  hfdtd_n2f_farfield for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it): same arguments, kernels and results, but
+ * the partial sums live in a library-owned static device array of 1048576
+ * doubles (8 MiB, held on every device the library runs on until the process
+ * ends) instead of a buffer allocated at the first call.  The call needs
+ * n_obs * (12 * ceil(n_face / 2048) + 15) doubles of it and returns
+ * HFDTD_EINVAL beyond that (the 256^3 benchmark's 362 points over its
+ * 221184 face samples need 474552).  It calls no allocator.  All calls on one
+ * device share the array: calls that overlap on one device (other engines'
+ * streams, other threads) race and must be serialised by the caller. */
+int hfdtd_n2f_farfield_noalloc(hfdtd_n2f *m, int freq, const double *obs, int n_obs,
+                               double k_wave, double eta0, int l_sign, int geom_fp64,
+                               double *eh);
 
 /* K-step buffering of the face DFT (extension; per-step accumulation is the
  * reference's monitors.py:571-624).  K = 0: per-step sums; K even, 2..256:

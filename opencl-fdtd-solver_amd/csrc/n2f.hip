@@ -10,6 +10,7 @@
 // deterministic two-stage reduction in FP64 instead of float CAS atomics.
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -562,16 +563,26 @@ static void free_buffering(hfdtd_n2f *m) {
   m->K = m->kfill = 0;
 }
 
+constexpr int kFarfieldChunk = 2048;  // face samples per stage-1 block
+
+// Doubles of scratch one far-field call needs: the stage-1 partials
+// (12 per observation and chunk of face samples), the observation points
+// and the E/H results.
+static size_t farfield_scratch_doubles(const hfdtd_n2f *m, int n_obs) {
+  const size_t nblk = std::max<size_t>(1, ceil_div(m->n_face, kFarfieldChunk));
+  return (size_t)n_obs * nblk * 12 + (size_t)n_obs * 3 + (size_t)n_obs * 12;
+}
+
+// The far field into ``eh`` with ``scratch`` (at least
+// farfield_scratch_doubles doubles of device memory); every word of it the
+// kernels read is written first in the same call.
 template <typename G, typename A>
 static int farfield_t(hfdtd_n2f *m, int fr, const double *obs, int n_obs, double k_wave,
-                      double eta0, int l_sign, double *eh) {
+                      double eta0, int l_sign, double *eh, double *scratch) {
   Engine *e = m->e;
-  const int chunk = 2048;
+  const int chunk = kFarfieldChunk;
   const int nblk = std::max(1, (int)ceil_div(m->n_face, chunk));
-  const size_t need = (size_t)n_obs * nblk * 12 + (size_t)n_obs * 3 + (size_t)n_obs * 12;
-  int rc = ensure_scratch(m, need);
-  if (rc) return rc;
-  double *part = m->scratch;
+  double *part = scratch;
   double *d_obs = part + (size_t)n_obs * nblk * 12;
   double *d_eh = d_obs + (size_t)n_obs * 3;
   HF_HIP(hipMemcpyAsync(d_obs, obs, (size_t)n_obs * 3 * sizeof(double), hipMemcpyHostToDevice,
@@ -586,6 +597,39 @@ static int farfield_t(hfdtd_n2f *m, int fr, const double *obs, int n_obs, double
                         e->stream));
   HF_HIP(hipStreamSynchronize(e->stream));
   return HFDTD_OK;
+}
+
+// Scratch of hfdtd_n2f_farfield_noalloc: one copy per device, held for the
+// life of the process.  Its name stands for the device array only in device
+// code; host code uses the address hipGetSymbolAddress returns for the
+// current device.
+constexpr size_t kNoallocScratchDoubles = size_t(1) << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of ``scratch`` on ``device`` (looked up once per device,
+// with that device current).
+static int noalloc_scratch(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(HFDTD_EUNSUPPORTED, "device ordinal too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    HF_HIP(hipSetDevice(device));
+    void *ptr = nullptr;
+    HF_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = (double *)ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return HFDTD_OK;
+}
+
+template <typename Run>
+static int farfield_dispatch(hfdtd_n2f *m, int freq, int geom_fp64, Run run) {
+  const bool g64 = geom_fp64 || m->e->fp64;
+  if (m->acc_elem == 8) return g64 ? run(double{}, double{}) : run(float{}, double{});
+  return g64 ? run(double{}, float{}) : run(float{}, float{});
 }
 
 }  // namespace hfdtd
@@ -758,12 +802,35 @@ int hfdtd_n2f_farfield(hfdtd_n2f *m, int freq, const double *obs, int n_obs, dou
   if (rc) return rc;
   Engine *e = m->e;
   HF_HIP(hipSetDevice(e->device));
-  const bool g64 = geom_fp64 || e->fp64;
-  if (m->acc_elem == 8)
-    return g64 ? farfield_t<double, double>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh)
-               : farfield_t<float, double>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh);
-  return g64 ? farfield_t<double, float>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh)
-             : farfield_t<float, float>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh);
+  rc = ensure_scratch(m, farfield_scratch_doubles(m, n_obs));
+  if (rc) return rc;
+  return farfield_dispatch(m, freq, geom_fp64, [&](auto g, auto a) {
+    return farfield_t<decltype(g), decltype(a)>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh,
+                                                m->scratch);
+  });
+}
+
+int hfdtd_n2f_farfield_noalloc(hfdtd_n2f *m, int freq, const double *obs, int n_obs,
+                               double k_wave, double eta0, int l_sign, int geom_fp64,
+                               double *eh) {
+  if (!m || !obs || !eh) return fail(HFDTD_EINVAL, "NULL argument");
+  if (n_obs < 1) return fail(HFDTD_EINVAL, "n_obs must be >= 1");
+  if (freq < 0 || freq >= m->nfreq) return fail(HFDTD_EINVAL, "frequency index out of range");
+  if (l_sign != 1 && l_sign != -1) return fail(HFDTD_EINVAL, "l_sign must be +1 or -1");
+  if (farfield_scratch_doubles(m, n_obs) > kNoallocScratchDoubles)
+    return fail(HFDTD_EINVAL,
+                "n_obs * (12 * ceil(n_face / 2048) + 15) exceeds the 1048576 doubles of the "
+                "allocation-free far field's scratch");
+  double *buf = nullptr;
+  int rc = noalloc_scratch(m->e->device, &buf);
+  if (rc) return rc;
+  rc = n2f_flush(m);
+  if (rc) return rc;
+  HF_HIP(hipSetDevice(m->e->device));
+  return farfield_dispatch(m, freq, geom_fp64, [&](auto g, auto a) {
+    return farfield_t<decltype(g), decltype(a)>(m, freq, obs, n_obs, k_wave, eta0, l_sign, eh,
+                                                buf);
+  });
 }
 
 int hfdtd_n2f_set_buffering(hfdtd_n2f *m, int K) {

@@ -115,6 +115,11 @@ EXPORTS = {
         [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
          C.c_void_p],
     ),  # fmt: skip
+    "hfdtd_n2f_farfield_noalloc": (
+        C.c_int,
+        [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+         C.c_void_p],
+    ),  # fmt: skip
     "hfdtd_probe_create": (
         C.c_int,
         [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)],
