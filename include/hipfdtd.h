@@ -255,6 +255,13 @@ int hfdtd_n2f_set_buffering(hfdtd_n2f *m, int K);
  * engine's stream); the readers above do it themselves.  No counterpart in
  * the reference, whose per-step sums need none. */
 int hfdtd_n2f_flush(hfdtd_n2f *m);
+/* No counterpart in the reference.  1 in *on when this monitor's z faces are
+ * recorded by the engine's march sweep: with K-step buffering on a float32
+ * engine, the first such monitor of the engine whose z faces lie at k >= 1
+ * and at least two cells apart; its gather then reads the two z faces from
+ * that record (the same values, contiguous) after march steps, and from the
+ * fields after any other change of the state. */
+int hfdtd_n2f_zrecord(const hfdtd_n2f *m, int *on);
 
 /* ── point probe (device-side DFT of one field sample) ──────────────────── */
 

@@ -631,6 +631,7 @@ int hfdtd_adjoint_save(hfdtd_adjoint *a, int c) {
 int hfdtd_adjoint_restore(hfdtd_adjoint *a, int c) {
   if (!a || c < 0 || c >= a->n_ckpt) return fail(HFDTD_EINVAL, "checkpoint index out of range");
   Engine *e = a->e;
+  e->touch();
   HF_HIP(hipSetDevice(e->device));
   const char *src = (const char *)a->ckpt + (size_t)c * (a->ckpt_state + a->ckpt_psi);
   HF_HIP(hipMemcpyAsync(e->state[e->cur], src, a->ckpt_state, hipMemcpyDeviceToDevice,
@@ -683,6 +684,7 @@ int hfdtd_adjoint_begin(hfdtd_adjoint *a, int64_t n_last) {
   if (!a || !a->np || !a->coef_set)
     return fail(HFDTD_EINVAL, "set the probes and the objective gradient first");
   Engine *e = a->e;
+  e->touch();
   if (!e->lossless)
     return fail(HFDTD_EUNSUPPORTED, "the adjoint gradient needs a lossless grid (no sigma)");
   HF_HIP(hipSetDevice(e->device));
@@ -704,6 +706,7 @@ int hfdtd_adjoint_reverse(hfdtd_adjoint *a, int64_t n_hi, int64_t n_lo, int slot
   if (slot_hi >= a->hist_slots || slot_hi - (n_hi - n_lo) < 0)
     return fail(HFDTD_EINVAL, "history slots do not cover the reverse range");
   Engine *e = a->e;
+  e->touch();
   HF_HIP(hipSetDevice(e->device));
   for (long long n = n_hi; n > n_lo; --n) {
     const int slot = (int)(slot_hi - (n_hi - n));

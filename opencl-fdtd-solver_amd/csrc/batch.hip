@@ -661,7 +661,10 @@ int hfdtd_batch_run(hfdtd_batch *b, int n_steps, int graph_steps) {
     return fail(HFDTD_EINVAL, "run goes past the end of the source/probe tables");
   if (b->nfreq > 0 && b->d_probe_acc == nullptr) return fail(HFDTD_EINVAL, "probes not set");
   HF_HIP(hipSetDevice(b->eng[0]->device));
-  for (Engine *e : b->eng) HF_HIP(hipStreamSynchronize(e->stream));
+  for (Engine *e : b->eng) {
+    HF_HIP(hipStreamSynchronize(e->stream));
+    e->touch();
+  }
   return b->fp64 ? batch_run_t<double>(b, n_steps, graph_steps)
                  : batch_run_t<float>(b, n_steps, graph_steps);
 }

@@ -41,6 +41,27 @@ constexpr int kPsiAxis[PSI_COUNT] = {0, 0, 1, 1, 2, 2, 0, 0, 1, 1, 2, 2};
 
 constexpr int kMatMax = 48;  // materials of an indexed grid at most (march sweep)
 
+// The z-face record of the march sweep.  A Huygens-box monitor whose
+// z faces lie at k = kf[0] and kf[1] (each at least 1, two or more apart)
+// asks the engine to keep, for every i in [i0, i1] and j in [j0, j1] (its
+// face cells and their +1 neighbours), the values its co-location reads
+// there: (Ex, Ey) at k and (Hx, Hy) at k and k - 1.  March steps write
+// (Ex, Ey, Hx, Hy) of both columns as float quads into buf,
+// [face][column: k - 1 | k][i - i0][j - j0]; the monitor's gather then reads
+// them contiguously instead of one cache line per sample and field.  The
+// record matches the state while `epoch` equals the engine's state_epoch,
+// which every change of the state advances.
+struct ZRec {
+  const void *owner = nullptr;  // the monitor it serves (nullptr: off)
+  int kf[2] = {};
+  int i0 = 0, i1 = -1, j0 = 0, j1 = -1;
+  float *buf = nullptr;
+  size_t bytes = 0;
+  uint64_t epoch = ~0ull;
+  // quads per (face, column) array
+  size_t quads() const { return (size_t)(i1 - i0 + 1) * (size_t)(j1 - j0 + 1); }
+};
+
 struct Engine {
   int nx = 0, ny = 0, nz = 0, npml = 0;
   int fp64 = 0;
@@ -97,6 +118,15 @@ struct Engine {
   bool mat_ok = false;
 
   uint64_t bytes = 0;
+
+  // Advanced by every call that changes E, H or psi (or what field() reads);
+  // a march step that writes the z-face record stamps it with the new value.
+  uint64_t state_epoch = 0;
+  ZRec zrec;
+  void touch() { ++state_epoch; }
+  bool zrec_valid(const void *owner) const {
+    return zrec.owner == owner && zrec.buf && zrec.epoch == state_epoch && !view_shadow;
+  }
 
   // HIP-event timing of the Yee sweeps (hfdtd_timing / hfdtd_timing_read):
   // events are recorded on `stream` around every sweep launch while enabled.

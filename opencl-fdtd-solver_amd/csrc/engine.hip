@@ -176,6 +176,11 @@ static void release(Engine *e) {
     e->mtab = nullptr;
   }
   e->mat_ok = false;
+  {
+    void *t = e->zrec.buf;
+    fr(t);
+    e->zrec = ZRec{};
+  }
   for (hipEvent_t ev : e->events) (void)hipEventDestroy(ev);
   e->events.clear();
   e->events_used = 0;
@@ -513,6 +518,7 @@ int hfdtd_write_volume(hfdtd_engine *h, int which, const void *host) {
   if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
   if (which >= HFDTD_CEX) e->mat_ok = false;  // coefficients no longer from the index
   HF_HIP(hipSetDevice(e->device));
+  if (which <= HFDTD_HZ) e->touch();
   HF_HIP(hipMemcpyAsync(slot, host, e->ncell * e->elem, hipMemcpyHostToDevice, e->stream));
   HF_HIP(hipStreamSynchronize(e->stream));
   if (which >= HFDTD_CAX) e->lossless = false;
@@ -557,6 +563,7 @@ int hfdtd_reset_fields(hfdtd_engine *h) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = &h->s;
   HF_HIP(hipSetDevice(e->device));
+  e->touch();
   int rc;
   if ((rc = fill_value(e, e->state[e->cur], 6 * e->vstride, 0.0))) return rc;
   if ((rc = fill_value(e, e->psi[e->cur], e->psi_off[PSI_COUNT], 0.0))) return rc;
@@ -625,6 +632,7 @@ int hfdtd_set_view(hfdtd_engine *h, int shadow) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
   if (shadow && !h->s.shadow) return fail(HFDTD_EINVAL, "no watched boxes: no shadow slab");
   h->s.view_shadow = shadow != 0;
+  h->s.touch();  // field() now reads another slab
   return HFDTD_OK;
 }
 

@@ -90,6 +90,31 @@ __device__ __forceinline__ Src<T> lds_src(const Src<T> *base, int t) {
 // resource's range and are dropped, loads return 0, neither touches memory.
 constexpr unsigned kDropOffset = 0x80000000u;
 
+// The engine's z-face record (ZRec in common.h) as a sweep argument; a
+// null buf records nothing.  Blocks whose columns hold a face's k or k - 1
+// (and whose rows and planes meet the record) run the recording variant:
+// after its E update each owned cell there stores its new (Ex, Ey, Hx, Hy)
+// with one 128-bit store.  Lanes and planes that record nothing get offsets
+// past the record's resource, whose size is exact (below 1 GB): their
+// stores are dropped, and no sum of the two drop offsets wraps back into
+// range.
+struct ZCap {
+  float *buf;
+  int kf0, kf1;  // the faces' k
+  int i0, i1, j0, j1;
+  unsigned bytes;
+};
+constexpr unsigned kCapDropLane = 0x80000000u;
+constexpr unsigned kCapDropPlane = 0x40000000u;
+
+__device__ __forceinline__ void bst_quad(Rsrc r, float a, float b, float c, float d, unsigned voff,
+                                         unsigned soff) {
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  const u4 v = {__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
+                __builtin_bit_cast(unsigned, c), __builtin_bit_cast(unsigned, d)};
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
+}
+
 // LDS of a block: new H and E of the planes, double-buffered by plane
 // parity, on a grid of rows j0-1 ... j0+4RW (then a trash row and one spare
 // row, so neighbour reads from the trash row stay inside the array) by
@@ -170,7 +195,8 @@ struct PsiPlane {
   bool dead;  // a plane past the chunk: its loads are dropped
 };
 
-template <typename T, int RW, bool LOSSY, bool X, bool Y, bool Z, bool NT, bool MAT = false>
+template <typename T, int RW, bool LOSSY, bool X, bool Y, bool Z, bool NT, bool MAT = false,
+          bool CAP = false>
 struct March {
   static constexpr int S = RW + 1;  // RW owned rows + the halo slot
   static constexpr int TJ = 4 * RW;
@@ -193,6 +219,15 @@ struct March {
   Prof1<T> pyh[S], pye[S], pzh[S], pze[S];
   T hyc[RW], hzc[RW];  // new H(i-1) at the owned cells
   T oe[S][3];          // E(i) at every slot (copied from the loads of plane i-1)
+  // the halo slot's per-lane offsets of its six field loads (E at i+1, old H
+  // at i) and of its x-stripe psi loads: a lane loads only what its role
+  // uses (see run), the others get kDropOffset and move no data
+  unsigned hoff[6], hoffx;
+  // z-face record (CAP): its resource and extents, and the per-lane byte
+  // offset of an owned cell's quad in it
+  Rsrc rcap;
+  ZCap zc;
+  unsigned zo[RW];
   unsigned padb;       // store offset of lanes and planes outside the grid
   int iend;            // first plane past this block's chunk
   bool sized;          // slab resources carry their sizes (slabs below 2 GB)
@@ -229,16 +264,16 @@ struct March {
   // at one valid element instead would send every wave of the grid to the
   // same cache line.
   __device__ __forceinline__ void load_h(MarchH<T> &s, const MarchSlot &m, unsigned pl,
-                                         unsigned pl1, const PsiPlane &pp) const {
+                                         unsigned pl1, const PsiPlane &pp, bool halo) const {
     const unsigned vs = vsb();
-    s.ex = bld<T, false>(r0, m.o4, pl1);
-    s.ey = bld<T, false>(r0, m.o4, vs + pl1);
-    s.ez = bld<T, false>(r0, m.o4, 2 * vs + pl1);
-    s.hx = bld<T, false>(r0, m.o4, 3 * vs + pl);
-    s.hy = bld<T, false>(r0, m.o4, 4 * vs + pl);
-    s.hz = bld<T, false>(r0, m.o4, 5 * vs + pl);
+    s.ex = bld<T, false>(r0, halo ? hoff[0] : m.o4, pl1);
+    s.ey = bld<T, false>(r0, halo ? hoff[1] : m.o4, vs + pl1);
+    s.ez = bld<T, false>(r0, halo ? hoff[2] : m.o4, 2 * vs + pl1);
+    s.hx = bld<T, false>(r0, halo ? hoff[3] : m.o4, 3 * vs + pl);
+    s.hy = bld<T, false>(r0, halo ? hoff[4] : m.o4, 4 * vs + pl);
+    s.hz = bld<T, false>(r0, halo ? hoff[5] : m.o4, 5 * vs + pl);
     if constexpr (X) {
-      const unsigned vx = pp.inx && !pp.dead ? m.o4 : kDropOffset;
+      const unsigned vx = pp.inx && !pp.dead ? (halo ? hoffx : m.o4) : kDropOffset;
       s.p[0] = bld<T, false>(rq0, vx, psib(0, pp.xs));
       s.p[1] = bld<T, false>(rq0, vx, psib(1, pp.xs));
     }
@@ -252,6 +287,13 @@ struct March {
       s.p[4] = bld<T, false>(rq0, vz, psib(4, pp.zs));
       s.p[5] = bld<T, false>(rq0, vz, psib(5, pp.zs));
     }
+  }
+
+  // plane part of a record store: dropped outside [i0, i1] and past the chunk
+  __device__ __forceinline__ unsigned cap_plane(int i, bool live) const {
+    return live && i >= zc.i0 && i <= zc.i1
+               ? (unsigned)(i - zc.i0) * (unsigned)(zc.j1 - zc.j0 + 1) * 16u
+               : kCapDropPlane;
   }
 
   __device__ __forceinline__ unsigned mat_plane(int i, bool dead) const {
@@ -296,7 +338,7 @@ struct March {
     const PsiPlane pp = psi_plane(i);
     const unsigned pl = load_plane(i, pp.dead), pl1 = load_plane(i + 1, pp.dead);
 #pragma unroll
-    for (int q = 0; q < S; ++q) load_h(s[q], sl[q], pl, pl1, pp);
+    for (int q = 0; q < S; ++q) load_h(s[q], sl[q], pl, pl1, pp, q == RW);
   }
 
   __device__ __forceinline__ void load_e_all(MarchE<T> (&s)[RW], int i) const {
@@ -486,6 +528,7 @@ struct March {
         psi_st(p[4], 10, ok && m.inz, m.qz4 + pp.zs * E);
         psi_st(p[5], 11, ok && m.inz, m.qz4 + pp.zs * E);
       }
+      if constexpr (CAP) bst_quad(rcap, ex, ey, h[q][0], h[q][1], zo[q], cap_plane(i, live));
       hyc[q] = h[q][1];
       hzc[q] = h[q][2];
     }
@@ -511,6 +554,11 @@ struct March {
     // j0-1+3w ...); lane 51 of wave 0 E only at (j0+TJ, k0-1); the rest idle.
     // H of E-only lanes and both of idle lanes go to the trash row.
     constexpr int trash_row = TJ + 2;
+    // fields a halo lane's role reads, bit c for Ex Ey Ez Hx Hy Hz: H on row
+    // j0-1 is needed for Hx and Hz (E's j-1 neighbours), H on column k0-1
+    // for Hx and Hy (E's k-1 neighbours), both from all of E; E-only lanes
+    // need the components their H neighbours read across the tile edge
+    unsigned need = 0;
 #pragma unroll
     for (int q = 0; q <= RW; ++q) {
       int j, k, hr, hc, er, ec;
@@ -526,11 +574,13 @@ struct March {
         k = k0 + 16 * w + lane;
         hr = er = 0;
         hc = ec = 16 * w + lane + 1;
+        need = 0b101111;  // E, Hx, Hz
       } else if (lane < 16 + RW) {
         j = j0 + RW * w + lane - 16;
         k = k0 - 1;
         hr = er = RW * w + lane - 16 + 1;
         hc = ec = 0;
+        need = 0b011111;  // E, Hx, Hy
       } else if (lane >= 32 && lane < 48) {
         j = j0 + TJ;
         k = k0 + 16 * w + lane - 32;
@@ -538,6 +588,7 @@ struct March {
         hc = lane;
         er = TJ + 1;
         ec = 16 * w + lane - 32 + 1;
+        need = 0b000101;  // Ex, Ez: j+1 neighbours of Hz, Hx on row j0+TJ-1
       } else if (lane >= 48 && lane < 51 && 3 * w + lane - 48 <= TJ) {
         j = j0 - 1 + 3 * w + lane - 48;
         k = k0 + 64;
@@ -545,6 +596,7 @@ struct March {
         hc = lane;
         er = 3 * w + lane - 48;
         ec = 65;
+        need = 0b000011;  // Ex, Ey: k+1 neighbours of Hy, Hx on column k0+63
       } else if (lane == 51 && w == 0) {
         j = j0 + TJ;
         k = k0 - 1;
@@ -552,6 +604,7 @@ struct March {
         hc = lane;
         er = TJ + 1;
         ec = 0;
+        need = 0b000100;  // Ez: j+1 neighbour of Hx at (j0+TJ-1, k0-1)
       } else {
         j = j0;
         k = k0;
@@ -565,13 +618,26 @@ struct March {
       m.o1 = (unsigned)(m.j * g.nz + m.k);
       m.o1j = (unsigned)(min(m.j + 1, g.ny - 1) * g.nz + m.k);
       m.o1k = (unsigned)(m.j * g.nz + min(m.k + 1, g.nz - 1));
-      m.iny = Y && n > 0 && (m.j < n || m.j >= g.ny - n);
-      m.inz = Z && n > 0 && (m.k < n || m.k >= g.nz - n);
+      // the halo slot's psi only for lanes that compute a needed H
+      const bool hpsi = q < RW || (need & 0b011000) != 0;
+      m.iny = Y && n > 0 && hpsi && (m.j < n || m.j >= g.ny - n);
+      m.inz = Z && n > 0 && hpsi && (m.k < n || m.k >= g.nz - n);
       m.qy4 = m.iny ? (unsigned)(stripe_local(m.j, n, g.ny) * g.nz + m.k) * E : kDropOffset;
       m.qz4 = m.inz ? (unsigned)(m.j * 2 * n + stripe_local(m.k, n, g.nz)) * E : kDropOffset;
       m.lh = hr * kMarchCols + hc;
       m.le = er * kMarchCols + ec;
       m.own = own;
+      if constexpr (CAP) {
+        if (q < RW) {
+          // record slot 2 f + 1 for the face's k, 2 f for its k - 1
+          const int f = (k == zc.kf0 || k == zc.kf0 - 1) ? 0 : (k == zc.kf1 || k == zc.kf1 - 1) ? 1 : -1;
+          const int kf = f == 0 ? zc.kf0 : zc.kf1;
+          const bool rec = f >= 0 && own && j >= zc.j0 && j <= zc.j1;
+          const unsigned P = (unsigned)(zc.i1 - zc.i0 + 1) * (unsigned)(zc.j1 - zc.j0 + 1);
+          zo[q] = rec ? ((unsigned)(2 * f + (k == kf ? 1 : 0)) * P + (unsigned)(j - zc.j0)) * 16u
+                      : kCapDropLane;
+        }
+      }
       pyh[q] = prof1(g, prof, 1, 1, m.j);
       pye[q] = prof1(g, prof, 0, 1, m.j);
       pzh[q] = prof1(g, prof, 1, 2, m.k);
@@ -581,6 +647,14 @@ struct March {
     // below 2 GB), else this lane's slot in the volume padding
     if (sized) padb = kDropOffset;
     else padb = ((unsigned)((size_t)g.nx * g.ny * g.nz) + (unsigned)lane) * E;
+    // the halo slot's field loads a lane's role does not read are dropped
+    // (only with sized resources; a dropped load returns 0, and on a dead
+    // plane, whose plane offset is already past the resource, the sum may
+    // wrap to an address inside the slab: a load of valid memory whose
+    // value is never used)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) hoff[c] = !sized || ((need >> c) & 1) ? sl[RW].o4 : kDropOffset;
+    hoffx = !sized || (need & 0b011000) ? sl[RW].o4 : kDropOffset;
 
     const unsigned vs = vsb();
     MarchH<T> A[S], B[S];
@@ -690,7 +764,7 @@ struct MarchChunks {
 // reads its psi in any variant.)  Fewer psi operations per plane also keep
 // a plane's loads and stores within the 63 a wavefront can count.
 template <typename T, int RW, bool LOSSY, bool NT, bool MAT>
-__global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dtm_dl, T dtm,
+__global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) void k_march(Geo g, int planes, T dtm_dl, T dtm,
                                                          T inv_dl, const T *__restrict__ s0,
                                                          T *__restrict__ s1,
                                                          const T *__restrict__ coef,
@@ -701,7 +775,7 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
                                                          int run, const MarchSources<T> src,
                                                          const unsigned char *__restrict__ mat,
                                                          const float *__restrict__ mtab,
-                                                         int nmat) {
+                                                         int nmat, const ZCap zc) {
   __shared__ MarchLds<T, RW> L;
   __shared__ Src<T> lsrc[kMarchSources];
   __shared__ float ltab[MAT ? kMatMax * kMatMax : 1];
@@ -753,12 +827,31 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
   const Rsrc rmat = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<unsigned char *>(mat), (short)0,
       MAT ? (int)min((unsigned long long)g.nx * g.ny * g.nz + 64ull, 0x7fffffffull) : 0, 0x00020000);
+  // the z-face record (FP32): blocks whose columns, rows and planes meet it
+  constexpr bool kCapable = sizeof(T) == 4;
+  bool cap = false;
+  if (kCapable && zc.buf) {
+    auto col = [&](int c) { return c >= k0 && c < k0 + 64; };
+    cap = (col(zc.kf0) || col(zc.kf0 - 1) || col(zc.kf1) || col(zc.kf1 - 1)) &&
+          j0 <= zc.j1 && j0 + 4 * RW - 1 >= zc.j0 && i0 <= zc.i1 && i1 - 1 >= zc.i0;
+  }
+  const Rsrc rcap = __builtin_amdgcn_make_buffer_rsrc(zc.buf, (short)0, (int)zc.bytes, 0x00020000);
+#define HF_MARCH_RUN1(X, Y, Z, C)                                                              \
+  {                                                                                           \
+    March<T, RW, LOSSY, X, Y, Z, NT, MAT, C> m{g,   dtm_dl, dtm,  inv_dl, r0,     r1,           \
+                                               rc,  rq0,    rq1,  prof,   lsrc,   src.n,        \
+                                               si0,    si1,  rmat, ltab, nmat};                 \
+    m.rcap = rcap;                                                                            \
+    m.zc = zc;                                                                                \
+    m.run(L, np, k0, j0, i0, i1);                                                             \
+  }
 #define HF_MARCH_RUN(X, Y, Z)                                                                  \
   {                                                                                           \
-    March<T, RW, LOSSY, X, Y, Z, NT, MAT> m{g,   dtm_dl, dtm,  inv_dl, r0,     r1,              \
-                                            rc,  rq0,    rq1,  prof,   lsrc,   src.n,           \
-                                            si0,    si1,  rmat, ltab, nmat};                    \
-    m.run(L, np, k0, j0, i0, i1);                                                             \
+    if constexpr (kCapable) {                                                                 \
+      if (cap) HF_MARCH_RUN1(X, Y, Z, true) else HF_MARCH_RUN1(X, Y, Z, false)                \
+    } else {                                                                                  \
+      HF_MARCH_RUN1(X, Y, Z, false)                                                           \
+    }                                                                                         \
   }
   if (x) {
     if (y) {
@@ -774,6 +867,7 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Geo g, int planes, T dt
     }
   }
 #undef HF_MARCH_RUN
+#undef HF_MARCH_RUN1
 }
 
 }  // namespace hfdtd

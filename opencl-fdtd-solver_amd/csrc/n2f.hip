@@ -191,6 +191,32 @@ struct PhaseRowEH {
   float e[32], h[32];  // cos, sin pairs (times dt); zero beyond 2 nfreq
 };
 
+// The engine's z-face record as the gather reads it (buf == nullptr: read
+// the z faces from the fields).
+struct ZRead {
+  const float4 *buf;
+  int i0, j0, nj;
+  long long quads;  // quads per (face, column) array
+};
+
+// Co-located z-face sample from the record: the same values and the same
+// operations as colocate_impl's z branch (the faces' k are at least 1, so
+// its k - 1 is the record's first column).
+__device__ __forceinline__ void colocate_zrec(const Dims3 &g, const ZRead &z, int f, int i, int j,
+                                              float v[4]) {
+  const int i1 = i + 1 < g.nx ? i + 1 : i;
+  const int j1 = j + 1 < g.ny ? j + 1 : j;
+  auto at = [&](int col, int a, int bj) {  // (Ex, Ey, Hx, Hy) at column k - 1 + col
+    return z.buf[(size_t)(2 * (f - 4) + col) * z.quads + (size_t)(a - z.i0) * z.nj + (bj - z.j0)];
+  };
+  const float4 c0 = at(1, i, j), cj = at(1, i, j1), ci = at(1, i1, j);
+  const float4 l0 = at(0, i, j), lj = at(0, i, j1), li = at(0, i1, j);
+  v[0] = avg2(c0.x, cj.x, j1 != j);
+  v[1] = avg2(c0.y, ci.y, i1 != i);
+  v[2] = avg4(c0.z, l0.z, cj.z, lj.z);
+  v[3] = avg4(c0.w, l0.w, ci.w, li.w);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_n2f_gather(Dims3 g, Box b, const T *__restrict__ Ex,
                                                     const T *__restrict__ Ey,
@@ -200,7 +226,7 @@ __global__ __launch_bounds__(256) void k_n2f_gather(Dims3 g, Box b, const T *__r
                                                     const T *__restrict__ Hz,
                                                     float *__restrict__ row, long long pe,
                                                     PhaseRowEH ph, float *__restrict__ we,
-                                                    float *__restrict__ wh) {
+                                                    float *__restrict__ wh, ZRead zr) {
   if (blockIdx.x == 0 && threadIdx.x < 32) {
     we[threadIdx.x] = ph.e[threadIdx.x];
     wh[threadIdx.x] = ph.h[threadIdx.x];
@@ -211,7 +237,15 @@ __global__ __launch_bounds__(256) void k_n2f_gather(Dims3 g, Box b, const T *__r
   const int f = decode_face(b, s, i, j, k);
   T v[4];
   int c[4];
-  colocate_impl<T>(g, f, i, j, k, Ex, Ey, Ez, Hx, Hy, Hz, v, c);
+  if constexpr (sizeof(T) == 4) {
+    if (zr.buf && f >= 4) {
+      colocate_zrec(g, zr, f, i, j, v);
+    } else {
+      colocate_impl<T>(g, f, i, j, k, Ex, Ey, Ez, Hx, Hy, Hz, v, c);
+    }
+  } else {
+    colocate_impl<T>(g, f, i, j, k, Ex, Ey, Ez, Hx, Hy, Hz, v, c);
+  }
   row[2 * s] = (float)v[0];
   row[2 * s + 1] = (float)v[1];
   row[pe + 2 * s] = (float)v[2];
@@ -545,18 +579,62 @@ static int gather_step(hfdtd_n2f *m, const double *phase) {
     ph.h[q] = fr < m->nfreq ? (float)phase[fr * 4 + 2 + c] : 0.0f;
   }
   const Dims3 g{e->nx, e->ny, e->nz};
+  ZRead zr{};
+  if (e->zrec_valid(m)) {  // the last step recorded this box's z faces
+    zr.buf = (const float4 *)e->zrec.buf;
+    zr.i0 = e->zrec.i0;
+    zr.j0 = e->zrec.j0;
+    zr.nj = e->zrec.j1 - e->zrec.j0 + 1;
+    zr.quads = (long long)e->zrec.quads();
+  }
   hipLaunchKernelGGL(k_n2f_gather<float>, dim3(ceil_div(m->n_face, 256)), dim3(256), 0,
                      e->stream, g, box_of(m), (const float *)e->field(0),
                      (const float *)e->field(1), (const float *)e->field(2),
                      (const float *)e->field(3), (const float *)e->field(4),
                      (const float *)e->field(5), m->buf + (size_t)m->kfill * m->ppad, m->pe, ph,
-                     m->we + (size_t)m->kfill * 32, m->wh + (size_t)m->kfill * 32);
+                     m->we + (size_t)m->kfill * 32, m->wh + (size_t)m->kfill * 32, zr);
   HF_HIP(hipGetLastError());
   if (++m->kfill == m->K) return n2f_flush(m);
   return HFDTD_OK;
 }
 
+// The engine's z-face record for this monitor's box: asked for with K-step
+// buffering on a float32 engine when no other monitor holds it, the z faces
+// lie at k >= 1 and are at least two cells apart (so the four recorded
+// columns are distinct); otherwise the gather reads the fields.
+static void zrec_acquire(hfdtd_n2f *m) {
+  Engine *e = m->e;
+  if (e->zrec.owner || e->fp64) return;
+  const int *b = m->box;
+  if (b[4] < 1 || b[5] < b[4] + 2) return;
+  ZRec z;
+  z.kf[0] = b[4];
+  z.kf[1] = b[5];
+  z.i0 = b[0];
+  z.i1 = std::min(b[1] + 1, e->nx - 1);
+  z.j0 = b[2];
+  z.j1 = std::min(b[3] + 1, e->ny - 1);
+  z.bytes = 2 * 2 * z.quads() * 4 * sizeof(float);
+  if (z.bytes >= 0x40000000ull) return;  // the sweep's drop offsets need a record below 1 GB
+  if (hipMalloc((void **)&z.buf, z.bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  z.owner = m;
+  z.epoch = ~0ull;  // nothing recorded yet
+  e->zrec = z;
+}
+
+static void zrec_release(hfdtd_n2f *m) {
+  ZRec &z = m->e->zrec;
+  if (z.owner != m) return;
+  (void)hipStreamSynchronize(m->e->stream);
+  if (z.buf) (void)hipFree(z.buf);
+  z = ZRec{};
+}
+
 static void free_buffering(hfdtd_n2f *m) {
+  zrec_release(m);
   for (float *p : {m->buf, m->we, m->wh})
     if (p) (void)hipFree(p);
   m->buf = m->we = m->wh = nullptr;
@@ -717,6 +795,12 @@ int hfdtd_n2f_accumulate(hfdtd_n2f *m, const double *phase) {
   return accumulate_t<float, float, false>(m, phase);
 }
 
+int hfdtd_n2f_zrecord(const hfdtd_n2f *m, int *on) {
+  if (!m || !on) return fail(HFDTD_EINVAL, "NULL argument");
+  *on = m->e->zrec.owner == m ? 1 : 0;
+  return HFDTD_OK;
+}
+
 int hfdtd_n2f_flush(hfdtd_n2f *m) {
   if (!m) return fail(HFDTD_EINVAL, "NULL argument");
   return n2f_flush(m);
@@ -860,6 +944,7 @@ int hfdtd_n2f_set_buffering(hfdtd_n2f *m, int K) {
     return hip_fail(err, "hfdtd_n2f_set_buffering");
   }
   m->K = K;
+  zrec_acquire(m);
   return HFDTD_OK;
 }
 

@@ -250,6 +250,20 @@ static int march_rw(Engine *e, const SrcList<T> &sl) {
   const bool inside = sl.n <= kMarchSources;
   ms.n = inside ? sl.n : 0;
   for (int q = 0; q < ms.n; ++q) ms.s[q] = sl.s[q];
+  // the z-face record, if a monitor keeps one (FP32)
+  ZCap zc{};
+  const ZRec &z = e->zrec;
+  const bool record = sizeof(T) == 4 && z.owner && z.buf;
+  if (record) {
+    zc.buf = z.buf;
+    zc.kf0 = z.kf[0];
+    zc.kf1 = z.kf[1];
+    zc.i0 = z.i0;
+    zc.i1 = z.i1;
+    zc.j0 = z.j0;
+    zc.j1 = z.j1;
+    zc.bytes = (unsigned)z.bytes;
+  }
   int rc = timing_mark(e);
   if (rc) return rc;
   const int o = e->cur ^ 1;
@@ -258,10 +272,14 @@ static int march_rw(Engine *e, const SrcList<T> &sl) {
                      (T *)e->state[o], (const T *)e->coef, (const T *)e->prof,
                      (const T *)e->psi[e->cur], (T *)e->psi[o], tiles_k, tiles_j, ch, e->xcd,
                      e->march_run, ms, MAT ? (const unsigned char *)e->mat : nullptr,
-                     MAT ? (const float *)e->mtab : nullptr, MAT ? e->nmat : 0);
+                     MAT ? (const float *)e->mtab : nullptr, MAT ? e->nmat : 0, zc);
   HF_HIP(hipGetLastError());
   rc = timing_mark(e);
-  if (rc || inside) return rc;
+  if (rc) return rc;
+  // the record holds this step's values unless the fix-up kernel below
+  // changes E on source cells afterwards
+  if (record && inside) e->zrec.epoch = e->state_epoch;
+  if (inside) return rc;
   return source_fixup<T, LOSSY>(e, sl);
 }
 
@@ -369,6 +387,7 @@ static int pair_t(Engine *e, const hfdtd_source *src_a, int na, const hfdtd_sour
 
 int launch_pair_steps(Engine *e, const hfdtd_source *sa, int na, const hfdtd_source *sb, int nb) {
   HF_HIP(hipSetDevice(e->device));
+  e->touch();
   if (e->fp64) return fail(HFDTD_EUNSUPPORTED, "the pair sweep is FP32 only");
   return pair_t<float>(e, sa, na, sb, nb);
 }
@@ -409,26 +428,31 @@ static int fused_t(Engine *e, const hfdtd_source *src, int n) {
 
 int launch_march_step(Engine *e, const hfdtd_source *src, int n) {
   HF_HIP(hipSetDevice(e->device));
+  e->touch();
   return e->fp64 ? march_t<double>(e, src, n) : march_t<float>(e, src, n);
 }
 
 int launch_update_H(Engine *e) {
   HF_HIP(hipSetDevice(e->device));
+  e->touch();
   return e->fp64 ? update_H_t<double>(e) : update_H_t<float>(e);
 }
 
 int launch_update_E(Engine *e) {
   HF_HIP(hipSetDevice(e->device));
+  e->touch();
   return e->fp64 ? update_E_t<double>(e) : update_E_t<float>(e);
 }
 
 int launch_sources(Engine *e, const hfdtd_source *src, int n) {
   HF_HIP(hipSetDevice(e->device));
+  e->touch();
   return e->fp64 ? sources_t<double>(e, src, n) : sources_t<float>(e, src, n);
 }
 
 int launch_fused_step(Engine *e, const hfdtd_source *src, int n) {
   HF_HIP(hipSetDevice(e->device));
+  e->touch();
   return e->fp64 ? fused_t<double>(e, src, n) : fused_t<float>(e, src, n);
 }
 

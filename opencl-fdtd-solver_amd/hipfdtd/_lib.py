@@ -129,6 +129,7 @@ EXPORTS = {
     "hfdtd_probe_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "hfdtd_n2f_set_buffering": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_n2f_flush": (C.c_int, [C.c_void_p]),
+    "hfdtd_n2f_zrecord": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hfdtd_dft_plane_create": (
         C.c_int,
         [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
