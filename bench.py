@@ -65,6 +65,7 @@ Usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 20] [--no-cpu]
        python bench.py --workload plane-dft [--steps 1500]
        python bench.py --workload adjoint [--steps 3000] [--repeat 2]
        python bench.py --workload batch [--steps 1200] [--graph-steps 100]
+       python bench.py --workload reference-sizes
 """
 
 from __future__ import annotations
@@ -558,6 +559,58 @@ def run_batch(args):
     }))  # fmt: skip
 
 
+def run_reference_sizes(args):
+    """The reference's own throughput benchmark (benchmarks/benchmark_vs_meep.py:62-135,
+    README.md:184-190) at its published sizes: vacuum, dl = 1 mm, npml 25, a
+    soft Ex sheet over the whole xy plane at z = nz - npml - 2 driven by a
+    6 GHz Gaussian-modulated sine (20 % bandwidth), no monitors; 10 warm-up
+    steps, then the median of 3 timed windows."""
+    import warnings
+
+    import hipfdtd
+
+    warnings.simplefilter("ignore", DeprecationWarning)
+    cases = [((400, 400, 400), 100), ((600, 600, 600), 100), ((750, 750, 750), 80)]
+    freq, npml, dl = 6e9, 25, 1e-3
+    fw = 0.2 * freq
+    t0, sig = 5.0 / (math.pi * fw), 1.0 / (math.pi * fw)
+    out = []
+    dev = ""
+    for shape, steps in cases:
+        sim = hipfdtd.HIPFDTD(shape, dl, npml=npml)
+        dev = sim.device_name
+        z_src = shape[2] - npml - 2
+
+        def src(f, z=z_src):
+            amp = math.exp(-0.5 * ((f.t - t0) / sig) ** 2) * math.sin(2 * math.pi * freq * f.t)
+            f.add_source_Ex(z, amp)  # the reference's legacy soft sheet
+
+        sim.add_source(src)
+        sim.run(10)
+        sim.synchronize()
+        rates = []
+        for _ in range(3):
+            t_a = time.perf_counter()
+            sim.run(steps)
+            sim.synchronize()
+            rates.append(float(np.prod(shape)) * steps / (time.perf_counter() - t_a) / 1e6)
+        out.append({"grid": list(shape), "steps": steps, "path": sim.path,
+                    "mcups_median": round(float(np.median(rates)), 1),
+                    "mcups": [round(r, 1) for r in rates]})  # fmt: skip
+        print(f"[reference-sizes] {shape} {sim.path}: {out[-1]['mcups_median']} Mcell/s",
+              file=sys.stderr, flush=True)  # fmt: skip
+        sim.close()
+    print(json.dumps({
+        "metric": "Mcell-updates/s", "unit": "Mcell/s", "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "reference throughput benchmark: vacuum, npml 25, soft Ex sheet "
+                               "over the whole xy plane, no monitors; median of 3 windows"},
+        "results": out,
+        "reference_published_rtx5080": {"400^3": 7970, "600^3": 8630, "750^3": 9430},
+        "device": dev,
+    }))  # fmt: skip
+
+
 TRAFFIC_FILES = {"fused": ROOT / "profiles" / "k_fused_256cube_hbm_traffic.txt",
                  "split": ROOT / "profiles" / "split_sweeps_256cube_hbm_traffic.txt",
                  "march": ROOT / "profiles" / "k_march_256cube_hbm_traffic.txt",
@@ -606,7 +659,7 @@ def main():
                     help="256^3 workload: steps of the FP32-vs-FP64 accuracy run (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", default="yee-256",
-                    choices=["yee-256", "plane-dft", "adjoint", "batch"])
+                    choices=["yee-256", "plane-dft", "adjoint", "batch", "reference-sizes"])
     ap.add_argument("--graph-steps", type=int, default=100, help="batch: steps per HIP graph")
     ap.add_argument("--repeat", type=int, default=2, help="adjoint: timed forward+backward runs")
     ap.add_argument("--march-tiling", default=None, metavar="ROWS,PLANES[,RUN[,NT]]",
@@ -627,6 +680,9 @@ def main():
         if args.steps == 2000:
             args.steps = 1200
         run_batch(args)
+        return
+    if args.workload == "reference-sizes":
+        run_reference_sizes(args)
         return
     if args.workload == "adjoint":
         if args.steps == 2000:

@@ -164,6 +164,12 @@ int hfdtd_set_split_schedule(hfdtd_engine *e, int run_blocks);
  * Used by the autotuner; no counterpart in the reference. */
 int hfdtd_set_march_tiling(hfdtd_engine *e, int rows_per_wave, int planes_i, int run_blocks,
                            int nontemporal);
+/* No counterpart in the reference.  Addressing of the march sweep: 0 (the
+ * default) one buffer resource per slab, with 32-bit offsets, on slabs below
+ * 4 GB, and one per volume on FP32 slabs of 4 GB and more (volumes below
+ * 2 GB, about 810^3 cells); 1 one per volume on any FP32 grid (same results;
+ * for testing that addressing on small grids). */
+int hfdtd_set_march_addressing(hfdtd_engine *e, int per_volume);
 
 /* Two Yee steps in one launch (the engine.py:724-735 step twice, minus the
  * callbacks): src_a[0..na) are the sources of the first step, src_b[0..nb)

@@ -105,6 +105,7 @@ struct Engine {
   bool view_shadow = false;
   int march_planes = 0;
   int march_run = 16;
+  bool march_per_volume = false;  // per-volume addressing on any FP32 slab (set by hand)
   int march_nt = -1;
   int march_occ[2][3] = {};  // resident march blocks per CU by [lossy][rows]
   int march_occ_mat[3] = {};  // the same for the material-indexed sweep by rows

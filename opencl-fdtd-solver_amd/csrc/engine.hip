@@ -278,6 +278,30 @@ static int build_material_index(Engine *e, const S *eps) {
   return HFDTD_OK;
 }
 
+// The index of the default medium (eps_r = 1 everywhere): every cell is
+// material 0, whose table entry dt/eps0 / (0.5 (1 + 1)) is the default ce
+// volumes' value dt/eps0 exactly, so the march sweep reads one byte per cell
+// before any set_epsilon (FP32 engines).
+static int uniform_material_index(Engine *e) {
+  if (!e->mat) {
+    int rc = dmalloc(e, &e->mat, e->ncell + 64);
+    if (rc) return rc;
+  }
+  if (!e->mtab) {
+    void *t = nullptr;
+    int rc = dmalloc(e, &t, (size_t)kMatMax * kMatMax * sizeof(float));
+    if (rc) return rc;
+    e->mtab = (float *)t;
+  }
+  const float ce = (float)(e->dt / kEPS0 / (0.5 * (1.0 + 1.0)));
+  HF_HIP(hipMemsetAsync(e->mat, 0, e->ncell + 64, e->stream));
+  HF_HIP(hipMemcpyAsync(e->mtab, &ce, sizeof(float), hipMemcpyHostToDevice, e->stream));
+  HF_HIP(hipStreamSynchronize(e->stream));
+  e->nmat = 1;
+  e->mat_ok = true;
+  return HFDTD_OK;
+}
+
 }  // namespace hfdtd
 
 using namespace hfdtd;
@@ -443,6 +467,7 @@ int hfdtd_create(int nx, int ny, int nz, int npml, double dl, double dt, int fp6
   if ((rc = fill_value(e, e->psi[0], e->psi_off[PSI_COUNT], 0.0))) return bail(rc);
   err = hipStreamSynchronize(e->stream);
   if (err != hipSuccess) return bail(hip_fail(err, "hipStreamSynchronize(create)"));
+  if (!fp64 && (rc = uniform_material_index(e))) return bail(rc);
   *out = h;
   return HFDTD_OK;
 }
@@ -701,6 +726,13 @@ int hfdtd_set_march_tiling(hfdtd_engine *h, int rows_per_wave, int planes_i, int
   if (planes_i) e->march_planes = planes_i < 0 ? 0 : planes_i;
   if (run_blocks) e->march_run = run_blocks;
   if (nontemporal != -2) e->march_nt = nontemporal;
+  return HFDTD_OK;
+}
+
+int hfdtd_set_march_addressing(hfdtd_engine *h, int per_volume) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  if (per_volume < 0 || per_volume > 1) return fail(HFDTD_EINVAL, "per_volume must be 0 or 1");
+  h->s.march_per_volume = per_volume != 0;
   return HFDTD_OK;
 }
 

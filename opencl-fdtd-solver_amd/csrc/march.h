@@ -196,7 +196,7 @@ struct PsiPlane {
 };
 
 template <typename T, int RW, bool LOSSY, bool X, bool Y, bool Z, bool NT, bool MAT = false,
-          bool CAP = false>
+          bool CAP = false, bool BIG = false>
 struct March {
   static constexpr int S = RW + 1;  // RW owned rows + the halo slot
   static constexpr int TJ = 4 * RW;
@@ -232,7 +232,33 @@ struct March {
   int iend;            // first plane past this block's chunk
   bool sized;          // slab resources carry their sizes (slabs below 2 GB)
 
+  // BIG: slabs of 4 GB and more, past 32-bit offsets.  Every access then
+  // goes through a resource on its own volume (each below 2 GB), made from
+  // the slab's base pointer where it is used.
+  const T *s0p, *cp;
+  T *s1p;
+
   __device__ __forceinline__ unsigned vsb() const { return (unsigned)g.vs * E; }
+  __device__ __forceinline__ Rsrc vol_rsrc(const T *base, int c) const {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base) + (size_t)c * g.vs, (short)0,
+                                             (int)(g.vs * E), 0x00020000);
+  }
+  // volume c (Ex .. Hz) of the slab read, at byte offsets voff (per lane)
+  // and pl (wave-uniform)
+  __device__ __forceinline__ T ld0(int c, unsigned voff, unsigned pl) const {
+    if constexpr (BIG) return bld<T, false>(vol_rsrc(s0p, c), voff, pl);
+    else return bld<T, false>(r0, voff, (unsigned)c * vsb() + pl);
+  }
+  // coefficient volume c (ce_x .. ca_z), streamed
+  __device__ __forceinline__ T ldc(int c, unsigned voff, unsigned pl) const {
+    if constexpr (BIG) return bld<T, NT>(vol_rsrc(cp, c), voff, pl);
+    else return bld<T, NT>(rc, voff, (unsigned)c * vsb() + pl);
+  }
+  // volume c of the slab written
+  __device__ __forceinline__ void st1(int c, T v, unsigned voff) const {
+    if constexpr (BIG) bst<T, NT>(vol_rsrc(s1p, c), v, voff, 0u);
+    else bst<T, NT>(r1, v, voff, (unsigned)c * vsb());
+  }
   __device__ __forceinline__ unsigned plane_b(int i) const {
     return (unsigned)min(i, g.nx - 1) * (unsigned)(g.ny * g.nz) * E;
   }
@@ -266,12 +292,12 @@ struct March {
   __device__ __forceinline__ void load_h(MarchH<T> &s, const MarchSlot &m, unsigned pl,
                                          unsigned pl1, const PsiPlane &pp, bool halo) const {
     const unsigned vs = vsb();
-    s.ex = bld<T, false>(r0, halo ? hoff[0] : m.o4, pl1);
-    s.ey = bld<T, false>(r0, halo ? hoff[1] : m.o4, vs + pl1);
-    s.ez = bld<T, false>(r0, halo ? hoff[2] : m.o4, 2 * vs + pl1);
-    s.hx = bld<T, false>(r0, halo ? hoff[3] : m.o4, 3 * vs + pl);
-    s.hy = bld<T, false>(r0, halo ? hoff[4] : m.o4, 4 * vs + pl);
-    s.hz = bld<T, false>(r0, halo ? hoff[5] : m.o4, 5 * vs + pl);
+    s.ex = ld0(0, halo ? hoff[0] : m.o4, pl1);
+    s.ey = ld0(1, halo ? hoff[1] : m.o4, pl1);
+    s.ez = ld0(2, halo ? hoff[2] : m.o4, pl1);
+    s.hx = ld0(3, halo ? hoff[3] : m.o4, pl);
+    s.hy = ld0(4, halo ? hoff[4] : m.o4, pl);
+    s.hz = ld0(5, halo ? hoff[5] : m.o4, pl);
     if constexpr (X) {
       const unsigned vx = pp.inx && !pp.dead ? (halo ? hoffx : m.o4) : kDropOffset;
       s.p[0] = bld<T, false>(rq0, vx, psib(0, pp.xs));
@@ -308,14 +334,14 @@ struct March {
       s.mj1 = __builtin_amdgcn_raw_buffer_load_b8(rmat, m.o1j, pm, 0);
       s.mk1 = __builtin_amdgcn_raw_buffer_load_b8(rmat, m.o1k, pm, 0);
     } else {
-      s.cx = bld<T, NT>(rc, m.o4, pl);
-      s.cy = bld<T, NT>(rc, m.o4, vs + pl);
-      s.cz = bld<T, NT>(rc, m.o4, 2 * vs + pl);
+      s.cx = ldc(0, m.o4, pl);
+      s.cy = ldc(1, m.o4, pl);
+      s.cz = ldc(2, m.o4, pl);
     }
     if constexpr (LOSSY) {
-      s.ax = bld<T, NT>(rc, m.o4, 3 * vs + pl);
-      s.ay = bld<T, NT>(rc, m.o4, 4 * vs + pl);
-      s.az = bld<T, NT>(rc, m.o4, 5 * vs + pl);
+      s.ax = ldc(3, m.o4, pl);
+      s.ay = ldc(4, m.o4, pl);
+      s.az = ldc(5, m.o4, pl);
     }
     if constexpr (X) {
       const unsigned vx = pp.inx && !pp.dead ? m.o4 : kDropOffset;
@@ -424,9 +450,9 @@ struct March {
       if (q < RW) {
         const bool ok = m.own && live;
         const unsigned v = ok ? m.o4 + pl : padb;
-        bst<T, NT>(r1, h[q][0], v, 3 * vs);
-        bst<T, NT>(r1, h[q][1], v, 4 * vs);
-        bst<T, NT>(r1, h[q][2], v, 5 * vs);
+        st1(3, h[q][0], v);
+        st1(4, h[q][1], v);
+        st1(5, h[q][2], v);
         if constexpr (X) {
           psi_st(p[0], 0, ok && pp.inx, m.o4 + pp.xs * E);
           psi_st(p[1], 1, ok && pp.inx, m.o4 + pp.xs * E);
@@ -513,9 +539,9 @@ struct March {
              hx_km, hy_km, hyc[q], hzc[q], ex, ey, ez, ax, ay, az, cx, cy, cz, p, inv_dl);
       const bool ok = m.own && live;
       const unsigned v = ok ? m.o4 + pl : padb;
-      bst<T, NT>(r1, ex, v, 0u);
-      bst<T, NT>(r1, ey, v, vs);
-      bst<T, NT>(r1, ez, v, 2 * vs);
+      st1(0, ex, v);
+      st1(1, ey, v);
+      st1(2, ez, v);
       if constexpr (X) {
         psi_st(p[0], 6, ok && pp.inx, m.o4 + pp.xs * E);
         psi_st(p[1], 7, ok && pp.inx, m.o4 + pp.xs * E);
@@ -544,7 +570,7 @@ struct March {
   __device__ __forceinline__ void run(MarchLds<T, RW> &L, int planes, int k0, int j0, int i0,
                                       int i1) {
     iend = i1;
-    sized = slab_bytes<T>(g) < 0x80000000ull;
+    sized = BIG ? (unsigned long long)g.vs * E < 0x80000000ull : slab_bytes<T>(g) < 0x80000000ull;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int n = g.npml;
@@ -665,9 +691,9 @@ struct March {
       const unsigned p0 = plane_b(i0);
 #pragma unroll
       for (int q = 0; q < S; ++q) {
-        oe[q][0] = bld<T, false>(r0, sl[q].o4, p0);
-        oe[q][1] = bld<T, false>(r0, sl[q].o4, vs + p0);
-        oe[q][2] = bld<T, false>(r0, sl[q].o4, 2 * vs + p0);
+        oe[q][0] = ld0(0, sl[q].o4, p0);
+        oe[q][1] = ld0(1, sl[q].o4, p0);
+        oe[q][2] = ld0(2, sl[q].o4, p0);
       }
       if constexpr (MAT) {
         const unsigned pm = mat_plane(i0, false);
@@ -702,12 +728,11 @@ struct March {
           p[4] = bld<T, false>(rq0, m.qz4, psib(4, pp.zs));
           p[5] = bld<T, false>(rq0, m.qz4, psib(5, pp.zs));
         }
-        T hx = bld<T, false>(r0, m.o4, 3 * vs + pm), hy = bld<T, false>(r0, m.o4, 4 * vs + pm);
-        T hz = bld<T, false>(r0, m.o4, 5 * vs + pm);
-        h_math(g, im, m.j, m.k, st, pxh, pyh[q], pzh[q], bld<T, false>(r0, m.o4, pm),
-               bld<T, false>(r0, m.o4, vs + pm), bld<T, false>(r0, m.o4, 2 * vs + pm),
-               bld<T, false>(r0, m.o4, pm + nzb), bld<T, false>(r0, m.o4, 2 * vs + pm + nzb),
-               bld<T, false>(r0, m.o4 + E, pm), bld<T, false>(r0, m.o4 + E, vs + pm), oe[q][1],
+        T hx = ld0(3, m.o4, pm), hy = ld0(4, m.o4, pm);
+        T hz = ld0(5, m.o4, pm);
+        h_math(g, im, m.j, m.k, st, pxh, pyh[q], pzh[q], ld0(0, m.o4, pm), ld0(1, m.o4, pm),
+               ld0(2, m.o4, pm), ld0(0, m.o4, pm + nzb), ld0(2, m.o4, pm + nzb),
+               ld0(0, m.o4 + E, pm), ld0(1, m.o4 + E, pm), oe[q][1],
                oe[q][2], hx, hy, hz, p, dtm_dl, dtm);
         hyc[q] = hy;
         hzc[q] = hz;
@@ -763,7 +788,7 @@ struct MarchChunks {
 // column) meet a z-stripe.  (The plane before the chunk, recomputed once,
 // reads its psi in any variant.)  Fewer psi operations per plane also keep
 // a plane's loads and stores within the 63 a wavefront can count.
-template <typename T, int RW, bool LOSSY, bool NT, bool MAT>
+template <typename T, int RW, bool LOSSY, bool NT, bool MAT, bool BIG = false>
 __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) void k_march(Geo g, int planes, T dtm_dl, T dtm,
                                                          T inv_dl, const T *__restrict__ s0,
                                                          T *__restrict__ s1,
@@ -828,7 +853,7 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
       const_cast<unsigned char *>(mat), (short)0,
       MAT ? (int)min((unsigned long long)g.nx * g.ny * g.nz + 64ull, 0x7fffffffull) : 0, 0x00020000);
   // the z-face record (FP32): blocks whose columns, rows and planes meet it
-  constexpr bool kCapable = sizeof(T) == 4;
+  constexpr bool kCapable = sizeof(T) == 4 && !BIG;
   bool cap = false;
   if (kCapable && zc.buf) {
     auto col = [&](int c) { return c >= k0 && c < k0 + 64; };
@@ -838,9 +863,12 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
   const Rsrc rcap = __builtin_amdgcn_make_buffer_rsrc(zc.buf, (short)0, (int)zc.bytes, 0x00020000);
 #define HF_MARCH_RUN1(X, Y, Z, C)                                                              \
   {                                                                                           \
-    March<T, RW, LOSSY, X, Y, Z, NT, MAT, C> m{g,   dtm_dl, dtm,  inv_dl, r0,     r1,           \
-                                               rc,  rq0,    rq1,  prof,   lsrc,   src.n,        \
-                                               si0,    si1,  rmat, ltab, nmat};                 \
+    March<T, RW, LOSSY, X, Y, Z, NT, MAT, C, BIG> m{g,   dtm_dl, dtm,  inv_dl, r0,     r1,      \
+                                                    rc,  rq0,    rq1,  prof,   lsrc,   src.n,   \
+                                                    si0,    si1,  rmat, ltab, nmat};            \
+    m.s0p = s0;                                                                               \
+    m.s1p = s1;                                                                               \
+    m.cp = coef;                                                                              \
     m.rcap = rcap;                                                                            \
     m.zc = zc;                                                                                \
     m.run(L, np, k0, j0, i0, i1);                                                             \

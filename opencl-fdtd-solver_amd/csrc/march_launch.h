@@ -1,0 +1,94 @@
+// Launcher of the march sweep (k_march, march.h) for one tiling and medium
+// kind.  Defined here and instantiated in march_f32_r1.hip, march_f32_r2.hip
+// and march_f64.hip, so the many sweep bodies compile in parallel; march.hip
+// picks the instantiation for a step.
+#pragma once
+
+#include "launch_common.h"
+#include "march.h"
+
+namespace hfdtd {
+
+template <typename T, int RW, bool LOSSY, bool NT, bool MAT = false, bool BIG = false>
+int march_rw(Engine *e, const SrcList<T> &sl) {
+  auto kern = k_march<T, RW, LOSSY, NT, MAT, BIG>;
+  int big_occ = 0;
+  int &occ = BIG ? big_occ : MAT ? e->march_occ_mat[RW] : e->march_occ[LOSSY ? 1 : 0][RW];
+  if (occ == 0) {
+    // resident blocks per CU from the kernel's own resource use: a block puts
+    // one wavefront on each of the four SIMDs, each SIMD holds 512 VGPRs per
+    // lane (allocated in granules of 8), a CU has 160 KB of LDS
+    hipFuncAttributes fa;
+    int per_cu = 1;
+    if (hipFuncGetAttributes(&fa, (const void *)kern) == hipSuccess && fa.numRegs > 0) {
+      const int vg = (fa.numRegs + 7) / 8 * 8;
+      per_cu = 512 / vg;
+      if (fa.sharedSizeBytes > 0)
+        per_cu = std::min(per_cu, (int)(160 * 1024 / fa.sharedSizeBytes));
+      per_cu = std::max(per_cu, 1);
+    }
+    occ = per_cu;
+  }
+  const int planes = e->march_planes_for(RW, occ);
+  const int tiles_k = (int)ceil_div(e->nz, 64), tiles_j = (int)ceil_div(e->ny, 4 * RW);
+  MarchChunks ch;
+  ch.split = e->npml > 0 && e->nx > 2 * e->npml ? 1 : 0;
+  ch.inner = (int)ceil_div(ch.split ? e->nx - 2 * e->npml : e->nx, planes);
+  const dim3 grid(tiles_k * tiles_j * (ch.inner + 2 * ch.split));
+  MarchSources<T> ms;
+  const bool inside = sl.n <= kMarchSources;
+  ms.n = inside ? sl.n : 0;
+  for (int q = 0; q < ms.n; ++q) ms.s[q] = sl.s[q];
+  // the z-face record, if a monitor keeps one (FP32)
+  ZCap zc{};
+  const ZRec &z = e->zrec;
+  const bool record = sizeof(T) == 4 && !BIG && z.owner && z.buf;
+  if (record) {
+    zc.buf = z.buf;
+    zc.kf0 = z.kf[0];
+    zc.kf1 = z.kf[1];
+    zc.i0 = z.i0;
+    zc.i1 = z.i1;
+    zc.j0 = z.j0;
+    zc.j1 = z.j1;
+    zc.bytes = (unsigned)z.bytes;
+  }
+  int rc = timing_mark(e);
+  if (rc) return rc;
+  const int o = e->cur ^ 1;
+  hipLaunchKernelGGL(kern, grid, dim3(kMarchThreads), 0, e->stream, geo_of(e), planes,
+                     T(e->dtm_dl), T(e->dtm), T(e->inv_dl), (const T *)e->state[e->cur],
+                     (T *)e->state[o], (const T *)e->coef, (const T *)e->prof,
+                     (const T *)e->psi[e->cur], (T *)e->psi[o], tiles_k, tiles_j, ch, e->xcd,
+                     e->march_run, ms, MAT ? (const unsigned char *)e->mat : nullptr,
+                     MAT ? (const float *)e->mtab : nullptr, MAT ? e->nmat : 0, zc);
+  HF_HIP(hipGetLastError());
+  rc = timing_mark(e);
+  if (rc) return rc;
+  // the record holds this step's values unless the fix-up kernel below
+  // changes E on source cells afterwards
+  if (record && inside) e->zrec.epoch = e->state_epoch;
+  if (inside) return rc;
+  return source_fixup<T, LOSSY>(e, sl);
+}
+
+#define HF_MARCH_RW(T, RW, LOSSY, NT, MAT) \
+  template int march_rw<T, RW, LOSSY, NT, MAT, false>(Engine *, const SrcList<T> &)
+// slabs of 4 GB and more (FP32, streaming)
+#define HF_MARCH_RW_BIG(X)                                                               \
+  X template int march_rw<float, 1, false, true, false, true>(Engine *, const SrcList<float> &); \
+  X template int march_rw<float, 1, true, true, false, true>(Engine *, const SrcList<float> &);  \
+  X template int march_rw<float, 1, false, true, true, true>(Engine *, const SrcList<float> &);  \
+  X template int march_rw<float, 2, false, true, false, true>(Engine *, const SrcList<float> &); \
+  X template int march_rw<float, 2, true, true, false, true>(Engine *, const SrcList<float> &);  \
+  X template int march_rw<float, 2, false, true, true, true>(Engine *, const SrcList<float> &)
+#define HF_MARCH_RW_SET(X, T, RW)   \
+  X HF_MARCH_RW(T, RW, false, true, false);  \
+  X HF_MARCH_RW(T, RW, false, false, false); \
+  X HF_MARCH_RW(T, RW, true, true, false);   \
+  X HF_MARCH_RW(T, RW, true, false, false)
+#define HF_MARCH_RW_MAT(X, T, RW)  \
+  X HF_MARCH_RW(T, RW, false, true, true); \
+  X HF_MARCH_RW(T, RW, false, false, true)
+
+}  // namespace hfdtd

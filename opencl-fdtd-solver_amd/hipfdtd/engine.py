@@ -94,6 +94,9 @@ class HIPFDTD(SourceMonitorMixin):
         psi_total = 4 * (self.psi_x_size + self.psi_y_size + self.psi_z_size) + 64
         self._fused_fits = max(6 * vstride, psi_total) * item < 2**32
         self._pair_fits = max(6 * vstride, psi_total) * item < 2**31 and item == 4
+        # the march sweep also takes FP32 slabs of 4 GB and more, addressing
+        # each volume (below 2 GB) on its own: up to about 810^3 cells
+        self._march_fits = self._fused_fits or (item == 4 and max(vstride, psi_total) * item < 2**31)
         self.path = "split"
         self.fused = False
         self.set_path(self._resolve_path(fused, path, dtype, (self.Nx, self.Ny, self.Nz)))
@@ -181,17 +184,20 @@ class HIPFDTD(SourceMonitorMixin):
         """Step path: 'split' (two sweeps, in place), 'fused' (LDS-DMA tiles),
         'march' (register-pipelined tiles) or 'pair' (``run`` advances two
         steps per launch, single steps take the march sweep); ``True`` /
-        ``False`` mean 'fused' / 'split'.  One-launch paths fall back to
-        'split' on grids beyond their 32-bit offsets, 'pair' to 'march' for
-        FP64 or slabs of 2 GB and more.  Takes effect at the next step; the
-        state carries over."""
+        ``False`` mean 'fused' / 'split'.  'fused' falls back to 'split' on
+        slabs of 4 GB and more, 'march' on FP64 slabs of 4 GB and more or
+        FP32 volumes of 2 GB and more, 'pair' to 'march' for FP64 or slabs of
+        2 GB and more.  Takes effect at the next step; the state carries
+        over."""
         if isinstance(path, (bool, np.bool_)):
             path = "fused" if path else "split"
         if path not in self.PATHS:
             raise ValueError(f"path must be one of {self.PATHS}; got {path!r}")
         if path == "pair" and not self._pair_fits:
             path = "march"
-        if path != "split" and not self._fused_fits:
+        if path == "march" and not self._march_fits:
+            path = "split"
+        if path == "fused" and not self._fused_fits:
             path = "split"
         self.path = path
         self.fused = path != "split"
@@ -530,6 +536,13 @@ class HIPFDTD(SourceMonitorMixin):
         _lib.check(self._lib.hfdtd_set_march_tiling(self._handle, int(rows_per_wave),
                                                     int(planes_i), int(run_blocks),
                                                     int(nontemporal)), "set_march_tiling")  # fmt: skip
+
+    def set_march_addressing(self, per_volume: bool) -> None:
+        """March sweep addressing: one buffer resource per volume on any FP32
+        grid (True), or only where the slabs reach 4 GB (False, default);
+        the results are the same."""
+        _lib.check(self._lib.hfdtd_set_march_addressing(self._handle, int(bool(per_volume))),
+                   "set_march_addressing")  # fmt: skip
 
     def material_count(self) -> int:
         """Materials of the march sweep's material index (0: the sweep reads

@@ -133,3 +133,24 @@ def test_fp64_engine_has_no_index():
     sim2 = hip().HIPFDTD(shape, DL, npml=6, path="march")
     sim2.set_epsilon(eps, np.zeros(shape, np.float32))  # lossy: ca volumes, no index
     assert sim2.material_count() == 0
+
+
+@pytest.mark.parametrize("rows", [0, 1, 2])
+def test_default_medium_is_indexed(rows):
+    """Before any set_epsilon the medium is eps_r = 1 everywhere: one
+    material, whose table entry equals the default ce volumes' dt/eps0."""
+    shape = (45, 35, 100)
+    print(f"[mat-default] rows={rows}", flush=True)
+    out = {}
+    for path in ("fused", "march"):
+        sim = hip().HIPFDTD(shape, DL, npml=6, path=path)
+        if path == "march":
+            sim.set_march_tiling(rows, 0, 0, -2)
+            assert sim.material_count() == 1
+        sim.add_source(lambda f: f.add_source_Jx(50, fo.gaussian_sine(f.t, 3e10, 1.5e10), 5, 40,
+                                                 4, 30, rim_taper=True))  # fmt: skip
+        sim.run(16)
+        out[path] = sim.fields()
+    _same(out["fused"], out["march"])
+    f64 = hip().HIPFDTD(shape, DL, npml=6, path="march", dtype=np.float64)
+    assert f64.material_count() == 0
