@@ -6,25 +6,35 @@ C ABI of ``include/hipfdtd.h``.  All device memory is owned by the library.
 
 Step execution.  The reference runs ``update_H``, then the source callbacks
 (at ``t + dt/2``), then ``update_E``, then the monitors
-(``engine.py:724-737``).  ``HIPFDTD`` keeps that order on two paths:
+(``engine.py:724-737``).  ``HIPFDTD`` keeps that order on every path:
 
-* split sweeps (``fused=False``): ``update_H``, the source callbacks (each
-  ``add_source_*`` launches at once), ``update_E``; one thread per cell over
-  the flattened (j, k) planes;
-* fused sweep (``fused=True``): during the source phase, ``add_source_*``
-  calls are recorded instead of launched; one sweep advances H and E
-  everywhere, then a small fix-up kernel recomputes E on the source cells
-  with the recorded sources added first, which is the reference's order
-  cell by cell.  A callback that touches field data (``Ex``, ``read_point``,
-  a field setter ...) first forces the H half to run on its own, so it sees
-  exactly the state the reference would show it; the step then finishes
-  with the split E sweep.  Grids whose field slab (six volumes) reaches 4 GB,
-  beyond the fused sweep's 32-bit offsets, use the split sweeps.
+* ``"split"``: ``update_H``, the source callbacks (each ``add_source_*``
+  launches at once), ``update_E``; one thread per cell over the flattened
+  (j, k) planes, in place;
+* ``"fused"``: during the source phase, ``add_source_*`` calls are recorded
+  instead of launched; one LDS-staged sweep advances H and E everywhere,
+  then a small fix-up kernel recomputes E on the source cells with the
+  recorded sources added first, which is the reference's order cell by
+  cell;
+* ``"march"``: one register-pipelined sweep advances H and E and applies up
+  to four recorded sources in its E half (more go through the fix-up
+  kernel); bit-identical to ``"fused"``;
+* ``"pair"`` (opt-in, FP32): ``run`` advances two steps per launch,
+  bit-identical to two march steps; monitors see the state between the two
+  steps through a shadow copy of the cells they name.
 
-``fused=None`` (the default) picks the path measured faster on MI355X: the
-split sweeps for FP32 (they stream at close to copy bandwidth and leave no
-lanes idle on extents that are not multiples of 64), the fused sweep for
-FP64.  ``hipfdtd.autotune.tune_engine`` measures both for a given grid.
+On the one-launch paths a callback that touches field data (``Ex``,
+``read_point``, a field setter ...) first forces the H half to run on its
+own, so it sees exactly the state the reference would show it; the step
+then finishes with the split E sweep.
+
+``path=None`` (the default) picks the path measured faster on MI355X: the
+march sweep from 200^3 cells up, below that the split sweeps for FP32 and
+the fused sweep for FP64.  Paths whose addressing does not reach a large
+grid are not offered for it (the fused and pair sweeps need the six-volume
+slab below 4 GB and 2 GB; the march sweep takes FP32 grids past that with
+one buffer resource per volume).  ``hipfdtd.autotune.tune_engine``
+measures the paths and tilings for a given grid.
 """
 
 from __future__ import annotations
