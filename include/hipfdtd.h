@@ -193,7 +193,8 @@ int hfdtd_set_view(hfdtd_engine *e, int shadow);
 /* Materials of the engine's material index: on a lossless FP32 grid with at
  * most 48 distinct eps_r values, hfdtd_set_materials also stores a byte of
  * material per cell and a table of ce per pair of materials, which the march
- * sweep reads instead of the ce volumes (identical values); 0 when the grid
+ * sweep and the split E sweeps read instead of the ce volumes (identical
+ * values); 0 when the grid
  * has no index (more values, lossy or FP64 grids, a coefficient volume
  * written by hand).  Extension, no counterpart in the reference. */
 int hfdtd_material_count(const hfdtd_engine *e, int *nmat);

@@ -49,6 +49,11 @@ struct SimDesc {
   T *psi[2];  // psi slab, by step parity
   const T *coef;
   const T *prof;
+  // lossless FP32 engines with a material index (split path): one byte of
+  // material per cell and the ce table; null otherwise
+  const unsigned char *mat;
+  const float *mtab;
+  int nmat;
   int tile_begin, tiles_k, tiles_j;
   int lossy;
   int has_src;
@@ -182,6 +187,13 @@ __global__ __launch_bounds__(kFlatBlock) void k_batch_split_E(BatchArgs<T> a) {
     const int q = (lt - i * d.tiles_k) * kFlatBlock + threadIdx.x;
     if (q >= g.ny * g.nz) continue;
     const int j = q / g.nz;
+    if constexpr (sizeof(T) == 4) {
+      if (d.mat) {  // lossless (ca = 1), whichever variant the batch runs
+        split_e_cell_mat(g, a.inv_dl, d.st[0], d.coef, d.mat, d.mtab, d.nmat, d.prof, d.psi[0], i,
+                         j, q - j * g.nz);
+        continue;
+      }
+    }
     split_e_cell<T, LOSSY>(g, a.inv_dl, d.st[0], d.coef, d.prof, d.psi[0], i, j, q - j * g.nz);
   }
 }
@@ -302,6 +314,10 @@ static int batch_build(hfdtd_batch *b) {
     d.psi[1] = (T *)e->psi[e->cur ^ 1];
     d.coef = (const T *)e->coef;
     d.prof = (const T *)e->prof;
+    const bool mat = b->path == HFDTD_PATH_SPLIT && sizeof(T) == 4 && e->lossless && e->mat_ok;
+    d.mat = mat ? (const unsigned char *)e->mat : nullptr;
+    d.mtab = mat ? (const float *)e->mtab : nullptr;
+    d.nmat = mat ? e->nmat : 0;
     d.tile_begin = begin;
     int nt;
     if (b->path == HFDTD_PATH_SPLIT) {  // blocks of 256 cells per plane

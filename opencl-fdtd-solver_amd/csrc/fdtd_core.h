@@ -268,16 +268,16 @@ __device__ __forceinline__ void split_h_cell(const Geo &g, T dtm_dl, T dtm, T *_
   psi_store(g, psi, 0, s, p);
 }
 
-// E(i,j,k) and its psi from H (reference yee_update.cl:120-238), in place.
+// E(i,j,k) and its psi from H (reference yee_update.cl:120-238), in place,
+// with the cell's edge coefficients cx, cy, cz given.
 template <typename T, bool LOSSY>
-__device__ __forceinline__ void split_e_cell(const Geo &g, T inv_dl, T *__restrict__ st,
+__device__ __forceinline__ void split_e_core(const Geo &g, T inv_dl, T *__restrict__ st,
                                              const T *__restrict__ coef,
                                              const T *__restrict__ prof, T *__restrict__ psi,
-                                             int i, int j, int k) {
+                                             int i, int j, int k, size_t idx, T cx, T cy, T cz) {
   T *Ex = st, *Ey = st + g.vs, *Ez = st + 2 * g.vs;
   const T *Hx = st + 3 * g.vs, *Hy = st + 4 * g.vs, *Hz = st + 5 * g.vs;
   const size_t sxy = (size_t)g.ny * g.nz;
-  const size_t idx = (size_t)i * sxy + (size_t)j * g.nz + k;
   const StripeIdx s = stripe_idx(g, i, j, k);
   T p[6] = {0, 0, 0, 0, 0, 0};
   psi_load(g, psi, 6, s, p);
@@ -288,12 +288,44 @@ __device__ __forceinline__ void split_e_cell(const Geo &g, T inv_dl, T *__restri
   e_math(g, i, j, k, s, prof1(g, prof, 0, 0, i), prof1(g, prof, 0, 1, j), prof1(g, prof, 0, 2, k),
          Hx[idx], Hy[idx], Hz[idx], j > 0 ? Hx[idx - g.nz] : T(0), j > 0 ? Hz[idx - g.nz] : T(0),
          k > 0 ? Hx[idx - 1] : T(0), k > 0 ? Hy[idx - 1] : T(0), i > 0 ? Hy[idx - sxy] : T(0),
-         i > 0 ? Hz[idx - sxy] : T(0), ex, ey, ez, ax, ay, az, coef[idx], coef[g.vs + idx],
-         coef[2 * g.vs + idx], p, inv_dl);
+         i > 0 ? Hz[idx - sxy] : T(0), ex, ey, ez, ax, ay, az, cx, cy, cz, p, inv_dl);
   Ex[idx] = ex;
   Ey[idx] = ey;
   Ez[idx] = ez;
   psi_store(g, psi, 6, s, p);
+}
+
+// E(i,j,k) with ce from the coefficient volumes.
+template <typename T, bool LOSSY>
+__device__ __forceinline__ void split_e_cell(const Geo &g, T inv_dl, T *__restrict__ st,
+                                             const T *__restrict__ coef,
+                                             const T *__restrict__ prof, T *__restrict__ psi,
+                                             int i, int j, int k) {
+  const size_t idx = ((size_t)i * g.ny + j) * g.nz + k;
+  split_e_core<T, LOSSY>(g, inv_dl, st, coef, prof, psi, i, j, k, idx, coef[idx],
+                         coef[g.vs + idx], coef[2 * g.vs + idx]);
+}
+
+// E(i,j,k) on a lossless FP32 grid with a material index: one byte of
+// material per cell, and ce of an edge from the table of its two cells'
+// materials (row: the cell's own, column: the cell at i+1, j+1 or k+1; on
+// the grid's last plane, row or column the cell itself, as the edge average
+// keeps the cell value there).  The table holds the values the ce volumes
+// hold, so results are the same.
+__device__ __forceinline__ void split_e_cell_mat(const Geo &g, float inv_dl,
+                                                 float *__restrict__ st,
+                                                 const float *__restrict__ coef,
+                                                 const unsigned char *__restrict__ mat,
+                                                 const float *__restrict__ mtab, int nmat,
+                                                 const float *__restrict__ prof,
+                                                 float *__restrict__ psi, int i, int j, int k) {
+  const size_t sxy = (size_t)g.ny * g.nz;
+  const size_t idx = (size_t)i * sxy + (size_t)j * g.nz + k;
+  const float *tb = mtab + (unsigned)mat[idx] * (unsigned)nmat;
+  const float cx = tb[mat[i + 1 < g.nx ? idx + sxy : idx]];
+  const float cy = tb[mat[j + 1 < g.ny ? idx + g.nz : idx]];
+  const float cz = tb[mat[k + 1 < g.nz ? idx + 1 : idx]];
+  split_e_core<float, false>(g, inv_dl, st, coef, prof, psi, i, j, k, idx, cx, cy, cz);
 }
 
 template <typename T>

@@ -45,18 +45,26 @@ __global__ __launch_bounds__(kFlatBlock) void k_update_H(Geo g, T dtm_dl, T dtm,
   split_h_cell(g, dtm_dl, dtm, st, prof, psi, i, j, q - j * g.nz);
 }
 
-template <typename T, bool LOSSY>
+// MAT: ce from the material index (lossless FP32 grids), not the volumes.
+template <typename T, bool LOSSY, bool MAT = false>
 __global__ __launch_bounds__(kFlatBlock) void k_update_E(Geo g, T inv_dl, T *__restrict__ st,
                                                           const T *__restrict__ coef,
                                                           const T *__restrict__ prof,
                                                           T *__restrict__ psi, int bpp,
-                                                          int xcd, int run) {
+                                                          int xcd, int run,
+                                                          const unsigned char *__restrict__ mat,
+                                                          const float *__restrict__ mtab,
+                                                          int nmat) {
+  static_assert(!MAT || (sizeof(T) == 4 && !LOSSY), "the material index is lossless FP32");
   const int L = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
   const int i = L / bpp;
   const int q = (L - i * bpp) * kFlatBlock + threadIdx.x;
   if (q >= g.ny * g.nz) return;
   const int j = q / g.nz;
-  split_e_cell<T, LOSSY>(g, inv_dl, st, coef, prof, psi, i, j, q - j * g.nz);
+  if constexpr (MAT)
+    split_e_cell_mat(g, inv_dl, st, coef, mat, mtab, nmat, prof, psi, i, j, q - j * g.nz);
+  else
+    split_e_cell<T, LOSSY>(g, inv_dl, st, coef, prof, psi, i, j, q - j * g.nz);
 }
 
 // ── sources ──────────────────────────────────────────────────────────────
@@ -130,11 +138,16 @@ template <typename T>
 static int update_E_t(Engine *e) {
   const unsigned bpp = ceil_div((size_t)e->ny * e->nz, kFlatBlock);
   auto kern = e->lossless ? k_update_E<T, false> : k_update_E<T, true>;
+  const bool mat = sizeof(T) == 4 && e->lossless && e->mat_ok;
+  if constexpr (sizeof(T) == 4)
+    if (mat) kern = k_update_E<float, false, true>;
   int rc = timing_mark(e);
   if (rc) return rc;
   hipLaunchKernelGGL(kern, dim3(bpp * (unsigned)e->nx), dim3(kFlatBlock), 0, e->stream,
                      geo_of(e), T(e->inv_dl), (T *)e->state[e->cur], (const T *)e->coef,
-                     (const T *)e->prof, (T *)e->psi[e->cur], (int)bpp, e->xcd, e->split_run);
+                     (const T *)e->prof, (T *)e->psi[e->cur], (int)bpp, e->xcd, e->split_run,
+                     mat ? (const unsigned char *)e->mat : nullptr,
+                     mat ? (const float *)e->mtab : nullptr, mat ? e->nmat : 0);
   HF_HIP(hipGetLastError());
   return timing_mark(e);
 }

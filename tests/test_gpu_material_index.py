@@ -1,11 +1,14 @@
-"""Material-indexed march sweep (lossless FP32 grids with few distinct eps_r).
+"""Material-indexed sweeps (lossless FP32 grids with few distinct eps_r).
 
 An edge's ce depends only on the eps_r of its two cells, so on a grid with
-at most 48 distinct values the march sweep reads one byte of material per
-cell and takes ce from a table computed exactly as the ce volumes are.
-Results must equal the fused sweep (which reads the ce volumes) bit for bit;
-grids with more values, lossy grids and grids whose ce volumes were written
-by hand fall back to the volumes.
+at most 48 distinct values the march sweep and the split E sweep (single
+engine and batched) read one byte of material per cell and take ce from a
+table computed exactly as the ce volumes are.  Results must equal the
+sweeps that read the ce volumes bit for bit: the fused sweep for the march
+sweep, and the split sweeps of the same engine with the index turned off
+(a ce volume rewritten with its own values) for the split sweeps.  Grids
+with more values, lossy grids and grids whose ce volumes were written by
+hand fall back to the volumes.
 """
 
 from __future__ import annotations
@@ -54,7 +57,7 @@ def _run(path, shape, npml, eps, nsrc, steps, poke=None, rows=0, planes=0, mater
     sim.set_epsilon(eps)
     if poke is not None:
         poke(sim)
-    if path == "march" and materials is not None:
+    if path in ("march", "split") and materials is not None:
         assert sim.material_count() == materials, (sim.material_count(), materials)
     nx, ny, nz = shape
 
@@ -92,6 +95,64 @@ def test_indexed_march_matches_fused(shape, npml, nvals, nsrc, eps_dtype, indexe
     want = nvals if indexed else 0
     _same(_run("fused", shape, npml, eps, nsrc, 16),
           _run("march", shape, npml, eps, nsrc, 16, rows=rows, materials=want))  # fmt: skip
+
+
+def _index_off(sim):
+    """Rewrite ce_x with its own values: the same coefficients, no index."""
+    sim.write_volume("ce_x", sim.read_volume("ce_x"))
+
+
+@pytest.mark.parametrize("shape,npml,nvals,nsrc,eps_dtype,indexed", [
+    ((45, 35, 100), 6, 5, 3, np.float32, True),
+    ((40, 52, 70), 8, 48, 4, np.float64, True),
+    ((40, 52, 70), 8, 49, 2, np.float32, False),
+    ((33, 40, 130), 0, 12, 6, np.float32, True),
+    ((29, 31, 17), 4, 2, 2, np.float32, True)])  # fmt: skip
+def test_indexed_split_matches_volume_split(shape, npml, nvals, nsrc, eps_dtype, indexed):
+    print(f"[mat-split] shape={shape} npml={npml} nvals={nvals} nsrc={nsrc}", flush=True)
+    eps = _boxes_eps(shape, nvals, 5, eps_dtype)
+    assert len(np.unique(eps)) == nvals
+    _same(_run("split", shape, npml, eps, nsrc, 16, materials=nvals if indexed else 0),
+          _run("split", shape, npml, eps, nsrc, 16, _index_off, materials=0))  # fmt: skip
+
+
+def test_indexed_batch_split_matches_volume_batch():
+    """A batch of indexed and unindexed engines on the split path against the
+    same batch with every index off, and a lone indexed engine's serial
+    split run."""
+    shapes = [(30, 26, 40), (22, 35, 19), (41, 24, 33)]
+    print(f"[mat-batch] shapes={shapes}", flush=True)
+    epss = [_boxes_eps(sh, 3 + q, 6 + q) for q, sh in enumerate(shapes)]
+
+    def run(off):
+        sims = []
+        for q, (sh, eps) in enumerate(zip(shapes, epss)):
+            sim = hip().HIPFDTD(sh, DL, npml=4)
+            sim.set_epsilon(eps)
+            if q in off:
+                _index_off(sim)
+            assert sim.material_count() == (0 if q in off else 3 + q)
+            sims.append(sim)
+        from hipfdtd.batch import BatchedFDTD
+
+        b = BatchedFDTD(sims)
+        b.set_path("split")
+        box = [(s[0] // 2, s[0] // 2 + 1, s[1] // 2, s[1] // 2 + 1, s[2] // 2, s[2] // 2 + 1)
+               for s in shapes]  # fmt: skip
+        b.set_sources([("z", bx) for bx in box],
+                      [lambda t, f=f: fo.gaussian_sine(t, f, 0.5 * f) for f in (2e10, 3e10, 4e10)],
+                      24)  # fmt: skip
+        b.run(24, graph_steps=8)
+        out = [s.fields() for s in sims]
+        b.close()
+        for s in sims:
+            s.close()
+        return out
+
+    got, want = run({1}), run({0, 1, 2})
+    for q in range(len(shapes)):
+        _same(got[q], want[q])
+        assert float(np.max(np.abs(got[q]["Ez"]))) > 0
 
 
 def test_hand_written_coefficients_turn_the_index_off():
