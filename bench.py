@@ -85,6 +85,7 @@ sys.path.insert(0, str(ROOT / "opencl-fdtd-solver_amd"))
 
 C0 = 299_792_458.0
 HBM_PEAK_GBS = 8000.0
+TIMING_EVERY = 15  # one sweep launch in 15 is timed with HIP events
 FIELDS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 
 
@@ -167,7 +168,11 @@ def run_gpu(args, wl: Workload):
     sim.run(args.warmup)  # the pair path takes two steps per launch in run()
     mon.flush()  # the window starts with an empty K-step buffer
     sim.synchronize()
-    sim.kernel_timing(True)
+    # HIP events on a sample of the window's sweep launches (every 15th: an
+    # odd stride times the split path's H and E sweeps alike); each event
+    # record costs the stream a few microseconds, so timing every launch
+    # would slow the window it measures
+    sim.kernel_timing(True, every=TIMING_EVERY)
     t0 = time.perf_counter()
     sim.run(args.steps)
     # every face-DFT sample of the window is folded into the sums inside it:
@@ -175,7 +180,8 @@ def run_gpu(args, wl: Workload):
     mon.flush()
     sim.synchronize()
     t1 = time.perf_counter()
-    kern_ms, launches = sim.kernel_timing_read()
+    kern_ms, timed = sim.kernel_timing_read()
+    launches = sim.kernel_timing_launches()
     sim.kernel_timing(False)
     print(f"[bench] timed {args.steps} steps in {t1 - t0:.3f} s", file=sys.stderr, flush=True)
 
@@ -184,7 +190,8 @@ def run_gpu(args, wl: Workload):
     eh = [mon.get_farfields(obs, freq_index=f) for f in range(wl.n_freq)]
     t3 = time.perf_counter()
     finite = bool(np.all(np.isfinite(eh)))
-    return dict(wall_s=t1 - t0, kern_ms=kern_ms, launches=launches, n2f_ms=(t3 - t2) * 1e3,
+    return dict(wall_s=t1 - t0, kern_ms=kern_ms, timed=timed, launches=launches,
+                n2f_ms=(t3 - t2) * 1e3,
                 finite=finite, device=sim.device_name, path=sim.path,
                 tuned=tuned)  # fmt: skip
 
@@ -702,8 +709,8 @@ def main():
     ms_step = res["wall_s"] / args.steps * 1e3
     b_alg = 72.0 * cells + 8.0 * wl.psi_entries()
     # sweep kernels per step: k_fused, k_march, k_update_H + k_update_E, or
-    # half a k_pair launch
-    step_kern_s = res["kern_ms"] / args.steps / 1e3
+    # half a k_pair launch (mean of the timed launches x launches per step)
+    step_kern_s = res["kern_ms"] / res["timed"] * res["launches"] / args.steps / 1e3
     achieved = b_alg / step_kern_s / 1e9
     traffic = measured_traffic(res["path"])
     kernel = KERNELS[res["path"]]
@@ -739,6 +746,7 @@ def main():
             "bytes_alg_per_step": b_alg,
             "sweep_us_per_step": round(step_kern_s * 1e6, 2),
             "launches_per_step": round(res["launches"] / args.steps, 3),
+            "timed_launches": res["timed"],
             # achieved/frac count algorithmic bytes; these count the bytes
             # the counters saw (traffic) over the same sweep time
             "traffic_gbs": None if traffic is None else round(traffic / step_kern_s / 1e9, 1),

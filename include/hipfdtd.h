@@ -202,11 +202,18 @@ int hfdtd_material_count(const hfdtd_engine *e, int *nmat);
 int hfdtd_set_pair_planes(hfdtd_engine *e, int planes_i);
 
 /* HIP-event timing of the Yee sweep kernels on the engine's stream: while
- * enabled, every update_H / update_E / fused sweep launch is bracketed by a
- * pair of events.  enable=1 clears earlier records. */
+ * enabled, every n-th update_H / update_E / fused / march / pair sweep launch
+ * (n = enable: 1 times every launch, 0 turns timing off) is bracketed by a
+ * pair of events.  Each event record costs the stream a few microseconds,
+ * so a sample (an odd n, which times the H and E sweeps of the split path
+ * alike) keeps the timed run close to an untimed one.  Enabling clears
+ * earlier records.  Extension, no counterpart in the reference. */
 int hfdtd_timing(hfdtd_engine *e, int enable);
-/* synchronizes the stream; total milliseconds and number of sweep launches */
+/* synchronizes the stream; total milliseconds and number of the timed sweep
+ * launches */
 int hfdtd_timing_read(hfdtd_engine *e, double *total_ms, int *launches);
+/* sweep launches while timing was on, timed or not.  Extension. */
+int hfdtd_timing_launches(const hfdtd_engine *e, long long *launches);
 
 /* ── near-to-far monitor (monitors.py:383-804, dft_farfield.cl:98-635) ── */
 

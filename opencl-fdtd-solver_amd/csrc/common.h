@@ -130,9 +130,15 @@ struct Engine {
   }
 
   // HIP-event timing of the Yee sweeps (hfdtd_timing / hfdtd_timing_read):
-  // events are recorded on `stream` around every sweep launch while enabled.
+  // while enabled, events are recorded on `stream` around every
+  // timing_every-th sweep launch.  timing_mark is called before and after
+  // each launch; timing_after says which of the two comes next, timing_take
+  // whether the launch in between is timed.
   bool timing = false;
-  std::vector<hipEvent_t> events;  // pool; [2q, 2q+1] bracket launch q
+  int timing_every = 1;
+  long long timing_launches = 0;  // sweep launches since timing was enabled
+  bool timing_after = false, timing_take = false;
+  std::vector<hipEvent_t> events;  // pool; [2q, 2q+1] bracket timed launch q
   size_t events_used = 0;
 
   char *vol(int slab_parity, int c) const {

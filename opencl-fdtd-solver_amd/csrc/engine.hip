@@ -190,6 +190,11 @@ static void release(Engine *e) {
 
 int timing_mark(Engine *e) {
   if (!e->timing) return HFDTD_OK;
+  if (!e->timing_after) {  // before a launch: is it one of the sample?
+    e->timing_take = e->timing_launches++ % e->timing_every == 0;
+  }
+  e->timing_after = !e->timing_after;
+  if (!e->timing_take) return HFDTD_OK;
   if (e->events_used == e->events.size()) {
     hipEvent_t ev;
     HF_HIP(hipEventCreate(&ev));
@@ -683,10 +688,22 @@ int hfdtd_synchronize(hfdtd_engine *h) {
 
 int hfdtd_timing(hfdtd_engine *h, int enable) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  if (enable < 0) return fail(HFDTD_EINVAL, "enable must be 0 (off) or n >= 1 (every n-th launch)");
   Engine *e = &h->s;
   if (enable && e->events_used % 2) return fail(HFDTD_EINVAL, "unbalanced timing events");
   e->timing = enable != 0;
-  if (enable) e->events_used = 0;
+  if (enable) {
+    e->events_used = 0;
+    e->timing_every = enable;
+    e->timing_launches = 0;
+    e->timing_after = e->timing_take = false;
+  }
+  return HFDTD_OK;
+}
+
+int hfdtd_timing_launches(const hfdtd_engine *h, long long *launches) {
+  if (!h || !launches) return fail(HFDTD_EINVAL, "NULL argument");
+  *launches = h->s.timing_launches;
   return HFDTD_OK;
 }
 

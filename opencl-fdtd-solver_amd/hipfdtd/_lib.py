@@ -98,6 +98,7 @@ EXPORTS = {
     "hfdtd_set_march_addressing": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "hfdtd_timing_launches": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
     "hfdtd_n2f_create": (
         C.c_int,
         [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_void_p)],

@@ -578,16 +578,26 @@ class HIPFDTD(SourceMonitorMixin):
         256-cell blocks per XCD (1 = hardware round robin, 0 keeps)."""
         _lib.check(self._lib.hfdtd_set_split_schedule(self._handle, int(run_blocks)))
 
-    def kernel_timing(self, enable: bool) -> None:
-        """Start (True) or stop (False) HIP-event timing of the sweep kernels."""
-        _lib.check(self._lib.hfdtd_timing(self._handle, int(bool(enable))), "timing")
+    def kernel_timing(self, enable: bool, every: int = 1) -> None:
+        """Start (True) or stop (False) HIP-event timing of the sweep kernels:
+        every ``every``-th sweep launch is bracketed by a pair of events (an
+        odd ``every`` samples the H and E sweeps of the split path alike)."""
+        if int(every) < 1:
+            raise ValueError("every must be at least 1")
+        _lib.check(self._lib.hfdtd_timing(self._handle, int(every) if enable else 0), "timing")
 
     def kernel_timing_read(self):
-        """(total milliseconds, launches) of the sweep kernels timed so far."""
+        """(total milliseconds, launches) of the sweep launches timed so far."""
         ms = C.c_double(0.0)
         n = C.c_int(0)
         _lib.check(self._lib.hfdtd_timing_read(self._handle, C.byref(ms), C.byref(n)), "timing")
         return float(ms.value), int(n.value)
+
+    def kernel_timing_launches(self) -> int:
+        """Sweep launches since timing was enabled, timed or not."""
+        n = C.c_longlong(0)
+        _lib.check(self._lib.hfdtd_timing_launches(self._handle, C.byref(n)), "timing")
+        return int(n.value)
 
     # ── field access (engine.py:747-789) ──
 

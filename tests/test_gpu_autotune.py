@@ -100,3 +100,35 @@ def test_tune_batch_applies_schedule(tmp_path):
     b.run(4)
     assert all(np.any(s.read_volume("Ez")) for s in sims)
     b.close()
+
+
+@pytest.mark.parametrize("path,per_step", [("split", 2), ("fused", 1), ("march", 1)])
+def test_sampled_sweep_timing(path, per_step):
+    """Timing every n-th sweep launch: the sample holds launches 0, n, 2n ...
+    of all the sweep launches while timing is on (the split path's H and E
+    sweeps alternate), each with a positive duration, and stepping is
+    unchanged by it."""
+    import hipfdtd
+
+    sim = hipfdtd.HIPFDTD(SHAPE, DL, npml=6, path=path)
+    ref = hipfdtd.HIPFDTD(SHAPE, DL, npml=6, path=path)
+    for s in (sim, ref):
+        s.add_source(src)
+        s.run(3)
+    steps, every = 10, 3
+    sim.kernel_timing(True, every=every)
+    sim.run(steps)
+    ms, timed = sim.kernel_timing_read()
+    launches = sim.kernel_timing_launches()
+    sim.kernel_timing(False)
+    print(f"[timing] {path}: {timed} of {launches} launches timed, {ms:.3f} ms", flush=True)
+    assert launches == steps * per_step
+    assert timed == -(-launches // every)
+    assert ms > 0
+    sim.run(2)  # timing off: no more launches counted or timed
+    assert sim.kernel_timing_read()[1] == timed and sim.kernel_timing_launches() == launches
+    ref.run(steps + 2)
+    for name in ("Ex", "Hz"):
+        assert np.array_equal(sim.read_volume(name), ref.read_volume(name)), name
+    with pytest.raises(ValueError):
+        sim.kernel_timing(True, every=0)
