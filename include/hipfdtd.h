@@ -198,6 +198,13 @@ int hfdtd_set_view(hfdtd_engine *e, int shadow);
  * has no index (more values, lossy or FP64 grids, a coefficient volume
  * written by hand).  Extension, no counterpart in the reference. */
 int hfdtd_material_count(const hfdtd_engine *e, int *nmat);
+/* Where the march sweep takes ce from: 0 the three ce volumes, 1 the
+ * material index, 2 the eps_r volume (lossless FP32 grids without an index
+ * whose eps_r values are exact in FP32: hfdtd_set_materials keeps them, and
+ * the sweep recomputes each edge's ce in FP64 as the coefficients were made,
+ * identical values).  Writing a coefficient volume returns it to 0 until the
+ * next hfdtd_set_materials.  Extension, no counterpart in the reference. */
+int hfdtd_ce_source(const hfdtd_engine *e, int *source);
 /* i planes per block of hfdtd_step_pair (0 = 24, at most 1024).  Extension. */
 int hfdtd_set_pair_planes(hfdtd_engine *e, int planes_i);
 

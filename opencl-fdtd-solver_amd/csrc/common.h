@@ -41,6 +41,14 @@ constexpr int kPsiAxis[PSI_COUNT] = {0, 0, 1, 1, 2, 2, 0, 0, 1, 1, 2, 2};
 
 constexpr int kMatMax = 48;  // materials of an indexed grid at most (march sweep)
 
+// Where the march sweep takes an edge's ce from on lossless FP32 grids: the
+// three ce volumes; the material index (a byte per cell and a table of ce
+// per pair of materials); or a volume of eps_r (an FP32 value per cell,
+// exact copies of the values the coefficients were made from), ce
+// recomputed per edge in FP64 and rounded, the arithmetic of the
+// coefficient kernel, so the values are those of the volumes.
+constexpr int kCeVolumes = 0, kCeIndex = 1, kCeEps = 2;
+
 // The z-face record of the march sweep.  A Huygens-box monitor whose
 // z faces lie at k = kf[0] and kf[1] (each at least 1, two or more apart)
 // asks the engine to keep, for every i in [i0, i1] and j in [j0, j1] (its
@@ -108,7 +116,7 @@ struct Engine {
   bool march_per_volume = false;  // per-volume addressing on any FP32 slab (set by hand)
   int march_nt = -1;
   int march_occ[2][3] = {};  // resident march blocks per CU by [lossy][rows]
-  int march_occ_mat[3] = {};  // the same for the material-indexed sweep by rows
+  int march_occ_mat[3][3] = {};  // the same for the index / eps_r sweeps by [ce source][rows]
   // Lossless FP32 grids with at most kMatMax distinct eps_r values: a byte of
   // material per cell and the table of ce for every pair of materials (an
   // edge's ce depends only on its two cells), which the march sweep reads
@@ -117,6 +125,14 @@ struct Engine {
   float *mtab = nullptr;
   int nmat = 0;
   bool mat_ok = false;
+  // Other lossless FP32 grids whose eps_r values are exact in FP32: the
+  // eps_r volume (ncell + 64 floats), from which the march sweep recomputes
+  // ce.  eps_ok while it matches the ce volumes.
+  void *epsv = nullptr;
+  bool eps_ok = false;
+  int ce_source() const {
+    return !lossless || fp64 ? kCeVolumes : mat_ok ? kCeIndex : eps_ok ? kCeEps : kCeVolumes;
+  }
 
   uint64_t bytes = 0;
 

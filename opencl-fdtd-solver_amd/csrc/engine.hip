@@ -176,6 +176,8 @@ static void release(Engine *e) {
     e->mtab = nullptr;
   }
   e->mat_ok = false;
+  fr(e->epsv);
+  e->eps_ok = false;
   {
     void *t = e->zrec.buf;
     fr(t);
@@ -305,6 +307,40 @@ static int uniform_material_index(Engine *e) {
   e->nmat = 1;
   e->mat_ok = true;
   return HFDTD_OK;
+}
+
+// The eps_r volume of a lossless FP32 grid without a material index: the
+// march sweep recomputes ce from it.  Only when every value is exact in
+// FP32 (always for FP32 input), so ce from it equals the ce volumes, which
+// were made from the values as given; d_eps holds them on the device.  Best
+// effort: without room for the volume the sweep reads the ce volumes.
+template <typename S>
+static void build_eps_volume(Engine *e, const S *host, const void *d_eps) {
+  e->eps_ok = false;
+  std::vector<float> conv;
+  if constexpr (sizeof(S) == 8) {
+    conv.resize(e->ncell);
+    for (size_t c = 0; c < e->ncell; ++c) {
+      const float f = (float)host[c];
+      if ((double)f != host[c]) return;  // not exact in FP32 (NaN included)
+      conv[c] = f;
+    }
+  }
+  if (((double)e->ncell + 64.0) * 4.0 >= 2147483648.0) return;  // one buffer resource reaches it
+  if (!e->epsv && dmalloc(e, &e->epsv, (e->ncell + 64) * sizeof(float)) != HFDTD_OK) {
+    (void)hipGetLastError();
+    return;
+  }
+  hipError_t err =
+      sizeof(S) == 8
+          ? hipMemcpyAsync(e->epsv, conv.data(), e->ncell * 4, hipMemcpyHostToDevice, e->stream)
+          : hipMemcpyAsync(e->epsv, d_eps, e->ncell * 4, hipMemcpyDeviceToDevice, e->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  e->eps_ok = true;
 }
 
 }  // namespace hfdtd
@@ -531,13 +567,18 @@ int hfdtd_set_materials(hfdtd_engine *h, const void *eps_r, const void *sigma, i
   }
   if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
   if (err != hipSuccess) rc = hip_fail(err, "set_materials");
-  (void)hipFree(d_eps);
-  if (d_sig) (void)hipFree(d_sig);
   e->lossless = sigma == nullptr;
   e->mat_ok = false;
+  e->eps_ok = false;
   if (rc == HFDTD_OK && !e->fp64 && sigma == nullptr)
     rc = src_fp64 ? build_material_index(e, (const double *)eps_r)
                   : build_material_index(e, (const float *)eps_r);
+  if (rc == HFDTD_OK && !e->fp64 && sigma == nullptr && !e->mat_ok) {
+    if (src_fp64) build_eps_volume(e, (const double *)eps_r, d_eps);
+    else build_eps_volume(e, (const float *)eps_r, d_eps);
+  }
+  (void)hipFree(d_eps);
+  if (d_sig) (void)hipFree(d_sig);
   return rc;
 }
 
@@ -546,7 +587,7 @@ int hfdtd_write_volume(hfdtd_engine *h, int which, const void *host) {
   Engine *e = &h->s;
   void *slot = volume_ptr(e, which);
   if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
-  if (which >= HFDTD_CEX) e->mat_ok = false;  // coefficients no longer from the index
+  if (which >= HFDTD_CEX) e->mat_ok = e->eps_ok = false;  // coefficients no longer from eps_r
   HF_HIP(hipSetDevice(e->device));
   if (which <= HFDTD_HZ) e->touch();
   HF_HIP(hipMemcpyAsync(slot, host, e->ncell * e->elem, hipMemcpyHostToDevice, e->stream));
@@ -669,6 +710,12 @@ int hfdtd_set_view(hfdtd_engine *h, int shadow) {
 int hfdtd_material_count(const hfdtd_engine *h, int *nmat) {
   if (!h || !nmat) return fail(HFDTD_EINVAL, "NULL argument");
   *nmat = h->s.mat_ok ? h->s.nmat : 0;
+  return HFDTD_OK;
+}
+
+int hfdtd_ce_source(const hfdtd_engine *h, int *source) {
+  if (!h || !source) return fail(HFDTD_EINVAL, "NULL argument");
+  *source = h->s.ce_source();
   return HFDTD_OK;
 }
 

@@ -141,7 +141,8 @@ struct MarchH {
 template <typename T>
 struct MarchE {
   T cx, cy, cz;  // ce(i)
-  unsigned mi1, mj1, mk1;  // indexed grids: material of the cells at i+1, j+1, k+1
+  // material (kCeIndex) or eps_r bits (kCeEps) of the cells at i+1, j+1, k+1
+  unsigned mi1, mj1, mk1;
   T ax, ay, az;  // ca(i) (lossy grids)
   T p[6];        // old psi of the E slots (PML tiles)
 };
@@ -153,7 +154,7 @@ struct MarchSlot {
   unsigned o4;   // byte offset of (j, k) inside a plane
   unsigned qy4;  // byte offset inside a y-stripe plane (kDropOffset outside the stripe)
   unsigned qz4;  // byte offset inside the z-stripe rows of a plane (kDropOffset outside)
-  unsigned o1, o1j, o1k;  // material-index offsets of (j, k), (j+1, k), (j, k+1), clamped
+  unsigned o1, o1j, o1k;  // ce-source byte offsets of (j, k), (j+1, k), (j, k+1), clamped
   bool iny, inz;
   bool own;      // an owned cell inside the grid
   int lh, le;    // LDS element index (row * kMarchCols + column) of its H and E
@@ -195,9 +196,11 @@ struct PsiPlane {
   bool dead;  // a plane past the chunk: its loads are dropped
 };
 
-template <typename T, int RW, bool LOSSY, bool X, bool Y, bool Z, bool NT, bool MAT = false,
+template <typename T, int RW, bool LOSSY, bool X, bool Y, bool Z, bool NT, int CEM = kCeVolumes,
           bool CAP = false, bool BIG = false>
 struct March {
+  static constexpr bool MAT = CEM != kCeVolumes;  // ce from a per-cell volume, not the ce volumes
+  static constexpr unsigned CB = CEM == kCeEps ? 4u : 1u;  // bytes per cell of that volume
   static constexpr int S = RW + 1;  // RW owned rows + the halo slot
   static constexpr int TJ = 4 * RW;
   static constexpr unsigned E = sizeof(T);
@@ -210,11 +213,13 @@ struct March {
   int nsrc;
   int src_i0, src_i1;  // planes on which some source can reach this tile
   // indexed grids: one byte of material per cell, ce of an edge from the
-  // table of its two cells' materials (LDS copy)
+  // table of its two cells' materials (LDS copy); eps_r grids: the eps_r
+  // volume, ce = cscale / (eps_r averaged over the edge's two cells)
   Rsrc rmat;
   const float *mtab;
   int nmat;
-  unsigned mc[RW];  // material of the owned cells on the current plane
+  double cscale;    // dt / eps0 (kCeEps)
+  unsigned mc[RW];  // material (or eps_r bits) of the owned cells on the current plane
   MarchSlot sl[S];
   Prof1<T> pyh[S], pye[S], pzh[S], pze[S];
   T hyc[RW], hzc[RW];  // new H(i-1) at the owned cells
@@ -323,16 +328,27 @@ struct March {
   }
 
   __device__ __forceinline__ unsigned mat_plane(int i, bool dead) const {
-    return dead ? kDropOffset : (unsigned)min(i, g.nx - 1) * (unsigned)(g.ny * g.nz);
+    return dead ? kDropOffset : (unsigned)min(i, g.nx - 1) * (unsigned)(g.ny * g.nz) * CB;
+  }
+  // one value of the material index or of the eps_r volume
+  __device__ __forceinline__ unsigned ld_mat(unsigned voff, unsigned soff) const {
+    if constexpr (CEM == kCeEps) return __builtin_amdgcn_raw_buffer_load_b32(rmat, voff, soff, 0);
+    else return __builtin_amdgcn_raw_buffer_load_b8(rmat, voff, soff, 0);
+  }
+  // ce of an edge between cells of eps_r a and b (bit patterns), as the
+  // coefficient kernel computes it
+  __device__ __forceinline__ float ce_eps(unsigned a, unsigned b) const {
+    const double ea = (double)__builtin_bit_cast(float, a), eb = (double)__builtin_bit_cast(float, b);
+    return (float)(cscale / (0.5 * (ea + eb)));
   }
 
   __device__ __forceinline__ void load_e(MarchE<T> &s, const MarchSlot &m, unsigned pl,
                                          unsigned pm, unsigned pm1, const PsiPlane &pp) const {
     const unsigned vs = vsb();
     if constexpr (MAT) {
-      s.mi1 = __builtin_amdgcn_raw_buffer_load_b8(rmat, m.o1, pm1, 0);
-      s.mj1 = __builtin_amdgcn_raw_buffer_load_b8(rmat, m.o1j, pm, 0);
-      s.mk1 = __builtin_amdgcn_raw_buffer_load_b8(rmat, m.o1k, pm, 0);
+      s.mi1 = ld_mat(m.o1, pm1);
+      s.mj1 = ld_mat(m.o1j, pm);
+      s.mk1 = ld_mat(m.o1k, pm);
     } else {
       s.cx = ldc(0, m.o4, pl);
       s.cy = ldc(1, m.o4, pl);
@@ -503,12 +519,20 @@ struct March {
       }
       T ex = oe[q][0], ey = oe[q][1], ez = oe[q][2];
       T cx, cy, cz;
-      if constexpr (MAT) {
+      if constexpr (CEM == kCeIndex) {
         const float *tb = mtab + mc[q] * (unsigned)nmat;
         cx = tb[ec[q].mi1];
         cy = tb[ec[q].mj1];
         cz = tb[ec[q].mk1];
         mc[q] = ec[q].mi1;  // the cell's material on the next plane
+      } else if constexpr (CEM == kCeEps) {
+        // on the grid's last plane, row or column the neighbour offsets are
+        // clamped to the cell itself: 0.5 (e + e) = e, the coefficient
+        // kernel's value there
+        cx = ce_eps(mc[q], ec[q].mi1);
+        cy = ce_eps(mc[q], ec[q].mj1);
+        cz = ce_eps(mc[q], ec[q].mk1);
+        mc[q] = ec[q].mi1;
       } else {
         cx = ec[q].cx;
         cy = ec[q].cy;
@@ -641,9 +665,9 @@ struct March {
       m.j = min(max(j, 0), g.ny - 1);
       m.k = min(max(k, 0), g.nz - 1);
       m.o4 = (unsigned)(m.j * g.nz + m.k) * E;
-      m.o1 = (unsigned)(m.j * g.nz + m.k);
-      m.o1j = (unsigned)(min(m.j + 1, g.ny - 1) * g.nz + m.k);
-      m.o1k = (unsigned)(m.j * g.nz + min(m.k + 1, g.nz - 1));
+      m.o1 = (unsigned)(m.j * g.nz + m.k) * CB;
+      m.o1j = (unsigned)(min(m.j + 1, g.ny - 1) * g.nz + m.k) * CB;
+      m.o1k = (unsigned)(m.j * g.nz + min(m.k + 1, g.nz - 1)) * CB;
       // the halo slot's psi only for lanes that compute a needed H
       const bool hpsi = q < RW || (need & 0b011000) != 0;
       m.iny = Y && n > 0 && hpsi && (m.j < n || m.j >= g.ny - n);
@@ -698,7 +722,7 @@ struct March {
       if constexpr (MAT) {
         const unsigned pm = mat_plane(i0, false);
 #pragma unroll
-        for (int q = 0; q < RW; ++q) mc[q] = __builtin_amdgcn_raw_buffer_load_b8(rmat, sl[q].o1, pm, 0);
+        for (int q = 0; q < RW; ++q) mc[q] = ld_mat(sl[q].o1, pm);
       }
     }
     load_h_all(A, i0);
@@ -788,7 +812,7 @@ struct MarchChunks {
 // column) meet a z-stripe.  (The plane before the chunk, recomputed once,
 // reads its psi in any variant.)  Fewer psi operations per plane also keep
 // a plane's loads and stores within the 63 a wavefront can count.
-template <typename T, int RW, bool LOSSY, bool NT, bool MAT, bool BIG = false>
+template <typename T, int RW, bool LOSSY, bool NT, int CEM, bool BIG = false>
 __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) void k_march(Geo g, int planes, T dtm_dl, T dtm,
                                                          T inv_dl, const T *__restrict__ s0,
                                                          T *__restrict__ s1,
@@ -798,17 +822,18 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
                                                          T *__restrict__ q1, int tiles_k,
                                                          int tiles_j, MarchChunks ch, int xcd,
                                                          int run, const MarchSources<T> src,
-                                                         const unsigned char *__restrict__ mat,
+                                                         const void *__restrict__ mat,
                                                          const float *__restrict__ mtab,
-                                                         int nmat, const ZCap zc) {
+                                                         int nmat, double cscale,
+                                                         const ZCap zc) {
   __shared__ MarchLds<T, RW> L;
   __shared__ Src<T> lsrc[kMarchSources];
-  __shared__ float ltab[MAT ? kMatMax * kMatMax : 1];
+  __shared__ float ltab[CEM == kCeIndex ? kMatMax * kMatMax : 1];
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int t = 0; t < kMarchSources; ++t) lsrc[t] = src.s[t];
   }  // visible after the barrier that ends the prologue
-  if constexpr (MAT) {
+  if constexpr (CEM == kCeIndex) {
     for (int t = threadIdx.x; t < nmat * nmat; t += kMarchThreads) ltab[t] = mtab[t];
   }
   const int B = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
@@ -848,10 +873,15 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
   const Rsrc rq0 = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(q0), (short)0, psi_bytes,
                                                      0x00020000);
   const Rsrc rq1 = __builtin_amdgcn_make_buffer_rsrc(q1, (short)0, psi_bytes, 0x00020000);
-  // the material index: one byte per cell, sized so loads of dead planes drop
+  // the material index (one byte per cell) or the eps_r volume (four),
+  // sized so loads of dead planes drop (the eps_r volume stays below 2 GB,
+  // checked on the host)
+  constexpr unsigned long long cb = CEM == kCeEps ? 4ull : 1ull;
   const Rsrc rmat = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<unsigned char *>(mat), (short)0,
-      MAT ? (int)min((unsigned long long)g.nx * g.ny * g.nz + 64ull, 0x7fffffffull) : 0, 0x00020000);
+      const_cast<void *>(mat), (short)0,
+      CEM != kCeVolumes ? (int)min(((unsigned long long)g.nx * g.ny * g.nz + 64ull) * cb, 0x7fffffffull)
+                        : 0,
+      0x00020000);
   // the z-face record (FP32): blocks whose columns, rows and planes meet it
   constexpr bool kCapable = sizeof(T) == 4 && !BIG;
   bool cap = false;
@@ -863,9 +893,10 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
   const Rsrc rcap = __builtin_amdgcn_make_buffer_rsrc(zc.buf, (short)0, (int)zc.bytes, 0x00020000);
 #define HF_MARCH_RUN1(X, Y, Z, C)                                                              \
   {                                                                                           \
-    March<T, RW, LOSSY, X, Y, Z, NT, MAT, C, BIG> m{g,   dtm_dl, dtm,  inv_dl, r0,     r1,      \
+    March<T, RW, LOSSY, X, Y, Z, NT, CEM, C, BIG> m{g,   dtm_dl, dtm,  inv_dl, r0,     r1,      \
                                                     rc,  rq0,    rq1,  prof,   lsrc,   src.n,   \
                                                     si0,    si1,  rmat, ltab, nmat};            \
+    m.cscale = cscale;                                                                        \
     m.s0p = s0;                                                                               \
     m.s1p = s1;                                                                               \
     m.cp = coef;                                                                              \

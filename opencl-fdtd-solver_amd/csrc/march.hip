@@ -22,14 +22,17 @@ static bool march_big_fits(const Engine *e) {
 // the launcher for slabs of 4 GB and more
 static int march_big(Engine *e, const SrcList<float> &sl) {
   const bool ll = e->lossless;
-  if (e->march_rows == 1 || (e->march_rows == 0 && ll && e->mat_ok)) {
-    if (ll && e->mat_ok) return march_rw<float, 1, false, true, true, true>(e, sl);
-    return ll ? march_rw<float, 1, false, true, false, true>(e, sl)
-              : march_rw<float, 1, true, true, false, true>(e, sl);
+  const int ce = e->ce_source();
+  if (e->march_rows == 1 || (e->march_rows == 0 && ce == kCeIndex)) {
+    if (ce == kCeIndex) return march_rw<float, 1, false, true, kCeIndex, true>(e, sl);
+    if (ce == kCeEps) return march_rw<float, 1, false, true, kCeEps, true>(e, sl);
+    return ll ? march_rw<float, 1, false, true, kCeVolumes, true>(e, sl)
+              : march_rw<float, 1, true, true, kCeVolumes, true>(e, sl);
   }
-  if (ll && e->mat_ok) return march_rw<float, 2, false, true, true, true>(e, sl);
-  return ll ? march_rw<float, 2, false, true, false, true>(e, sl)
-            : march_rw<float, 2, true, true, false, true>(e, sl);
+  if (ce == kCeIndex) return march_rw<float, 2, false, true, kCeIndex, true>(e, sl);
+  if (ce == kCeEps) return march_rw<float, 2, false, true, kCeEps, true>(e, sl);
+  return ll ? march_rw<float, 2, false, true, kCeVolumes, true>(e, sl)
+            : march_rw<float, 2, true, true, kCeVolumes, true>(e, sl);
 }
 
 template <typename T>
@@ -57,16 +60,22 @@ static int march_t(Engine *e, const hfdtd_source *src, int n) {
   if constexpr (sizeof(T) == 8) {
     rc = ll ? (nt ? march_rw<T, 1, false, true>(e, sl) : march_rw<T, 1, false, false>(e, sl))
             : (nt ? march_rw<T, 1, true, true>(e, sl) : march_rw<T, 1, true, false>(e, sl));
-  } else if (e->march_rows == 1 || (e->march_rows == 0 && ll && e->mat_ok)) {
+  } else if (e->march_rows == 1 || (e->march_rows == 0 && e->ce_source() == kCeIndex)) {
     // one row per wavefront: four blocks per CU; with the material index's
     // light loads that occupancy pays (256^3: 233 against 249 us)
-    if (ll && e->mat_ok)
-      rc = nt ? march_rw<T, 1, false, true, true>(e, sl) : march_rw<T, 1, false, false, true>(e, sl);
+    if (e->ce_source() == kCeIndex)
+      rc = nt ? march_rw<T, 1, false, true, kCeIndex>(e, sl)
+              : march_rw<T, 1, false, false, kCeIndex>(e, sl);
+    else if (e->ce_source() == kCeEps)
+      rc = nt ? march_rw<T, 1, false, true, kCeEps>(e, sl) : march_rw<T, 1, false, false, kCeEps>(e, sl);
     else
       rc = ll ? (nt ? march_rw<T, 1, false, true>(e, sl) : march_rw<T, 1, false, false>(e, sl))
               : (nt ? march_rw<T, 1, true, true>(e, sl) : march_rw<T, 1, true, false>(e, sl));
-  } else if (ll && e->mat_ok) {  // lossless with a material index: ce from its table
-    rc = nt ? march_rw<T, 2, false, true, true>(e, sl) : march_rw<T, 2, false, false, true>(e, sl);
+  } else if (e->ce_source() == kCeIndex) {  // lossless with a material index: ce from its table
+    rc = nt ? march_rw<T, 2, false, true, kCeIndex>(e, sl)
+            : march_rw<T, 2, false, false, kCeIndex>(e, sl);
+  } else if (e->ce_source() == kCeEps) {  // lossless, eps_r volume: ce recomputed per edge
+    rc = nt ? march_rw<T, 2, false, true, kCeEps>(e, sl) : march_rw<T, 2, false, false, kCeEps>(e, sl);
   } else {
     rc = ll ? (nt ? march_rw<T, 2, false, true>(e, sl) : march_rw<T, 2, false, false>(e, sl))
             : (nt ? march_rw<T, 2, true, true>(e, sl) : march_rw<T, 2, true, false>(e, sl));

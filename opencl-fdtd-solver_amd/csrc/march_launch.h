@@ -9,11 +9,12 @@
 
 namespace hfdtd {
 
-template <typename T, int RW, bool LOSSY, bool NT, bool MAT = false, bool BIG = false>
+template <typename T, int RW, bool LOSSY, bool NT, int CEM = kCeVolumes, bool BIG = false>
 int march_rw(Engine *e, const SrcList<T> &sl) {
-  auto kern = k_march<T, RW, LOSSY, NT, MAT, BIG>;
+  auto kern = k_march<T, RW, LOSSY, NT, CEM, BIG>;
   int big_occ = 0;
-  int &occ = BIG ? big_occ : MAT ? e->march_occ_mat[RW] : e->march_occ[LOSSY ? 1 : 0][RW];
+  int &occ = BIG ? big_occ
+                 : CEM != kCeVolumes ? e->march_occ_mat[CEM][RW] : e->march_occ[LOSSY ? 1 : 0][RW];
   if (occ == 0) {
     // resident blocks per CU from the kernel's own resource use: a block puts
     // one wavefront on each of the four SIMDs, each SIMD holds 512 VGPRs per
@@ -60,8 +61,11 @@ int march_rw(Engine *e, const SrcList<T> &sl) {
                      T(e->dtm_dl), T(e->dtm), T(e->inv_dl), (const T *)e->state[e->cur],
                      (T *)e->state[o], (const T *)e->coef, (const T *)e->prof,
                      (const T *)e->psi[e->cur], (T *)e->psi[o], tiles_k, tiles_j, ch, e->xcd,
-                     e->march_run, ms, MAT ? (const unsigned char *)e->mat : nullptr,
-                     MAT ? (const float *)e->mtab : nullptr, MAT ? e->nmat : 0, zc);
+                     e->march_run, ms,
+                     CEM == kCeIndex ? (const void *)e->mat
+                                     : CEM == kCeEps ? (const void *)e->epsv : nullptr,
+                     CEM == kCeIndex ? (const float *)e->mtab : nullptr,
+                     CEM == kCeIndex ? e->nmat : 0, e->dt / kEPS0, zc);
   HF_HIP(hipGetLastError());
   rc = timing_mark(e);
   if (rc) return rc;
@@ -72,23 +76,26 @@ int march_rw(Engine *e, const SrcList<T> &sl) {
   return source_fixup<T, LOSSY>(e, sl);
 }
 
-#define HF_MARCH_RW(T, RW, LOSSY, NT, MAT) \
-  template int march_rw<T, RW, LOSSY, NT, MAT, false>(Engine *, const SrcList<T> &)
+#define HF_MARCH_RW(T, RW, LOSSY, NT, CEM) \
+  template int march_rw<T, RW, LOSSY, NT, CEM, false>(Engine *, const SrcList<T> &)
 // slabs of 4 GB and more (FP32, streaming)
-#define HF_MARCH_RW_BIG(X)                                                               \
-  X template int march_rw<float, 1, false, true, false, true>(Engine *, const SrcList<float> &); \
-  X template int march_rw<float, 1, true, true, false, true>(Engine *, const SrcList<float> &);  \
-  X template int march_rw<float, 1, false, true, true, true>(Engine *, const SrcList<float> &);  \
-  X template int march_rw<float, 2, false, true, false, true>(Engine *, const SrcList<float> &); \
-  X template int march_rw<float, 2, true, true, false, true>(Engine *, const SrcList<float> &);  \
-  X template int march_rw<float, 2, false, true, true, true>(Engine *, const SrcList<float> &)
-#define HF_MARCH_RW_SET(X, T, RW)   \
-  X HF_MARCH_RW(T, RW, false, true, false);  \
-  X HF_MARCH_RW(T, RW, false, false, false); \
-  X HF_MARCH_RW(T, RW, true, true, false);   \
-  X HF_MARCH_RW(T, RW, true, false, false)
-#define HF_MARCH_RW_MAT(X, T, RW)  \
-  X HF_MARCH_RW(T, RW, false, true, true); \
-  X HF_MARCH_RW(T, RW, false, false, true)
+#define HF_MARCH_RW_BIG1(X, RW)                                                                   \
+  X template int march_rw<float, RW, false, true, kCeVolumes, true>(Engine *, const SrcList<float> &); \
+  X template int march_rw<float, RW, true, true, kCeVolumes, true>(Engine *, const SrcList<float> &);  \
+  X template int march_rw<float, RW, false, true, kCeIndex, true>(Engine *, const SrcList<float> &);   \
+  X template int march_rw<float, RW, false, true, kCeEps, true>(Engine *, const SrcList<float> &)
+#define HF_MARCH_RW_BIG(X) \
+  HF_MARCH_RW_BIG1(X, 1);  \
+  HF_MARCH_RW_BIG1(X, 2)
+#define HF_MARCH_RW_SET(X, T, RW)        \
+  X HF_MARCH_RW(T, RW, false, true, kCeVolumes);  \
+  X HF_MARCH_RW(T, RW, false, false, kCeVolumes); \
+  X HF_MARCH_RW(T, RW, true, true, kCeVolumes);   \
+  X HF_MARCH_RW(T, RW, true, false, kCeVolumes)
+#define HF_MARCH_RW_MAT(X, T, RW)       \
+  X HF_MARCH_RW(T, RW, false, true, kCeIndex);  \
+  X HF_MARCH_RW(T, RW, false, false, kCeIndex); \
+  X HF_MARCH_RW(T, RW, false, true, kCeEps);    \
+  X HF_MARCH_RW(T, RW, false, false, kCeEps)
 
 }  // namespace hfdtd

@@ -94,6 +94,7 @@ EXPORTS = {
     "hfdtd_set_view": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_set_pair_planes": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_material_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "hfdtd_ce_source": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hfdtd_set_march_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "hfdtd_set_march_addressing": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing": (C.c_int, [C.c_void_p, C.c_int]),

@@ -562,6 +562,17 @@ class HIPFDTD(SourceMonitorMixin):
         _lib.check(self._lib.hfdtd_material_count(self._handle, C.byref(n)), "material_count")
         return int(n.value)
 
+    CE_SOURCES = ("volumes", "index", "eps")
+
+    def ce_source(self) -> str:
+        """Where the march sweep takes ce from: ``'volumes'`` (the three ce
+        volumes), ``'index'`` (the material index) or ``'eps'`` (the eps_r
+        volume of a lossless FP32 grid without an index, ce recomputed per
+        edge exactly as the coefficients were made)."""
+        n = C.c_int(0)
+        _lib.check(self._lib.hfdtd_ce_source(self._handle, C.byref(n)), "ce_source")
+        return self.CE_SOURCES[n.value]
+
     def set_pair_planes(self, planes_i: int = 0) -> None:
         """Planes of i per block of the pair sweep (0 = 24)."""
         _lib.check(self._lib.hfdtd_set_pair_planes(self._handle, int(planes_i)), "set_pair_planes")

@@ -97,6 +97,53 @@ def test_indexed_march_matches_fused(shape, npml, nvals, nsrc, eps_dtype, indexe
           _run("march", shape, npml, eps, nsrc, 16, rows=rows, materials=want))  # fmt: skip
 
 
+@pytest.mark.parametrize("rows", [0, 1, 2])
+@pytest.mark.parametrize("case", ["f32", "f64-exact", "f64-inexact", "poked", "npml0"])
+def test_eps_volume_march_matches_fused(case, rows):
+    """A continuous random medium (every cell its own eps_r, no index): the
+    march sweep recomputes ce from the eps_r volume when the values are
+    exact in FP32, and reads the ce volumes otherwise; both equal the fused
+    sweep bit for bit."""
+    shape, npml = ((37, 45, 70), 0) if case == "npml0" else ((37, 45, 70), 6)
+    rng = np.random.default_rng(11)
+    eps = rng.uniform(1.0, 7.0, size=shape)
+    if case != "f64-inexact":
+        eps = eps.astype(np.float32)
+        if case == "f64-exact":
+            eps = eps.astype(np.float64)
+    want = "volumes" if case in ("f64-inexact", "poked") else "eps"
+    print(f"[eps] {case} rows={rows} -> {want}", flush=True)
+
+    def check(sim):
+        assert sim.material_count() == 0
+        assert sim.ce_source() == want, (sim.ce_source(), want)
+
+    def poke(sim):
+        if case == "poked":
+            _index_off(sim)
+        check(sim)
+
+    _same(_run("fused", shape, npml, eps, 3, 16),
+          _run("march", shape, npml, eps, 3, 16, poke, rows=rows, materials=0))  # fmt: skip
+
+
+def test_ce_source_follows_the_medium():
+    shape = (20, 24, 30)
+    sim = hip().HIPFDTD(shape, DL, npml=4)
+    assert sim.ce_source() == "index"  # the default medium: one material
+    rng = np.random.default_rng(2)
+    sim.set_epsilon(rng.uniform(1, 3, size=shape).astype(np.float32))
+    assert sim.ce_source() == "eps"
+    sim.set_epsilon(_boxes_eps(shape, 4, 1))
+    assert sim.ce_source() == "index"
+    sim.set_epsilon(rng.uniform(1, 3, size=shape).astype(np.float32),
+                    np.full(shape, 0.1, np.float32))  # fmt: skip
+    assert sim.ce_source() == "volumes"
+    d = hip().HIPFDTD(shape, DL, npml=4, dtype=np.float64)
+    d.set_epsilon(rng.uniform(1, 3, size=shape).astype(np.float32))
+    assert d.ce_source() == "volumes"
+
+
 def _index_off(sim):
     """Rewrite ce_x with its own values: the same coefficients, no index."""
     sim.write_volume("ce_x", sim.read_volume("ce_x"))
