@@ -560,7 +560,7 @@ def run_batch(args):
         "batch_hbm_gbs_alg": round(b_alg / step_s / 1e9, 1),
         "batch_hbm_frac": round(b_alg / step_s / 1e9 / HBM_PEAK_GBS, 4),
         "probe_dft_max_rel_err_vs_serial": err,
-        "autotune": {k: tuned[k] for k in ("path", "blocks", "rows", "planes", "ms")},
+        "autotune": {k: tuned[k] for k in ("path", "blocks", "rows", "planes", "width", "ms")},
         "autotune_table": tuned["table"],
         "device": dev,
     }))  # fmt: skip

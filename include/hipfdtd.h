@@ -169,6 +169,12 @@ int hfdtd_set_march_tiling(hfdtd_engine *e, int rows_per_wave, int planes_i, int
  * 4 GB, and one per volume on FP32 slabs of 4 GB and more (volumes below
  * 2 GB, about 810^3 cells); 1 one per volume on any FP32 grid (same results;
  * for testing that addressing on small grids). */
+/* March-sweep tile width: 64 (default), 32 or 16 columns of k, with 64 / width
+ * rows of j per wavefront slot (0 = 64).  Narrow tiles leave fewer lanes idle
+ * on extents that are not multiples of 64 and recompute a smaller halo; they
+ * apply to FP32 grids with slabs below 4 GB and one row per wavefront (FP64
+ * and larger grids keep 64).  Results are the same for every width. */
+int hfdtd_set_march_width(hfdtd_engine *e, int width);
 int hfdtd_set_march_addressing(hfdtd_engine *e, int per_volume);
 
 /* Two Yee steps in one launch (the engine.py:724-735 step twice, minus the
@@ -324,8 +330,8 @@ int hfdtd_dft_read(hfdtd_dft *d, double *out);
 /* ── batched sweep of independent simulations ─────────────────────────── */
 
 /* No counterpart in the reference (it runs one grid at a time).  Groups n
- * engines of any sizes (same dl, dtype and device) so one persistent launch
- * per step advances all of them with the fused sweep; the engines stay usable
+ * engines of any sizes (same dl, dtype and device) so one launch per step
+ * advances all of them (path below); the engines stay usable
  * for reads afterwards.  Each engine may carry one current/soft source whose
  * amplitude per step comes from a table, and one point probe whose per-step
  * phases come from a table. */
@@ -348,8 +354,17 @@ int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i)
 /* HFDTD_PATH_FUSED (default): one persistent fused-tile launch per step;
  * HFDTD_PATH_SPLIT: persistent split sweeps over blocks of 256 cells of the
  * flattened (j, k) planes (H, sources, E, probes), which leave no lanes idle
- * on extents that are not multiples of 64 */
+ * on extents that are not multiples of 64;
+ * HFDTD_PATH_MARCH: one launch per step of march tiles over every engine,
+ * with each engine's source applied and its probe recorded inside the
+ * sweep (field and psi slabs below 4 GB / 2 GB per engine) */
 int hfdtd_batch_set_path(hfdtd_batch *b, int path);
+/* HFDTD_PATH_MARCH launch shape: j rows per wavefront (1 or 2; FP64: 1),
+ * i planes per chunk (2..256, rounded up to even), consecutive work items
+ * per XCD run, tile width in k (64, 32 or 16 columns; narrower tiles with
+ * one row per wavefront, FP32 only); 0 keeps the current value (defaults 1,
+ * 32, 16, 64) */
+int hfdtd_batch_set_march(hfdtd_batch *b, int rows, int planes, int run, int width);
 
 /* ── adjoint gradient of probe-DFT objectives ─────────────────────────────
  * No counterpart in the reference (it has no differentiation).  Objective:

@@ -13,7 +13,9 @@
 #include <algorithm>
 #include <vector>
 
+#include "batch_march.h"
 #include "fdtd_core.h"
+#include "launch_common.h"
 
 struct hfdtd_probe {
   hfdtd::Engine *e = nullptr;
@@ -243,6 +245,11 @@ __global__ void k_step_advance(int *step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(step, 1);
 }
 
+// After a run of n batched march launches (which read *step + q).
+__global__ void k_step_add(int *step, int n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(step, n);
+}
+
 }  // namespace hfdtd
 
 struct hfdtd_batch {
@@ -264,7 +271,12 @@ struct hfdtd_batch {
   int n_phase = 0;
   double *d_probe_acc = nullptr;
   int blocks = 0, rows = 4, planes = 16;
-  int path = HFDTD_PATH_FUSED;  // fused tiles, or the flat split sweeps
+  int path = HFDTD_PATH_FUSED;  // fused tiles, the flat split sweeps, or march tiles
+  // march path: rows per wavefront (1 or 2), i planes per chunk, consecutive
+  // work items per XCD run; the medium kind and loss of the kernel it runs
+  int mrows = 1, mplanes = 32, mrun = 16, mwidth = 64;
+  int mce = 0;
+  bool mlossy = false;
   bool lossy = false;
   hipGraphExec_t gexec = nullptr;
 };
@@ -374,6 +386,167 @@ static int batch_build(hfdtd_batch *b) {
   return HFDTD_OK;
 }
 
+// March path: per-grid descriptors (parity origin = current step) and the
+// work list of march tiles.  The kernel reads the material index when every
+// grid has one, else the ce volumes (with ca when some grid is lossy).
+template <typename T>
+static int batch_build_march(hfdtd_batch *b) {
+  const int n = (int)b->eng.size();
+  const int RW = b->mrows;
+  std::vector<BMSim<T>> desc(n);
+  std::vector<int> item_sim;
+  int begin = 0;
+  bool all_mat = sizeof(T) == 4;
+  b->mlossy = false;
+  for (int s = 0; s < n; ++s) {
+    Engine *e = b->eng[s];
+    if (!e->fused_fits() || ((double)e->psi_off[PSI_COUNT] + 64.0) * e->elem >= 2147483648.0)
+      return fail(HFDTD_EUNSUPPORTED,
+                  "batched march tiles need field slabs below 4 GB and psi slabs below 2 GB");
+    int rc = ensure_alt(e);
+    if (rc) return rc;
+    b->mlossy = b->mlossy || !e->lossless;
+    all_mat = all_mat && e->ce_source() == kCeIndex;
+  }
+  b->mce = all_mat ? kCeIndex : kCeVolumes;
+  for (int s = 0; s < n; ++s) {
+    Engine *e = b->eng[s];
+    BMSim<T> &d = desc[s];
+    d.g = geo_of(e);
+    d.st[0] = (T *)e->state[e->cur];
+    d.st[1] = (T *)e->state[e->cur ^ 1];
+    d.psi[0] = (T *)e->psi[e->cur];
+    d.psi[1] = (T *)e->psi[e->cur ^ 1];
+    d.coef = (const T *)e->coef;
+    d.prof = (const T *)e->prof;
+    d.mat = all_mat ? e->mat : nullptr;
+    d.mtab = all_mat ? e->mtab : nullptr;
+    d.nmat = all_mat ? e->nmat : 0;
+    d.planes = (b->mplanes + 1) / 2 * 2;
+    d.tiles_k = (int)ceil_div(e->nz, b->mwidth);
+    d.tiles_j = (int)ceil_div(e->ny, 4 * RW * (64 / b->mwidth));
+    d.ch.split = e->npml > 0 && e->nx > 2 * e->npml ? 1 : 0;
+    d.ch.inner = (int)ceil_div(d.ch.split ? e->nx - 2 * e->npml : e->nx, d.planes);
+    const int nt = d.tiles_k * d.tiles_j * (d.ch.inner + 2 * d.ch.split);
+    d.item0 = begin;
+    item_sim.insert(item_sim.end(), nt, s);
+    begin += nt;
+    d.has_src = b->has_src.empty() ? 0 : b->has_src[s];
+    if (d.has_src) {
+      hfdtd_source in = b->src[s];
+      in.amp = 0.0;  // from the table
+      if (!make_src<T>(e, in, d.src)) return fail(HFDTD_EINVAL, "invalid source descriptor");
+    } else {
+      d.src = Src<T>{};
+    }
+    d.has_probe = 0;
+    d.pi = d.pj = d.pk = -1;
+    if (!b->probe.empty() && b->probe[3 * s] >= 0) {
+      d.has_probe = 1;
+      d.pi = b->probe[3 * s];
+      d.pj = b->probe[3 * s + 1];
+      d.pk = b->probe[3 * s + 2];
+    }
+  }
+  int rc = batch_free_graph(b);
+  if (rc) return rc;
+  if (b->d_desc) HF_HIP(hipFree(b->d_desc));
+  if (b->d_tile_sim) HF_HIP(hipFree(b->d_tile_sim));
+  b->d_desc = nullptr;
+  b->d_tile_sim = nullptr;
+  HF_HIP(hipMalloc(&b->d_desc, sizeof(BMSim<T>) * n));
+  HF_HIP(hipMalloc((void **)&b->d_tile_sim, sizeof(int) * item_sim.size()));
+  HF_HIP(hipMemcpyAsync(b->d_desc, desc.data(), sizeof(BMSim<T>) * n, hipMemcpyHostToDevice,
+                        b->stream));
+  HF_HIP(hipMemcpyAsync(b->d_tile_sim, item_sim.data(), sizeof(int) * item_sim.size(),
+                        hipMemcpyHostToDevice, b->stream));
+  HF_HIP(hipStreamSynchronize(b->stream));
+  b->n_tiles = (int)item_sim.size();
+  return HFDTD_OK;
+}
+
+template <typename T>
+static BMArgs<T> batch_march_args(const hfdtd_batch *b) {
+  const Engine *e = b->eng[0];
+  BMArgs<T> a;
+  a.sims = (const BMSim<T> *)b->d_desc;
+  a.item_sim = b->d_tile_sim;
+  a.n_items = b->n_tiles;
+  a.step = b->d_step;
+  a.step0 = b->step_host;
+  a.dtm_dl = T(e->dtm_dl);
+  a.dtm = T(e->dtm);
+  a.inv_dl = T(e->inv_dl);
+  a.amp = b->d_amp;
+  a.phase = b->d_phase;
+  a.probe_acc = b->nfreq > 0 ? b->d_probe_acc : nullptr;
+  a.n_table = std::max(b->n_amp, b->n_phase);
+  a.nfreq = b->nfreq;
+  a.comp = b->comp;
+  a.xcd = e->xcd;
+  a.run = b->mrun;
+  return a;
+}
+
+// Launch q of a run of batched march steps.
+template <typename T>
+static int batch_launch_march(const hfdtd_batch *b, const BMArgs<T> &a, int q) {
+  if constexpr (sizeof(T) == 8) {
+    return b->mlossy ? launch_batch_march<T, 1, true, kCeVolumes, 64>(a, q, b->stream)
+                     : launch_batch_march<T, 1, false, kCeVolumes, 64>(a, q, b->stream);
+  } else {
+#define HF_BM(RW, W)                                                                     \
+  return b->mce == kCeIndex ? launch_batch_march<T, RW, false, kCeIndex, W>(a, q, b->stream)  \
+         : b->mlossy        ? launch_batch_march<T, RW, true, kCeVolumes, W>(a, q, b->stream) \
+                            : launch_batch_march<T, RW, false, kCeVolumes, W>(a, q, b->stream)
+    if (b->mrows == 2) HF_BM(2, 64);
+    if (b->mwidth == 32) HF_BM(1, 32);
+    if (b->mwidth == 16) HF_BM(1, 16);
+    HF_BM(1, 64);
+#undef HF_BM
+  }
+}
+
+template <typename T>
+static int batch_run_march(hfdtd_batch *b, int n_steps, int graph_steps) {
+  int rc = batch_build_march<T>(b);
+  if (rc) return rc;
+  const BMArgs<T> a = batch_march_args<T>(b);
+  int done = 0;
+  if (graph_steps > 0 && n_steps >= graph_steps) {
+    hipGraph_t graph;
+    HF_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int q = 0; q < graph_steps && rc == HFDTD_OK; ++q) rc = batch_launch_march<T>(b, a, q);
+    if (rc == HFDTD_OK) {
+      hipLaunchKernelGGL(k_step_add, dim3(1), dim3(64), 0, b->stream, b->d_step, graph_steps);
+      if (hipGetLastError() != hipSuccess) rc = fail(HFDTD_EHIP, "k_step_add launch");
+    }
+    if (rc) {
+      hipGraph_t junk;
+      (void)hipStreamEndCapture(b->stream, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      return rc;
+    }
+    HF_HIP(hipStreamEndCapture(b->stream, &graph));
+    hipError_t err = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (err != hipSuccess) return hip_fail(err, "hipGraphInstantiate");
+    for (; done + graph_steps <= n_steps; done += graph_steps)
+      HF_HIP(hipGraphLaunch(b->gexec, b->stream));
+  }
+  if (done < n_steps) {
+    for (int q = 0; q < n_steps - done; ++q)
+      if ((rc = batch_launch_march<T>(b, a, q))) return rc;
+    hipLaunchKernelGGL(k_step_add, dim3(1), dim3(64), 0, b->stream, b->d_step, n_steps - done);
+    HF_HIP(hipGetLastError());
+  }
+  HF_HIP(hipStreamSynchronize(b->stream));
+  b->step_host += n_steps;
+  if (n_steps & 1)
+    for (Engine *e : b->eng) e->cur ^= 1;
+  return HFDTD_OK;
+}
+
 template <typename T>
 static BatchArgs<T> batch_args(const hfdtd_batch *b) {
   const Engine *e = b->eng[0];
@@ -457,6 +630,7 @@ static int batch_launch_step(hfdtd_batch *b, const BatchArgs<T> &a) {
 
 template <typename T>
 static int batch_run_t(hfdtd_batch *b, int n_steps, int graph_steps) {
+  if (b->path == HFDTD_PATH_MARCH) return batch_run_march<T>(b, n_steps, graph_steps);
   int rc = batch_build<T>(b);
   if (rc) return rc;
   const BatchArgs<T> a = batch_args<T>(b);
@@ -697,8 +871,9 @@ int hfdtd_batch_read_probes(hfdtd_batch *b, double *out) {
 }
 
 int hfdtd_batch_set_path(hfdtd_batch *b, int path) {
-  if (!b || (path != HFDTD_PATH_FUSED && path != HFDTD_PATH_SPLIT))
-    return fail(HFDTD_EINVAL, "path must be HFDTD_PATH_FUSED or HFDTD_PATH_SPLIT");
+  if (!b || (path != HFDTD_PATH_FUSED && path != HFDTD_PATH_SPLIT && path != HFDTD_PATH_MARCH))
+    return fail(HFDTD_EINVAL,
+                "path must be HFDTD_PATH_FUSED, HFDTD_PATH_SPLIT or HFDTD_PATH_MARCH");
   b->path = path;
   return batch_free_graph(b);
 }
@@ -712,6 +887,22 @@ int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i)
   if (rows_j) b->rows = rows_j;
   if (planes_i) b->planes = planes_i;
   return HFDTD_OK;
+}
+
+int hfdtd_batch_set_march(hfdtd_batch *b, int rows, int planes, int run, int width) {
+  if (!b || rows < 0 || rows > 2 || planes < 0 || planes > 256 || run < 0 ||
+      (width != 0 && width != 64 && width != 32 && width != 16))
+    return fail(HFDTD_EINVAL, "march rows must be 1 or 2, planes in [2, 256], run >= 1, width "
+                              "64, 32 or 16 (0 = keep)");
+  if (b->fp64 && (rows == 2 || (width && width != 64)))
+    return fail(HFDTD_EINVAL, "FP64 batches use one row per wavefront and 64-column tiles");
+  const int r = rows ? rows : b->mrows, w = width ? width : b->mwidth;
+  if (r == 2 && w != 64) return fail(HFDTD_EINVAL, "two rows per wavefront need 64-column tiles");
+  if (width) b->mwidth = width;
+  if (rows) b->mrows = rows;
+  if (planes) b->mplanes = std::max(2, planes);
+  if (run) b->mrun = run;
+  return batch_free_graph(b);
 }
 
 }  // extern "C"

@@ -115,8 +115,13 @@ struct Engine {
   int march_run = 16;
   bool march_per_volume = false;  // per-volume addressing on any FP32 slab (set by hand)
   int march_nt = -1;
-  int march_occ[2][3] = {};  // resident march blocks per CU by [lossy][rows]
-  int march_occ_mat[3][3] = {};  // the same for the index / eps_r sweeps by [ce source][rows]
+  // march tiles of 64, 32 or 16 columns of k (FP32; 64 / W rows of j per
+  // wavefront slot)
+  int march_width = 64;
+  // resident march blocks per CU by [width 64/32/16][lossy][rows], and the
+  // same for the index / eps_r sweeps by [width][ce source][rows]
+  int march_occ[3][2][3] = {};
+  int march_occ_mat[3][3][3] = {};
   // Lossless FP32 grids with at most kMatMax distinct eps_r values: a byte of
   // material per cell and the table of ce for every pair of materials (an
   // edge's ce depends only on its two cells), which the march sweep reads
@@ -184,11 +189,11 @@ struct Engine {
   // plane before the chunk and its prologue (measured at 256^3: short
   // chunks cost more than one plane each).  With PML the x-stripes form
   // chunks of their own and P chunks the planes between.
-  int march_planes_for(int rows_per_wave, int per_cu) const {
+  int march_planes_for(int rows_per_wave, int per_cu, int width = 64) const {
     if (march_planes > 0) return (march_planes + 1) / 2 * 2;
     const long long slots = (long long)(per_cu > 0 ? per_cu : 1) * cus;
-    const long long tiles =
-        (long long)((nz + 63) / 64) * ((ny + 4 * rows_per_wave - 1) / (4 * rows_per_wave));
+    const int tj = 4 * rows_per_wave * (64 / width);
+    const long long tiles = (long long)((nz + width - 1) / width) * ((ny + tj - 1) / tj);
     int best = 64;
     long long best_t = -1;
     const bool split = npml > 0 && nx > 2 * npml;  // x-stripes in chunks of their own

@@ -793,6 +793,14 @@ int hfdtd_set_march_tiling(hfdtd_engine *h, int rows_per_wave, int planes_i, int
   return HFDTD_OK;
 }
 
+int hfdtd_set_march_width(hfdtd_engine *h, int width) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  if (width != 0 && width != 64 && width != 32 && width != 16)
+    return fail(HFDTD_EINVAL, "march width must be 64, 32 or 16 (0 = 64)");
+  h->s.march_width = width ? width : 64;
+  return HFDTD_OK;
+}
+
 int hfdtd_set_march_addressing(hfdtd_engine *h, int per_volume) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
   if (per_volume < 0 || per_volume > 1) return fail(HFDTD_EINVAL, "per_volume must be 0 or 1");

@@ -115,16 +115,28 @@ __device__ __forceinline__ void bst_quad(Rsrc r, float a, float b, float c, floa
   __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
 }
 
+// Tile shape: W columns of k (64, 32 or 16) by 4 * RW * (64 / W) rows of j.
+// A wavefront's slot holds 64 / W rows of W columns (lane = row * W + column
+// within the slot), so narrow grids leave fewer lanes idle and the tile's
+// recomputed halo shrinks as it gets squarer.
+template <int RW, int W>
+struct MarchShape {
+  static_assert(W == 64 || W == 32 || W == 16, "march tiles are 64, 32 or 16 columns wide");
+  static constexpr int G = 64 / W;        // rows of one slot
+  static constexpr int TJ = 4 * RW * G;   // rows of the tile
+  static constexpr int C = W + 2;         // LDS columns: k0-1 ... k0+W
+  static constexpr int R = TJ + 4;        // LDS rows: j0-1 ... j0+TJ, a trash and a spare row
+};
+
 // LDS of a block: new H and E of the planes, double-buffered by plane
-// parity, on a grid of rows j0-1 ... j0+4RW (then a trash row and one spare
+// parity, on a grid of rows j0-1 ... j0+TJ (then a trash row and one spare
 // row, so neighbour reads from the trash row stay inside the array) by
-// columns k0-1 ... k0+64.
-constexpr int kMarchCols = 66;
-template <typename T, int RW>
+// columns k0-1 ... k0+W.
+template <typename T, int RW, int W = 64>
 struct MarchLds {
-  static constexpr int R = 4 * RW + 4;
-  T h[2][3][R][kMarchCols];  // new H(i): [parity][Hx Hy Hz]
-  T e[2][3][R][kMarchCols];  // E(i) (old), the j+1 / k+1 neighbours of H
+  static constexpr int R = MarchShape<RW, W>::R, C = MarchShape<RW, W>::C;
+  T h[2][3][R][C];  // new H(i): [parity][Hx Hy Hz]
+  T e[2][3][R][C];  // E(i) (old), the j+1 / k+1 neighbours of H
 };
 
 // Plane-i data of one slot, loaded one plane ahead in two parts: the H part
@@ -157,7 +169,7 @@ struct MarchSlot {
   unsigned o1, o1j, o1k;  // ce-source byte offsets of (j, k), (j+1, k), (j, k+1), clamped
   bool iny, inz;
   bool own;      // an owned cell inside the grid
-  int lh, le;    // LDS element index (row * kMarchCols + column) of its H and E
+  int lh, le;    // LDS element index (row * LDS columns + column) of its H and E
 };
 
 // Raw buffer loads and stores; NT marks them non-temporal (streamed once).
@@ -196,13 +208,26 @@ struct PsiPlane {
   bool dead;  // a plane past the chunk: its loads are dropped
 };
 
+// A point probe of the batched sweep: one E component of one cell.  The
+// block that owns the cell keeps the component's new value and, after its
+// planes, adds value x phase to the probe's DFT (FP64), as the probe
+// kernels of the split sweeps do with the stored value.
+struct MarchProbe {
+  int i, j, k, comp;
+  int nf;             // frequencies
+  const double *ph;   // nf x {re, im}: the phase factors of this step
+  double *acc;        // nf x {re, im}
+};
+
 template <typename T, int RW, bool LOSSY, bool X, bool Y, bool Z, bool NT, int CEM = kCeVolumes,
-          bool CAP = false, bool BIG = false>
+          bool CAP = false, bool BIG = false, bool PRB = false, int W = 64>
 struct March {
   static constexpr bool MAT = CEM != kCeVolumes;  // ce from a per-cell volume, not the ce volumes
   static constexpr unsigned CB = CEM == kCeEps ? 4u : 1u;  // bytes per cell of that volume
   static constexpr int S = RW + 1;  // RW owned rows + the halo slot
-  static constexpr int TJ = 4 * RW;
+  static constexpr int TJ = MarchShape<RW, W>::TJ;
+  static constexpr int G = MarchShape<RW, W>::G;
+  static constexpr int LC = MarchShape<RW, W>::C;  // LDS row stride
   static constexpr unsigned E = sizeof(T);
 
   Geo g;
@@ -233,6 +258,11 @@ struct March {
   Rsrc rcap;
   ZCap zc;
   unsigned zo[RW];
+  // probe (PRB): the probe, whether owned slot q holds its cell, and the
+  // component's new value there
+  MarchProbe pr;
+  bool phit[RW];
+  T pv;
   unsigned padb;       // store offset of lanes and planes outside the grid
   int iend;            // first plane past this block's chunk
   bool sized;          // slab resources carry their sizes (slabs below 2 GB)
@@ -414,7 +444,7 @@ struct March {
   // i+1.  E of the plane is carried in oe, so no value of a set stays live
   // while its next loads are in flight: the sets keep their registers from
   // one plane to the next and the loop needs no copies of in-flight values.
-  __device__ __forceinline__ void step(MarchLds<T, RW> &L, int buf, int i, MarchH<T> (&hc)[S],
+  __device__ __forceinline__ void step(MarchLds<T, RW, W> &L, int buf, int i, MarchH<T> (&hc)[S],
                                        MarchH<T> (&hn)[S], MarchE<T> (&ec)[RW],
                                        MarchE<T> (&en)[RW]) {
     const unsigned sxy = (unsigned)(g.ny * g.nz);
@@ -454,8 +484,8 @@ struct March {
       h[q][2] = hc[q].hz;
       // E(i) at j+1 and k+1 (written to LDS during the previous plane); past
       // the grid's last row / column they are clamped copies h_math ignores
-      const T exj = (&L.e[buf][0][0][0])[m.le + kMarchCols];
-      const T ezj = (&L.e[buf][2][0][0])[m.le + kMarchCols];
+      const T exj = (&L.e[buf][0][0][0])[m.le + LC];
+      const T ezj = (&L.e[buf][2][0][0])[m.le + LC];
       const T exk = (&L.e[buf][0][0][0])[m.le + 1];
       const T eyk = (&L.e[buf][1][0][0])[m.le + 1];
       h_math(g, ic, m.j, m.k, st, pxh, pyh[q], pzh[q], oe[q][0], oe[q][1], oe[q][2], exj, ezj,
@@ -500,8 +530,8 @@ struct March {
 #pragma unroll
     for (int q = 0; q < RW; ++q) {
       const MarchSlot &m = sl[q];
-      const T hz_jm = (&L.h[buf][2][0][0])[m.lh - kMarchCols];
-      const T hx_jm = (&L.h[buf][0][0][0])[m.lh - kMarchCols];
+      const T hz_jm = (&L.h[buf][2][0][0])[m.lh - LC];
+      const T hx_jm = (&L.h[buf][0][0][0])[m.lh - LC];
       const T hy_km = (&L.h[buf][1][0][0])[m.lh - 1], hx_km = (&L.h[buf][0][0][0])[m.lh - 1];
       const StripeIdx st = stripes(m, pp.inx);
       T p[6] = {0, 0, 0, 0, 0, 0};
@@ -579,6 +609,9 @@ struct March {
         psi_st(p[5], 11, ok && m.inz, m.qz4 + pp.zs * E);
       }
       if constexpr (CAP) bst_quad(rcap, ex, ey, h[q][0], h[q][1], zo[q], cap_plane(i, live));
+      if constexpr (PRB) {
+        if (ic == pr.i && live) pv = phit[q] ? (pr.comp == 0 ? ex : pr.comp == 1 ? ey : ez) : pv;
+      }
       hyc[q] = h[q][1];
       hzc[q] = h[q][2];
     }
@@ -591,19 +624,24 @@ struct March {
   }
 
   // Slot set-up, the carry H(i0-1) and the first loads, then the planes.
-  __device__ __forceinline__ void run(MarchLds<T, RW> &L, int planes, int k0, int j0, int i0,
+  __device__ __forceinline__ void run(MarchLds<T, RW, W> &L, int planes, int k0, int j0, int i0,
                                       int i1) {
     iend = i1;
+    if constexpr (PRB) pv = T(0);
     sized = BIG ? (unsigned long long)g.vs * E < 0x80000000ull : slab_bytes<T>(g) < 0x80000000ull;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int n = g.npml;
-    // halo slot lanes of wave w: 0-15 H on row j0-1 (columns k0+16w ...);
-    // 16 .. 16+RW-1 H on column k0-1 (the wave's rows); 32-47 E only on
-    // row j0+TJ (columns k0+16w ...); 48-50 E only on column k0+64 (rows
-    // j0-1+3w ...); lane 51 of wave 0 E only at (j0+TJ, k0-1); the rest idle.
-    // H of E-only lanes and both of idle lanes go to the trash row.
+    // halo slot lanes of wave w (QW = W/4 columns per wave, PE rows per
+    // wave): 0 .. QW-1 H on row j0-1 (columns k0+QW w ...); 16 .. 16+RW G-1
+    // H on column k0-1 (the wave's rows); 32 .. 32+QW-1 E only on row j0+TJ
+    // (columns k0+QW w ...); 48 .. 48+PE-1 E only on column k0+W (rows
+    // j0-1+PE w ...); lane 48+PE of wave 0 E only at (j0+TJ, k0-1); the rest
+    // idle.  H of E-only lanes and both of idle lanes go to the trash row.
     constexpr int trash_row = TJ + 2;
+    constexpr int QW = W / 4;
+    constexpr int PE = (TJ + 2 + 3) / 4 > 3 ? (TJ + 2 + 3) / 4 : 3;
+    const int tc = lane % LC;  // a trash-row column
     // fields a halo lane's role reads, bit c for Ex Ey Ez Hx Hy Hz: H on row
     // j0-1 is needed for Hx and Hz (E's j-1 neighbours), H on column k0-1
     // for Hx and Hy (E's k-1 neighbours), both from all of E; E-only lanes
@@ -614,44 +652,45 @@ struct March {
       int j, k, hr, hc, er, ec;
       bool own = false;
       if (q < RW) {
-        j = j0 + RW * w + q;
-        k = k0 + lane;
-        hr = er = RW * w + q + 1;
-        hc = ec = lane + 1;
+        const int lr = (RW * w + q) * G + lane / W, c = lane % W;  // tile row and column
+        j = j0 + lr;
+        k = k0 + c;
+        hr = er = lr + 1;
+        hc = ec = c + 1;
         own = j < g.ny && k < g.nz;
-      } else if (lane < 16) {
+      } else if (lane < QW) {
         j = j0 - 1;
-        k = k0 + 16 * w + lane;
+        k = k0 + QW * w + lane;
         hr = er = 0;
-        hc = ec = 16 * w + lane + 1;
+        hc = ec = QW * w + lane + 1;
         need = 0b101111;  // E, Hx, Hz
-      } else if (lane < 16 + RW) {
-        j = j0 + RW * w + lane - 16;
+      } else if (lane >= 16 && lane < 16 + RW * G) {
+        j = j0 + RW * G * w + lane - 16;
         k = k0 - 1;
-        hr = er = RW * w + lane - 16 + 1;
+        hr = er = RW * G * w + lane - 16 + 1;
         hc = ec = 0;
         need = 0b011111;  // E, Hx, Hy
-      } else if (lane >= 32 && lane < 48) {
+      } else if (lane >= 32 && lane < 32 + QW) {
         j = j0 + TJ;
-        k = k0 + 16 * w + lane - 32;
+        k = k0 + QW * w + lane - 32;
         hr = trash_row;
-        hc = lane;
+        hc = tc;
         er = TJ + 1;
-        ec = 16 * w + lane - 32 + 1;
+        ec = QW * w + lane - 32 + 1;
         need = 0b000101;  // Ex, Ez: j+1 neighbours of Hz, Hx on row j0+TJ-1
-      } else if (lane >= 48 && lane < 51 && 3 * w + lane - 48 <= TJ) {
-        j = j0 - 1 + 3 * w + lane - 48;
-        k = k0 + 64;
+      } else if (lane >= 48 && lane < 48 + PE && PE * w + lane - 48 <= TJ) {
+        j = j0 - 1 + PE * w + lane - 48;
+        k = k0 + W;
         hr = trash_row;
-        hc = lane;
-        er = 3 * w + lane - 48;
-        ec = 65;
-        need = 0b000011;  // Ex, Ey: k+1 neighbours of Hy, Hx on column k0+63
-      } else if (lane == 51 && w == 0) {
+        hc = tc;
+        er = PE * w + lane - 48;
+        ec = W + 1;
+        need = 0b000011;  // Ex, Ey: k+1 neighbours of Hy, Hx on column k0+W-1
+      } else if (lane == 48 + PE && w == 0) {
         j = j0 + TJ;
         k = k0 - 1;
         hr = trash_row;
-        hc = lane;
+        hc = tc;
         er = TJ + 1;
         ec = 0;
         need = 0b000100;  // Ez: j+1 neighbour of Hx at (j0+TJ-1, k0-1)
@@ -659,7 +698,7 @@ struct March {
         j = j0;
         k = k0;
         hr = er = trash_row;
-        hc = ec = lane;
+        hc = ec = tc;
       }
       MarchSlot &m = sl[q];
       m.j = min(max(j, 0), g.ny - 1);
@@ -674,8 +713,8 @@ struct March {
       m.inz = Z && n > 0 && hpsi && (m.k < n || m.k >= g.nz - n);
       m.qy4 = m.iny ? (unsigned)(stripe_local(m.j, n, g.ny) * g.nz + m.k) * E : kDropOffset;
       m.qz4 = m.inz ? (unsigned)(m.j * 2 * n + stripe_local(m.k, n, g.nz)) * E : kDropOffset;
-      m.lh = hr * kMarchCols + hc;
-      m.le = er * kMarchCols + ec;
+      m.lh = hr * LC + hc;
+      m.le = er * LC + ec;
       m.own = own;
       if constexpr (CAP) {
         if (q < RW) {
@@ -687,6 +726,9 @@ struct March {
           zo[q] = rec ? ((unsigned)(2 * f + (k == kf ? 1 : 0)) * P + (unsigned)(j - zc.j0)) * 16u
                       : kCapDropLane;
         }
+      }
+      if constexpr (PRB) {
+        if (q < RW) phit[q] = own && j == pr.j && k == pr.k;
       }
       pyh[q] = prof1(g, prof, 1, 1, m.j);
       pye[q] = prof1(g, prof, 0, 1, m.j);
@@ -779,6 +821,18 @@ struct March {
       step(L, 1, i0 + t + 1, B, A, EB, EA);
       t += 2;
     } while (t < planes);
+    if constexpr (PRB) {
+      bool hit = false;
+#pragma unroll
+      for (int q = 0; q < RW; ++q) hit = hit || phit[q];
+      if (hit) {  // one lane of the grid: the probe's plane lies in [i0, i1)
+        const double v = (double)pv;
+        for (int f = 0; f < pr.nf; ++f) {
+          pr.acc[2 * f] += v * pr.ph[2 * f];
+          pr.acc[2 * f + 1] += v * pr.ph[2 * f + 1];
+        }
+      }
+    }
   }
 };
 
@@ -806,63 +860,32 @@ struct MarchChunks {
   }
 };
 
-// One block per (k tile, j tile, i chunk).  A block takes the variant with
-// the psi it needs: X when its chunk meets an x-stripe, Y when its rows
-// (with the halo row) meet a y-stripe, Z when its columns (with the halo
-// column) meet a z-stripe.  (The plane before the chunk, recomputed once,
-// reads its psi in any variant.)  Fewer psi operations per plane also keep
-// a plane's loads and stores within the 63 a wavefront can count.
-template <typename T, int RW, bool LOSSY, bool NT, int CEM, bool BIG = false>
-__global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) void k_march(Geo g, int planes, T dtm_dl, T dtm,
-                                                         T inv_dl, const T *__restrict__ s0,
-                                                         T *__restrict__ s1,
-                                                         const T *__restrict__ coef,
-                                                         const T *__restrict__ prof,
-                                                         const T *__restrict__ q0,
-                                                         T *__restrict__ q1, int tiles_k,
-                                                         int tiles_j, MarchChunks ch, int xcd,
-                                                         int run, const MarchSources<T> src,
-                                                         const void *__restrict__ mat,
-                                                         const float *__restrict__ mtab,
-                                                         int nmat, double cscale,
-                                                         const ZCap zc) {
-  __shared__ MarchLds<T, RW> L;
-  __shared__ Src<T> lsrc[kMarchSources];
-  __shared__ float ltab[CEM == kCeIndex ? kMatMax * kMatMax : 1];
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int t = 0; t < kMarchSources; ++t) lsrc[t] = src.s[t];
-  }  // visible after the barrier that ends the prologue
-  if constexpr (CEM == kCeIndex) {
-    for (int t = threadIdx.x; t < nmat * nmat; t += kMarchThreads) ltab[t] = mtab[t];
-  }
-  const int B = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
-  const int bk = B % tiles_k, bc = B / (tiles_k * tiles_j);
-  // XCD x runs the same j-tiles of every chunk unless the tile order rotates
-  // from chunk to chunk: then the CPML rows at both ends of j land on a
-  // different XCD in each chunk instead of loading the same two XCDs
-  const int bj = ((B / tiles_k) % tiles_j + bc * max(1, run / tiles_k)) % tiles_j;
-  const int k0 = bk * 64, j0 = bj * 4 * RW;
-  int i0, i1;
-  ch.range(g, planes, bc, i0, i1);
-  // planes on which a source of the step can reach a slot of this tile: the
-  // sources whose box meets rows j0-1 ... j0+4RW and columns k0-1 ... k0+64
-  // (every slot's clamped cell lies there); the per-cell tests of the sweep
-  // still decide, so a tile no source touches skips the source loop
-  int si0 = g.nx, si1 = 0;
-#pragma unroll
-  for (int t = 0; t < kMarchSources; ++t) {
-    if (t < src.n && src.s[t].cj0 <= j0 + 4 * RW && src.s[t].cj1 > j0 - 1 &&
-        src.s[t].ck0 <= k0 + 64 && src.s[t].ck1 > k0 - 1) {
-      si0 = min(si0, src.s[t].ci0);
-      si1 = max(si1, src.s[t].ci1);
-    }
-  }
+// One tile of the sweep: rows j0 ... j0+TJ-1, columns k0 ... k0+W-1,
+// planes [i0, i1).  The block takes the variant with the psi it needs: X
+// when its chunk meets an x-stripe, Y when its rows (with the halo row)
+// meet a y-stripe, Z when its columns (with the halo column) meet a
+// z-stripe.  (The plane before the chunk, recomputed once, reads its psi in
+// any variant.)  Fewer psi operations per plane also keep a plane's loads
+// and stores within the 63 a wavefront can count.  EXTRA names the one
+// optional per-block feature a kernel carries: 1 the z-face record (taken
+// by blocks that meet it), 2 the probe (taken by the block that owns its
+// cell); `extra` says whether this block takes it.
+template <typename T, int RW, bool LOSSY, bool NT, int CEM, bool BIG, int EXTRA, int W = 64>
+__device__ __forceinline__ void march_tile(MarchLds<T, RW, W> &L, const Src<T> *lsrc, int nsrc,
+                                           int si0, int si1, const float *ltab, const Geo &g,
+                                           int k0, int j0, int i0, int i1, T dtm_dl, T dtm,
+                                           T inv_dl, const T *__restrict__ s0,
+                                           T *__restrict__ s1, const T *__restrict__ coef,
+                                           const T *__restrict__ prof,
+                                           const T *__restrict__ q0, T *__restrict__ q1,
+                                           const void *__restrict__ mat, int nmat, double cscale,
+                                           const ZCap &zc, bool extra, const MarchProbe &pr) {
   const int np = (i1 - i0 + 1) / 2 * 2;  // even; planes past i1 are masked
   const int n = g.npml;
   const bool x = n > 0 && (i0 < n || i1 > g.nx - n);
-  const bool y = n > 0 && (j0 - 1 < n || min(j0 + 4 * RW - 1, g.ny - 1) >= g.ny - n);
-  const bool z = n > 0 && (k0 - 1 < n || min(k0 + 63, g.nz - 1) >= g.nz - n);
+  constexpr int TJ = MarchShape<RW, W>::TJ;
+  const bool y = n > 0 && (j0 - 1 < n || min(j0 + TJ - 1, g.ny - 1) >= g.ny - n);
+  const bool z = n > 0 && (k0 - 1 < n || min(k0 + W - 1, g.nz - 1) >= g.nz - n);
   const unsigned long long sb = slab_bytes<T>(g);
   const int sbr = sb < 0x80000000ull ? (int)sb : -1;  // the coefficient slab has the same size
   const Rsrc r0 = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(s0), (short)0, sbr, 0x00020000);
@@ -882,34 +905,31 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
       CEM != kCeVolumes ? (int)min(((unsigned long long)g.nx * g.ny * g.nz + 64ull) * cb, 0x7fffffffull)
                         : 0,
       0x00020000);
-  // the z-face record (FP32): blocks whose columns, rows and planes meet it
-  constexpr bool kCapable = sizeof(T) == 4 && !BIG;
-  bool cap = false;
-  if (kCapable && zc.buf) {
-    auto col = [&](int c) { return c >= k0 && c < k0 + 64; };
-    cap = (col(zc.kf0) || col(zc.kf0 - 1) || col(zc.kf1) || col(zc.kf1 - 1)) &&
-          j0 <= zc.j1 && j0 + 4 * RW - 1 >= zc.j0 && i0 <= zc.i1 && i1 - 1 >= zc.i0;
-  }
+  // the z-face record (FP32)
+  constexpr bool kCapable = EXTRA == 1 && sizeof(T) == 4 && !BIG;
   const Rsrc rcap = __builtin_amdgcn_make_buffer_rsrc(zc.buf, (short)0, (int)zc.bytes, 0x00020000);
-#define HF_MARCH_RUN1(X, Y, Z, C)                                                              \
+#define HF_MARCH_RUN1(X, Y, Z, C, P)                                                           \
   {                                                                                           \
-    March<T, RW, LOSSY, X, Y, Z, NT, CEM, C, BIG> m{g,   dtm_dl, dtm,  inv_dl, r0,     r1,      \
-                                                    rc,  rq0,    rq1,  prof,   lsrc,   src.n,   \
-                                                    si0,    si1,  rmat, ltab, nmat};            \
+    March<T, RW, LOSSY, X, Y, Z, NT, CEM, C, BIG, P, W> m{g, dtm_dl, dtm, inv_dl, r0,    r1,   \
+                                                       rc,  rq0,    rq1,  prof,   lsrc,   nsrc, \
+                                                       si0,    si1,  rmat, ltab, nmat};         \
     m.cscale = cscale;                                                                        \
     m.s0p = s0;                                                                               \
     m.s1p = s1;                                                                               \
     m.cp = coef;                                                                              \
     m.rcap = rcap;                                                                            \
     m.zc = zc;                                                                                \
+    m.pr = pr;                                                                                \
     m.run(L, np, k0, j0, i0, i1);                                                             \
   }
 #define HF_MARCH_RUN(X, Y, Z)                                                                  \
   {                                                                                           \
     if constexpr (kCapable) {                                                                 \
-      if (cap) HF_MARCH_RUN1(X, Y, Z, true) else HF_MARCH_RUN1(X, Y, Z, false)                \
+      if (extra) HF_MARCH_RUN1(X, Y, Z, true, false) else HF_MARCH_RUN1(X, Y, Z, false, false) \
+    } else if constexpr (EXTRA == 2) {                                                        \
+      if (extra) HF_MARCH_RUN1(X, Y, Z, false, true) else HF_MARCH_RUN1(X, Y, Z, false, false) \
     } else {                                                                                  \
-      HF_MARCH_RUN1(X, Y, Z, false)                                                           \
+      HF_MARCH_RUN1(X, Y, Z, false, false)                                                    \
     }                                                                                         \
   }
   if (x) {
@@ -927,6 +947,81 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
   }
 #undef HF_MARCH_RUN
 #undef HF_MARCH_RUN1
+}
+
+// Tile B of a grid (k tiles fastest, then j tiles, then i chunks): its
+// first column, first row and plane range.  XCD x runs the same j-tiles of
+// every chunk unless the tile order rotates from chunk to chunk: then the
+// CPML rows at both ends of j land on a different XCD in each chunk instead
+// of loading the same two XCDs.
+template <int RW, int W = 64>
+__device__ __forceinline__ void march_tile_of(const Geo &g, int B, int planes, int tiles_k,
+                                              int tiles_j, const MarchChunks &ch, int run,
+                                              int &k0, int &j0, int &i0, int &i1) {
+  const int bk = B % tiles_k, bc = B / (tiles_k * tiles_j);
+  const int bj = ((B / tiles_k) % tiles_j + bc * max(1, run / tiles_k)) % tiles_j;
+  k0 = bk * W;
+  j0 = bj * MarchShape<RW, W>::TJ;
+  ch.range(g, planes, bc, i0, i1);
+}
+
+// Planes on which one of the step's sources can reach a slot of the tile:
+// the sources whose box meets rows j0-1 ... j0+TJ and columns k0-1 ...
+// k0+W (every slot's clamped cell lies there); the per-cell tests of the
+// sweep still decide, so a tile no source touches skips the source loop.
+template <typename T, int RW, int W = 64>
+__device__ __forceinline__ void march_src_window(const Src<T> &s, bool on, int k0, int j0,
+                                                 int &si0, int &si1) {
+  if (on && s.cj0 <= j0 + MarchShape<RW, W>::TJ && s.cj1 > j0 - 1 && s.ck0 <= k0 + W &&
+      s.ck1 > k0 - 1) {
+    si0 = min(si0, s.ci0);
+    si1 = max(si1, s.ci1);
+  }
+}
+
+// One block per (k tile, j tile, i chunk).
+template <typename T, int RW, bool LOSSY, bool NT, int CEM, bool BIG = false, int W = 64>
+__global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) void k_march(Geo g, int planes, T dtm_dl, T dtm,
+                                                         T inv_dl, const T *__restrict__ s0,
+                                                         T *__restrict__ s1,
+                                                         const T *__restrict__ coef,
+                                                         const T *__restrict__ prof,
+                                                         const T *__restrict__ q0,
+                                                         T *__restrict__ q1, int tiles_k,
+                                                         int tiles_j, MarchChunks ch, int xcd,
+                                                         int run, const MarchSources<T> src,
+                                                         const void *__restrict__ mat,
+                                                         const float *__restrict__ mtab,
+                                                         int nmat, double cscale,
+                                                         const ZCap zc) {
+  __shared__ MarchLds<T, RW, W> L;
+  __shared__ Src<T> lsrc[kMarchSources];
+  __shared__ float ltab[CEM == kCeIndex ? kMatMax * kMatMax : 1];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int t = 0; t < kMarchSources; ++t) lsrc[t] = src.s[t];
+  }  // visible after the barrier that ends the prologue
+  if constexpr (CEM == kCeIndex) {
+    for (int t = threadIdx.x; t < nmat * nmat; t += kMarchThreads) ltab[t] = mtab[t];
+  }
+  const int B = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
+  int k0, j0, i0, i1;
+  march_tile_of<RW, W>(g, B, planes, tiles_k, tiles_j, ch, run, k0, j0, i0, i1);
+  int si0 = g.nx, si1 = 0;
+#pragma unroll
+  for (int t = 0; t < kMarchSources; ++t)
+    march_src_window<T, RW, W>(src.s[t], t < src.n, k0, j0, si0, si1);
+  // the z-face record (FP32): blocks whose columns, rows and planes meet it
+  bool cap = false;
+  if (sizeof(T) == 4 && !BIG && zc.buf) {
+    auto col = [&](int c) { return c >= k0 && c < k0 + W; };
+    cap = (col(zc.kf0) || col(zc.kf0 - 1) || col(zc.kf1) || col(zc.kf1 - 1)) &&
+          j0 <= zc.j1 && j0 + MarchShape<RW, W>::TJ - 1 >= zc.j0 && i0 <= zc.i1 &&
+          i1 - 1 >= zc.i0;
+  }
+  march_tile<T, RW, LOSSY, NT, CEM, BIG, 1, W>(L, lsrc, src.n, si0, si1, ltab, g, k0, j0, i0, i1,
+                                            dtm_dl, dtm, inv_dl, s0, s1, coef, prof, q0, q1, mat,
+                                            nmat, cscale, zc, cap, MarchProbe{});
 }
 
 }  // namespace hfdtd

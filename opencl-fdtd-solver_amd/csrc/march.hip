@@ -1,16 +1,20 @@
 // March sweep: the step launcher (tiling and medium kind pick one of the
 // launchers instantiated in march_f32_r1.hip, march_f32_r2.hip,
-// march_f64.hip).
+// march_f32_w32.hip, march_f32_w16.hip, march_f64.hip, march_f32_big.hip).
 
 #include "march_launch.h"
 
 namespace hfdtd {
 
-HF_MARCH_RW_SET(extern, float, 1);
-HF_MARCH_RW_MAT(extern, float, 1);
-HF_MARCH_RW_SET(extern, float, 2);
-HF_MARCH_RW_MAT(extern, float, 2);
-HF_MARCH_RW_SET(extern, double, 1);
+HF_MARCH_RW_SET(extern, float, 1, 64);
+HF_MARCH_RW_MAT(extern, float, 1, 64);
+HF_MARCH_RW_SET(extern, float, 2, 64);
+HF_MARCH_RW_MAT(extern, float, 2, 64);
+HF_MARCH_RW_SET(extern, double, 1, 64);
+HF_MARCH_RW_SET(extern, float, 1, 32);
+HF_MARCH_RW_MAT(extern, float, 1, 32);
+HF_MARCH_RW_SET(extern, float, 1, 16);
+HF_MARCH_RW_MAT(extern, float, 1, 16);
 HF_MARCH_RW_BIG(extern);
 
 // slabs of 4 GB and more: FP32 volumes below 2 GB each, psi slab below 2 GB
@@ -33,6 +37,23 @@ static int march_big(Engine *e, const SrcList<float> &sl) {
   if (ce == kCeEps) return march_rw<float, 2, false, true, kCeEps, true>(e, sl);
   return ll ? march_rw<float, 2, false, true, kCeVolumes, true>(e, sl)
             : march_rw<float, 2, true, true, kCeVolumes, true>(e, sl);
+}
+
+// FP32 tiles narrower than 64 columns (one row per wavefront slot)
+template <int W>
+static int march_narrow(Engine *e, const SrcList<float> &sl, bool nt) {
+  const int ce = e->ce_source();
+  if (ce == kCeIndex)
+    return nt ? march_rw<float, 1, false, true, kCeIndex, false, W>(e, sl)
+              : march_rw<float, 1, false, false, kCeIndex, false, W>(e, sl);
+  if (ce == kCeEps)
+    return nt ? march_rw<float, 1, false, true, kCeEps, false, W>(e, sl)
+              : march_rw<float, 1, false, false, kCeEps, false, W>(e, sl);
+  if (e->lossless)
+    return nt ? march_rw<float, 1, false, true, kCeVolumes, false, W>(e, sl)
+              : march_rw<float, 1, false, false, kCeVolumes, false, W>(e, sl);
+  return nt ? march_rw<float, 1, true, true, kCeVolumes, false, W>(e, sl)
+            : march_rw<float, 1, true, false, kCeVolumes, false, W>(e, sl);
 }
 
 template <typename T>
@@ -60,6 +81,8 @@ static int march_t(Engine *e, const hfdtd_source *src, int n) {
   if constexpr (sizeof(T) == 8) {
     rc = ll ? (nt ? march_rw<T, 1, false, true>(e, sl) : march_rw<T, 1, false, false>(e, sl))
             : (nt ? march_rw<T, 1, true, true>(e, sl) : march_rw<T, 1, true, false>(e, sl));
+  } else if (e->march_width != 64) {
+    rc = e->march_width == 32 ? march_narrow<32>(e, sl, nt) : march_narrow<16>(e, sl, nt);
   } else if (e->march_rows == 1 || (e->march_rows == 0 && e->ce_source() == kCeIndex)) {
     // one row per wavefront: four blocks per CU; with the material index's
     // light loads that occupancy pays (256^3: 233 against 249 us)

@@ -3,7 +3,7 @@
 
 namespace hfdtd {
 
-HF_MARCH_RW_SET(, float, 1);
-HF_MARCH_RW_MAT(, float, 1);
+HF_MARCH_RW_SET(, float, 1, 64);
+HF_MARCH_RW_MAT(, float, 1, 64);
 
 }  // namespace hfdtd

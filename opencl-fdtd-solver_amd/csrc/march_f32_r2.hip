@@ -3,7 +3,7 @@
 
 namespace hfdtd {
 
-HF_MARCH_RW_SET(, float, 2);
-HF_MARCH_RW_MAT(, float, 2);
+HF_MARCH_RW_SET(, float, 2, 64);
+HF_MARCH_RW_MAT(, float, 2, 64);
 
 }  // namespace hfdtd

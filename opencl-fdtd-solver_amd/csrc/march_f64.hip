@@ -3,6 +3,6 @@
 
 namespace hfdtd {
 
-HF_MARCH_RW_SET(, double, 1);
+HF_MARCH_RW_SET(, double, 1, 64);
 
 }  // namespace hfdtd

@@ -9,12 +9,15 @@
 
 namespace hfdtd {
 
-template <typename T, int RW, bool LOSSY, bool NT, int CEM = kCeVolumes, bool BIG = false>
+template <typename T, int RW, bool LOSSY, bool NT, int CEM = kCeVolumes, bool BIG = false,
+          int W = 64>
 int march_rw(Engine *e, const SrcList<T> &sl) {
-  auto kern = k_march<T, RW, LOSSY, NT, CEM, BIG>;
+  auto kern = k_march<T, RW, LOSSY, NT, CEM, BIG, W>;
+  constexpr int wi = W == 64 ? 0 : W == 32 ? 1 : 2;
   int big_occ = 0;
-  int &occ = BIG ? big_occ
-                 : CEM != kCeVolumes ? e->march_occ_mat[CEM][RW] : e->march_occ[LOSSY ? 1 : 0][RW];
+  int &occ = BIG                 ? big_occ
+             : CEM != kCeVolumes ? e->march_occ_mat[wi][CEM][RW]
+                                 : e->march_occ[wi][LOSSY ? 1 : 0][RW];
   if (occ == 0) {
     // resident blocks per CU from the kernel's own resource use: a block puts
     // one wavefront on each of the four SIMDs, each SIMD holds 512 VGPRs per
@@ -30,8 +33,9 @@ int march_rw(Engine *e, const SrcList<T> &sl) {
     }
     occ = per_cu;
   }
-  const int planes = e->march_planes_for(RW, occ);
-  const int tiles_k = (int)ceil_div(e->nz, 64), tiles_j = (int)ceil_div(e->ny, 4 * RW);
+  const int planes = e->march_planes_for(RW, occ, W);
+  const int tiles_k = (int)ceil_div(e->nz, W);
+  const int tiles_j = (int)ceil_div(e->ny, MarchShape<RW, W>::TJ);
   MarchChunks ch;
   ch.split = e->npml > 0 && e->nx > 2 * e->npml ? 1 : 0;
   ch.inner = (int)ceil_div(ch.split ? e->nx - 2 * e->npml : e->nx, planes);
@@ -76,8 +80,8 @@ int march_rw(Engine *e, const SrcList<T> &sl) {
   return source_fixup<T, LOSSY>(e, sl);
 }
 
-#define HF_MARCH_RW(T, RW, LOSSY, NT, CEM) \
-  template int march_rw<T, RW, LOSSY, NT, CEM, false>(Engine *, const SrcList<T> &)
+#define HF_MARCH_RW(T, RW, LOSSY, NT, CEM, W) \
+  template int march_rw<T, RW, LOSSY, NT, CEM, false, W>(Engine *, const SrcList<T> &)
 // slabs of 4 GB and more (FP32, streaming)
 #define HF_MARCH_RW_BIG1(X, RW)                                                                   \
   X template int march_rw<float, RW, false, true, kCeVolumes, true>(Engine *, const SrcList<float> &); \
@@ -87,15 +91,16 @@ int march_rw(Engine *e, const SrcList<T> &sl) {
 #define HF_MARCH_RW_BIG(X) \
   HF_MARCH_RW_BIG1(X, 1);  \
   HF_MARCH_RW_BIG1(X, 2)
-#define HF_MARCH_RW_SET(X, T, RW)        \
-  X HF_MARCH_RW(T, RW, false, true, kCeVolumes);  \
-  X HF_MARCH_RW(T, RW, false, false, kCeVolumes); \
-  X HF_MARCH_RW(T, RW, true, true, kCeVolumes);   \
-  X HF_MARCH_RW(T, RW, true, false, kCeVolumes)
-#define HF_MARCH_RW_MAT(X, T, RW)       \
-  X HF_MARCH_RW(T, RW, false, true, kCeIndex);  \
-  X HF_MARCH_RW(T, RW, false, false, kCeIndex); \
-  X HF_MARCH_RW(T, RW, false, true, kCeEps);    \
-  X HF_MARCH_RW(T, RW, false, false, kCeEps)
+// W: the tile width (64, 32 or 16 columns)
+#define HF_MARCH_RW_SET(X, T, RW, W)        \
+  X HF_MARCH_RW(T, RW, false, true, kCeVolumes, W);  \
+  X HF_MARCH_RW(T, RW, false, false, kCeVolumes, W); \
+  X HF_MARCH_RW(T, RW, true, true, kCeVolumes, W);   \
+  X HF_MARCH_RW(T, RW, true, false, kCeVolumes, W)
+#define HF_MARCH_RW_MAT(X, T, RW, W)       \
+  X HF_MARCH_RW(T, RW, false, true, kCeIndex, W);  \
+  X HF_MARCH_RW(T, RW, false, false, kCeIndex, W); \
+  X HF_MARCH_RW(T, RW, false, true, kCeEps, W);    \
+  X HF_MARCH_RW(T, RW, false, false, kCeEps, W)
 
 }  // namespace hfdtd

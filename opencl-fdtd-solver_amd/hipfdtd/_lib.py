@@ -97,6 +97,7 @@ EXPORTS = {
     "hfdtd_ce_source": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hfdtd_set_march_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "hfdtd_set_march_addressing": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfdtd_set_march_width": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "hfdtd_timing_launches": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
@@ -162,6 +163,7 @@ EXPORTS = {
     "hfdtd_batch_read_probes": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "hfdtd_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "hfdtd_batch_set_path": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfdtd_batch_set_march": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "hfdtd_adjoint_create": (
         C.c_int,
         [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_void_p)],

@@ -10,8 +10,8 @@ depends on the grid extents, the dtype, lossy or not, and the device.
 ``tune_engine(sim)`` times the split sweeps and every fused-sweep and
 march-sweep candidate (HIP events on the engine stream) on a scratch engine
 of the same shape and applies the fastest; ``tune_batch(batch)`` does the same for a
-``BatchedFDTD`` (fused tiles or flat split sweeps) before its tables are
-set.  All stepping happens on zero fields with no sources, which
+``BatchedFDTD`` (fused tiles, flat split sweeps or march tiles) before its
+tables are set.  All stepping happens on zero fields with no sources, which
 leaves the fields exactly zero, so tuning never changes a simulation.
 Results are cached in a JSON file keyed by (kind, shapes, dtype, npml, lossy,
 device, library ABI); ``HIPFDTD_TUNE_CACHE`` names the file (default
@@ -43,6 +43,10 @@ DEFAULT_PATHS = ("fused", "split", "march")
 BATCH_ROWS_FP32 = (2, 4, 6, 8)
 BATCH_ROWS_FP64 = (2, 4)
 BATCH_PLANES = (8, 16, 32, 64)
+# batched march candidates: (rows per wavefront, planes per chunk, tile width)
+BATCH_MARCH_FP32 = ((1, 32, 64), (2, 32, 64), (1, 24, 32), (1, 32, 32), (1, 48, 32),
+                    (1, 32, 16), (1, 48, 16))
+BATCH_MARCH_FP64 = ((1, 16, 64), (1, 32, 64), (1, 48, 64))
 
 
 def cache_path() -> Path:
@@ -194,16 +198,19 @@ def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=No
     return dict(entry, cached=False)
 
 
-def tune_batch(batch, *, rows=None, planes=None, blocks=(0,), paths=("fused", "split"),
-               steps=10, warmup=2, use_cache=True, write_cache=True, cache_file=None) -> dict:  # fmt: skip
-    """Pick the path ('fused' tiles or 'split' flat sweeps) and (blocks,
-    rows_j, planes_i) of a ``BatchedFDTD`` and apply them.
+def tune_batch(batch, *, rows=None, planes=None, blocks=(0,), paths=("fused", "split", "march"),
+               march=None, steps=10, warmup=2, use_cache=True, write_cache=True,
+               cache_file=None) -> dict:  # fmt: skip
+    """Pick the path ('fused' tiles, 'split' flat sweeps or 'march' tiles)
+    and its launch shape for a ``BatchedFDTD`` and apply them: (blocks,
+    rows_j, planes_i) for the fused path, (rows per wavefront, planes per
+    chunk, tile width) from ``march`` for the march path.
 
     Must run before the batch's source / probe tables are set, while every
     simulation is still at step 0 with zero fields (stepping them then leaves
     them unchanged).  Returns the choice, its ms per step and the table
-    [[path, blocks, rows, planes, ms], ...]; rows and planes do not apply to
-    the split path.
+    [[path, blocks, rows, planes, width, ms], ...]; rows and planes do not
+    apply to the split path, width only to the march path.
     """
     sims = batch.sims
     if batch._n_table or any(s.step_num for s in sims):
@@ -211,36 +218,44 @@ def tune_batch(batch, *, rows=None, planes=None, blocks=(0,), paths=("fused", "s
     s0 = sims[0]
     key = cache_key("batch", [(s.Nx, s.Ny, s.Nz) for s in sims], s0.dtype, s0.npml,
                     any(s.lossy for s in sims), s0.device_name)  # fmt: skip
+    fp64 = s0.dtype == np.dtype(np.float64)
+
+    def apply(path, b, r, p, w):
+        batch.set_path(path)
+        if path == "march":
+            batch.set_march(r, p, 0, w)
+        else:
+            batch.set_launch(b, r or 0, p or 0)
+
     if use_cache:
         hit = load_cache(cache_file).get(key)
-        if hit:
-            batch.set_path(hit["path"])
-            batch.set_launch(hit["blocks"], hit["rows"] or 0, hit["planes"] or 0)
+        if hit and hit["path"] in paths and "width" in hit:
+            apply(hit["path"], hit["blocks"], hit["rows"], hit["planes"], hit["width"])
             return dict(hit, cached=True)
-    fp64 = s0.dtype == np.dtype(np.float64)
     rows = rows or (BATCH_ROWS_FP64 if fp64 else BATCH_ROWS_FP32)
     planes = planes or BATCH_PLANES
+    march = march or (BATCH_MARCH_FP64 if fp64 else BATCH_MARCH_FP32)
     lib = batch._lib
 
-    def timed(path, b, r, p):
-        batch.set_path(path)
-        batch.set_launch(b, r or 0, p or 0)
+    def timed(path, b, r, p, w):
+        apply(path, b, r, p, w)
         _lib.check(lib.hfdtd_batch_run(batch._handle, warmup, 0), "batch tune")
         t0 = time.perf_counter()
         _lib.check(lib.hfdtd_batch_run(batch._handle, steps, 0), "batch tune")
-        return [path, b, r, p, round((time.perf_counter() - t0) / steps * 1e3, 4)]
+        return [path, b, r, p, w, round((time.perf_counter() - t0) / steps * 1e3, 4)]
 
     table = []
     for b in blocks:
         if "fused" in paths:
-            table += [timed("fused", b, r, p) for r in rows for p in planes]
+            table += [timed("fused", b, r, p, None) for r in rows for p in planes]
         if "split" in paths:
-            table.append(timed("split", b, None, None))
-    path, b, r, p, ms = min(table, key=lambda e: e[4])
-    batch.set_path(path)
-    batch.set_launch(b, r or 0, p or 0)
-    entry = {"path": path, "blocks": b, "rows": r, "planes": p, "ms": ms, "table": table,
-             "when": time.strftime("%Y-%m-%dT%H:%M:%S")}  # fmt: skip
+            table.append(timed("split", b, None, None, None))
+    if "march" in paths:
+        table += [timed("march", 0, r, p, w) for r, p, w in march]
+    path, b, r, p, w, ms = min(table, key=lambda e: e[5])
+    apply(path, b, r, p, w)
+    entry = {"path": path, "blocks": b, "rows": r, "planes": p, "width": w, "ms": ms,
+             "table": table, "when": time.strftime("%Y-%m-%dT%H:%M:%S")}  # fmt: skip
     if write_cache:
         save_cache({key: entry}, cache_file)
     return dict(entry, cached=False)

@@ -100,14 +100,24 @@ class BatchedFDTD:
                                                     int(planes_i)))  # fmt: skip
 
     def set_path(self, path: str) -> None:
-        """'fused' (one persistent fused-tile launch per step, the default) or
+        """'fused' (one persistent fused-tile launch per step, the default),
         'split' (persistent split sweeps over flattened planes: H, sources,
-        E, probes; no idle lanes on extents that are not multiples of 64)."""
-        if path not in ("fused", "split"):
-            raise ValueError("path must be 'fused' or 'split'")
-        code = _lib.PATH_FUSED if path == "fused" else _lib.PATH_SPLIT
-        _lib.check(self._lib.hfdtd_batch_set_path(self._handle, code), "batch_set_path")
+        E, probes; no idle lanes on extents that are not multiples of 64) or
+        'march' (one launch per step of march tiles over every grid, sources
+        and probes handled inside the sweep)."""
+        codes = {"fused": _lib.PATH_FUSED, "split": _lib.PATH_SPLIT, "march": _lib.PATH_MARCH}
+        if path not in codes:
+            raise ValueError("path must be 'fused', 'split' or 'march'")
+        _lib.check(self._lib.hfdtd_batch_set_path(self._handle, codes[path]), "batch_set_path")
         self.path = path
+
+    def set_march(self, rows: int = 0, planes: int = 0, run: int = 0, width: int = 0) -> None:
+        """Launch shape of the 'march' path: j rows per wavefront (1 or 2),
+        i planes per chunk, consecutive work items per XCD run, tile width
+        in k (64, 32 or 16 columns; narrower tiles leave fewer lanes idle on
+        extents that are not multiples of 64); 0 keeps a setting."""
+        _lib.check(self._lib.hfdtd_batch_set_march(self._handle, int(rows), int(planes), int(run),
+                                                   int(width)), "batch_set_march")  # fmt: skip
 
     def run(self, n_steps: int, graph_steps: int = 0):
         """Advance every simulation ``n_steps`` steps (one launch per step, or

@@ -547,6 +547,12 @@ class HIPFDTD(SourceMonitorMixin):
                                                     int(planes_i), int(run_blocks),
                                                     int(nontemporal)), "set_march_tiling")  # fmt: skip
 
+    def set_march_width(self, width: int = 0) -> None:
+        """March-sweep tile width: 64 (default), 32 or 16 columns of k, with
+        64 / width rows of j per wavefront slot (FP32 grids with slabs below
+        4 GB; others keep 64).  The results do not depend on it."""
+        _lib.check(self._lib.hfdtd_set_march_width(self._handle, int(width)), "set_march_width")
+
     def set_march_addressing(self, per_volume: bool) -> None:
         """March sweep addressing: one buffer resource per volume on any FP32
         grid (True), or only where the slabs reach 4 GB (False, default);

@@ -127,6 +127,132 @@ def test_batch_matches_serial_engines(graph_steps, path):
     batch.close()
 
 
+def _march_batch_case(shapes, media, n_steps, graph_steps, rows, planes, run, fp64=False,
+                      lossy=(), split=None, width=0):
+    """Serial march engines and one batched march run of the same grids,
+    sources and probes; returns [(serial fields, serial probe DFT)], the
+    batch's sims and its probe DFTs."""
+    dtype = np.float64 if fp64 else np.float32
+    nf = 4
+    freqs = [list(np.linspace(0.7, 1.3, nf) * f) for _, f in media]
+
+    def src_box(shape):
+        c = [s // 2 for s in shape]
+        return (c[0], c[0] + 1, c[1], c[1] + 1, c[2], c[2] + 1)
+
+    def probe_cell(shape):
+        return (shape[0] // 2 + 3, shape[1] // 2 - 2, shape[2] // 2 + 1)
+
+    def engine(q, shape, eps):
+        sim = hip().HIPFDTD(shape, 1e-3, npml=8, dtype=dtype, path="march")
+        if q in lossy:
+            sig = np.zeros(shape, np.float32)
+            sig[2:-2, 3:-3, 4:-4] = 0.05
+            sim.set_epsilon(eps, sig)
+        else:
+            sim.set_epsilon(eps)
+        return sim
+
+    serial = []
+    for q, (shape, (eps, f), fr) in enumerate(zip(shapes, media, freqs)):
+        sim = engine(q, shape, eps)
+        wf = waveform(f)
+        sim.add_source(lambda s, wf=wf, b=src_box(shape): s.add_source_J("z", wf(s.t), b))
+        pr = hip().probes.HIPPointProbe(sim, probe_cell(shape), "Ez", fr)
+        sim.run(n_steps)
+        serial.append((sim.fields(), pr.dft()))
+        sim.close()
+    sims = [engine(q, shape, eps) for q, (shape, (eps, _)) in enumerate(zip(shapes, media))]
+    batch = hip().batch.BatchedFDTD(sims)
+    batch.set_path("march")
+    batch.set_march(rows, planes, run, width)
+    batch.set_sources([("z", src_box(s)) for s in shapes], [waveform(f) for _, f in media],
+                      n_steps)  # fmt: skip
+    batch.set_probes([probe_cell(s) for s in shapes], "Ez", freqs, n_steps)
+    first = n_steps // 2 + 1 if split is None else split  # an odd first run: parity carries over
+    batch.run(first, graph_steps)
+    batch.run(n_steps - first, graph_steps)
+    dfts = batch.probe_dfts()
+    batch.close()
+    return serial, sims, dfts
+
+
+@pytest.mark.parametrize("medium,graph_steps,rows,planes,run", [
+    ("index", 0, 1, 32, 16), ("index", 10, 2, 24, 8), ("index", 10, 1, 8, 1),
+    ("volumes", 10, 1, 30, 16), ("lossy", 0, 2, 32, 16), ("lossy", 10, 1, 64, 5),
+    ("fp64", 10, 1, 32, 16), ("fp64-lossy", 0, 1, 16, 16)])  # fmt: skip
+def test_batch_march_matches_serial_march_engines(medium, graph_steps, rows, planes, run):
+    """The batched march sweep against the same grids stepped one by one on
+    the single-engine march sweep: fields and probe DFTs bit for bit, on
+    the material index (every grid indexed), the ce volumes (one grid with
+    more than 48 eps values), with ca (one lossy grid) and in FP64."""
+    rng = np.random.default_rng(7)
+    shapes = [tuple(int(v) for v in rng.integers(20, 90, size=3)) for _ in range(5)]
+    media = [sphere_case(rng, s) for s in shapes]
+    if medium == "volumes":
+        eps = media[1][0].copy()
+        eps[2:-2, 2:-2, 2:-2] += rng.uniform(0, 1e-3, size=eps[2:-2, 2:-2, 2:-2].shape)
+        media[1] = (eps.astype(np.float32), media[1][1])
+    lossy = (2,) if "lossy" in medium else ()
+    n_steps = 41
+    print(f"[batch-march] {medium} shapes={shapes} graph={graph_steps} rows={rows} "
+          f"planes={planes} run={run}", flush=True)  # fmt: skip
+    serial, sims, dfts = _march_batch_case(shapes, media, n_steps, graph_steps, rows, planes, run,
+                                           fp64=medium.startswith("fp64"), lossy=lossy)  # fmt: skip
+    for q, sim in enumerate(sims):
+        if medium == "index":
+            assert sim.material_count() > 0
+        ref_f, ref_p = serial[q]
+        got = sim.fields()
+        for name in ref_f:
+            np.testing.assert_array_equal(got[name], ref_f[name], err_msg=f"grid {q} {name}")
+        np.testing.assert_array_equal(dfts[q], ref_p, err_msg=f"grid {q} probe")
+        assert np.max(np.abs(ref_p)) > 0
+        assert sim.step_num == n_steps and math.isclose(sim.t, n_steps * sim.dt, rel_tol=1e-12)
+        sim.close()
+
+
+def test_batch_march_against_oracle():
+    """Two grids batched on the march path against the oracle stepping each
+    grid on its own (fields and probe DFT within 1e-5)."""
+    rng = np.random.default_rng(9)
+    shapes = [(30, 41, 70), (52, 27, 33)]
+    media = [sphere_case(rng, s) for s in shapes]
+    n_steps = 60
+    _, sims, dfts = _march_batch_case(shapes, media, n_steps, 0, 1, 16, 16)
+    for q, (shape, (eps, f)) in enumerate(zip(shapes, media)):
+        ora = fo.OracleFDTD(shape, 1e-3, npml=8)
+        ora.set_epsilon(eps)
+        c = [s // 2 for s in shape]
+        box = (c[0], c[0] + 1, c[1], c[1] + 1, c[2], c[2] + 1)
+        wf = waveform(f)
+        ora.add_source(lambda s, wf=wf, b=box: s.add_source_J("z", wf(s.t), b))
+        cell = (shape[0] // 2 + 3, shape[1] // 2 - 2, shape[2] // 2 + 1)
+        om = 2 * np.pi * np.linspace(0.7, 1.3, 4) * f
+        acc = np.zeros(4, complex)
+        for _ in range(n_steps):
+            ora.step()
+            acc += float(ora.Ez[cell]) * np.exp(1j * om * ora.t) * ora.dt
+        assert group_error(sims[q].fields(), ora.fields()) < 1e-5, q
+        assert np.max(np.abs(dfts[q] - acc)) / np.max(np.abs(acc)) < 1e-5, q
+        sims[q].close()
+
+
+def test_batch_march_rejects_bad_shapes():
+    sims = [hip().HIPFDTD((16, 16, 16), 1e-3, npml=2) for _ in range(2)]
+    batch = hip().batch.BatchedFDTD(sims)
+    with pytest.raises(ValueError, match="rows"):
+        batch.set_march(rows=3)
+    with pytest.raises(ValueError, match="path"):
+        batch.set_path("pair")
+    batch.close()
+    s64 = [hip().HIPFDTD((16, 16, 16), 1e-3, npml=2, dtype=np.float64) for _ in range(2)]
+    b64 = hip().batch.BatchedFDTD(s64)
+    with pytest.raises(ValueError, match="FP64"):
+        b64.set_march(rows=2)
+    b64.close()
+
+
 def test_batch_rejects_mixed_grids():
     a = hip().HIPFDTD((16, 16, 16), 1e-3, npml=2)
     b = hip().HIPFDTD((16, 16, 16), 2e-3, npml=2)
