@@ -49,8 +49,11 @@ against central differences (δε = 1e-3) of FP64 runs.
 ``--workload batch`` measures 64 simulations of uneven sizes (sides
 U{48..160}, seed 4, npml 8, a centred dielectric sphere, a centred Jz dipole
 and a 4-frequency Ez probe each, 1200 steps) advanced together by the
-batched persistent sweep (schedule picked by the autotuner, HIP-graph
-replays), against the same 64 runs one after another on the plain engine.
+batched sweep (path and schedule picked by the autotuner among the split,
+fused and march batch paths; HIP-graph replays), against the same 64 runs
+one after another on the plain engine: with its device point probe, on the
+fused sweep, and through the reference's plugin API alone (source callback,
+host DFT monitor reading the probe cell each step).
 
 ``--workload plane-dft`` (not the default line) measures the 512^3 plane-DFT
 configuration: smooth Gaussian random εr (seed 3, σ = 8 cells, clipped to
@@ -516,11 +519,42 @@ def run_batch(args):
             sim.close()
         return total, dfts, dev
 
+    def serial_plain_api():
+        """The plain engine driven only through the reference's plugin API:
+        the source as a callback, the probe DFT as a host monitor reading
+        the cell with read_point after every step (the reference has no
+        device-side probe)."""
+        total, dfts = 0.0, []
+        for c, eps in zip(cfg, epss):
+            sim = hipfdtd.HIPFDTD(c["shape"], dl, npml=npml)
+            sim.set_epsilon(eps)
+            w, bx = wave(c), box(c)
+            sim.add_source(lambda f, w=w, bx=bx: f.add_source_J("z", w(f.t), bx))
+            om = 2 * np.pi * np.asarray(c["freqs"])
+            acc = np.zeros(len(om), complex)
+            cell = tuple(c["probe"])
+
+            def probe(f, cell=cell, om=om, acc=acc):
+                acc += float(f.read_point("Ez", *cell)) * np.exp(1j * om * f.t) * f.dt
+
+            sim.add_monitor(probe)
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.run(steps)
+            sim.synchronize()
+            total += time.perf_counter() - t0
+            dfts.append(acc)
+            sim.close()
+        return total, dfts
+
     # the plain engine's default path (split sweeps for FP32), and the fused sweep
     t_serial, serial, dev = serial_run(None)
     t_fused, _, _ = serial_run(True)
-    print(f"[batch] serial {t_serial:.2f} s (fused sweep {t_fused:.2f} s)", file=sys.stderr,
-          flush=True)  # fmt: skip
+    t_plain, plain = serial_plain_api()
+    plain_err = float(np.max(np.abs(np.array(plain) - np.array(serial))
+                             / np.max(np.abs(np.array(serial)), axis=1, keepdims=True)))  # fmt: skip
+    print(f"[batch] serial {t_serial:.2f} s (fused sweep {t_fused:.2f} s, plugin API with a host "
+          f"probe {t_plain:.2f} s)", file=sys.stderr, flush=True)  # fmt: skip
 
     sims = []
     for c, eps in zip(cfg, epss):
@@ -556,6 +590,11 @@ def run_batch(args):
         "serial_path": "default (split sweeps for FP32)",
         "serial_fused_sweep_s": round(t_fused, 3),
         "batched_vs_serial_fused_sweep": round(t_fused / t_batch, 2),
+        "serial_plain_api_s": round(t_plain, 3),
+        "serial_plain_api": "source callback + host DFT monitor reading the probe cell with "
+                            "read_point each step (the reference's plugin API)",
+        "batched_vs_serial_plain_api": round(t_plain / t_batch, 2),
+        "plain_api_probe_max_rel_err_vs_device_probe": plain_err,
         "batch_ms_per_step": round(step_s * 1e3, 3),
         "batch_hbm_gbs_alg": round(b_alg / step_s / 1e9, 1),
         "batch_hbm_frac": round(b_alg / step_s / 1e9 / HBM_PEAK_GBS, 4),

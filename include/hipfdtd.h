@@ -169,11 +169,11 @@ int hfdtd_set_march_tiling(hfdtd_engine *e, int rows_per_wave, int planes_i, int
  * 4 GB, and one per volume on FP32 slabs of 4 GB and more (volumes below
  * 2 GB, about 810^3 cells); 1 one per volume on any FP32 grid (same results;
  * for testing that addressing on small grids). */
-/* March-sweep tile width: 64 (default), 32 or 16 columns of k, with 64 / width
- * rows of j per wavefront slot (0 = 64).  Narrow tiles leave fewer lanes idle
- * on extents that are not multiples of 64 and recompute a smaller halo; they
- * apply to FP32 grids with slabs below 4 GB and one row per wavefront (FP64
- * and larger grids keep 64).  Results are the same for every width. */
+/* March-sweep tile width: 64 (default) or 32 columns of k, with 64 / width
+ * rows of j per wavefront slot (0 = 64).  32-column tiles take fewer tiles
+ * where nz modulo 64 lies in (0, 32] (thin grids: nz <= 32, nz = 96 ...);
+ * they apply to FP32 grids with slabs below 4 GB and one row per wavefront
+ * (FP64 and larger grids keep 64).  Results are the same for every width. */
 int hfdtd_set_march_width(hfdtd_engine *e, int width);
 int hfdtd_set_march_addressing(hfdtd_engine *e, int per_volume);
 
@@ -361,9 +361,9 @@ int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i)
 int hfdtd_batch_set_path(hfdtd_batch *b, int path);
 /* HFDTD_PATH_MARCH launch shape: j rows per wavefront (1 or 2; FP64: 1),
  * i planes per chunk (2..256, rounded up to even), consecutive work items
- * per XCD run, tile width in k (64, 32 or 16 columns; narrower tiles with
- * one row per wavefront, FP32 only); 0 keeps the current value (defaults 1,
- * 32, 16, 64) */
+ * per XCD run, tile width in k (64 or 32 columns; 32 with one row per
+ * wavefront, FP32 only); 0 keeps the current value (defaults 1, 32, 16,
+ * 64) */
 int hfdtd_batch_set_march(hfdtd_batch *b, int rows, int planes, int run, int width);
 
 /* ── adjoint gradient of probe-DFT objectives ─────────────────────────────

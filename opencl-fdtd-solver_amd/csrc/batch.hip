@@ -501,7 +501,6 @@ static int batch_launch_march(const hfdtd_batch *b, const BMArgs<T> &a, int q) {
                             : launch_batch_march<T, RW, false, kCeVolumes, W>(a, q, b->stream)
     if (b->mrows == 2) HF_BM(2, 64);
     if (b->mwidth == 32) HF_BM(1, 32);
-    if (b->mwidth == 16) HF_BM(1, 16);
     HF_BM(1, 64);
 #undef HF_BM
   }
@@ -891,9 +890,9 @@ int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i)
 
 int hfdtd_batch_set_march(hfdtd_batch *b, int rows, int planes, int run, int width) {
   if (!b || rows < 0 || rows > 2 || planes < 0 || planes > 256 || run < 0 ||
-      (width != 0 && width != 64 && width != 32 && width != 16))
+      (width != 0 && width != 64 && width != 32))
     return fail(HFDTD_EINVAL, "march rows must be 1 or 2, planes in [2, 256], run >= 1, width "
-                              "64, 32 or 16 (0 = keep)");
+                              "64 or 32 (0 = keep)");
   if (b->fp64 && (rows == 2 || (width && width != 64)))
     return fail(HFDTD_EINVAL, "FP64 batches use one row per wavefront and 64-column tiles");
   const int r = rows ? rows : b->mrows, w = width ? width : b->mwidth;

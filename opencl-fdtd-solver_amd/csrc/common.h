@@ -115,13 +115,13 @@ struct Engine {
   int march_run = 16;
   bool march_per_volume = false;  // per-volume addressing on any FP32 slab (set by hand)
   int march_nt = -1;
-  // march tiles of 64, 32 or 16 columns of k (FP32; 64 / W rows of j per
+  // march tiles of 64 or 32 columns of k (FP32; 64 / W rows of j per
   // wavefront slot)
   int march_width = 64;
-  // resident march blocks per CU by [width 64/32/16][lossy][rows], and the
+  // resident march blocks per CU by [width 64/32][lossy][rows], and the
   // same for the index / eps_r sweeps by [width][ce source][rows]
-  int march_occ[3][2][3] = {};
-  int march_occ_mat[3][3][3] = {};
+  int march_occ[2][2][3] = {};
+  int march_occ_mat[2][3][3] = {};
   // Lossless FP32 grids with at most kMatMax distinct eps_r values: a byte of
   // material per cell and the table of ce for every pair of materials (an
   // edge's ce depends only on its two cells), which the march sweep reads

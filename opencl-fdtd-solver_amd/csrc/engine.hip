@@ -795,8 +795,8 @@ int hfdtd_set_march_tiling(hfdtd_engine *h, int rows_per_wave, int planes_i, int
 
 int hfdtd_set_march_width(hfdtd_engine *h, int width) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
-  if (width != 0 && width != 64 && width != 32 && width != 16)
-    return fail(HFDTD_EINVAL, "march width must be 64, 32 or 16 (0 = 64)");
+  if (width != 0 && width != 64 && width != 32)
+    return fail(HFDTD_EINVAL, "march width must be 64 or 32 (0 = 64)");
   h->s.march_width = width ? width : 64;
   return HFDTD_OK;
 }

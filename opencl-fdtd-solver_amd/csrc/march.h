@@ -115,13 +115,13 @@ __device__ __forceinline__ void bst_quad(Rsrc r, float a, float b, float c, floa
   __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
 }
 
-// Tile shape: W columns of k (64, 32 or 16) by 4 * RW * (64 / W) rows of j.
+// Tile shape: W columns of k (64 or 32) by 4 * RW * (64 / W) rows of j.
 // A wavefront's slot holds 64 / W rows of W columns (lane = row * W + column
 // within the slot), so narrow grids leave fewer lanes idle and the tile's
 // recomputed halo shrinks as it gets squarer.
 template <int RW, int W>
 struct MarchShape {
-  static_assert(W == 64 || W == 32 || W == 16, "march tiles are 64, 32 or 16 columns wide");
+  static_assert(W == 64 || W == 32, "march tiles are 64 or 32 columns wide");
   static constexpr int G = 64 / W;        // rows of one slot
   static constexpr int TJ = 4 * RW * G;   // rows of the tile
   static constexpr int C = W + 2;         // LDS columns: k0-1 ... k0+W

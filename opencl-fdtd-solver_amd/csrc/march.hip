@@ -1,6 +1,6 @@
 // March sweep: the step launcher (tiling and medium kind pick one of the
 // launchers instantiated in march_f32_r1.hip, march_f32_r2.hip,
-// march_f32_w32.hip, march_f32_w16.hip, march_f64.hip, march_f32_big.hip).
+// march_f32_w32.hip, march_f64.hip, march_f32_big.hip).
 
 #include "march_launch.h"
 
@@ -13,8 +13,6 @@ HF_MARCH_RW_MAT(extern, float, 2, 64);
 HF_MARCH_RW_SET(extern, double, 1, 64);
 HF_MARCH_RW_SET(extern, float, 1, 32);
 HF_MARCH_RW_MAT(extern, float, 1, 32);
-HF_MARCH_RW_SET(extern, float, 1, 16);
-HF_MARCH_RW_MAT(extern, float, 1, 16);
 HF_MARCH_RW_BIG(extern);
 
 // slabs of 4 GB and more: FP32 volumes below 2 GB each, psi slab below 2 GB
@@ -82,7 +80,7 @@ static int march_t(Engine *e, const hfdtd_source *src, int n) {
     rc = ll ? (nt ? march_rw<T, 1, false, true>(e, sl) : march_rw<T, 1, false, false>(e, sl))
             : (nt ? march_rw<T, 1, true, true>(e, sl) : march_rw<T, 1, true, false>(e, sl));
   } else if (e->march_width != 64) {
-    rc = e->march_width == 32 ? march_narrow<32>(e, sl, nt) : march_narrow<16>(e, sl, nt);
+    rc = march_narrow<32>(e, sl, nt);
   } else if (e->march_rows == 1 || (e->march_rows == 0 && e->ce_source() == kCeIndex)) {
     // one row per wavefront: four blocks per CU; with the material index's
     // light loads that occupancy pays (256^3: 233 against 249 us)

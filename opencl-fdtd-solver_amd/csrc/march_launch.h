@@ -13,7 +13,7 @@ template <typename T, int RW, bool LOSSY, bool NT, int CEM = kCeVolumes, bool BI
           int W = 64>
 int march_rw(Engine *e, const SrcList<T> &sl) {
   auto kern = k_march<T, RW, LOSSY, NT, CEM, BIG, W>;
-  constexpr int wi = W == 64 ? 0 : W == 32 ? 1 : 2;
+  constexpr int wi = W == 64 ? 0 : 1;
   int big_occ = 0;
   int &occ = BIG                 ? big_occ
              : CEM != kCeVolumes ? e->march_occ_mat[wi][CEM][RW]
@@ -91,7 +91,7 @@ int march_rw(Engine *e, const SrcList<T> &sl) {
 #define HF_MARCH_RW_BIG(X) \
   HF_MARCH_RW_BIG1(X, 1);  \
   HF_MARCH_RW_BIG1(X, 2)
-// W: the tile width (64, 32 or 16 columns)
+// W: the tile width (64 or 32 columns)
 #define HF_MARCH_RW_SET(X, T, RW, W)        \
   X HF_MARCH_RW(T, RW, false, true, kCeVolumes, W);  \
   X HF_MARCH_RW(T, RW, false, false, kCeVolumes, W); \

@@ -38,14 +38,17 @@ PLANES = (-1, 8, 16, 24, 32, 48, 64)
 MARCH_ROWS_FP32 = (1, 2)
 MARCH_ROWS_FP64 = (1,)
 MARCH_PLANES = (-1, 16, 24, 32)
+# march tile widths (columns of k): 32-column tiles (two rows of j per
+# wavefront slot, one row per wavefront) are candidates on FP32 grids
+MARCH_WIDTHS_FP32 = (64, 32)
+MARCH_WIDTHS_FP64 = (64,)
 PAIR_PLANES = (16, 24, 32)
 DEFAULT_PATHS = ("fused", "split", "march")
 BATCH_ROWS_FP32 = (2, 4, 6, 8)
 BATCH_ROWS_FP64 = (2, 4)
 BATCH_PLANES = (8, 16, 32, 64)
 # batched march candidates: (rows per wavefront, planes per chunk, tile width)
-BATCH_MARCH_FP32 = ((1, 32, 64), (2, 32, 64), (1, 24, 32), (1, 32, 32), (1, 48, 32),
-                    (1, 32, 16), (1, 48, 16))
+BATCH_MARCH_FP32 = ((1, 24, 64), (1, 32, 64), (2, 32, 64), (1, 24, 32), (1, 32, 32))
 BATCH_MARCH_FP64 = ((1, 16, 64), (1, 32, 64), (1, 48, 64))
 
 
@@ -110,33 +113,37 @@ def _time_engine(sim, steps, warmup):
 def apply(sim, result) -> None:
     """Apply a ``tune_engine`` result (path and launch shape) to another
     engine of the same shape and medium kind."""
-    _apply(sim, result["path"], result["rows"], result["planes"])
+    _apply(sim, result["path"], result["rows"], result["planes"], result.get("width") or 64)
 
 
-def _apply(sim, path, r, p):
+def _apply(sim, path, r, p, w=64):
     sim.set_path(path)
     if path == "fused":
         sim.set_fused_tiling(r, p, 0)
     elif path == "march":
+        sim.set_march_width(w)
         sim.set_march_tiling(r, p, 0, -2)
     elif path == "pair":
         sim.set_pair_planes(p)
 
 
 def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=None,
-                pair_planes=None, paths=DEFAULT_PATHS, steps=20, warmup=3, use_cache=True,
-                write_cache=True, cache_file=None) -> dict:  # fmt: skip
+                march_widths=None, pair_planes=None, paths=DEFAULT_PATHS, steps=20, warmup=3,
+                use_cache=True, write_cache=True, cache_file=None) -> dict:  # fmt: skip
     """Pick the step path and its launch shape for ``sim``, and apply them.
 
     Times, on a scratch engine of the same shape, the split sweeps (one
     candidate), the LDS-DMA fused sweep for ``rows`` (j rows per block) x
     ``planes`` candidates and the march sweep for ``march_rows`` (j rows per
-    wavefront) x ``march_planes`` candidates; with "pair" in ``paths``
+    wavefront) x ``march_planes`` candidates on 64-column tiles, and for
+    ``march_planes`` on the narrower ``march_widths`` (one row per
+    wavefront, when 1 is among ``march_rows``); with "pair" in ``paths``
     (opt-in) also the two-steps-per-launch sweep for ``pair_planes``.  The
     timing runs without monitors: the pair sweep's record of watched cells
     for monitors is not included.  Returns {"path": "split" | "fused" |
-    "march" | "pair", "rows", "planes", "us": fastest µs of sweep kernels
-    per step, "table": [[path, rows, planes, µs], ...], "cached": bool};
+    "march" | "pair", "rows", "planes", "width" (march tile width, else
+    None), "us": fastest µs of sweep kernels per step, "table": [[path,
+    rows, planes, width, µs], ...], "cached": bool};
     ``planes`` -1 is the grid-derived default.  One-launch candidates are
     skipped for grids beyond their slab limits.
     """
@@ -151,14 +158,15 @@ def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=No
     key = cache_key(kind, [shape], sim.dtype, sim.npml, sim.lossy, sim.device_name)
     if use_cache:
         hit = load_cache(cache_file).get(key)
-        if hit:
-            _apply(sim, hit["path"], hit["rows"], hit["planes"])
+        if hit and "width" in hit:
+            _apply(sim, hit["path"], hit["rows"], hit["planes"], hit["width"] or 64)
             return dict(hit, cached=True)
     fp64 = sim.dtype == np.dtype(np.float64)
     rows = rows or (FP64_ROWS if fp64 else FP32_ROWS)
     planes = planes or PLANES
     march_rows = march_rows or (MARCH_ROWS_FP64 if fp64 else MARCH_ROWS_FP32)
     march_planes = march_planes or MARCH_PLANES
+    march_widths = march_widths or (MARCH_WIDTHS_FP64 if fp64 else MARCH_WIDTHS_FP32)
     pair_planes = pair_planes or PAIR_PLANES
     scratch = HIPFDTD(shape, sim.dl, npml=sim.npml, dtype=sim.dtype, device=sim.device,
                       fused=False)  # fmt: skip
@@ -168,30 +176,36 @@ def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=No
             scratch.set_epsilon(np.ones(shape, np.float32), np.zeros(shape, np.float32))
         elif indexed:
             scratch.set_epsilon(np.ones(shape, np.float32))  # one material: indexed
+        def timed(path, r, p, w):
+            return [path, r, p, w, round(_time_engine(scratch, steps, warmup), 3)]
+
         if "split" in paths:
-            table.append(["split", None, None, round(_time_engine(scratch, steps, warmup), 3)])
+            table.append(timed("split", None, None, None))
         scratch.set_path("fused")
         if "fused" in paths and scratch.fused:
             for r in rows:
                 for p in planes:
                     scratch.set_fused_tiling(r, p, 0)
-                    table.append(["fused", r, p, round(_time_engine(scratch, steps, warmup), 3)])
+                    table.append(timed("fused", r, p, None))
         scratch.set_path("march")
         if "march" in paths and scratch.path == "march":
-            for r in march_rows:
-                for p in march_planes:
-                    scratch.set_march_tiling(r, p, 0, -2)
-                    table.append(["march", r, p, round(_time_engine(scratch, steps, warmup), 3)])
+            for w in march_widths:
+                scratch.set_march_width(w)
+                for r in march_rows if w == 64 else [r for r in march_rows if r == 1]:
+                    for p in march_planes:
+                        scratch.set_march_tiling(r, p, 0, -2)
+                        table.append(timed("march", r, p, w))
+            scratch.set_march_width(64)
         scratch.set_path("pair")
         if "pair" in paths and scratch.path == "pair":
             for p in pair_planes:
                 scratch.set_pair_planes(p)
-                table.append(["pair", None, p, round(_time_engine(scratch, steps, warmup), 3)])
+                table.append(timed("pair", None, p, None))
     finally:
         scratch.close()
-    path, r, p, us = min(table, key=lambda e: e[3])
-    _apply(sim, path, r, p)
-    entry = {"path": path, "rows": r, "planes": p, "us": us, "table": table,
+    path, r, p, w, us = min(table, key=lambda e: e[4])
+    _apply(sim, path, r, p, w or 64)
+    entry = {"path": path, "rows": r, "planes": p, "width": w, "us": us, "table": table,
              "when": time.strftime("%Y-%m-%dT%H:%M:%S")}  # fmt: skip
     if write_cache:
         save_cache({key: entry}, cache_file)

@@ -114,8 +114,8 @@ class BatchedFDTD:
     def set_march(self, rows: int = 0, planes: int = 0, run: int = 0, width: int = 0) -> None:
         """Launch shape of the 'march' path: j rows per wavefront (1 or 2),
         i planes per chunk, consecutive work items per XCD run, tile width
-        in k (64, 32 or 16 columns; narrower tiles leave fewer lanes idle on
-        extents that are not multiples of 64); 0 keeps a setting."""
+        in k (64 or 32 columns; 32-column tiles take fewer tiles where nz
+        modulo 64 lies in (0, 32]); 0 keeps a setting."""
         _lib.check(self._lib.hfdtd_batch_set_march(self._handle, int(rows), int(planes), int(run),
                                                    int(width)), "batch_set_march")  # fmt: skip
 

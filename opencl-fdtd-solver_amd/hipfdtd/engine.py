@@ -548,9 +548,10 @@ class HIPFDTD(SourceMonitorMixin):
                                                     int(nontemporal)), "set_march_tiling")  # fmt: skip
 
     def set_march_width(self, width: int = 0) -> None:
-        """March-sweep tile width: 64 (default), 32 or 16 columns of k, with
+        """March-sweep tile width: 64 (default) or 32 columns of k, with
         64 / width rows of j per wavefront slot (FP32 grids with slabs below
-        4 GB; others keep 64).  The results do not depend on it."""
+        4 GB; others keep 64).  32-column tiles take fewer tiles where nz
+        modulo 64 lies in (0, 32].  The results do not depend on it."""
         _lib.check(self._lib.hfdtd_set_march_width(self._handle, int(width)), "set_march_width")
 
     def set_march_addressing(self, per_volume: bool) -> None:

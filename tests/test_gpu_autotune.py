@@ -36,8 +36,11 @@ def test_tune_engine_applies_and_caches(tmp_path):
                       cache_file=cache)  # fmt: skip
     print("[autotune] engine", res["path"], res["rows"], res["planes"], res["us"], flush=True)
     assert not res["cached"]
-    assert [res["path"], res["rows"], res["planes"], res["us"]] in res["table"]
-    assert len(res["table"]) == 1 + 6 * 3 + 2 * 2
+    assert [res["path"], res["rows"], res["planes"], res["width"], res["us"]] in res["table"]
+    # split; fused 6 rows x 3 planes; march 2 rows x 2 planes on 64-column
+    # tiles and 2 planes on 32-column tiles
+    assert len(res["table"]) == 1 + 6 * 3 + 2 * 2 + 2
+    assert {e[3] for e in res["table"] if e[0] == "march"} == {64, 32}
     assert sim.path == res["path"]
     assert cache.exists()
     again = tune_engine(sim, cache_file=cache)

@@ -243,6 +243,10 @@ def test_batch_march_rejects_bad_shapes():
     batch = hip().batch.BatchedFDTD(sims)
     with pytest.raises(ValueError, match="rows"):
         batch.set_march(rows=3)
+    with pytest.raises(ValueError, match="width"):
+        batch.set_march(width=16)
+    with pytest.raises(ValueError, match="64-column"):
+        batch.set_march(rows=2, width=32)
     with pytest.raises(ValueError, match="path"):
         batch.set_path("pair")
     batch.close()
