@@ -68,6 +68,7 @@ Usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 20] [--no-cpu]
        python bench.py --workload plane-dft [--steps 1500]
        python bench.py --workload adjoint [--steps 3000] [--repeat 2]
        python bench.py --workload batch [--steps 1200] [--graph-steps 100]
+                       [--batch-path auto|split|fused|march] [--no-serial]
        python bench.py --workload reference-sizes
 """
 
@@ -548,13 +549,17 @@ def run_batch(args):
         return total, dfts
 
     # the plain engine's default path (split sweeps for FP32), and the fused sweep
-    t_serial, serial, dev = serial_run(None)
-    t_fused, _, _ = serial_run(True)
-    t_plain, plain = serial_plain_api()
-    plain_err = float(np.max(np.abs(np.array(plain) - np.array(serial))
-                             / np.max(np.abs(np.array(serial)), axis=1, keepdims=True)))  # fmt: skip
-    print(f"[batch] serial {t_serial:.2f} s (fused sweep {t_fused:.2f} s, plugin API with a host "
-          f"probe {t_plain:.2f} s)", file=sys.stderr, flush=True)  # fmt: skip
+    if args.no_serial:  # profiling runs: the batch alone
+        t_serial = t_fused = t_plain = plain_err = None
+        serial, dev = None, ""
+    else:
+        t_serial, serial, dev = serial_run(None)
+        t_fused, _, _ = serial_run(True)
+        t_plain, plain = serial_plain_api()
+        plain_err = float(np.max(np.abs(np.array(plain) - np.array(serial))
+                                 / np.max(np.abs(np.array(serial)), axis=1, keepdims=True)))  # fmt: skip
+        print(f"[batch] serial {t_serial:.2f} s (fused sweep {t_fused:.2f} s, plugin API with a "
+              f"host probe {t_plain:.2f} s)", file=sys.stderr, flush=True)  # fmt: skip
 
     sims = []
     for c, eps in zip(cfg, epss):
@@ -562,7 +567,9 @@ def run_batch(args):
         sim.set_epsilon(eps)
         sims.append(sim)
     batch = BatchedFDTD(sims)
-    tuned = tune_batch(batch, use_cache=False, write_cache=False)
+    paths = ("fused", "split", "march") if args.batch_path == "auto" else (args.batch_path,)
+    tuned = tune_batch(batch, paths=paths, use_cache=False, write_cache=False)
+    dev = dev or sims[0].device_name
     batch.set_sources([("z", box(c)) for c in cfg], [wave(c) for c in cfg], steps)
     batch.set_probes([c["probe"] for c in cfg], "Ez", [c["freqs"] for c in cfg], steps)
     sims[0].synchronize()
@@ -570,30 +577,33 @@ def run_batch(args):
     batch.run(steps, graph_steps=args.graph_steps)
     t_batch = time.perf_counter() - t0
     got = batch.probe_dfts()
-    ref = np.array(serial)
-    err = float(np.max(np.abs(got - ref) / np.max(np.abs(ref), axis=1, keepdims=True)))
+    err = None
+    if serial is not None:
+        ref = np.array(serial)
+        err = float(np.max(np.abs(got - ref) / np.max(np.abs(ref), axis=1, keepdims=True)))
     batch.close()
     for s in sims:
         s.close()
     b_alg = 72.0 * cells + 8.0 * psi
     step_s = t_batch / steps
     print(json.dumps({
-        "metric": "batched/serial throughput", "value": round(t_serial / t_batch, 2), "unit": "x",
+        "metric": "batched/serial throughput",
+        "value": round(t_serial / t_batch, 2) if t_serial else None, "unit": "x",
         "higher_is_better": True, "steps": steps, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "64 grids, sides U{48..160} (seed 4), npml 8, centred sphere "
                                "r U[5,30] eps_r U[1.5,8], centred Jz dipole f ~ U(c0/30dl, "
                                "c0/12dl), Ez probe, 4-frequency DFT",
                    "total_cells": int(cells), "graph_steps": args.graph_steps},
         "batch_mcell_per_s": round(cells * steps / t_batch / 1e6, 1),
-        "serial_mcell_per_s": round(cells * steps / t_serial / 1e6, 1),
-        "batch_s": round(t_batch, 3), "serial_s": round(t_serial, 3),
+        "serial_mcell_per_s": round(cells * steps / t_serial / 1e6, 1) if t_serial else None,
+        "batch_s": round(t_batch, 3), "serial_s": round(t_serial, 3) if t_serial else None,
         "serial_path": "default (split sweeps for FP32)",
-        "serial_fused_sweep_s": round(t_fused, 3),
-        "batched_vs_serial_fused_sweep": round(t_fused / t_batch, 2),
-        "serial_plain_api_s": round(t_plain, 3),
+        "serial_fused_sweep_s": round(t_fused, 3) if t_fused else None,
+        "batched_vs_serial_fused_sweep": round(t_fused / t_batch, 2) if t_fused else None,
+        "serial_plain_api_s": round(t_plain, 3) if t_plain else None,
         "serial_plain_api": "source callback + host DFT monitor reading the probe cell with "
                             "read_point each step (the reference's plugin API)",
-        "batched_vs_serial_plain_api": round(t_plain / t_batch, 2),
+        "batched_vs_serial_plain_api": round(t_plain / t_batch, 2) if t_plain else None,
         "plain_api_probe_max_rel_err_vs_device_probe": plain_err,
         "batch_ms_per_step": round(step_s * 1e3, 3),
         "batch_hbm_gbs_alg": round(b_alg / step_s / 1e9, 1),
@@ -707,6 +717,10 @@ def main():
     ap.add_argument("--workload", default="yee-256",
                     choices=["yee-256", "plane-dft", "adjoint", "batch", "reference-sizes"])
     ap.add_argument("--graph-steps", type=int, default=100, help="batch: steps per HIP graph")
+    ap.add_argument("--batch-path", default="auto", choices=["auto", "split", "fused", "march"],
+                    help="batch: tune only this path (auto: all three)")
+    ap.add_argument("--no-serial", action="store_true",
+                    help="batch: skip the serial runs (profiling the batch alone)")
     ap.add_argument("--repeat", type=int, default=2, help="adjoint: timed forward+backward runs")
     ap.add_argument("--march-tiling", default=None, metavar="ROWS,PLANES[,RUN[,NT]]",
                     help="with --path march: rows per wavefront, planes per block, blocks per XCD "
