@@ -1,12 +1,11 @@
 // Point probe and the batched sweep of many independent simulations.
 //
-// The batched sweep advances a set of engines of uneven sizes with one
-// persistent launch per step: a flat work list holds every (engine, tile)
-// pair, each block loops over work items with stride gridDim.x and runs the
-// same fused tile body as the single-engine sweep; a second small launch then
-// applies each engine's source and records its probe.  The step index lives in
-// device memory (bumped by a one-thread kernel after each step), so a run of
-// K steps can be captured once as a HIP graph and replayed.  Source
+// The batched sweeps advance a set of engines of uneven sizes together: a
+// flat work list holds every (engine, tile) pair of the path (fused tiles,
+// 256-cell blocks of the split sweeps' flattened planes, or march tiles in
+// batch_march.h).  The step index lives in device memory: launch q of a run
+// reads *step + q, and a one-thread kernel adds the run's length after it,
+// so a run of K steps is captured once as a HIP graph and replayed.  Source
 // amplitudes and probe phases come from per-engine tables indexed by that
 // step, which keeps every launch argument fixed across replays.
 
@@ -94,8 +93,8 @@ struct BatchSrcP {
 // LOSSY selects the update with the ca load; it is exact for lossless
 // engines too (ca = 1), so one variant serves a mixed batch.
 template <typename T, int TJ, bool LOSSY>
-__global__ __launch_bounds__(64 * (TJ + 1)) void k_batch_step(BatchArgs<T> a) {
-  const int par = (*a.step - a.step0) & 1;
+__global__ __launch_bounds__(64 * (TJ + 1)) void k_batch_step(BatchArgs<T> a, int q) {
+  const int par = (*a.step + q - a.step0) & 1;
   for (int n = blockIdx.x; n < a.n_tiles; n += gridDim.x) {
     const int t = xcd_remap(n, a.n_tiles, a.xcd);
     const int s = a.tile_sim[t];
@@ -115,8 +114,8 @@ __global__ __launch_bounds__(64 * (TJ + 1)) void k_batch_step(BatchArgs<T> a) {
 // After the batched sweep: one block per engine redoes E on the cells of its
 // source box (source_fixup_cell), then records its probe cell.
 template <typename T>
-__global__ __launch_bounds__(256) void k_batch_fixup(BatchArgs<T> a) {
-  const int step = *a.step;
+__global__ __launch_bounds__(256) void k_batch_fixup(BatchArgs<T> a, int q) {
+  const int step = *a.step + q;
   const int par = (step - a.step0) & 1;
   const int s = blockIdx.x;
   const SimDesc<T> &d = a.sims[s];
@@ -178,74 +177,92 @@ __global__ __launch_bounds__(kFlatBlock) void k_batch_split_H(BatchArgs<T> a) {
   }
 }
 
+// The engine's source of the step (added to E before the cell's update, as
+// the single-engine split path adds it between its sweeps: sources.cl:7-49
+// via src_term, with the cell's ce) and its probe (value x phase added to the
+// probe's FP64 DFT after the update, by the one thread that owns the cell).
+template <typename T>
+struct BatchEHook {
+  bool on;   // the source applies to this item
+  Src<T> s;
+  T amp;
+  int pi, pj, pk, comp, nf;  // pi < 0: no probe
+  const double *ph;
+  double *acc;
+  __device__ __forceinline__ void pre(int i, int j, int k, T cx, T cy, T cz, T &ex, T &ey,
+                                      T &ez) const {
+    if (!on) return;
+    if (i < s.ci0 || i >= s.ci1 || j < s.cj0 || j >= s.cj1 || k < s.ck0 || k >= s.ck1) return;
+    const T v = src_term<T>(s, i, j, k, s.comp == 0 ? cx : s.comp == 1 ? cy : cz, amp);
+    if (s.comp == 0) ex += v;
+    else if (s.comp == 1) ey += v;
+    else ez += v;
+  }
+  __device__ __forceinline__ void post(int i, int j, int k, T ex, T ey, T ez) const {
+    if (i != pi || j != pj || k != pk) return;
+    const double v = (double)(comp == 0 ? ex : comp == 1 ? ey : ez);
+    for (int f = 0; f < nf; ++f) {
+      acc[2 * f] += v * ph[2 * f];
+      acc[2 * f + 1] += v * ph[2 * f + 1];
+    }
+  }
+};
+
+// E half of the batched split step, with each engine's source and probe.
+// Work items whose plane holds neither run the plain update; the others
+// carry the hook (copied once per item).  FP32 keeps eight waves per SIMD
+// (64 VGPRs), the plain split E sweep's occupancy.
 template <typename T, bool LOSSY>
-__global__ __launch_bounds__(kFlatBlock) void k_batch_split_E(BatchArgs<T> a) {
+__global__ __launch_bounds__(kFlatBlock, sizeof(T) == 4 ? 8 : 1) void k_batch_split_E(BatchArgs<T> a,
+                                                                                    int qs) {
+  const int step = *a.step + qs;
   for (int m = blockIdx.x; m < a.n_tiles; m += gridDim.x) {
     const int n = gridDim.x == a.n_tiles ? xcd_runs(m, a.n_tiles, a.xcd, a.run) : m;
-    const SimDesc<T> &d = a.sims[a.tile_sim[n]];
+    const int sim = __builtin_amdgcn_readfirstlane(a.tile_sim[n]);
+    const SimDesc<T> &d = a.sims[sim];
     const Geo g = d.g;
     const int lt = n - d.tile_begin;
     const int i = lt / d.tiles_k;
     const int q = (lt - i * d.tiles_k) * kFlatBlock + threadIdx.x;
+    const bool src_here = a.amp && d.has_src && i >= d.src.ci0 && i < d.src.ci1;
+    const bool prb_here = a.probe_acc && d.has_probe && i == d.pi;
     if (q >= g.ny * g.nz) continue;
-    const int j = q / g.nz;
+    const int j = q / g.nz, k = q - j * g.nz;
+    if (src_here || prb_here) {
+      const size_t row = (size_t)sim * a.n_table + step;
+      BatchEHook<T> hk;
+      hk.on = src_here;
+      hk.s = d.src;
+      hk.amp = src_here ? T(a.amp[row]) : T(0);
+      hk.pi = prb_here ? d.pi : -1;
+      hk.pj = d.pj;
+      hk.pk = d.pk;
+      hk.comp = a.comp;
+      hk.nf = a.nfreq;
+      hk.ph = a.phase + row * a.nfreq * 2;
+      hk.acc = a.probe_acc + (size_t)sim * a.nfreq * 2;
+      if constexpr (sizeof(T) == 4) {
+        if (d.mat) {  // lossless (ca = 1), whichever variant the batch runs
+          split_e_cell_mat(g, a.inv_dl, d.st[0], d.coef, d.mat, d.mtab, d.nmat, d.prof, d.psi[0],
+                           i, j, k, hk);
+          continue;
+        }
+      }
+      split_e_cell<T, LOSSY>(g, a.inv_dl, d.st[0], d.coef, d.prof, d.psi[0], i, j, k, hk);
+      continue;
+    }
     if constexpr (sizeof(T) == 4) {
-      if (d.mat) {  // lossless (ca = 1), whichever variant the batch runs
+      if (d.mat) {
         split_e_cell_mat(g, a.inv_dl, d.st[0], d.coef, d.mat, d.mtab, d.nmat, d.prof, d.psi[0], i,
-                         j, q - j * g.nz);
+                         j, k);
         continue;
       }
     }
-    split_e_cell<T, LOSSY>(g, a.inv_dl, d.st[0], d.coef, d.prof, d.psi[0], i, j, q - j * g.nz);
+    split_e_cell<T, LOSSY>(g, a.inv_dl, d.st[0], d.coef, d.prof, d.psi[0], i, j, k);
   }
 }
 
-// Between the halves: one block per engine adds its source to E
-// (sources.cl:7-49 via src_term, as the single-engine split path does).
-template <typename T>
-__global__ __launch_bounds__(256) void k_batch_split_src(BatchArgs<T> a) {
-  const SimDesc<T> &d = a.sims[blockIdx.x];
-  if (!d.has_src) return;
-  const Geo g = d.g;
-  const Src<T> src = d.src;
-  const T amp = T(a.amp[(size_t)blockIdx.x * a.n_table + *a.step]);
-  const int ni = src.ci1 - src.ci0, nj = src.cj1 - src.cj0, nk = src.ck1 - src.ck0;
-  const long long total = (long long)ni * nj * nk;
-  T *F = d.st[0] + src.comp * g.vs;
-  const T *ce = d.coef + src.comp * g.vs;
-  for (long long t = threadIdx.x; t < total; t += blockDim.x) {
-    const int k = src.ck0 + (int)(t % nk);
-    const int j = src.cj0 + (int)((t / nk) % nj);
-    const int i = src.ci0 + (int)(t / ((long long)nk * nj));
-    const size_t idx = ((size_t)i * g.ny + j) * g.nz + k;
-    F[idx] += src_term<T>(src, i, j, k, ce[idx], amp);
-  }
-}
-
-// After the E half: one thread per engine records its probe.
-template <typename T>
-__global__ void k_batch_split_probe(BatchArgs<T> a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n_sims) return;
-  const SimDesc<T> &d = a.sims[s];
-  if (!d.has_probe) return;
-  const Geo g = d.g;
-  const size_t row = (size_t)s * a.n_table + *a.step;
-  const size_t idx = ((size_t)d.pi * g.ny + d.pj) * g.nz + d.pk;
-  const double v = (double)d.st[0][a.comp * g.vs + idx];
-  const double *ph = a.phase + row * a.nfreq * 2;
-  double *acc = a.probe_acc + (size_t)s * a.nfreq * 2;
-  for (int f = 0; f < a.nfreq; ++f) {
-    acc[2 * f] += v * ph[2 * f];
-    acc[2 * f + 1] += v * ph[2 * f + 1];
-  }
-}
-
-__global__ void k_step_advance(int *step) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(step, 1);
-}
-
-// After a run of n batched march launches (which read *step + q).
+// After a run of n batched launches (which read *step + q).
 __global__ void k_step_add(int *step, int n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(step, n);
 }
@@ -507,46 +524,6 @@ static int batch_launch_march(const hfdtd_batch *b, const BMArgs<T> &a, int q) {
 }
 
 template <typename T>
-static int batch_run_march(hfdtd_batch *b, int n_steps, int graph_steps) {
-  int rc = batch_build_march<T>(b);
-  if (rc) return rc;
-  const BMArgs<T> a = batch_march_args<T>(b);
-  int done = 0;
-  if (graph_steps > 0 && n_steps >= graph_steps) {
-    hipGraph_t graph;
-    HF_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int q = 0; q < graph_steps && rc == HFDTD_OK; ++q) rc = batch_launch_march<T>(b, a, q);
-    if (rc == HFDTD_OK) {
-      hipLaunchKernelGGL(k_step_add, dim3(1), dim3(64), 0, b->stream, b->d_step, graph_steps);
-      if (hipGetLastError() != hipSuccess) rc = fail(HFDTD_EHIP, "k_step_add launch");
-    }
-    if (rc) {
-      hipGraph_t junk;
-      (void)hipStreamEndCapture(b->stream, &junk);
-      if (junk) (void)hipGraphDestroy(junk);
-      return rc;
-    }
-    HF_HIP(hipStreamEndCapture(b->stream, &graph));
-    hipError_t err = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (err != hipSuccess) return hip_fail(err, "hipGraphInstantiate");
-    for (; done + graph_steps <= n_steps; done += graph_steps)
-      HF_HIP(hipGraphLaunch(b->gexec, b->stream));
-  }
-  if (done < n_steps) {
-    for (int q = 0; q < n_steps - done; ++q)
-      if ((rc = batch_launch_march<T>(b, a, q))) return rc;
-    hipLaunchKernelGGL(k_step_add, dim3(1), dim3(64), 0, b->stream, b->d_step, n_steps - done);
-    HF_HIP(hipGetLastError());
-  }
-  HF_HIP(hipStreamSynchronize(b->stream));
-  b->step_host += n_steps;
-  if (n_steps & 1)
-    for (Engine *e : b->eng) e->cur ^= 1;
-  return HFDTD_OK;
-}
-
-template <typename T>
 static BatchArgs<T> batch_args(const hfdtd_batch *b) {
   const Engine *e = b->eng[0];
   BatchArgs<T> a;
@@ -571,39 +548,34 @@ static BatchArgs<T> batch_args(const hfdtd_batch *b) {
   return a;
 }
 
+// Step q of a run on the split path: the H sweep, then the E sweep with the
+// engines' sources and probes.
 template <typename T>
-static int batch_launch_split(hfdtd_batch *b, const BatchArgs<T> &a) {
+static int batch_launch_split(hfdtd_batch *b, const BatchArgs<T> &a, int q) {
   const int grid = b->blocks > 0 ? std::min(b->blocks, b->n_tiles) : b->n_tiles;
-  const int n = (int)b->eng.size();
   hipLaunchKernelGGL(k_batch_split_H<T>, dim3(grid), dim3(kFlatBlock), 0, b->stream, a);
   HF_HIP(hipGetLastError());
-  if (a.amp) {
-    hipLaunchKernelGGL(k_batch_split_src<T>, dim3(n), dim3(256), 0, b->stream, a);
-    HF_HIP(hipGetLastError());
-  }
   if (b->lossy)
-    hipLaunchKernelGGL((k_batch_split_E<T, true>), dim3(grid), dim3(kFlatBlock), 0, b->stream, a);
+    hipLaunchKernelGGL((k_batch_split_E<T, true>), dim3(grid), dim3(kFlatBlock), 0, b->stream, a,
+                       q);
   else
-    hipLaunchKernelGGL((k_batch_split_E<T, false>), dim3(grid), dim3(kFlatBlock), 0, b->stream, a);
-  HF_HIP(hipGetLastError());
-  if (a.probe_acc) {
-    hipLaunchKernelGGL(k_batch_split_probe<T>, dim3(ceil_div(n, 64)), dim3(64), 0, b->stream, a);
-    HF_HIP(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, b->stream, b->d_step);
+    hipLaunchKernelGGL((k_batch_split_E<T, false>), dim3(grid), dim3(kFlatBlock), 0, b->stream, a,
+                       q);
   HF_HIP(hipGetLastError());
   return HFDTD_OK;
 }
 
+// Step q of a run on the fused path: the fused tiles, then one block per
+// engine for its source and probe.
 template <typename T>
-static int batch_launch_step(hfdtd_batch *b, const BatchArgs<T> &a) {
-  if (b->path == HFDTD_PATH_SPLIT) return batch_launch_split<T>(b, a);
+static int batch_launch_step(hfdtd_batch *b, const BatchArgs<T> &a, int q) {
+  if (b->path == HFDTD_PATH_SPLIT) return batch_launch_split<T>(b, a, q);
   const int grid = b->blocks > 0 ? std::min(b->blocks, b->n_tiles) : b->n_tiles;
 #define HF_BATCH(TJ)                                                                          \
   if (b->lossy)                                                                               \
-    hipLaunchKernelGGL((k_batch_step<T, TJ, true>), dim3(grid), dim3(64, TJ + 1), 0, b->stream, a); \
+    hipLaunchKernelGGL((k_batch_step<T, TJ, true>), dim3(grid), dim3(64, TJ + 1), 0, b->stream, a, q); \
   else                                                                                        \
-    hipLaunchKernelGGL((k_batch_step<T, TJ, false>), dim3(grid), dim3(64, TJ + 1), 0, b->stream, a)
+    hipLaunchKernelGGL((k_batch_step<T, TJ, false>), dim3(grid), dim3(64, TJ + 1), 0, b->stream, a, q)
   if constexpr (sizeof(T) == 8) {  // FP64: tiles of at most 4 rows fit the LDS
     if (b->rows == 2) HF_BATCH(2);
     else HF_BATCH(4);
@@ -619,32 +591,31 @@ static int batch_launch_step(hfdtd_batch *b, const BatchArgs<T> &a) {
   HF_HIP(hipGetLastError());
   if (a.amp || a.probe_acc) {
     hipLaunchKernelGGL(k_batch_fixup<T>, dim3((unsigned)b->eng.size()), dim3(256), 0, b->stream,
-                       a);
+                       a, q);
     HF_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, b->stream, b->d_step);
-  HF_HIP(hipGetLastError());
   return HFDTD_OK;
 }
 
-template <typename T>
-static int batch_run_t(hfdtd_batch *b, int n_steps, int graph_steps) {
-  if (b->path == HFDTD_PATH_MARCH) return batch_run_march<T>(b, n_steps, graph_steps);
-  int rc = batch_build<T>(b);
-  if (rc) return rc;
-  const BatchArgs<T> a = batch_args<T>(b);
-  int done = 0;
+// n_steps launches of launch(q), q counting within each graph or direct run,
+// with *step advanced after each run; graph_steps > 0 captures that many
+// steps once and replays them (remainder launched directly).
+template <typename F>
+static int batch_run_steps(hfdtd_batch *b, int n_steps, int graph_steps, F launch) {
+  int done = 0, rc = HFDTD_OK;
   if (graph_steps > 0 && n_steps >= graph_steps) {
     hipGraph_t graph;
     HF_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int q = 0; q < graph_steps; ++q) {
-      rc = batch_launch_step<T>(b, a);
-      if (rc) {
-        hipGraph_t junk;
-        (void)hipStreamEndCapture(b->stream, &junk);
-        if (junk) (void)hipGraphDestroy(junk);
-        return rc;
-      }
+    for (int q = 0; q < graph_steps && rc == HFDTD_OK; ++q) rc = launch(q);
+    if (rc == HFDTD_OK) {
+      hipLaunchKernelGGL(k_step_add, dim3(1), dim3(64), 0, b->stream, b->d_step, graph_steps);
+      if (hipGetLastError() != hipSuccess) rc = fail(HFDTD_EHIP, "k_step_add launch");
+    }
+    if (rc) {
+      hipGraph_t junk;
+      (void)hipStreamEndCapture(b->stream, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      return rc;
     }
     HF_HIP(hipStreamEndCapture(b->stream, &graph));
     hipError_t err = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
@@ -653,11 +624,33 @@ static int batch_run_t(hfdtd_batch *b, int n_steps, int graph_steps) {
     for (; done + graph_steps <= n_steps; done += graph_steps)
       HF_HIP(hipGraphLaunch(b->gexec, b->stream));
   }
-  for (; done < n_steps; ++done)
-    if ((rc = batch_launch_step<T>(b, a))) return rc;
+  if (done < n_steps) {
+    for (int q = 0; q < n_steps - done; ++q)
+      if ((rc = launch(q))) return rc;
+    hipLaunchKernelGGL(k_step_add, dim3(1), dim3(64), 0, b->stream, b->d_step, n_steps - done);
+    HF_HIP(hipGetLastError());
+  }
   HF_HIP(hipStreamSynchronize(b->stream));
+  return HFDTD_OK;
+}
+
+template <typename T>
+static int batch_run_t(hfdtd_batch *b, int n_steps, int graph_steps) {
+  int rc;
+  if (b->path == HFDTD_PATH_MARCH) {
+    if ((rc = batch_build_march<T>(b))) return rc;
+    const BMArgs<T> a = batch_march_args<T>(b);
+    rc = batch_run_steps(b, n_steps, graph_steps,
+                         [&](int q) { return batch_launch_march<T>(b, a, q); });
+  } else {
+    if ((rc = batch_build<T>(b))) return rc;
+    const BatchArgs<T> a = batch_args<T>(b);
+    rc = batch_run_steps(b, n_steps, graph_steps,
+                         [&](int q) { return batch_launch_step<T>(b, a, q); });
+  }
+  if (rc) return rc;
   b->step_host += n_steps;
-  if (b->path == HFDTD_PATH_FUSED && (n_steps & 1))
+  if (b->path != HFDTD_PATH_SPLIT && (n_steps & 1))
     for (Engine *e : b->eng) e->cur ^= 1;
   return HFDTD_OK;
 }

@@ -90,8 +90,12 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
     pr.ph = a.phase + row * a.nfreq * 2;
     pr.acc = a.probe_acc + (size_t)s * a.nfreq * 2;
   }
+  // the source count as the engine's flag (0 or 1, an int the compiler
+  // cannot bound), so the sweep keeps its source loop a loop: with a count
+  // known to be at most one the compiler peels it into the plane body,
+  // where it raises register pressure past the budget (VGPR spills)
   march_tile<T, RW, LOSSY, true, CEM, false, 2, W>(
-      L, lsrc, src_on ? 1 : 0, si0, si1, ltab, g, k0, j0, i0, i1, a.dtm_dl, a.dtm, a.inv_dl,
+      L, lsrc, a.amp ? d.has_src : 0, si0, si1, ltab, g, k0, j0, i0, i1, a.dtm_dl, a.dtm, a.inv_dl,
       d.st[par], d.st[par ^ 1], d.coef, d.prof, d.psi[par], d.psi[par ^ 1], d.mat, d.nmat, 0.0,
       ZCap{}, prb, pr);
 }

@@ -270,11 +270,22 @@ __device__ __forceinline__ void split_h_cell(const Geo &g, T dtm_dl, T dtm, T *_
 
 // E(i,j,k) and its psi from H (reference yee_update.cl:120-238), in place,
 // with the cell's edge coefficients cx, cy, cz given.
-template <typename T, bool LOSSY>
+// Per-cell hooks of the split E update: pre() may change E at the cell
+// before its update (after it is loaded, with the cell's ce), post() sees
+// the updated E.  The plain sweeps use none.
+struct NoCellHook {
+  template <typename T>
+  __device__ __forceinline__ void pre(int, int, int, T, T, T, T &, T &, T &) const {}
+  template <typename T>
+  __device__ __forceinline__ void post(int, int, int, T, T, T) const {}
+};
+
+template <typename T, bool LOSSY, typename Hook = NoCellHook>
 __device__ __forceinline__ void split_e_core(const Geo &g, T inv_dl, T *__restrict__ st,
                                              const T *__restrict__ coef,
                                              const T *__restrict__ prof, T *__restrict__ psi,
-                                             int i, int j, int k, size_t idx, T cx, T cy, T cz) {
+                                             int i, int j, int k, size_t idx, T cx, T cy, T cz,
+                                             const Hook &hook = Hook()) {
   T *Ex = st, *Ey = st + g.vs, *Ez = st + 2 * g.vs;
   const T *Hx = st + 3 * g.vs, *Hy = st + 4 * g.vs, *Hz = st + 5 * g.vs;
   const size_t sxy = (size_t)g.ny * g.nz;
@@ -282,6 +293,7 @@ __device__ __forceinline__ void split_e_core(const Geo &g, T inv_dl, T *__restri
   T p[6] = {0, 0, 0, 0, 0, 0};
   psi_load(g, psi, 6, s, p);
   T ex = Ex[idx], ey = Ey[idx], ez = Ez[idx];
+  hook.pre(i, j, k, cx, cy, cz, ex, ey, ez);
   const T ax = LOSSY ? coef[3 * g.vs + idx] : T(1);
   const T ay = LOSSY ? coef[4 * g.vs + idx] : T(1);
   const T az = LOSSY ? coef[5 * g.vs + idx] : T(1);
@@ -293,17 +305,18 @@ __device__ __forceinline__ void split_e_core(const Geo &g, T inv_dl, T *__restri
   Ey[idx] = ey;
   Ez[idx] = ez;
   psi_store(g, psi, 6, s, p);
+  hook.post(i, j, k, ex, ey, ez);
 }
 
 // E(i,j,k) with ce from the coefficient volumes.
-template <typename T, bool LOSSY>
+template <typename T, bool LOSSY, typename Hook = NoCellHook>
 __device__ __forceinline__ void split_e_cell(const Geo &g, T inv_dl, T *__restrict__ st,
                                              const T *__restrict__ coef,
                                              const T *__restrict__ prof, T *__restrict__ psi,
-                                             int i, int j, int k) {
+                                             int i, int j, int k, const Hook &hook = Hook()) {
   const size_t idx = ((size_t)i * g.ny + j) * g.nz + k;
-  split_e_core<T, LOSSY>(g, inv_dl, st, coef, prof, psi, i, j, k, idx, coef[idx],
-                         coef[g.vs + idx], coef[2 * g.vs + idx]);
+  split_e_core<T, LOSSY, Hook>(g, inv_dl, st, coef, prof, psi, i, j, k, idx, coef[idx],
+                               coef[g.vs + idx], coef[2 * g.vs + idx], hook);
 }
 
 // E(i,j,k) on a lossless FP32 grid with a material index: one byte of
@@ -312,20 +325,22 @@ __device__ __forceinline__ void split_e_cell(const Geo &g, T inv_dl, T *__restri
 // the grid's last plane, row or column the cell itself, as the edge average
 // keeps the cell value there).  The table holds the values the ce volumes
 // hold, so results are the same.
+template <typename Hook = NoCellHook>
 __device__ __forceinline__ void split_e_cell_mat(const Geo &g, float inv_dl,
                                                  float *__restrict__ st,
                                                  const float *__restrict__ coef,
                                                  const unsigned char *__restrict__ mat,
                                                  const float *__restrict__ mtab, int nmat,
                                                  const float *__restrict__ prof,
-                                                 float *__restrict__ psi, int i, int j, int k) {
+                                                 float *__restrict__ psi, int i, int j, int k,
+                                                 const Hook &hook = Hook()) {
   const size_t sxy = (size_t)g.ny * g.nz;
   const size_t idx = (size_t)i * sxy + (size_t)j * g.nz + k;
   const float *tb = mtab + (unsigned)mat[idx] * (unsigned)nmat;
   const float cx = tb[mat[i + 1 < g.nx ? idx + sxy : idx]];
   const float cy = tb[mat[j + 1 < g.ny ? idx + g.nz : idx]];
   const float cz = tb[mat[k + 1 < g.nz ? idx + 1 : idx]];
-  split_e_core<float, false>(g, inv_dl, st, coef, prof, psi, i, j, k, idx, cx, cy, cz);
+  split_e_core<float, false, Hook>(g, inv_dl, st, coef, prof, psi, i, j, k, idx, cx, cy, cz, hook);
 }
 
 template <typename T>
