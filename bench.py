@@ -173,9 +173,10 @@ def run_gpu(args, wl: Workload):
     mon.flush()  # the window starts with an empty K-step buffer
     sim.synchronize()
     # HIP events on a sample of the window's sweep launches (every 15th: an
-    # odd stride times the split path's H and E sweeps alike); each event
-    # record costs the stream a few microseconds, so timing every launch
-    # would slow the window it measures
+    # odd stride times the split path's H and E sweeps alike), recorded by
+    # each sampled launch around its kernel; a timed launch costs the stream
+    # a few microseconds, so timing every launch would slow the window it
+    # measures
     sim.kernel_timing(True, every=TIMING_EVERY)
     t0 = time.perf_counter()
     sim.run(args.steps)

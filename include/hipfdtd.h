@@ -216,11 +216,12 @@ int hfdtd_set_pair_planes(hfdtd_engine *e, int planes_i);
 
 /* HIP-event timing of the Yee sweep kernels on the engine's stream: while
  * enabled, every n-th update_H / update_E / fused / march / pair sweep launch
- * (n = enable: 1 times every launch, 0 turns timing off) is bracketed by a
- * pair of events.  Each event record costs the stream a few microseconds,
- * so a sample (an odd n, which times the H and E sweeps of the split path
- * alike) keeps the timed run close to an untimed one.  Enabling clears
- * earlier records.  Extension, no counterpart in the reference. */
+ * (n = enable: 1 times every launch, 0 turns timing off) records a start and
+ * a stop event around its kernel (hipExtLaunchKernelGGL).  A timed launch
+ * costs the stream a few microseconds, so a sample (an odd n, which times
+ * the H and E sweeps of the split path alike) keeps the timed run close to
+ * an untimed one.  Enabling clears earlier records.  Extension, no
+ * counterpart in the reference. */
 int hfdtd_timing(hfdtd_engine *e, int enable);
 /* synchronizes the stream; total milliseconds and number of the timed sweep
  * launches */

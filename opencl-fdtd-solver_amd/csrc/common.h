@@ -1,6 +1,7 @@
 // Internal definitions shared by the libhipfdtd translation units.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -151,14 +152,11 @@ struct Engine {
   }
 
   // HIP-event timing of the Yee sweeps (hfdtd_timing / hfdtd_timing_read):
-  // while enabled, events are recorded on `stream` around every
-  // timing_every-th sweep launch.  timing_mark is called before and after
-  // each launch; timing_after says which of the two comes next, timing_take
-  // whether the launch in between is timed.
+  // while enabled, every timing_every-th sweep launch records a start and a
+  // stop event around its kernel (timing_events hands them out).
   bool timing = false;
   int timing_every = 1;
   long long timing_launches = 0;  // sweep launches since timing was enabled
-  bool timing_after = false, timing_take = false;
   std::vector<hipEvent_t> events;  // pool; [2q, 2q+1] bracket timed launch q
   size_t events_used = 0;
 
@@ -234,8 +232,10 @@ struct Engine {
   }
 };
 
-// Record the next timing event on the engine stream (no-op when disabled).
-int timing_mark(Engine *e);
+// The start / stop events of the next sweep launch when timing samples it
+// (both null otherwise); the launch itself records them
+// (hipExtLaunchKernelGGL), around the kernel's execution.
+int timing_events(Engine *e, hipEvent_t &start, hipEvent_t &stop);
 
 // Launch the split H / E sweeps and the source kernel (yee.hip).
 int launch_update_H(Engine *e);

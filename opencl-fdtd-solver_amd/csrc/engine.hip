@@ -190,19 +190,17 @@ static void release(Engine *e) {
   e->stream = nullptr;
 }
 
-int timing_mark(Engine *e) {
+int timing_events(Engine *e, hipEvent_t &start, hipEvent_t &stop) {
+  start = stop = nullptr;
   if (!e->timing) return HFDTD_OK;
-  if (!e->timing_after) {  // before a launch: is it one of the sample?
-    e->timing_take = e->timing_launches++ % e->timing_every == 0;
-  }
-  e->timing_after = !e->timing_after;
-  if (!e->timing_take) return HFDTD_OK;
-  if (e->events_used == e->events.size()) {
+  if (e->timing_launches++ % e->timing_every != 0) return HFDTD_OK;
+  while (e->events.size() < e->events_used + 2) {
     hipEvent_t ev;
     HF_HIP(hipEventCreate(&ev));
     e->events.push_back(ev);
   }
-  HF_HIP(hipEventRecord(e->events[e->events_used++], e->stream));
+  start = e->events[e->events_used++];
+  stop = e->events[e->events_used++];
   return HFDTD_OK;
 }
 
@@ -737,13 +735,11 @@ int hfdtd_timing(hfdtd_engine *h, int enable) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
   if (enable < 0) return fail(HFDTD_EINVAL, "enable must be 0 (off) or n >= 1 (every n-th launch)");
   Engine *e = &h->s;
-  if (enable && e->events_used % 2) return fail(HFDTD_EINVAL, "unbalanced timing events");
   e->timing = enable != 0;
   if (enable) {
     e->events_used = 0;
     e->timing_every = enable;
     e->timing_launches = 0;
-    e->timing_after = e->timing_take = false;
   }
   return HFDTD_OK;
 }

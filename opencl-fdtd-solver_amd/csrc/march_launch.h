@@ -58,21 +58,20 @@ int march_rw(Engine *e, const SrcList<T> &sl) {
     zc.j1 = z.j1;
     zc.bytes = (unsigned)z.bytes;
   }
-  int rc = timing_mark(e);
+  hipEvent_t t0, t1;
+  int rc = timing_events(e, t0, t1);
   if (rc) return rc;
   const int o = e->cur ^ 1;
-  hipLaunchKernelGGL(kern, grid, dim3(kMarchThreads), 0, e->stream, geo_of(e), planes,
-                     T(e->dtm_dl), T(e->dtm), T(e->inv_dl), (const T *)e->state[e->cur],
-                     (T *)e->state[o], (const T *)e->coef, (const T *)e->prof,
-                     (const T *)e->psi[e->cur], (T *)e->psi[o], tiles_k, tiles_j, ch, e->xcd,
-                     e->march_run, ms,
-                     CEM == kCeIndex ? (const void *)e->mat
-                                     : CEM == kCeEps ? (const void *)e->epsv : nullptr,
-                     CEM == kCeIndex ? (const float *)e->mtab : nullptr,
-                     CEM == kCeIndex ? e->nmat : 0, e->dt / kEPS0, zc);
+  hipExtLaunchKernelGGL(kern, grid, dim3(kMarchThreads), 0, e->stream, t0, t1, 0, geo_of(e),
+                        planes, T(e->dtm_dl), T(e->dtm), T(e->inv_dl),
+                        (const T *)e->state[e->cur], (T *)e->state[o], (const T *)e->coef,
+                        (const T *)e->prof, (const T *)e->psi[e->cur], (T *)e->psi[o], tiles_k,
+                        tiles_j, ch, e->xcd, e->march_run, ms,
+                        CEM == kCeIndex ? (const void *)e->mat
+                                        : CEM == kCeEps ? (const void *)e->epsv : nullptr,
+                        CEM == kCeIndex ? (const float *)e->mtab : nullptr,
+                        CEM == kCeIndex ? e->nmat : 0, e->dt / kEPS0, zc);
   HF_HIP(hipGetLastError());
-  rc = timing_mark(e);
-  if (rc) return rc;
   // the record holds this step's values unless the fix-up kernel below
   // changes E on source cells afterwards
   if (record && inside) e->zrec.epoch = e->state_epoch;

@@ -20,16 +20,18 @@ static int pair_launch(Engine *e, const PairSources<T> &ps) {
     for (int c = 0; c < 6; ++c) pw.b[b][c] = e->watch_b[b][c];
     pw.c[b] = e->watch_c[b];
   }
-  int rc = timing_mark(e);
+  hipEvent_t t0, t1;
+  int rc = timing_events(e, t0, t1);
   if (rc) return rc;
   const int o = e->cur ^ 1;
-  hipLaunchKernelGGL((k_pair<T, NW, RW, LOSSY>), grid, dim3(NW * 64), 0, e->stream, geo_of(e),
-                     planes, T(e->dtm_dl), T(e->dtm), T(e->inv_dl), (const T *)e->state[e->cur],
-                     (T *)e->state[o], (const T *)e->coef, (const T *)e->prof,
-                     (const T *)e->psi[e->cur], (T *)e->psi[o], pw.n ? (T *)e->shadow : nullptr,
-                     tiles_k, tiles_j, e->xcd, e->march_run, ps, pw);
+  hipExtLaunchKernelGGL((k_pair<T, NW, RW, LOSSY>), grid, dim3(NW * 64), 0, e->stream, t0, t1, 0,
+                        geo_of(e), planes, T(e->dtm_dl), T(e->dtm), T(e->inv_dl),
+                        (const T *)e->state[e->cur], (T *)e->state[o], (const T *)e->coef,
+                        (const T *)e->prof, (const T *)e->psi[e->cur], (T *)e->psi[o],
+                        pw.n ? (T *)e->shadow : nullptr, tiles_k, tiles_j, e->xcd, e->march_run,
+                        ps, pw);
   HF_HIP(hipGetLastError());
-  return timing_mark(e);
+  return HFDTD_OK;
 }
 
 // 16 wavefronts of one row each: every variant fits 128 VGPRs, so a CU

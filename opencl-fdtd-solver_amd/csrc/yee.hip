@@ -125,13 +125,15 @@ __global__ __launch_bounds__(256) void k_source_fixup(Geo g, T inv_dl, const T *
 template <typename T>
 static int update_H_t(Engine *e) {
   const unsigned bpp = ceil_div((size_t)e->ny * e->nz, kFlatBlock);
-  int rc = timing_mark(e);
+  hipEvent_t t0, t1;
+  int rc = timing_events(e, t0, t1);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_update_H<T>, dim3(bpp * (unsigned)e->nx), dim3(kFlatBlock), 0, e->stream,
-                     geo_of(e), T(e->dtm_dl), T(e->dtm), (T *)e->state[e->cur],
-                     (const T *)e->prof, (T *)e->psi[e->cur], (int)bpp, e->xcd, e->split_run);
+  hipExtLaunchKernelGGL(k_update_H<T>, dim3(bpp * (unsigned)e->nx), dim3(kFlatBlock), 0,
+                        e->stream, t0, t1, 0, geo_of(e), T(e->dtm_dl), T(e->dtm),
+                        (T *)e->state[e->cur], (const T *)e->prof, (T *)e->psi[e->cur], (int)bpp,
+                        e->xcd, e->split_run);
   HF_HIP(hipGetLastError());
-  return timing_mark(e);
+  return HFDTD_OK;
 }
 
 template <typename T>
@@ -141,15 +143,16 @@ static int update_E_t(Engine *e) {
   const bool mat = sizeof(T) == 4 && e->lossless && e->mat_ok;
   if constexpr (sizeof(T) == 4)
     if (mat) kern = k_update_E<float, false, true>;
-  int rc = timing_mark(e);
+  hipEvent_t t0, t1;
+  int rc = timing_events(e, t0, t1);
   if (rc) return rc;
-  hipLaunchKernelGGL(kern, dim3(bpp * (unsigned)e->nx), dim3(kFlatBlock), 0, e->stream,
-                     geo_of(e), T(e->inv_dl), (T *)e->state[e->cur], (const T *)e->coef,
-                     (const T *)e->prof, (T *)e->psi[e->cur], (int)bpp, e->xcd, e->split_run,
-                     mat ? (const unsigned char *)e->mat : nullptr,
-                     mat ? (const float *)e->mtab : nullptr, mat ? e->nmat : 0);
+  hipExtLaunchKernelGGL(kern, dim3(bpp * (unsigned)e->nx), dim3(kFlatBlock), 0, e->stream, t0, t1,
+                        0, geo_of(e), T(e->inv_dl), (T *)e->state[e->cur], (const T *)e->coef,
+                        (const T *)e->prof, (T *)e->psi[e->cur], (int)bpp, e->xcd, e->split_run,
+                        mat ? (const unsigned char *)e->mat : nullptr,
+                        mat ? (const float *)e->mtab : nullptr, mat ? e->nmat : 0);
   HF_HIP(hipGetLastError());
-  return timing_mark(e);
+  return HFDTD_OK;
 }
 
 template <typename T>
@@ -204,16 +207,15 @@ static int fused_tj(Engine *e, const SrcList<T> &sl) {
   const int planes = e->fused_planes(TJ, occ);
   const int tiles_k = (int)ceil_div(e->nz, 64), tiles_j = (int)ceil_div(e->ny, TJ);
   const dim3 grid(tiles_k * tiles_j * ceil_div(e->nx, planes));
-  int rc = timing_mark(e);
+  hipEvent_t t0, t1;
+  int rc = timing_events(e, t0, t1);
   if (rc) return rc;
   const int o = e->cur ^ 1;
-  hipLaunchKernelGGL(kern, grid, dim3(64, TJ + 1), 0, e->stream, geo_of(e), planes, T(e->dtm_dl),
-                     T(e->dtm), T(e->inv_dl), (const T *)e->state[e->cur], (T *)e->state[o],
-                     (const T *)e->coef, (const T *)e->prof, (const T *)e->psi[e->cur],
-                     (T *)e->psi[o], tiles_k, tiles_j, e->xcd);
+  hipExtLaunchKernelGGL(kern, grid, dim3(64, TJ + 1), 0, e->stream, t0, t1, 0, geo_of(e), planes,
+                        T(e->dtm_dl), T(e->dtm), T(e->inv_dl), (const T *)e->state[e->cur],
+                        (T *)e->state[o], (const T *)e->coef, (const T *)e->prof,
+                        (const T *)e->psi[e->cur], (T *)e->psi[o], tiles_k, tiles_j, e->xcd);
   HF_HIP(hipGetLastError());
-  rc = timing_mark(e);
-  if (rc) return rc;
   return source_fixup<T, LOSSY>(e, sl);
 }
 
