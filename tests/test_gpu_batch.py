@@ -212,6 +212,64 @@ def test_batch_march_matches_serial_march_engines(medium, graph_steps, rows, pla
         sim.close()
 
 
+@pytest.mark.parametrize("fp64,lossy,graph_steps", [(True, False, 0), (True, True, 10),
+                                                    (False, True, 10)])  # fmt: skip
+def test_batch_split_fp64_and_lossy_match_serial(fp64, lossy, graph_steps):
+    """The two-launch split batch (sources and probes inside the E sweep) on
+    FP64 engines and with a lossy grid (the kernel with ca): fields and probe
+    DFTs bit for bit against serial split engines."""
+    rng = np.random.default_rng(21)
+    shapes = [tuple(int(v) for v in rng.integers(18, 60, size=3)) for _ in range(4)]
+    media = [sphere_case(rng, s) for s in shapes]
+    dtype = np.float64 if fp64 else np.float32
+    n_steps, nf = 37, 3
+    freqs = [list(np.linspace(0.8, 1.2, nf) * f) for _, f in media]
+
+    def box(shape):
+        c = [v // 2 for v in shape]
+        return (c[0] - 1, c[0] + 1, c[1], c[1] + 2, c[2], c[2] + 1)
+
+    def cell(shape):
+        return (shape[0] // 2 - 2, shape[1] // 2 + 1, shape[2] // 2 + 2)
+
+    def engine(q, shape, eps):
+        sim = hip().HIPFDTD(shape, 1e-3, npml=6, dtype=dtype, path="split")
+        if lossy and q == 1:
+            sig = np.zeros(shape, np.float32)
+            sig[3:-3, 2:-2, 4:-4] = 0.08
+            sim.set_epsilon(eps, sig)
+        else:
+            sim.set_epsilon(eps)
+        return sim
+
+    serial = []
+    for q, (shape, (eps, f), fr) in enumerate(zip(shapes, media, freqs)):
+        sim = engine(q, shape, eps)
+        wf = waveform(f)
+        sim.add_source(lambda s, wf=wf, b=box(shape): s.add_source_J("y", wf(s.t), b))
+        pr = hip().probes.HIPPointProbe(sim, cell(shape), "Ey", fr)
+        sim.run(n_steps)
+        serial.append((sim.fields(), pr.dft()))
+        sim.close()
+    sims = [engine(q, shape, eps) for q, (shape, (eps, _)) in enumerate(zip(shapes, media))]
+    batch = hip().batch.BatchedFDTD(sims)
+    batch.set_path("split")
+    batch.set_sources([("y", box(s)) for s in shapes], [waveform(f) for _, f in media], n_steps)
+    batch.set_probes([cell(s) for s in shapes], "Ey", freqs, n_steps)
+    batch.run(15, graph_steps)
+    batch.run(n_steps - 15, graph_steps)
+    dfts = batch.probe_dfts()
+    batch.close()
+    for q, sim in enumerate(sims):
+        ref_f, ref_p = serial[q]
+        got = sim.fields()
+        for name in ref_f:
+            np.testing.assert_array_equal(got[name], ref_f[name], err_msg=f"grid {q} {name}")
+        np.testing.assert_array_equal(dfts[q], ref_p, err_msg=f"grid {q} probe")
+        assert np.max(np.abs(ref_p)) > 0
+        sim.close()
+
+
 def test_batch_march_against_oracle():
     """Two grids batched on the march path against the oracle stepping each
     grid on its own (fields and probe DFT within 1e-5)."""
