@@ -360,6 +360,10 @@ int hfdtd_batch_set_launch(hfdtd_batch *b, int blocks, int rows_j, int planes_i)
  * with each engine's source applied and its probe recorded inside the
  * sweep (field and psi slabs below 4 GB / 2 GB per engine) */
 int hfdtd_batch_set_path(hfdtd_batch *b, int path);
+/* HFDTD_PATH_SPLIT work items: runs of 256 cells of one plane per block (1..8;
+ * 0 keeps the current value, default 2).  Longer items pay a block's set-up
+ * once for more cells and leave more threads idle at each plane's end. */
+int hfdtd_batch_set_split(hfdtd_batch *b, int runs);
 /* HFDTD_PATH_MARCH launch shape: j rows per wavefront (1 or 2; FP64: 1),
  * i planes per chunk (2..256, rounded up to even), consecutive work items
  * per XCD run, tile width in k (64 or 32 columns; 32 with one row per

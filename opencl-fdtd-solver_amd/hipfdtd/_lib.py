@@ -164,6 +164,7 @@ EXPORTS = {
     "hfdtd_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "hfdtd_batch_set_path": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_batch_set_march": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "hfdtd_batch_set_split": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_adjoint_create": (
         C.c_int,
         [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(C.c_void_p)],

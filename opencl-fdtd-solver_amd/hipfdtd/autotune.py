@@ -50,6 +50,8 @@ BATCH_PLANES = (8, 16, 32, 64)
 # batched march candidates: (rows per wavefront, planes per chunk, tile width)
 BATCH_MARCH_FP32 = ((1, 24, 64), (1, 32, 64), (2, 32, 64), (1, 24, 32), (1, 32, 32))
 BATCH_MARCH_FP64 = ((1, 16, 64), (1, 32, 64), (1, 48, 64))
+# batched split sweeps: runs of 256 cells per work item
+BATCH_SPLIT_RUNS = (1, 2, 4)
 
 
 def cache_path() -> Path:
@@ -213,12 +215,14 @@ def tune_engine(sim, *, rows=None, planes=None, march_rows=None, march_planes=No
 
 
 def tune_batch(batch, *, rows=None, planes=None, blocks=(0,), paths=("fused", "split", "march"),
-               march=None, steps=10, warmup=2, use_cache=True, write_cache=True,
-               cache_file=None) -> dict:  # fmt: skip
+               march=None, split_runs=None, steps=10, warmup=2, use_cache=True,
+               write_cache=True, cache_file=None) -> dict:  # fmt: skip
     """Pick the path ('fused' tiles, 'split' flat sweeps or 'march' tiles)
     and its launch shape for a ``BatchedFDTD`` and apply them: (blocks,
-    rows_j, planes_i) for the fused path, (rows per wavefront, planes per
-    chunk, tile width) from ``march`` for the march path.
+    rows_j, planes_i) for the fused path, runs of 256 cells per work item
+    from ``split_runs`` for the split path (in the table's planes column),
+    (rows per wavefront, planes per chunk, tile width) from ``march`` for the
+    march path.
 
     Must run before the batch's source / probe tables are set, while every
     simulation is still at step 0 with zero fields (stepping them then leaves
@@ -238,6 +242,9 @@ def tune_batch(batch, *, rows=None, planes=None, blocks=(0,), paths=("fused", "s
         batch.set_path(path)
         if path == "march":
             batch.set_march(r, p, 0, w)
+        elif path == "split":
+            batch.set_launch(b, 0, 0)
+            batch.set_split(p or 1)
         else:
             batch.set_launch(b, r or 0, p or 0)
 
@@ -249,6 +256,7 @@ def tune_batch(batch, *, rows=None, planes=None, blocks=(0,), paths=("fused", "s
     rows = rows or (BATCH_ROWS_FP64 if fp64 else BATCH_ROWS_FP32)
     planes = planes or BATCH_PLANES
     march = march or (BATCH_MARCH_FP64 if fp64 else BATCH_MARCH_FP32)
+    split_runs = split_runs or BATCH_SPLIT_RUNS
     lib = batch._lib
 
     def timed(path, b, r, p, w):
@@ -263,7 +271,7 @@ def tune_batch(batch, *, rows=None, planes=None, blocks=(0,), paths=("fused", "s
         if "fused" in paths:
             table += [timed("fused", b, r, p, None) for r in rows for p in planes]
         if "split" in paths:
-            table.append(timed("split", b, None, None, None))
+            table += [timed("split", b, None, r, None) for r in split_runs]
     if "march" in paths:
         table += [timed("march", 0, r, p, w) for r, p, w in march]
     path, b, r, p, w, ms = min(table, key=lambda e: e[5])

@@ -111,6 +111,11 @@ class BatchedFDTD:
         _lib.check(self._lib.hfdtd_batch_set_path(self._handle, codes[path]), "batch_set_path")
         self.path = path
 
+    def set_split(self, runs: int = 0) -> None:
+        """Work items of the 'split' path: runs of 256 cells of one plane per
+        block (1 to 8; 0 keeps the setting, default 2)."""
+        _lib.check(self._lib.hfdtd_batch_set_split(self._handle, int(runs)), "batch_set_split")
+
     def set_march(self, rows: int = 0, planes: int = 0, run: int = 0, width: int = 0) -> None:
         """Launch shape of the 'march' path: j rows per wavefront (1 or 2),
         i planes per chunk, consecutive work items per XCD run, tile width

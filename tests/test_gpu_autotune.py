@@ -87,13 +87,14 @@ def test_tune_batch_applies_schedule(tmp_path):
 
     sims = [hipfdtd.HIPFDTD(s, DL, npml=4) for s in ((24, 30, 40), (32, 20, 64), (20, 20, 20))]
     b = BatchedFDTD(sims)
-    res = tune_batch(b, rows=(2, 4), planes=(8, 16), march=((1, 16, 32), (2, 8, 64)), steps=3,
-                     warmup=1, cache_file=tmp_path / "tune.json")  # fmt: skip
+    res = tune_batch(b, rows=(2, 4), planes=(8, 16), march=((1, 16, 32), (2, 8, 64)),
+                     split_runs=(1, 4), steps=3, warmup=1, cache_file=tmp_path / "tune.json")  # fmt: skip
     print("[autotune] batch", res["path"], res["blocks"], res["rows"], res["planes"], res["ms"],
           flush=True)  # fmt: skip
-    assert len(res["table"]) == 7
+    assert len(res["table"]) == 8
     fused = {(2, 8), (2, 16), (4, 8), (4, 16)}
-    assert (res["path"] == "split" or (res["path"] == "fused" and (res["rows"], res["planes"]) in fused)
+    assert ((res["path"] == "split" and res["planes"] in (1, 4))
+            or (res["path"] == "fused" and (res["rows"], res["planes"]) in fused)
             or (res["path"] == "march"
                 and (res["rows"], res["planes"], res["width"]) in {(1, 16, 32), (2, 8, 64)}))  # fmt: skip
     assert b.path == res["path"]

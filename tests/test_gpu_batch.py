@@ -212,12 +212,13 @@ def test_batch_march_matches_serial_march_engines(medium, graph_steps, rows, pla
         sim.close()
 
 
-@pytest.mark.parametrize("fp64,lossy,graph_steps", [(True, False, 0), (True, True, 10),
-                                                    (False, True, 10)])  # fmt: skip
-def test_batch_split_fp64_and_lossy_match_serial(fp64, lossy, graph_steps):
-    """The two-launch split batch (sources and probes inside the E sweep) on
-    FP64 engines and with a lossy grid (the kernel with ca): fields and probe
-    DFTs bit for bit against serial split engines."""
+@pytest.mark.parametrize("fp64,lossy,graph_steps,runs", [
+    (True, False, 0, 1), (True, True, 10, 4), (False, True, 10, 3), (False, False, 10, 8)])  # fmt: skip
+def test_batch_split_fp64_and_lossy_match_serial(fp64, lossy, graph_steps, runs):
+    """The split batch (sources and probes in the hook pass over their
+    planes) on FP64 engines, with a lossy grid (the kernels with ca) and with
+    1 to 8 runs of 256 cells per work item: fields and probe DFTs bit for bit
+    against serial split engines."""
     rng = np.random.default_rng(21)
     shapes = [tuple(int(v) for v in rng.integers(18, 60, size=3)) for _ in range(4)]
     media = [sphere_case(rng, s) for s in shapes]
@@ -254,6 +255,7 @@ def test_batch_split_fp64_and_lossy_match_serial(fp64, lossy, graph_steps):
     sims = [engine(q, shape, eps) for q, (shape, (eps, _)) in enumerate(zip(shapes, media))]
     batch = hip().batch.BatchedFDTD(sims)
     batch.set_path("split")
+    batch.set_split(runs)
     batch.set_sources([("y", box(s)) for s in shapes], [waveform(f) for _, f in media], n_steps)
     batch.set_probes([cell(s) for s in shapes], "Ey", freqs, n_steps)
     batch.run(15, graph_steps)
@@ -303,6 +305,8 @@ def test_batch_march_rejects_bad_shapes():
         batch.set_march(rows=3)
     with pytest.raises(ValueError, match="width"):
         batch.set_march(width=16)
+    with pytest.raises(ValueError, match="runs"):
+        batch.set_split(9)
     with pytest.raises(ValueError, match="64-column"):
         batch.set_march(rows=2, width=32)
     with pytest.raises(ValueError, match="path"):
