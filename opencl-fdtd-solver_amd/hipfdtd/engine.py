@@ -408,6 +408,45 @@ class HIPFDTD(SourceMonitorMixin):
                 _LOG.info("  step %d/%d  t=%.3e s", i, n_steps, self.t)
             i += 1
 
+    def run_device(self, n_steps, source=None, probe=None, *, graph_steps=100):
+        """Advance ``n_steps`` steps with a device-resident source and probe,
+        the step loop captured once as a HIP graph of ``graph_steps`` steps
+        and replayed (extension: the reference evaluates its sources and
+        monitors in Python every step).  ``source`` is ``(component, box,
+        waveform)`` or ``(component, box, waveform, 'soft')``: a current (or
+        soft) source on the half-open box whose ``waveform(t)`` is sampled
+        on the host once per step, at t + dt/2, into a table (no rim taper).
+        ``probe`` is ``((i, j, k), 'Ex' | 'Ey' | 'Ez', freqs)``; returns its
+        DFT Σ E e^{jωt} dt over these steps (complex128, one per frequency),
+        or None without a probe.  The results equal ``run`` with the same
+        source as a callback and a ``HIPPointProbe``.  Registered source and
+        monitor callbacks cannot run inside a graph: the call refuses an
+        engine that has any."""
+        if self._sources or self._monitors:
+            raise ValueError("run_device runs no callbacks: clear_sources() / clear_monitors() "
+                             "first, or use run()")
+        from .batch import BatchedFDTD
+
+        n = int(n_steps)
+        if n <= 0:
+            return None
+        batch = BatchedFDTD([self])
+        try:
+            batch.set_path("march" if self.path == "march" else "split")
+            # longer work items on larger planes (fewer block set-ups per cell)
+            batch.set_split(4 if self.Ny * self.Nz >= 8192 else 2)
+            if source is not None:
+                comp, box, wave = source[0], source[1], source[2]
+                spec = (comp, box, "soft") if len(source) > 3 and source[3] == "soft" else (comp, box)
+                batch.set_sources([spec], [wave], n)
+            if probe is not None:
+                cell, comp, freqs = probe
+                batch.set_probes([tuple(cell)], comp, [list(freqs)], n)
+            batch.run(n, graph_steps if graph_steps and n >= graph_steps else 0)
+            return batch.probe_dfts()[0] if probe is not None else None
+        finally:
+            batch.close()
+
     def _pair_watch(self):
         """Cells the monitors read, as boxes (i0, i1, j0, j1, k0, k1[, mask])
         with an optional mask of the components read (bit c for Ex .. Hz,

@@ -324,3 +324,41 @@ def test_batch_rejects_mixed_grids():
     b = hip().HIPFDTD((16, 16, 16), 2e-3, npml=2)
     with pytest.raises(ValueError, match="same dl"):
         hip().batch.BatchedFDTD([a, b])
+
+
+@pytest.mark.parametrize("shape,path,graph_steps", [((40, 36, 44), "split", 16),
+                                                    ((48, 40, 70), "split", 0),
+                                                    ((64, 64, 64), "march", 10)])  # fmt: skip
+def test_run_device_matches_eager_run(shape, path, graph_steps):
+    """HIPFDTD.run_device (a graph-replayed batch of one, source and probe on
+    the device) against run() with the same source as a callback and a
+    point-probe monitor: fields and the probe DFT bit for bit; the engine's
+    time and step count advance alike, and stepping continues normally."""
+    rng = np.random.default_rng(31)
+    eps, f = sphere_case(rng, shape)
+    c = [v // 2 for v in shape]
+    box = (c[0], c[0] + 1, c[1] - 1, c[1] + 1, c[2], c[2] + 1)
+    cell = (c[0] + 2, c[1] - 3, c[2] + 1)
+    freqs = [0.8 * f, f, 1.2 * f]
+    wf = waveform(f)
+    n_steps = 33
+    ref = hip().HIPFDTD(shape, 1e-3, npml=6, path=path)
+    ref.set_epsilon(eps)
+    ref.add_source(lambda s: s.add_source_J("x", wf(s.t), box))
+    pr = hip().probes.HIPPointProbe(ref, cell, "Ex", freqs)
+    ref.run(n_steps)
+    want_f, want_p = ref.fields(), pr.dft()
+    sim = hip().HIPFDTD(shape, 1e-3, npml=6, path=path)
+    sim.set_epsilon(eps)
+    got_p = sim.run_device(n_steps, ("x", box, wf), (cell, "Ex", freqs), graph_steps=graph_steps)
+    got_f = sim.fields()
+    for n in FIELDS:
+        np.testing.assert_array_equal(got_f[n], want_f[n], err_msg=n)
+    np.testing.assert_array_equal(got_p, want_p)
+    assert sim.step_num == ref.step_num and sim.t == ref.t
+    sim.run(3)  # the engine steps on after a device run
+    assert sim.step_num == n_steps + 3
+    with pytest.raises(ValueError, match="callbacks"):
+        ref.run_device(2, ("x", box, wf))
+    ref.close()
+    sim.close()
