@@ -91,6 +91,21 @@ def test_engine_without_device_raises_runtime_error():
         HIPFDTD((8, 8, 8), 1e-3, npml=1)
 
 
+def test_memory_error_when_over_budget(monkeypatch):
+    """tests/test_unit_engine.py:244-257: a 64 MiB device and a 400^3 grid
+    fail in the constructor with a MemoryError naming the usable budget,
+    before anything is allocated (the device query is stubbed; no HIP call
+    beyond loading the library is made)."""
+    from hipfdtd import HIPFDTD, _lib
+
+    monkeypatch.setattr(_lib, "device_count", lambda: 1)
+    monkeypatch.setattr(_lib, "device_info", lambda d=0: (64 * 1024 * 1024, 0, "MockTinyGPU"))
+    with pytest.raises(MemoryError) as err:
+        HIPFDTD((400, 400, 400), 1e-3, npml=20)
+    msg = str(err.value).lower()
+    assert "usable" in msg and "mocktinygpu" in msg
+
+
 def test_memory_estimate_counts_all_arrays():
     from hipfdtd import HIPFDTD
 
