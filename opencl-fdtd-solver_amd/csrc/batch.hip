@@ -87,6 +87,26 @@ struct BatchArgs {
   int n_hook;
 };
 
+// A launch never writes its work list or descriptors: reading them through
+// the constant address space lets them come in by scalar loads into SGPRs
+// (read through a generic pointer, part of them come in by vector loads
+// after the first stores, each wavefront's copy in VGPRs).
+#define HF_CONST __attribute__((address_space(4)))
+template <typename S>
+__device__ __forceinline__ const HF_CONST S *as_const(const S *p) {
+  return (const HF_CONST S *)p;
+}
+template <typename S>
+__device__ __forceinline__ S load_const(const HF_CONST S *p) {
+  static_assert(sizeof(S) % 4 == 0, "read in dwords");
+  S out;
+  const HF_CONST unsigned *q = (const HF_CONST unsigned *)p;
+  unsigned *o = reinterpret_cast<unsigned *>(&out);
+#pragma unroll
+  for (int w = 0; w < (int)(sizeof(S) / 4); ++w) o[w] = q[w];
+  return out;
+}
+
 template <typename T>
 struct BatchSrcP {
   const Src<T> *s;
@@ -183,18 +203,22 @@ template <typename T>
 __global__ __launch_bounds__(kFlatBlock) void k_batch_split_H(BatchArgs<T> a) {
   for (int m = blockIdx.x; m < a.n_tiles; m += gridDim.x) {
     const int n = gridDim.x == a.n_tiles ? xcd_runs(m, a.n_tiles, a.xcd, a.run) : m;
-    const int ts = __builtin_amdgcn_readfirstlane(a.tile_sim[n]);
-    const SimDesc<T> &d = a.sims[ts & ~kHookBit];
-    const Geo g = d.g;
-    const int lt = n - d.tile_begin;
-    const int i = lt / d.tiles_k;
-    const int q0 = (lt - i * d.tiles_k) * (kFlatBlock * a.runs) + threadIdx.x;
+    const int ts = as_const(a.tile_sim)[n];
+    const HF_CONST SimDesc<T> *d = as_const(a.sims) + (ts & ~kHookBit);
+    const Geo g = load_const(&d->g);
+    T *const st = d->st[0];
+    const T *const prof = d->prof;
+    T *const psi = d->psi[0];
+    const int tiles_k = d->tiles_k;
+    const int lt = n - d->tile_begin;
+    const int i = lt / tiles_k;
+    const int q0 = (lt - i * tiles_k) * (kFlatBlock * a.runs) + threadIdx.x;
     const int ncell = g.ny * g.nz;
     for (int r = 0; r < a.runs; ++r) {
       const int q = q0 + r * kFlatBlock;
       if (q >= ncell) break;
       const int j = q / g.nz;
-      split_h_cell(g, a.dtm_dl, a.dtm, d.st[0], d.prof, d.psi[0], i, j, q - j * g.nz);
+      split_h_cell(g, a.dtm_dl, a.dtm, st, prof, psi, i, j, q - j * g.nz);
     }
   }
 }
@@ -238,26 +262,38 @@ template <typename T, bool LOSSY>
 __global__ __launch_bounds__(kFlatBlock) void k_batch_split_E(BatchArgs<T> a) {
   for (int m = blockIdx.x; m < a.n_tiles; m += gridDim.x) {
     const int n = gridDim.x == a.n_tiles ? xcd_runs(m, a.n_tiles, a.xcd, a.run) : m;
-    const int ts = __builtin_amdgcn_readfirstlane(a.tile_sim[n]);
+    const int ts = as_const(a.tile_sim)[n];
     if (ts & kHookBit) continue;
-    const SimDesc<T> &d = a.sims[ts];
-    const Geo g = d.g;
-    const int lt = n - d.tile_begin;
-    const int i = lt / d.tiles_k;
-    const int q0 = (lt - i * d.tiles_k) * (kFlatBlock * a.runs) + threadIdx.x;
+    const HF_CONST SimDesc<T> *d = as_const(a.sims) + ts;
+    const Geo g = load_const(&d->g);
+    T *const st = d->st[0];
+    const T *const coef = d->coef;
+    const T *const prof = d->prof;
+    T *const psi = d->psi[0];
+    const int tiles_k = d->tiles_k;
+    const int lt = n - d->tile_begin;
+    const int i = lt / tiles_k;
+    const int q0 = (lt - i * tiles_k) * (kFlatBlock * a.runs) + threadIdx.x;
     const int ncell = g.ny * g.nz;
+    if constexpr (sizeof(T) == 4) {
+      const unsigned char *const mat = d->mat;
+      if (mat) {  // lossless (ca = 1), whichever variant the batch runs
+        const float *const mtab = d->mtab;
+        const int nmat = d->nmat;
+        for (int r = 0; r < a.runs; ++r) {
+          const int q = q0 + r * kFlatBlock;
+          if (q >= ncell) break;
+          const int j = q / g.nz, k = q - j * g.nz;
+          split_e_cell_mat(g, a.inv_dl, st, coef, mat, mtab, nmat, prof, psi, i, j, k);
+        }
+        continue;
+      }
+    }
     for (int r = 0; r < a.runs; ++r) {
       const int q = q0 + r * kFlatBlock;
       if (q >= ncell) break;
       const int j = q / g.nz, k = q - j * g.nz;
-      if constexpr (sizeof(T) == 4) {
-        if (d.mat) {  // lossless (ca = 1), whichever variant the batch runs
-          split_e_cell_mat(g, a.inv_dl, d.st[0], d.coef, d.mat, d.mtab, d.nmat, d.prof, d.psi[0],
-                           i, j, k);
-          continue;
-        }
-      }
-      split_e_cell<T, LOSSY>(g, a.inv_dl, d.st[0], d.coef, d.prof, d.psi[0], i, j, k);
+      split_e_cell<T, LOSSY>(g, a.inv_dl, st, coef, prof, psi, i, j, k);
     }
   }
 }
