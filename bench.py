@@ -65,6 +65,7 @@ error against the FP64 sums.
 
 Usage: python bench.py [--gpus 1] [--steps 2000] [--warmup 20] [--no-cpu]
        python bench.py --copy-calibration
+       python bench.py --workload stream-ceiling
        python bench.py --workload plane-dft [--steps 1500]
        python bench.py --workload adjoint [--steps 3000] [--repeat 2]
        python bench.py --workload batch [--steps 1200] [--graph-steps 100]
@@ -704,6 +705,37 @@ def copy_calibration(n_bytes=1 << 30, reps=5):
     print(json.dumps({"copy_bytes_read": n_bytes, "copy_bytes_written": n_bytes, "reps": reps}))
 
 
+def stream_ceiling(n_bytes=1 << 30, reps=10):
+    """What plain streaming kernels reach on this device: the library's
+    stream probe (16-byte loads and stores, grid-stride loops) over 1 GiB
+    arrays (256 MiB for the six-array kernel), at several grid sizes, the best
+    of each kind.  copy = 1 read + 1 write stream, triad = 2 + 1, six = six
+    arrays read and six written (the balance of a Yee step's field traffic),
+    read = a sum.  It is the practical ceiling for a sweep's mix of reads
+    and writes, below the 8 TB/s peak."""
+    import ctypes as C
+
+    from hipfdtd import _lib
+
+    lib = _lib.load()
+    kinds = {"copy": (0, n_bytes, 2.0), "triad": (1, n_bytes, 3.0),
+             "six_by_six": (2, n_bytes // 4, 12.0), "read": (3, n_bytes, 1.0)}  # fmt: skip
+    out = {}
+    for name, (kind, nb, streams) in kinds.items():
+        best = None
+        for blocks in (256, 512, 1024, 2048, 4096):
+            sec = C.c_double(0.0)
+            _lib.check(lib.hfdtd_stream_probe(0, kind, nb, blocks, reps, C.byref(sec)),
+                       "hfdtd_stream_probe")  # fmt: skip
+            tbs = streams * nb / sec.value / 1e12
+            if best is None or tbs > best["tb_per_s"]:
+                best = {"tb_per_s": round(tbs, 3), "blocks": blocks,
+                        "us": round(sec.value * 1e6, 1)}  # fmt: skip
+        out[name] = best
+    print(json.dumps({"stream_ceiling": out, "bytes_per_array": {k: v[1] for k, v in kinds.items()},
+                      "reps": reps, "peak_tb_per_s": HBM_PEAK_GBS / 1e3}))  # fmt: skip
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -716,7 +748,8 @@ def main():
                     help="256^3 workload: steps of the FP32-vs-FP64 accuracy run (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", default="yee-256",
-                    choices=["yee-256", "plane-dft", "adjoint", "batch", "reference-sizes"])
+                    choices=["yee-256", "plane-dft", "adjoint", "batch", "reference-sizes",
+                             "stream-ceiling"])
     ap.add_argument("--graph-steps", type=int, default=100, help="batch: steps per HIP graph")
     ap.add_argument("--batch-path", default="auto", choices=["auto", "split", "fused", "march"],
                     help="batch: tune only this path (auto: all three)")
@@ -731,6 +764,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "stream-ceiling":
+        stream_ceiling()
         return
     if args.workload == "plane-dft":
         if args.steps == 2000:

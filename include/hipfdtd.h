@@ -97,6 +97,15 @@ int hfdtd_device_count(int *count);
 /* replaces device.global_mem_size / device.name (engine.py:337-363) */
 int hfdtd_device_info(int device, uint64_t *total_bytes, uint64_t *free_bytes, char *name,
                       int name_len);
+/* Streaming-bandwidth probe (extension, no counterpart in the reference; for
+ * reading a sweep's measured traffic against what plain streaming kernels
+ * reach on the device): allocates its own buffers of `bytes` per array
+ * (a multiple of 64, 4 KiB to 16 GiB), times `reps` launches of `blocks`
+ * 256-thread workgroups and stores the mean seconds per launch.  kind: 0 copy
+ * (bytes read + bytes written), 1 triad (2 + 1 arrays), 2 six arrays read and
+ * six written, 3 read only (a sum). */
+int hfdtd_stream_probe(int device, int kind, uint64_t bytes, int blocks, int reps,
+                       double *seconds);
 
 /* Host-only (no GPU needed): 1-D CFS-CPML profile b, c, kappa of length n in
  * float (fp64=0) or double (fp64=1); replaces cpml.build_cpml_1d

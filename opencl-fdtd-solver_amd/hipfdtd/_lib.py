@@ -64,6 +64,9 @@ EXPORTS = {
         C.c_int,
         [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_char_p, C.c_int],
     ),
+    "hfdtd_stream_probe": (
+        C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_double)],
+    ),
     "hfdtd_cpml_profile": (
         C.c_int,
         [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
