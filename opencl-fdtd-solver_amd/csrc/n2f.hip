@@ -258,6 +258,10 @@ typedef float n2f_f32x16 __attribute__((ext_vector_type(16)));
 // multiple of 32); v_mfma_f32_32x32x2f32 as in k_dft_contract, output
 // D[i][j] in register r of lane l with i = (r/4)*8 + (l/32)*4 + r%4,
 // j = l%32 = 2 f + {re, im}.  W rows at or beyond kvalid count as zero.
+// The block goes through LDS to fold D into the sums: row i of a wavefront's
+// tile is sample s0 + i/2, tangential component i%2, so for one component
+// and frequency its 16 samples' (re, im) pairs are 32 consecutive sums, and
+// each lane then reads and writes whole runs of them.
 template <typename A, bool KAHAN>
 __global__ __launch_bounds__(256) void k_n2f_contract(Box b, const float *__restrict__ buf,
                                                       long long ppad, long long pe,
@@ -265,35 +269,43 @@ __global__ __launch_bounds__(256) void k_n2f_contract(Box b, const float *__rest
                                                       const float *__restrict__ wh, int kvalid,
                                                       int nfreq, A *__restrict__ acc,
                                                       A *__restrict__ cmp) {
-  const int lane = threadIdx.x & 63;
-  const long long p0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
-  if (p0 >= ppad) return;  // wave-uniform
+  __shared__ float tile[4][32][33];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long p0 = ((long long)blockIdx.x * 4 + wave) * 32;
+  const bool live = p0 < ppad;  // wave-uniform
   const bool hrow = p0 >= pe;
-  const float *W = hrow ? wh : we;
   const int half = lane >> 5, col = lane & 31;
-  n2f_f32x16 d = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const int pairs = (kvalid + 1) / 2;
-  for (int t = 0; t < pairs; ++t) {
-    const int k = 2 * t + half;
-    const float a = buf[(long long)k * ppad + p0 + col];
-    const float w = k < kvalid ? W[k * 32 + col] : 0.0f;
-    d = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w, d, 0, 0, 0);
-  }
-  if (col >= 2 * nfreq) return;
-  const int fr = col >> 1, q = col & 1;
-  const size_t nfc = (size_t)b.n_face;
+  if (live) {
+    const float *W = hrow ? wh : we;
+    n2f_f32x16 d = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int pairs = (kvalid + 1) / 2;
+#pragma unroll 4
+    for (int t = 0; t < pairs; ++t) {
+      const int k = 2 * t + half;
+      const float a = buf[(long long)k * ppad + p0 + col];
+      const float w = k < kvalid ? W[k * 32 + col] : 0.0f;
+      d = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w, d, 0, 0, 0);
+    }
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const long long local = p0 + (r / 4) * 8 + half * 4 + (r % 4) - (hrow ? pe : 0);
-    if (local >= 2 * (long long)b.n_face) continue;
-    const int s = (int)(local >> 1), tg = (int)(local & 1);
-    int i, j, k;
-    const int f = decode_face(b, s, i, j, k);
+    for (int r = 0; r < 16; ++r) tile[wave][(r / 4) * 8 + half * 4 + (r % 4)][col] = d[r];
+  }
+  __syncthreads();
+  if (!live) return;
+  const long long row0 = p0 - (hrow ? pe : 0);  // even: the tile's first row
+  const int s0 = (int)(row0 >> 1);
+  const size_t nfc = (size_t)b.n_face;
+  for (int v = 0; v < nfreq; ++v) {
+    // lanes 0-31: (component 0, frequency v); lanes 32-63: (component 1, v)
+    const int tg = half, q = lane & 1, sl = (lane >> 1) & 15;
+    const int s = s0 + sl;
+    if (s >= b.n_face) continue;
+    const int f = s < b.off[2] ? (s < b.off[1] ? 0 : 1)
+                               : s < b.off[4] ? (s < b.off[3] ? 2 : 3) : (s < b.off[5] ? 4 : 5);
     // tangential components of the face: E (t=0,1) then H (t=0,1)
     const int ce = f <= 1 ? 1 + tg : (f <= 3 ? 2 * tg : tg);
     const int comp = hrow ? ce + 3 : ce;
-    const size_t o = (((size_t)comp * nfreq + fr) * nfc + s) * 2 + q;
-    const A term = (A)d[r];
+    const size_t o = (((size_t)comp * nfreq + v) * nfc + s) * 2 + q;
+    const A term = (A)tile[wave][2 * sl + tg][2 * v + q];
     if (!KAHAN) {
       acc[o] += term;
     } else {
