@@ -419,8 +419,13 @@ class HIPFDTD(SourceMonitorMixin):
         ``probe`` is ``((i, j, k), 'Ex' | 'Ey' | 'Ez', freqs)``; returns its
         DFT Σ E e^{jωt} dt over these steps (complex128, one per frequency),
         or None without a probe.  The results equal ``run`` with the same
-        source as a callback and a ``HIPPointProbe``.  Registered source and
-        monitor callbacks cannot run inside a graph: the call refuses an
+        source as a callback and a ``HIPPointProbe``, bit for bit: a split
+        engine runs batched split sweeps, a march, fused or pair engine
+        batched march tiles (the march sweep's arithmetic, which the fused
+        and pair sweeps share).  An engine too large for batched march tiles
+        (field slab of 4 GB or more, psi slab of 2 GB or more) runs batched
+        split sweeps, equal to its own path to rounding.  Registered source
+        and monitor callbacks cannot run inside a graph: the call refuses an
         engine that has any."""
         if self._sources or self._monitors:
             raise ValueError("run_device runs no callbacks: clear_sources() / clear_monitors() "
@@ -430,9 +435,12 @@ class HIPFDTD(SourceMonitorMixin):
         n = int(n_steps)
         if n <= 0:
             return None
+        item = np.dtype(self.dtype).itemsize
+        psi_bytes = (4 * (self.psi_x_size + self.psi_y_size + self.psi_z_size) + 64) * item
+        march = self.path != "split" and self._fused_fits and psi_bytes < 2**31
         batch = BatchedFDTD([self])
         try:
-            batch.set_path("march" if self.path == "march" else "split")
+            batch.set_path("march" if march else "split")
             # longer work items on larger planes (fewer block set-ups per cell)
             batch.set_split(4 if self.Ny * self.Nz >= 8192 else 2)
             if source is not None:

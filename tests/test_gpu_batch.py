@@ -326,10 +326,11 @@ def test_batch_rejects_mixed_grids():
         hip().batch.BatchedFDTD([a, b])
 
 
-@pytest.mark.parametrize("shape,path,graph_steps", [((40, 36, 44), "split", 16),
-                                                    ((48, 40, 70), "split", 0),
-                                                    ((64, 64, 64), "march", 10)])  # fmt: skip
-def test_run_device_matches_eager_run(shape, path, graph_steps):
+@pytest.mark.parametrize("shape,path,graph_steps,dtype", [
+    ((40, 36, 44), "split", 16, np.float32), ((48, 40, 70), "split", 0, np.float32),
+    ((64, 64, 64), "march", 10, np.float32), ((40, 36, 44), "fused", 16, np.float32),
+    ((36, 30, 50), "fused", 10, np.float64), ((44, 40, 48), "pair", 8, np.float32)])  # fmt: skip
+def test_run_device_matches_eager_run(shape, path, graph_steps, dtype):
     """HIPFDTD.run_device (a graph-replayed batch of one, source and probe on
     the device) against run() with the same source as a callback and a
     point-probe monitor: fields and the probe DFT bit for bit; the engine's
@@ -342,13 +343,15 @@ def test_run_device_matches_eager_run(shape, path, graph_steps):
     freqs = [0.8 * f, f, 1.2 * f]
     wf = waveform(f)
     n_steps = 33
-    ref = hip().HIPFDTD(shape, 1e-3, npml=6, path=path)
+    print(f"[run-device] shape={shape} path={path} graph_steps={graph_steps} "
+          f"dtype={np.dtype(dtype).name}", flush=True)  # fmt: skip
+    ref = hip().HIPFDTD(shape, 1e-3, npml=6, path=path, dtype=dtype)
     ref.set_epsilon(eps)
     ref.add_source(lambda s: s.add_source_J("x", wf(s.t), box))
     pr = hip().probes.HIPPointProbe(ref, cell, "Ex", freqs)
     ref.run(n_steps)
     want_f, want_p = ref.fields(), pr.dft()
-    sim = hip().HIPFDTD(shape, 1e-3, npml=6, path=path)
+    sim = hip().HIPFDTD(shape, 1e-3, npml=6, path=path, dtype=dtype)
     sim.set_epsilon(eps)
     got_p = sim.run_device(n_steps, ("x", box, wf), (cell, "Ex", freqs), graph_steps=graph_steps)
     got_f = sim.fields()
