@@ -343,6 +343,26 @@ __global__ void k_step_add(int *step, int n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(step, n);
 }
 
+// One row copy of the batched march's padding: rows x w elements from src
+// (row pitch sp) to dst (row pitch dp).
+struct PadJob {
+  const float *src;
+  float *dst;
+  long long rows;
+  int w, sp, dp;
+};
+
+// Job blockIdx.y, grid-stride over its elements.
+__global__ __launch_bounds__(256) void k_pad_rows(const PadJob *__restrict__ jobs) {
+  const PadJob J = jobs[blockIdx.y];
+  const long long n = J.rows * J.w;
+  for (long long t = blockIdx.x * 256ll + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+    const long long r = t / J.w;
+    const int c = (int)(t - r * J.w);
+    J.dst[r * J.dp + c] = J.src[r * J.sp + c];
+  }
+}
+
 }  // namespace hfdtd
 
 struct hfdtd_batch {
@@ -350,6 +370,7 @@ struct hfdtd_batch {
   hipStream_t stream = nullptr;
   int fp64 = 0;
   void *d_desc = nullptr;
+  const void *mbase = nullptr;  // march path: the address descriptor offsets count from
   int *d_tile_sim = nullptr;
   int n_tiles = 0;
   int *d_hook_items = nullptr;  // split path: items holding a source or probe
@@ -374,6 +395,16 @@ struct hfdtd_batch {
   int mce = 0;
   bool mlossy = false;
   bool lossy = false;
+  // march path, material-indexed FP32 batches: engines whose nz is not a
+  // multiple of 32 run on copies with rows padded to whole 128-byte lines
+  // (state and psi of both parities, then the material bytes), filled from
+  // the engine before a run and written back after it
+  int mpad = 1;
+  std::vector<float *> pad_buf;          // per engine, null: runs on its own slabs
+  std::vector<long long> pad_slab;       // elements of one padded state slab
+  std::vector<long long> pad_psi;        // elements of one padded psi slab
+  hfdtd::PadJob *d_jobs = nullptr;       // the row copies of one direction
+  int n_jobs_cap = 0;
   hipGraphExec_t gexec = nullptr;
 };
 
@@ -408,13 +439,7 @@ static int batch_build(hfdtd_batch *b) {
       if (rc) return rc;
     }
     SimDesc<T> &d = desc[s];
-    d.g.nx = e->nx;
-    d.g.ny = e->ny;
-    d.g.nz = e->nz;
-    d.g.npml = e->npml;
-    d.g.vs = (long long)e->vstride;
-    d.g.plen = (int)e->prof_len();
-    for (int q = 0; q <= PSI_COUNT; ++q) d.g.psi_off[q] = (int)e->psi_off[q];
+    d.g = geo_of(e);
     // parity 0 (relative to step0) is the engine's current state
     d.st[0] = (T *)e->state[e->cur];
     d.st[1] = (T *)e->state[e->cur ^ 1];
@@ -511,10 +536,142 @@ static int batch_build(hfdtd_batch *b) {
 // March path: per-grid descriptors (parity origin = current step) and the
 // work list of march tiles.  The kernel reads the material index when every
 // grid has one, else the ce volumes (with ca when some grid is lossy).
+// Host copy of psi_slot (fdtd_core.h): element offset of psi slot s.
+static long long psi_slot_host(const Geo &g, int s) {
+  const long long n2 = 2ll * g.npml;
+  const long long sx = n2 * g.ny * g.sk, sy = (long long)g.nx * n2 * g.sk;
+  const long long sz = (long long)g.nx * g.ny * n2;
+  const int t = s % 6;
+  const long long in_group = t == 0 ? 0 : t == 1 ? sx : t == 2 ? 2 * sx : t == 3 ? 2 * sx + sy
+                             : t == 4 ? 2 * (sx + sy) : 2 * (sx + sy) + sz;
+  return (long long)(s / 6) * 2 * (sx + sy + sz) + in_group;
+}
+
+// Geometry of an engine's padded copy: rows of whole 128-byte lines.
+static Geo padded_geo(const Engine *e) {
+  Geo g = geo_of(e);
+  g.sk = (e->nz + 31) / 32 * 32;
+  g.vs = ((long long)e->nx * e->ny * g.sk + 64 + 63) / 64 * 64;
+  return g;
+}
+
+// Row copies between engine s's slabs of parity ep (its state[cur ^ ep],
+// psi[cur ^ ep]) and its padded copy of parity q, in the direction to_pad.
+static void pad_jobs(const hfdtd_batch *b, int s, int ep, int q, bool to_pad,
+                     std::vector<PadJob> &jobs) {
+  const Engine *e = b->eng[s];
+  const Geo gp = padded_geo(e);
+  float *buf = b->pad_buf[s];
+  const long long slab = b->pad_slab[s], psi = b->pad_psi[s];
+  auto add = [&](float *eng_p, float *pad_p, long long rows, int w, int ew, int pw) {
+    if (rows <= 0 || w <= 0) return;
+    jobs.push_back(to_pad ? PadJob{eng_p, pad_p, rows, w, ew, pw}
+                          : PadJob{pad_p, eng_p, rows, w, pw, ew});
+  };
+  const long long rows = (long long)e->nx * e->ny;
+  for (int c = 0; c < 6; ++c)
+    add((float *)e->state[e->cur ^ ep] + (size_t)c * e->vstride, buf + q * slab + c * gp.vs, rows,
+        e->nz, e->nz, gp.sk);
+  const int n2 = 2 * e->npml;
+  for (int t = 0; t < PSI_COUNT; ++t) {
+    float *ep_ = (float *)e->psi[e->cur ^ ep] + e->psi_off[t];
+    float *pp = buf + 2 * slab + q * psi + psi_slot_host(gp, t);
+    const int u = t % 6;
+    if (u < 2) add(ep_, pp, (long long)n2 * e->ny, e->nz, e->nz, gp.sk);
+    else if (u < 4) add(ep_, pp, (long long)e->nx * n2, e->nz, e->nz, gp.sk);
+    else {
+      const int w = (int)(rows * n2);
+      add(ep_, pp, 1, w, w, w);
+    }
+  }
+}
+
+static int pad_run(hfdtd_batch *b, const std::vector<PadJob> &jobs) {
+  if (jobs.empty()) return HFDTD_OK;
+  if ((int)jobs.size() > b->n_jobs_cap) {
+    if (b->d_jobs) HF_HIP(hipFree(b->d_jobs));
+    b->d_jobs = nullptr;
+    b->n_jobs_cap = 0;
+    HF_HIP(hipMalloc((void **)&b->d_jobs, sizeof(PadJob) * jobs.size()));
+    b->n_jobs_cap = (int)jobs.size();
+  }
+  HF_HIP(hipMemcpyAsync(b->d_jobs, jobs.data(), sizeof(PadJob) * jobs.size(),
+                        hipMemcpyHostToDevice, b->stream));
+  long long most = 0;
+  for (const PadJob &j : jobs) most = std::max(most, j.rows * j.w);
+  const unsigned gx = (unsigned)std::min<long long>(ceil_div(most, 256), 2048);
+  hipLaunchKernelGGL(k_pad_rows, dim3(gx, (unsigned)jobs.size()), dim3(256), 0, b->stream,
+                     (const PadJob *)b->d_jobs);
+  HF_HIP(hipGetLastError());
+  return HFDTD_OK;
+}
+
+// Padded copies for the engines of an indexed FP32 march batch whose nz is
+// not a multiple of 32 (rows then start mid-line and a 64-column tile row
+// touches three lines instead of two): allocated once, filled here from
+// the engines' current state.  Returns with pad_buf[s] null for the
+// engines that run on their own slabs.
+static int pad_setup(hfdtd_batch *b, bool indexed) {
+  const int n = (int)b->eng.size();
+  b->pad_buf.resize(n, nullptr);
+  b->pad_slab.resize(n, 0);
+  b->pad_psi.resize(n, 0);
+  std::vector<PadJob> jobs;
+  for (int s = 0; s < n; ++s) {
+    Engine *e = b->eng[s];
+    bool want = b->mpad && indexed && e->elem == 4 && e->nz % 32 != 0;
+    const Geo g0 = geo_of(e), gp = padded_geo(e);
+    for (int q = 0; want && q <= PSI_COUNT; ++q) want = psi_slot_host(g0, q) == e->psi_off[q];
+    const long long slab = 6 * gp.vs, psi = psi_slot_host(gp, PSI_COUNT) + 64;
+    want = want && slab * 4 < 0x80000000ll && psi * 4 < 0x80000000ll;
+    if (!want) {
+      if (b->pad_buf[s]) HF_HIP(hipFree(b->pad_buf[s]));
+      b->pad_buf[s] = nullptr;
+      continue;
+    }
+    const long long mat_bytes = ((long long)e->nx * e->ny * gp.sk + 64 + 3) / 4 * 4;
+    if (!b->pad_buf[s]) {
+      const size_t bytes = (size_t)(2 * slab + 2 * psi) * 4 + (size_t)mat_bytes;
+      HF_HIP(hipMalloc((void **)&b->pad_buf[s], bytes));
+      HF_HIP(hipMemsetAsync(b->pad_buf[s], 0, bytes, b->stream));  // pad columns stay zero
+    }
+    b->pad_slab[s] = slab;
+    b->pad_psi[s] = psi;
+    HF_HIP(hipMemcpy2DAsync(b->pad_buf[s] + 2 * slab + 2 * psi, (size_t)gp.sk, e->mat,
+                            (size_t)e->nz, (size_t)e->nz, (size_t)e->nx * e->ny,
+                            hipMemcpyDeviceToDevice, b->stream));
+    pad_jobs(b, s, 0, 0, true, jobs);
+  }
+  return pad_run(b, jobs);
+}
+
+// After n steps: the padded engines' state and psi of parity n & 1 back into
+// their own slabs of that parity (the caller then flips cur when n is odd).
+static int pad_writeback(hfdtd_batch *b, int n_steps) {
+  if (n_steps <= 0) return HFDTD_OK;
+  std::vector<PadJob> jobs;
+  for (int s = 0; s < (int)b->pad_buf.size(); ++s)
+    if (b->pad_buf[s]) pad_jobs(b, s, n_steps & 1, n_steps & 1, false, jobs);
+  int rc = pad_run(b, jobs);
+  if (rc) return rc;
+  HF_HIP(hipStreamSynchronize(b->stream));
+  return HFDTD_OK;
+}
+
+// Element offset of p from base (both device addresses; every slab is
+// aligned to its element size).
+template <typename U>
+static long long elem_off(const void *base, const void *p) {
+  return ((long long)(uintptr_t)p - (long long)(uintptr_t)base) / (long long)sizeof(U);
+}
+
 template <typename T>
 static int batch_build_march(hfdtd_batch *b) {
   const int n = (int)b->eng.size();
   const int RW = b->mrows;
+  // the launch's base address: every slab is an element offset from it
+  const void *base = b->eng[0]->state[0];
+  b->mbase = base;
   std::vector<BMSim<T>> desc(n);
   std::vector<int> item_sim;
   int begin = 0;
@@ -531,17 +688,29 @@ static int batch_build_march(hfdtd_batch *b) {
     all_mat = all_mat && e->ce_source() == kCeIndex;
   }
   b->mce = all_mat ? kCeIndex : kCeVolumes;
+  int prc = pad_setup(b, all_mat);
+  if (prc) return prc;
   for (int s = 0; s < n; ++s) {
     Engine *e = b->eng[s];
     BMSim<T> &d = desc[s];
-    d.g = geo_of(e);
-    d.st[0] = (T *)e->state[e->cur];
-    d.st[1] = (T *)e->state[e->cur ^ 1];
-    d.psi[0] = (T *)e->psi[e->cur];
-    d.psi[1] = (T *)e->psi[e->cur ^ 1];
-    d.coef = (const T *)e->coef;
-    d.prof = (const T *)e->prof;
-    d.mat = all_mat ? e->mat : nullptr;
+    d.coef = elem_off<T>(base, e->coef);
+    d.prof = elem_off<T>(base, e->prof);
+    if (float *buf = b->pad_buf[s]) {  // the padded copy (indexed FP32: no coefficient reads)
+      const long long slab = b->pad_slab[s], psi = b->pad_psi[s];
+      d.g = padded_geo(e);
+      d.st[0] = elem_off<T>(base, buf);
+      d.st[1] = elem_off<T>(base, buf + slab);
+      d.psi[0] = elem_off<T>(base, buf + 2 * slab);
+      d.psi[1] = elem_off<T>(base, buf + 2 * slab + psi);
+      d.mat = elem_off<unsigned char>(base, buf + 2 * slab + 2 * psi);
+    } else {
+      d.g = geo_of(e);
+      d.st[0] = elem_off<T>(base, e->state[e->cur]);
+      d.st[1] = elem_off<T>(base, e->state[e->cur ^ 1]);
+      d.psi[0] = elem_off<T>(base, e->psi[e->cur]);
+      d.psi[1] = elem_off<T>(base, e->psi[e->cur ^ 1]);
+      d.mat = all_mat ? elem_off<unsigned char>(base, e->mat) : 0;
+    }
     d.mtab = all_mat ? e->mtab : nullptr;
     d.nmat = all_mat ? e->nmat : 0;
     d.planes = (b->mplanes + 1) / 2 * 2;
@@ -592,6 +761,7 @@ static BMArgs<T> batch_march_args(const hfdtd_batch *b) {
   const Engine *e = b->eng[0];
   BMArgs<T> a;
   a.sims = (const BMSim<T> *)b->d_desc;
+  a.base = b->mbase;
   a.item_sim = b->d_tile_sim;
   a.n_items = b->n_tiles;
   a.step = b->d_step;
@@ -757,6 +927,7 @@ static int batch_run_t(hfdtd_batch *b, int n_steps, int graph_steps) {
     const BMArgs<T> a = batch_march_args<T>(b);
     rc = batch_run_steps(b, n_steps, graph_steps,
                          [&](int q) { return batch_launch_march<T>(b, a, q); });
+    if (rc == HFDTD_OK) rc = pad_writeback(b, n_steps);
   } else {
     if ((rc = batch_build<T>(b))) return rc;
     const BatchArgs<T> a = batch_args<T>(b);
@@ -884,7 +1055,9 @@ int hfdtd_batch_destroy(hfdtd_batch *b) {
   (void)batch_free_graph(b);
   for (void *p : {(void *)b->d_desc, (void *)b->d_tile_sim, (void *)b->d_hook_items,
                   (void *)b->d_step, (void *)b->d_amp,
-                  (void *)b->d_phase, (void *)b->d_probe_acc})
+                  (void *)b->d_phase, (void *)b->d_probe_acc, (void *)b->d_jobs})
+    if (p) (void)hipFree(p);
+  for (float *p : b->pad_buf)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1017,6 +1190,12 @@ int hfdtd_batch_set_march(hfdtd_batch *b, int rows, int planes, int run, int wid
   if (rows) b->mrows = rows;
   if (planes) b->mplanes = std::max(2, planes);
   if (run) b->mrun = run;
+  return batch_free_graph(b);
+}
+
+int hfdtd_batch_set_march_padding(hfdtd_batch *b, int on) {
+  if (!b || (on != 0 && on != 1)) return fail(HFDTD_EINVAL, "padding must be 0 or 1");
+  b->mpad = on;
   return batch_free_graph(b);
 }
 

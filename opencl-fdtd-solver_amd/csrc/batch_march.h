@@ -11,6 +11,17 @@
 // *step + q, where q is the launch's place in its run or graph and one
 // small kernel advances *step after the run: every launch argument stays
 // fixed across graph replays.
+//
+// A grid's slabs are addressed as element offsets from base pointers that
+// come in as separate __restrict__ kernel arguments (one per role: state
+// read, state written, psi read, psi written, coefficients, profiles,
+// material index, probe sums), all holding the same address.  The compiler
+// can then tell the tile's stores from its loads, as in the single-engine
+// sweep, and keeps the next plane's loads in flight across the current
+// plane's stores; with pointers read from the descriptors it must assume
+// they alias, and the sweep waited for every load (15 % slower on one
+// 256^3 grid).  The roles never share a slab within a launch: the state and
+// psi written are the other parity of the ones read.
 #pragma once
 
 #include "march.h"
@@ -21,11 +32,12 @@ namespace hfdtd {
 template <typename T>
 struct BMSim {
   Geo g;
-  T *st[2];   // state slab, by step parity (relative to step0)
-  T *psi[2];  // psi slab, by step parity
-  const T *coef;
-  const T *prof;
-  const void *mat;    // material index (kCeIndex kernels), else null
+  // element offsets from the launch's base address (bytes for mat)
+  long long st[2];   // state slab, by step parity (relative to step0)
+  long long psi[2];  // psi slab, by step parity
+  long long coef;
+  long long prof;
+  long long mat;      // material index (kCeIndex kernels)
   const float *mtab;  // its ce table (nmat x nmat)
   int nmat;
   int tiles_k, tiles_j, planes;
@@ -39,6 +51,7 @@ struct BMSim {
 template <typename T>
 struct BMArgs {
   const BMSim<T> *sims;
+  const void *base;     // the address the descriptors' offsets count from
   const int *item_sim;  // grid of each work item
   int n_items;
   const int *step;  // steps taken since the tables were set, before this run
@@ -53,7 +66,9 @@ struct BMArgs {
 
 template <typename T, int RW, bool LOSSY, int CEM, int W>
 __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) void k_batch_march(
-    BMArgs<T> a, int q) {
+    BMArgs<T> a, int q, const T *__restrict__ rb, T *__restrict__ wb, const T *__restrict__ qrb,
+    T *__restrict__ qwb, const T *__restrict__ cb, const T *__restrict__ pb,
+    const unsigned char *__restrict__ mb, double *__restrict__ pacc) {
   constexpr int TJ = MarchShape<RW, W>::TJ;
   __shared__ MarchLds<T, RW, W> L;
   __shared__ Src<T> lsrc[1];
@@ -88,7 +103,7 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
     pr.comp = a.comp;
     pr.nf = a.nfreq;
     pr.ph = a.phase + row * a.nfreq * 2;
-    pr.acc = a.probe_acc + (size_t)s * a.nfreq * 2;
+    pr.acc = pacc + (size_t)s * a.nfreq * 2;
   }
   // the source count as the engine's flag (0 or 1, an int the compiler
   // cannot bound), so the sweep keeps its source loop a loop: with a count
@@ -96,15 +111,17 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
   // where it raises register pressure past the budget (VGPR spills)
   march_tile<T, RW, LOSSY, true, CEM, false, 2, W>(
       L, lsrc, a.amp ? d.has_src : 0, si0, si1, ltab, g, k0, j0, i0, i1, a.dtm_dl, a.dtm, a.inv_dl,
-      d.st[par], d.st[par ^ 1], d.coef, d.prof, d.psi[par], d.psi[par ^ 1], d.mat, d.nmat, 0.0,
-      ZCap{}, prb, pr);
+      rb + d.st[par], wb + d.st[par ^ 1], cb + d.coef, pb + d.prof, qrb + d.psi[par],
+      qwb + d.psi[par ^ 1], mb + d.mat, d.nmat, 0.0, ZCap{}, prb, pr);
 }
 
 // Launch one batched march step (instantiated in batch_march_*.hip).
 template <typename T, int RW, bool LOSSY, int CEM, int W>
 int launch_batch_march(const BMArgs<T> &a, int q, hipStream_t stream) {
+  const T *r = (const T *)a.base;
   hipLaunchKernelGGL((k_batch_march<T, RW, LOSSY, CEM, W>), dim3(a.n_items), dim3(kMarchThreads),
-                     0, stream, a, q);
+                     0, stream, a, q, r, (T *)r, r, (T *)r, r, r, (const unsigned char *)a.base,
+                     a.probe_acc);
   HF_HIP(hipGetLastError());
   return HFDTD_OK;
 }

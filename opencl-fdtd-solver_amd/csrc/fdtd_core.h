@@ -15,6 +15,11 @@ namespace hfdtd {
 // Grid geometry and slab offsets (kernel argument, passed by value).
 struct Geo {
   int nx, ny, nz, npml;
+  // elements between consecutive j rows of the volumes the march sweep
+  // addresses (and of its x- and y-stripe psi): nz for an engine's own
+  // slabs, nz rounded up to whole 128-byte rows in the batched march's
+  // padded copies
+  int sk;
   long long vs;                // elements between volumes of a slab
   int plen;                    // nx + ny + nz (profile slab row length)
   int psi_off[PSI_COUNT + 1];  // element offsets of the psi slots
@@ -25,6 +30,7 @@ inline Geo geo_of(const Engine *e) {
   g.nx = e->nx;
   g.ny = e->ny;
   g.nz = e->nz;
+  g.sk = e->nz;
   g.npml = e->npml;
   g.vs = (long long)e->vstride;
   g.plen = (int)e->prof_len();
@@ -571,7 +577,7 @@ __device__ __forceinline__ int dma(Rsrc rs, T *dst, const LaneOff<T> &vo, unsign
 // keep twelve offsets live in scalar registers.
 __device__ __forceinline__ unsigned psi_slot(const Geo &g, int s) {
   const unsigned n2 = 2u * (unsigned)g.npml;
-  const unsigned sx = n2 * (unsigned)(g.ny * g.nz), sy = (unsigned)g.nx * n2 * (unsigned)g.nz;
+  const unsigned sx = n2 * (unsigned)(g.ny * g.sk), sy = (unsigned)g.nx * n2 * (unsigned)g.sk;
   const unsigned sz = (unsigned)(g.nx * g.ny) * n2;
   const int t = s % 6;
   const unsigned in_group = t == 0 ? 0u : t == 1 ? sx : t == 2 ? 2 * sx : t == 3 ? 2 * sx + sy

@@ -295,7 +295,7 @@ struct March {
     else bst<T, NT>(r1, v, voff, (unsigned)c * vsb());
   }
   __device__ __forceinline__ unsigned plane_b(int i) const {
-    return (unsigned)min(i, g.nx - 1) * (unsigned)(g.ny * g.nz) * E;
+    return (unsigned)min(i, g.nx - 1) * (unsigned)(g.ny * g.sk) * E;
   }
   // byte offset of psi slot s plus a part given in elements
   __device__ __forceinline__ unsigned psib(int s, unsigned elems) const {
@@ -305,8 +305,8 @@ struct March {
     const int n = g.npml, ic = min(i, g.nx - 1);
     PsiPlane pp;
     pp.inx = n > 0 && (ic < n || ic >= g.nx - n);
-    pp.xs = pp.inx ? (unsigned)stripe_local(ic, n, g.nx) * (unsigned)(g.ny * g.nz) : 0u;
-    pp.ys = (unsigned)ic * 2u * (unsigned)n * (unsigned)g.nz;
+    pp.xs = pp.inx ? (unsigned)stripe_local(ic, n, g.nx) * (unsigned)(g.ny * g.sk) : 0u;
+    pp.ys = (unsigned)ic * 2u * (unsigned)n * (unsigned)g.sk;
     pp.zs = (unsigned)ic * (unsigned)g.ny * 2u * (unsigned)n;
     pp.dead = sized && i >= iend;
     return pp;
@@ -358,7 +358,7 @@ struct March {
   }
 
   __device__ __forceinline__ unsigned mat_plane(int i, bool dead) const {
-    return dead ? kDropOffset : (unsigned)min(i, g.nx - 1) * (unsigned)(g.ny * g.nz) * CB;
+    return dead ? kDropOffset : (unsigned)min(i, g.nx - 1) * (unsigned)(g.ny * g.sk) * CB;
   }
   // one value of the material index or of the eps_r volume
   __device__ __forceinline__ unsigned ld_mat(unsigned voff, unsigned soff) const {
@@ -447,7 +447,7 @@ struct March {
   __device__ __forceinline__ void step(MarchLds<T, RW, W> &L, int buf, int i, MarchH<T> (&hc)[S],
                                        MarchH<T> (&hn)[S], MarchE<T> (&ec)[RW],
                                        MarchE<T> (&en)[RW]) {
-    const unsigned sxy = (unsigned)(g.ny * g.nz);
+    const unsigned sxy = (unsigned)(g.ny * g.sk);
     const unsigned vs = vsb();
     const int ic = min(i, g.nx - 1);
     const bool live = i < iend;
@@ -703,15 +703,15 @@ struct March {
       MarchSlot &m = sl[q];
       m.j = min(max(j, 0), g.ny - 1);
       m.k = min(max(k, 0), g.nz - 1);
-      m.o4 = (unsigned)(m.j * g.nz + m.k) * E;
-      m.o1 = (unsigned)(m.j * g.nz + m.k) * CB;
-      m.o1j = (unsigned)(min(m.j + 1, g.ny - 1) * g.nz + m.k) * CB;
-      m.o1k = (unsigned)(m.j * g.nz + min(m.k + 1, g.nz - 1)) * CB;
+      m.o4 = (unsigned)(m.j * g.sk + m.k) * E;
+      m.o1 = (unsigned)(m.j * g.sk + m.k) * CB;
+      m.o1j = (unsigned)(min(m.j + 1, g.ny - 1) * g.sk + m.k) * CB;
+      m.o1k = (unsigned)(m.j * g.sk + min(m.k + 1, g.nz - 1)) * CB;
       // the halo slot's psi only for lanes that compute a needed H
       const bool hpsi = q < RW || (need & 0b011000) != 0;
       m.iny = Y && n > 0 && hpsi && (m.j < n || m.j >= g.ny - n);
       m.inz = Z && n > 0 && hpsi && (m.k < n || m.k >= g.nz - n);
-      m.qy4 = m.iny ? (unsigned)(stripe_local(m.j, n, g.ny) * g.nz + m.k) * E : kDropOffset;
+      m.qy4 = m.iny ? (unsigned)(stripe_local(m.j, n, g.ny) * g.sk + m.k) * E : kDropOffset;
       m.qz4 = m.inz ? (unsigned)(m.j * 2 * n + stripe_local(m.k, n, g.nz)) * E : kDropOffset;
       m.lh = hr * LC + hc;
       m.le = er * LC + ec;
@@ -738,7 +738,7 @@ struct March {
     // dropped by the hardware when the slab resource carries its size (slabs
     // below 2 GB), else this lane's slot in the volume padding
     if (sized) padb = kDropOffset;
-    else padb = ((unsigned)((size_t)g.nx * g.ny * g.nz) + (unsigned)lane) * E;
+    else padb = ((unsigned)((size_t)g.nx * g.ny * g.sk) + (unsigned)lane) * E;
     // the halo slot's field loads a lane's role does not read are dropped
     // (only with sized resources; a dropped load returns 0, and on a dead
     // plane, whose plane offset is already past the resource, the sum may
@@ -773,7 +773,7 @@ struct March {
     // i-1 term on plane 0)
     {
       const int im = max(i0 - 1, 0);
-      const unsigned pm = plane_b(im), nzb = (unsigned)g.nz * E;
+      const unsigned pm = plane_b(im), nzb = (unsigned)g.sk * E;
       const PsiPlane pp = psi_plane(im);
       const Prof1<T> pxh = prof1(g, prof, 1, 0, im);
 #pragma unroll
@@ -902,7 +902,7 @@ __device__ __forceinline__ void march_tile(MarchLds<T, RW, W> &L, const Src<T> *
   constexpr unsigned long long cb = CEM == kCeEps ? 4ull : 1ull;
   const Rsrc rmat = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<void *>(mat), (short)0,
-      CEM != kCeVolumes ? (int)min(((unsigned long long)g.nx * g.ny * g.nz + 64ull) * cb, 0x7fffffffull)
+      CEM != kCeVolumes ? (int)min(((unsigned long long)g.nx * g.ny * g.sk + 64ull) * cb, 0x7fffffffull)
                         : 0,
       0x00020000);
   // the z-face record (FP32)

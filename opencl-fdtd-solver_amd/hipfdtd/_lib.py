@@ -167,6 +167,7 @@ EXPORTS = {
     "hfdtd_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "hfdtd_batch_set_path": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_batch_set_march": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "hfdtd_batch_set_march_padding": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_batch_set_split": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_adjoint_create": (
         C.c_int,

@@ -124,6 +124,14 @@ class BatchedFDTD:
         _lib.check(self._lib.hfdtd_batch_set_march(self._handle, int(rows), int(planes), int(run),
                                                    int(width)), "batch_set_march")  # fmt: skip
 
+    def set_march_padding(self, on: bool) -> None:
+        """Row padding of the 'march' path (on by default): on a batch of
+        material-indexed FP32 engines, engines whose nz is not a multiple of
+        32 run on copies with rows padded to whole 128-byte lines, filled
+        before each run and written back after it (identical results)."""
+        _lib.check(self._lib.hfdtd_batch_set_march_padding(self._handle, int(bool(on))),
+                   "batch_set_march_padding")  # fmt: skip
+
     def run(self, n_steps: int, graph_steps: int = 0):
         """Advance every simulation ``n_steps`` steps (one launch per step, or
         graph replays of ``graph_steps`` steps)."""

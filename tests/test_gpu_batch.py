@@ -128,7 +128,7 @@ def test_batch_matches_serial_engines(graph_steps, path):
 
 
 def _march_batch_case(shapes, media, n_steps, graph_steps, rows, planes, run, fp64=False,
-                      lossy=(), split=None, width=0):
+                      lossy=(), split=None, width=0, pad=None):
     """Serial march engines and one batched march run of the same grids,
     sources and probes; returns [(serial fields, serial probe DFT)], the
     batch's sims and its probe DFTs."""
@@ -166,6 +166,8 @@ def _march_batch_case(shapes, media, n_steps, graph_steps, rows, planes, run, fp
     batch = hip().batch.BatchedFDTD(sims)
     batch.set_path("march")
     batch.set_march(rows, planes, run, width)
+    if pad is not None:
+        batch.set_march_padding(pad)
     batch.set_sources([("z", src_box(s)) for s in shapes], [waveform(f) for _, f in media],
                       n_steps)  # fmt: skip
     batch.set_probes([probe_cell(s) for s in shapes], "Ez", freqs, n_steps)
@@ -175,6 +177,32 @@ def _march_batch_case(shapes, media, n_steps, graph_steps, rows, planes, run, fp
     dfts = batch.probe_dfts()
     batch.close()
     return serial, sims, dfts
+
+
+@pytest.mark.parametrize("pad,width,graph_steps", [(True, 64, 10), (False, 64, 10),
+                                                    (True, 32, 0), (True, 64, 7)])  # fmt: skip
+def test_batch_march_row_padding(pad, width, graph_steps):
+    """Engines whose nz is not a multiple of 32 run on copies with padded
+    rows (material-indexed FP32 batches): fields and probe DFTs equal the
+    serial march engines bit for bit with the padding on and off, over two
+    runs whose first has an odd step count (the copies are written back at
+    the parity the run ends on)."""
+    rng = np.random.default_rng(17)
+    shapes = [(40, 37, 33), (29, 44, 47), (36, 30, 64), (31, 35, 95), (27, 41, 100)]
+    media = [sphere_case(rng, sh) for sh in shapes]
+    print(f"[batch-pad] pad={pad} width={width} graph_steps={graph_steps}", flush=True)
+    serial, sims, dfts = _march_batch_case(shapes, media, 31, graph_steps, 1, 16, 16,
+                                           width=width, pad=pad)  # fmt: skip
+    for q, sim in enumerate(sims):
+        assert sim.material_count() > 0
+        ref_f, ref_p = serial[q]
+        got = sim.fields()
+        for n in FIELDS:
+            np.testing.assert_array_equal(got[n], ref_f[n], err_msg=f"grid {q} {n}")
+        np.testing.assert_array_equal(dfts[q], ref_p, err_msg=f"grid {q} probe")
+        assert sim.step_num == 31
+        sim.run(2)  # the engine steps on from the written-back state
+        sim.close()
 
 
 @pytest.mark.parametrize("medium,graph_steps,rows,planes,run", [
