@@ -48,7 +48,7 @@ BATCH_ROWS_FP32 = (2, 4, 6, 8)
 BATCH_ROWS_FP64 = (2, 4)
 BATCH_PLANES = (8, 16, 32, 64)
 # batched march candidates: (rows per wavefront, planes per chunk, tile width)
-BATCH_MARCH_FP32 = ((1, 24, 64), (1, 32, 64), (2, 32, 64), (1, 24, 32), (1, 32, 32))
+BATCH_MARCH_FP32 = ((1, 16, 64), (1, 24, 64), (1, 32, 64), (2, 32, 64), (1, 16, 32), (1, 24, 32))
 BATCH_MARCH_FP64 = ((1, 16, 64), (1, 32, 64), (1, 48, 64))
 # batched split sweeps: runs of 256 cells per work item
 BATCH_SPLIT_RUNS = (1, 2, 4)
