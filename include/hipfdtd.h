@@ -379,12 +379,13 @@ int hfdtd_batch_set_split(hfdtd_batch *b, int runs);
  * wavefront, FP32 only); 0 keeps the current value (defaults 1, 32, 16,
  * 64) */
 int hfdtd_batch_set_march(hfdtd_batch *b, int rows, int planes, int run, int width);
-/* HFDTD_PATH_MARCH row padding (1 = on, the default; 0 = off): in a batch
- * whose engines all read a material index (FP32), an engine whose nz is not
- * a multiple of 32 runs on a copy of its state, psi and index with rows
- * padded to whole 128-byte lines (filled before a run, written back after
- * it; the results are identical).  Unaligned rows make a 64-column tile row
- * touch three cache lines instead of two. */
+/* HFDTD_PATH_MARCH row padding (1 = on, the default; 0 = off).  The march
+ * path runs every engine on a copy in a batch-owned arena (state and psi of
+ * both parities, coefficients, profiles, material index; filled before a
+ * run, written back after it).  With padding on, the copy's rows are padded
+ * to whole 128-byte lines where nz does not fill them: unaligned rows make a
+ * 64-column tile row touch three cache lines instead of two.  Results are
+ * identical either way. */
 int hfdtd_batch_set_march_padding(hfdtd_batch *b, int on);
 
 /* ── adjoint gradient of probe-DFT objectives ─────────────────────────────

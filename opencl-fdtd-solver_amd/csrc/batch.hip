@@ -343,7 +343,15 @@ __global__ void k_step_add(int *step, int n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(step, n);
 }
 
-// One row copy of the batched march's padding: rows x w elements from src
+// One engine's part of the batched march's arena: its geometry there (row
+// stride sk, volume stride vs) and the element offsets of its pieces.
+struct ArenaPart {
+  Geo g;
+  long long st, psi, coef, prof, mat;  // mat in bytes
+  long long slab, psi_len;             // elements of one state slab, one psi slab
+};
+
+// One row copy between an engine and the arena: rows x w floats from src
 // (row pitch sp) to dst (row pitch dp).
 struct PadJob {
   const float *src;
@@ -356,6 +364,14 @@ struct PadJob {
 __global__ __launch_bounds__(256) void k_pad_rows(const PadJob *__restrict__ jobs) {
   const PadJob J = jobs[blockIdx.y];
   const long long n = J.rows * J.w;
+  if (n < 0x7fffffffll) {  // 32-bit index arithmetic
+    const unsigned w = (unsigned)J.w, nn = (unsigned)n;
+    for (unsigned t = blockIdx.x * 256u + threadIdx.x; t < nn; t += gridDim.x * 256u) {
+      const unsigned r = t / w, c = t - r * w;
+      J.dst[(size_t)r * J.dp + c] = J.src[(size_t)r * J.sp + c];
+    }
+    return;
+  }
   for (long long t = blockIdx.x * 256ll + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
     const long long r = t / J.w;
     const int c = (int)(t - r * J.w);
@@ -370,7 +386,6 @@ struct hfdtd_batch {
   hipStream_t stream = nullptr;
   int fp64 = 0;
   void *d_desc = nullptr;
-  const void *mbase = nullptr;  // march path: the address descriptor offsets count from
   int *d_tile_sim = nullptr;
   int n_tiles = 0;
   int *d_hook_items = nullptr;  // split path: items holding a source or probe
@@ -395,15 +410,15 @@ struct hfdtd_batch {
   int mce = 0;
   bool mlossy = false;
   bool lossy = false;
-  // march path, material-indexed FP32 batches: engines whose nz is not a
-  // multiple of 32 run on copies with rows padded to whole 128-byte lines
-  // (state and psi of both parities, then the material bytes), filled from
-  // the engine before a run and written back after it
+  // march path: every engine runs on a copy in one batch-owned arena
+  // (state and psi of both parities, coefficients, profiles, material
+  // index), filled from the engine before a run and written back after it;
+  // rows padded to whole 128-byte lines where nz does not fill them (mpad)
   int mpad = 1;
-  std::vector<float *> pad_buf;          // per engine, null: runs on its own slabs
-  std::vector<long long> pad_slab;       // elements of one padded state slab
-  std::vector<long long> pad_psi;        // elements of one padded psi slab
-  hfdtd::PadJob *d_jobs = nullptr;       // the row copies of one direction
+  void *arena = nullptr;
+  size_t arena_bytes = 0;
+  std::vector<hfdtd::ArenaPart> parts;  // per engine
+  hfdtd::PadJob *d_jobs = nullptr;      // the row copies of one direction
   int n_jobs_cap = 0;
   hipGraphExec_t gexec = nullptr;
 };
@@ -547,42 +562,77 @@ static long long psi_slot_host(const Geo &g, int s) {
   return (long long)(s / 6) * 2 * (sx + sy + sz) + in_group;
 }
 
-// Geometry of an engine's padded copy: rows of whole 128-byte lines.
-static Geo padded_geo(const Engine *e) {
-  Geo g = geo_of(e);
-  g.sk = (e->nz + 31) / 32 * 32;
-  g.vs = ((long long)e->nx * e->ny * g.sk + 64 + 63) / 64 * 64;
-  return g;
+// The arena's layout (in elements of T, every piece aligned to 64): per
+// engine the state slabs of both parities, the psi slabs of both parities,
+// the coefficient slab (ce volumes kernels), the profiles and the material
+// index (indexed kernels).  Rows are padded to whole 128-byte lines when
+// pad is set and nz does not fill them.
+template <typename T>
+static size_t arena_layout(hfdtd_batch *b, bool indexed) {
+  const size_t n = b->eng.size();
+  b->parts.assign(n, ArenaPart{});
+  const long long per_line = 128 / (long long)sizeof(T);
+  auto up = [](long long v) { return (v + 63) / 64 * 64; };
+  long long at = 0;
+  for (size_t s = 0; s < n; ++s) {
+    const Engine *e = b->eng[s];
+    ArenaPart &a = b->parts[s];
+    a.g = geo_of(e);
+    if (b->mpad) a.g.sk = (int)((e->nz + per_line - 1) / per_line * per_line);
+    a.g.vs = up((long long)e->nx * e->ny * a.g.sk + 64);
+    a.slab = 6 * a.g.vs;
+    a.psi_len = up(psi_slot_host(a.g, PSI_COUNT) + 64);
+    a.st = at;
+    at += 2 * a.slab;
+    a.psi = at;
+    at += 2 * a.psi_len;
+    a.coef = at;
+    if (!indexed) at += a.slab;
+    a.prof = at;
+    at += up(6 * (long long)e->prof_len());
+    a.mat = at * (long long)sizeof(T);
+    if (indexed) at += up(((long long)e->nx * e->ny * a.g.sk + 64 + sizeof(T) - 1) / sizeof(T));
+  }
+  return (size_t)at * sizeof(T);
 }
 
-// Row copies between engine s's slabs of parity ep (its state[cur ^ ep],
-// psi[cur ^ ep]) and its padded copy of parity q, in the direction to_pad.
-static void pad_jobs(const hfdtd_batch *b, int s, int ep, int q, bool to_pad,
-                     std::vector<PadJob> &jobs) {
+// Row copies between engine s's slabs of parity ep (its state[cur ^ ep] and
+// psi[cur ^ ep]) and its arena part of parity q, in the direction to_arena;
+// with coef, also its coefficient slab and profiles (into the arena).
+template <typename T>
+static void arena_jobs(const hfdtd_batch *b, int s, int ep, int q, bool to_arena, bool coef,
+                       std::vector<PadJob> &jobs) {
   const Engine *e = b->eng[s];
-  const Geo gp = padded_geo(e);
-  float *buf = b->pad_buf[s];
-  const long long slab = b->pad_slab[s], psi = b->pad_psi[s];
-  auto add = [&](float *eng_p, float *pad_p, long long rows, int w, int ew, int pw) {
+  const ArenaPart &a = b->parts[s];
+  T *base = (T *)b->arena;
+  constexpr int F = (int)(sizeof(T) / sizeof(float));  // floats per element
+  auto add = [&](const void *eng_p, T *ar_p, long long rows, long long w, long long ew,
+                 long long aw) {
     if (rows <= 0 || w <= 0) return;
-    jobs.push_back(to_pad ? PadJob{eng_p, pad_p, rows, w, ew, pw}
-                          : PadJob{pad_p, eng_p, rows, w, pw, ew});
+    float *ep_ = (float *)const_cast<void *>(eng_p), *ap = (float *)ar_p;
+    jobs.push_back(to_arena ? PadJob{ep_, ap, rows, (int)(w * F), (int)(ew * F), (int)(aw * F)}
+                            : PadJob{ap, ep_, rows, (int)(w * F), (int)(aw * F), (int)(ew * F)});
   };
   const long long rows = (long long)e->nx * e->ny;
   for (int c = 0; c < 6; ++c)
-    add((float *)e->state[e->cur ^ ep] + (size_t)c * e->vstride, buf + q * slab + c * gp.vs, rows,
-        e->nz, e->nz, gp.sk);
-  const int n2 = 2 * e->npml;
+    add((const T *)e->state[e->cur ^ ep] + (size_t)c * e->vstride,
+        base + a.st + q * a.slab + c * a.g.vs, rows, e->nz, e->nz, a.g.sk);
+  const long long n2 = 2ll * e->npml;
   for (int t = 0; t < PSI_COUNT; ++t) {
-    float *ep_ = (float *)e->psi[e->cur ^ ep] + e->psi_off[t];
-    float *pp = buf + 2 * slab + q * psi + psi_slot_host(gp, t);
+    const T *eng_p = (const T *)e->psi[e->cur ^ ep] + e->psi_off[t];
+    T *ar_p = base + a.psi + q * a.psi_len + psi_slot_host(a.g, t);
     const int u = t % 6;
-    if (u < 2) add(ep_, pp, (long long)n2 * e->ny, e->nz, e->nz, gp.sk);
-    else if (u < 4) add(ep_, pp, (long long)e->nx * n2, e->nz, e->nz, gp.sk);
-    else {
-      const int w = (int)(rows * n2);
-      add(ep_, pp, 1, w, w, w);
-    }
+    if (u < 2) add(eng_p, ar_p, n2 * e->ny, e->nz, e->nz, a.g.sk);
+    else if (u < 4) add(eng_p, ar_p, (long long)e->nx * n2, e->nz, e->nz, a.g.sk);
+    else add(eng_p, ar_p, 1, rows * n2, rows * n2, rows * n2);
+  }
+  if (coef) {
+    if (a.prof != a.coef)  // the ce volumes kernels read the coefficient slab
+      for (int c = 0; c < 6; ++c)
+        add((const T *)e->coef + (size_t)c * e->vstride, base + a.coef + c * a.g.vs, rows, e->nz,
+            e->nz, a.g.sk);
+    const long long pl = 6 * (long long)e->prof_len();
+    add(e->prof, base + a.prof, 1, pl, pl, pl);
   }
 }
 
@@ -599,79 +649,58 @@ static int pad_run(hfdtd_batch *b, const std::vector<PadJob> &jobs) {
                         hipMemcpyHostToDevice, b->stream));
   long long most = 0;
   for (const PadJob &j : jobs) most = std::max(most, j.rows * j.w);
-  const unsigned gx = (unsigned)std::min<long long>(ceil_div(most, 256), 2048);
+  const unsigned gx = (unsigned)std::min<long long>((most + 255) / 256, 2048);
   hipLaunchKernelGGL(k_pad_rows, dim3(gx, (unsigned)jobs.size()), dim3(256), 0, b->stream,
                      (const PadJob *)b->d_jobs);
   HF_HIP(hipGetLastError());
   return HFDTD_OK;
 }
 
-// Padded copies for the engines of an indexed FP32 march batch whose nz is
-// not a multiple of 32 (rows then start mid-line and a 64-column tile row
-// touches three lines instead of two): allocated once, filled here from
-// the engines' current state.  Returns with pad_buf[s] null for the
-// engines that run on their own slabs.
-static int pad_setup(hfdtd_batch *b, bool indexed) {
-  const int n = (int)b->eng.size();
-  b->pad_buf.resize(n, nullptr);
-  b->pad_slab.resize(n, 0);
-  b->pad_psi.resize(n, 0);
+// Lay the arena out, (re)allocate it when it grows (zeroed: pad columns stay
+// zero) and fill it from the engines' current state, coefficients,
+// profiles and material index.
+template <typename T>
+static int arena_setup(hfdtd_batch *b, bool indexed) {
+  const size_t bytes = arena_layout<T>(b, indexed);
+  if (bytes > b->arena_bytes) {
+    if (b->arena) HF_HIP(hipFree(b->arena));
+    b->arena = nullptr;
+    b->arena_bytes = 0;
+    HF_HIP(hipMalloc(&b->arena, bytes));
+    HF_HIP(hipMemsetAsync(b->arena, 0, bytes, b->stream));
+    b->arena_bytes = bytes;
+  }
   std::vector<PadJob> jobs;
-  for (int s = 0; s < n; ++s) {
-    Engine *e = b->eng[s];
-    bool want = b->mpad && indexed && e->elem == 4 && e->nz % 32 != 0;
-    const Geo g0 = geo_of(e), gp = padded_geo(e);
-    for (int q = 0; want && q <= PSI_COUNT; ++q) want = psi_slot_host(g0, q) == e->psi_off[q];
-    const long long slab = 6 * gp.vs, psi = psi_slot_host(gp, PSI_COUNT) + 64;
-    want = want && slab * 4 < 0x80000000ll && psi * 4 < 0x80000000ll;
-    if (!want) {
-      if (b->pad_buf[s]) HF_HIP(hipFree(b->pad_buf[s]));
-      b->pad_buf[s] = nullptr;
-      continue;
-    }
-    const long long mat_bytes = ((long long)e->nx * e->ny * gp.sk + 64 + 3) / 4 * 4;
-    if (!b->pad_buf[s]) {
-      const size_t bytes = (size_t)(2 * slab + 2 * psi) * 4 + (size_t)mat_bytes;
-      HF_HIP(hipMalloc((void **)&b->pad_buf[s], bytes));
-      HF_HIP(hipMemsetAsync(b->pad_buf[s], 0, bytes, b->stream));  // pad columns stay zero
-    }
-    b->pad_slab[s] = slab;
-    b->pad_psi[s] = psi;
-    HF_HIP(hipMemcpy2DAsync(b->pad_buf[s] + 2 * slab + 2 * psi, (size_t)gp.sk, e->mat,
-                            (size_t)e->nz, (size_t)e->nz, (size_t)e->nx * e->ny,
-                            hipMemcpyDeviceToDevice, b->stream));
-    pad_jobs(b, s, 0, 0, true, jobs);
+  for (int s = 0; s < (int)b->eng.size(); ++s) {
+    const Engine *e = b->eng[s];
+    const ArenaPart &a = b->parts[s];
+    if (indexed)
+      HF_HIP(hipMemcpy2DAsync((char *)b->arena + a.mat, (size_t)a.g.sk, e->mat, (size_t)e->nz,
+                              (size_t)e->nz, (size_t)e->nx * e->ny, hipMemcpyDeviceToDevice,
+                              b->stream));
+    arena_jobs<T>(b, s, 0, 0, true, true, jobs);
   }
   return pad_run(b, jobs);
 }
 
-// After n steps: the padded engines' state and psi of parity n & 1 back into
-// their own slabs of that parity (the caller then flips cur when n is odd).
-static int pad_writeback(hfdtd_batch *b, int n_steps) {
+// After n steps: the state and psi of parity n & 1 back into each engine's
+// slabs of that parity (the caller then flips cur when n is odd).
+template <typename T>
+static int arena_writeback(hfdtd_batch *b, int n_steps) {
   if (n_steps <= 0) return HFDTD_OK;
   std::vector<PadJob> jobs;
-  for (int s = 0; s < (int)b->pad_buf.size(); ++s)
-    if (b->pad_buf[s]) pad_jobs(b, s, n_steps & 1, n_steps & 1, false, jobs);
+  for (int s = 0; s < (int)b->eng.size(); ++s)
+    arena_jobs<T>(b, s, n_steps & 1, n_steps & 1, false, false, jobs);
   int rc = pad_run(b, jobs);
   if (rc) return rc;
   HF_HIP(hipStreamSynchronize(b->stream));
   return HFDTD_OK;
 }
 
-// Element offset of p from base (both device addresses; every slab is
-// aligned to its element size).
-template <typename U>
-static long long elem_off(const void *base, const void *p) {
-  return ((long long)(uintptr_t)p - (long long)(uintptr_t)base) / (long long)sizeof(U);
-}
-
 template <typename T>
 static int batch_build_march(hfdtd_batch *b) {
   const int n = (int)b->eng.size();
   const int RW = b->mrows;
-  // the launch's base address: every slab is an element offset from it
-  const void *base = b->eng[0]->state[0];
-  b->mbase = base;
   std::vector<BMSim<T>> desc(n);
   std::vector<int> item_sim;
   int begin = 0;
@@ -688,29 +717,25 @@ static int batch_build_march(hfdtd_batch *b) {
     all_mat = all_mat && e->ce_source() == kCeIndex;
   }
   b->mce = all_mat ? kCeIndex : kCeVolumes;
-  int prc = pad_setup(b, all_mat);
+  arena_layout<T>(b, all_mat);
+  for (const ArenaPart &a : b->parts)  // the sweep's 32-bit slab offsets
+    if ((double)a.slab * sizeof(T) >= 4294967296.0 || (double)a.psi_len * sizeof(T) >= 2147483648.0)
+      return fail(HFDTD_EUNSUPPORTED,
+                  "batched march tiles need field slabs below 4 GB and psi slabs below 2 GB");
+  int prc = arena_setup<T>(b, all_mat);
   if (prc) return prc;
   for (int s = 0; s < n; ++s) {
     Engine *e = b->eng[s];
+    const ArenaPart &a = b->parts[s];
     BMSim<T> &d = desc[s];
-    d.coef = elem_off<T>(base, e->coef);
-    d.prof = elem_off<T>(base, e->prof);
-    if (float *buf = b->pad_buf[s]) {  // the padded copy (indexed FP32: no coefficient reads)
-      const long long slab = b->pad_slab[s], psi = b->pad_psi[s];
-      d.g = padded_geo(e);
-      d.st[0] = elem_off<T>(base, buf);
-      d.st[1] = elem_off<T>(base, buf + slab);
-      d.psi[0] = elem_off<T>(base, buf + 2 * slab);
-      d.psi[1] = elem_off<T>(base, buf + 2 * slab + psi);
-      d.mat = elem_off<unsigned char>(base, buf + 2 * slab + 2 * psi);
-    } else {
-      d.g = geo_of(e);
-      d.st[0] = elem_off<T>(base, e->state[e->cur]);
-      d.st[1] = elem_off<T>(base, e->state[e->cur ^ 1]);
-      d.psi[0] = elem_off<T>(base, e->psi[e->cur]);
-      d.psi[1] = elem_off<T>(base, e->psi[e->cur ^ 1]);
-      d.mat = all_mat ? elem_off<unsigned char>(base, e->mat) : 0;
-    }
+    d.g = a.g;
+    d.st[0] = a.st;
+    d.st[1] = a.st + a.slab;
+    d.psi[0] = a.psi;
+    d.psi[1] = a.psi + a.psi_len;
+    d.coef = a.coef;
+    d.prof = a.prof;
+    d.mat = a.mat;
     d.mtab = all_mat ? e->mtab : nullptr;
     d.nmat = all_mat ? e->nmat : 0;
     d.planes = (b->mplanes + 1) / 2 * 2;
@@ -761,7 +786,7 @@ static BMArgs<T> batch_march_args(const hfdtd_batch *b) {
   const Engine *e = b->eng[0];
   BMArgs<T> a;
   a.sims = (const BMSim<T> *)b->d_desc;
-  a.base = b->mbase;
+  a.base = b->arena;
   a.item_sim = b->d_tile_sim;
   a.n_items = b->n_tiles;
   a.step = b->d_step;
@@ -927,7 +952,7 @@ static int batch_run_t(hfdtd_batch *b, int n_steps, int graph_steps) {
     const BMArgs<T> a = batch_march_args<T>(b);
     rc = batch_run_steps(b, n_steps, graph_steps,
                          [&](int q) { return batch_launch_march<T>(b, a, q); });
-    if (rc == HFDTD_OK) rc = pad_writeback(b, n_steps);
+    if (rc == HFDTD_OK) rc = arena_writeback<T>(b, n_steps);
   } else {
     if ((rc = batch_build<T>(b))) return rc;
     const BatchArgs<T> a = batch_args<T>(b);
@@ -1057,8 +1082,7 @@ int hfdtd_batch_destroy(hfdtd_batch *b) {
                   (void *)b->d_step, (void *)b->d_amp,
                   (void *)b->d_phase, (void *)b->d_probe_acc, (void *)b->d_jobs})
     if (p) (void)hipFree(p);
-  for (float *p : b->pad_buf)
-    if (p) (void)hipFree(p);
+  if (b->arena) (void)hipFree(b->arena);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return HFDTD_OK;

@@ -12,16 +12,22 @@
 // small kernel advances *step after the run: every launch argument stays
 // fixed across graph replays.
 //
-// A grid's slabs are addressed as element offsets from base pointers that
-// come in as separate __restrict__ kernel arguments (one per role: state
-// read, state written, psi read, psi written, coefficients, profiles,
-// material index, probe sums), all holding the same address.  The compiler
-// can then tell the tile's stores from its loads, as in the single-engine
-// sweep, and keeps the next plane's loads in flight across the current
-// plane's stores; with pointers read from the descriptors it must assume
-// they alias, and the sweep waited for every load (15 % slower on one
-// 256^3 grid).  The roles never share a slab within a launch: the state and
-// psi written are the other parity of the ones read.
+// The sweep runs on the batch's arena, one allocation holding a copy of
+// every grid's state and psi (both parities), coefficients, profiles and
+// material index, filled from the engines before a run and written back
+// after it; rows are padded to whole 128-byte lines where nz does not fill
+// them (an unaligned row makes a 64-column tile row touch three lines
+// instead of two).  A grid's pieces are element offsets into the arena,
+// whose address comes in as separate __restrict__ kernel arguments, one
+// per role (state read, state written, psi read, psi written,
+// coefficients, profiles, material index); the probe sums come in as one
+// more.  The compiler can then tell the tile's stores from its loads, as in
+// the single-engine sweep, and keeps the next plane's loads in flight
+// across the current plane's stores: with pointers read from the
+// descriptors it must assume they alias, and the sweep waited for every
+// load (15 % slower on one 256^3 grid).  Within a launch the roles never
+// share memory: the state and psi written are the other parity of the
+// ones read.
 #pragma once
 
 #include "march.h"
@@ -32,7 +38,7 @@ namespace hfdtd {
 template <typename T>
 struct BMSim {
   Geo g;
-  // element offsets from the launch's base address (bytes for mat)
+  // element offsets into the arena (bytes for mat)
   long long st[2];   // state slab, by step parity (relative to step0)
   long long psi[2];  // psi slab, by step parity
   long long coef;
@@ -51,7 +57,7 @@ struct BMSim {
 template <typename T>
 struct BMArgs {
   const BMSim<T> *sims;
-  const void *base;     // the address the descriptors' offsets count from
+  const void *base;     // the arena
   const int *item_sim;  // grid of each work item
   int n_items;
   const int *step;  // steps taken since the tables were set, before this run

@@ -125,10 +125,11 @@ class BatchedFDTD:
                                                    int(width)), "batch_set_march")  # fmt: skip
 
     def set_march_padding(self, on: bool) -> None:
-        """Row padding of the 'march' path (on by default): on a batch of
-        material-indexed FP32 engines, engines whose nz is not a multiple of
-        32 run on copies with rows padded to whole 128-byte lines, filled
-        before each run and written back after it (identical results)."""
+        """Row padding of the 'march' path (on by default).  That path runs
+        every engine on a copy in a batch-owned arena, filled before each
+        run and written back after it; with padding the copy's rows are
+        padded to whole 128-byte lines where nz does not fill them
+        (identical results)."""
         _lib.check(self._lib.hfdtd_batch_set_march_padding(self._handle, int(bool(on))),
                    "batch_set_march_padding")  # fmt: skip
 
