@@ -60,11 +60,20 @@ __device__ __forceinline__ T pick3(bool c0, bool c1, T a, T b, T c) {
                                    (__builtin_bit_cast(U, c) & m2));
 }
 
+// The LDS copy of the sources seen through an LDS pointer: volatile reads
+// through a generic pointer become flat loads, which count against the
+// vector-memory counter, so each one waited for every field load in flight.
+template <typename T>
+__device__ __forceinline__ const volatile __attribute__((address_space(3))) Src<T> *
+lds_src_ptr(const Src<T> *base, int t) {
+  return (const volatile __attribute__((address_space(3))) Src<T> *)(base + t);
+}
+
 // Source t of the LDS copy, read field by field through volatile loads (so
 // the compiler re-reads them where they are used instead of holding them).
 template <typename T>
 __device__ __forceinline__ Src<T> lds_src(const Src<T> *base, int t) {
-  const volatile Src<T> *p = base + t;
+  const volatile __attribute__((address_space(3))) Src<T> *p = lds_src_ptr(base, t);
   Src<T> s;
   s.comp = p->comp;
   s.mode = p->mode;
@@ -572,7 +581,7 @@ struct March {
       // order, as the reference adds them between its H and E kernels; the
       // plane and row tests are wave-uniform
       for (int t = 0; t < (ic >= src_i0 && ic < src_i1 ? nsrc : 0); ++t) {
-        const volatile Src<T> *vp = lsrc + t;
+        const auto *vp = lds_src_ptr(lsrc, t);
         if (ic < vp->ci0 || ic >= vp->ci1 || m.j < vp->cj0 || m.j >= vp->cj1) continue;
         const Src<T> sq = lds_src(lsrc, t);
         if (m.k >= sq.ck0 && m.k < sq.ck1) {

@@ -230,7 +230,7 @@ struct Pair {
                                               int q, T cx, T cy, T cz, T &ex, T &ey,
                                               T &ez) const {
     for (int t = 0; t < (i >= ilo && i < ihi ? n : 0); ++t) {
-      const volatile Src<T> *vp = ls + t;
+      const auto *vp = lds_src_ptr(ls, t);
       if (i < vp->ci0 || i >= vp->ci1 || jr[q] < vp->cj0 || jr[q] >= vp->cj1) continue;
       const Src<T> sq = lds_src(ls, t);
       if (kc >= sq.ck0 && kc < sq.ck1) {
