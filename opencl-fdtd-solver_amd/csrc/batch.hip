@@ -650,9 +650,12 @@ static int pad_run(hfdtd_batch *b, const std::vector<PadJob> &jobs) {
   long long most = 0;
   for (const PadJob &j : jobs) most = std::max(most, j.rows * j.w);
   const unsigned gx = (unsigned)std::min<long long>((most + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_pad_rows, dim3(gx, (unsigned)jobs.size()), dim3(256), 0, b->stream,
-                     (const PadJob *)b->d_jobs);
-  HF_HIP(hipGetLastError());
+  for (size_t j0 = 0; j0 < jobs.size(); j0 += 65535) {  // grid y of one launch: 65535 jobs
+    const unsigned gy = (unsigned)std::min<size_t>(jobs.size() - j0, 65535);
+    hipLaunchKernelGGL(k_pad_rows, dim3(gx, gy), dim3(256), 0, b->stream,
+                       (const PadJob *)b->d_jobs + j0);
+    HF_HIP(hipGetLastError());
+  }
   return HFDTD_OK;
 }
 
