@@ -424,9 +424,11 @@ class HIPFDTD(SourceMonitorMixin):
         batched march tiles (the march sweep's arithmetic, which the fused
         and pair sweeps share).  An engine too large for batched march tiles
         (field slab of 4 GB or more, psi slab of 2 GB or more) runs batched
-        split sweeps, equal to its own path to rounding.  Registered source
-        and monitor callbacks cannot run inside a graph: the call refuses an
-        engine that has any."""
+        split sweeps, equal to its own path to rounding.  Batched march
+        tiles run on a copy of the engine (state, psi, coefficients) made
+        for the call and written back at its end, which needs that much
+        free device memory.  Registered source and monitor callbacks cannot
+        run inside a graph: the call refuses an engine that has any."""
         if self._sources or self._monitors:
             raise ValueError("run_device runs no callbacks: clear_sources() / clear_monitors() "
                              "first, or use run()")
