@@ -186,6 +186,23 @@ int hfdtd_set_march_tiling(hfdtd_engine *e, int rows_per_wave, int planes_i, int
 int hfdtd_set_march_width(hfdtd_engine *e, int width);
 int hfdtd_set_march_addressing(hfdtd_engine *e, int per_volume);
 
+/* Row-padded runs (extension, no counterpart in the reference: its kernels
+ * index the dense layout).  Where nz does not fill whole 128-byte lines
+ * (FP32: nz not a multiple of 32; FP64: of 16), a march tile's rows start
+ * mid-line and each 64-column row segment touches three cache lines instead
+ * of two.  hfdtd_padded_begin copies the state, psi and the coefficient
+ * source the march sweep reads into a copy with padded rows; HFDTD_PATH_MARCH
+ * steps with at most 4 sources then run on the copy, with identical results.
+ * Every other call on the engine, its monitors, probes, plane DFTs, adjoint
+ * or batches first ends the run (the copy's state and psi written back), as
+ * does hfdtd_padded_end.  A no-op (not live) where rows already fill whole
+ * lines; HFDTD_EUNSUPPORTED while a monitor records z faces in the sweep or
+ * when the copy would exceed the sweep's slab limits; HFDTD_ENOMEM without
+ * memory for the copy (one more state and psi, both parities). */
+int hfdtd_padded_begin(hfdtd_engine *e);
+int hfdtd_padded_end(hfdtd_engine *e);
+/* 1 while a row-padded run is live, else 0 */
+int hfdtd_padded_live(const hfdtd_engine *e, int *live);
 /* Two Yee steps in one launch (the engine.py:724-735 step twice, minus the
  * callbacks): src_a[0..na) are the sources of the first step, src_b[0..nb)
  * those of the second, at most 4 each.  Results equal two hfdtd_step calls

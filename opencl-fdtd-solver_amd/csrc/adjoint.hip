@@ -416,6 +416,7 @@ size_t psi_elems(const Engine *e) { return e->psi_off[PSI_COUNT] + 64; }
 template <typename T>
 int record_t(hfdtd_adjoint *a, int slot) {
   Engine *e = a->e;
+  HF_SETTLE(e);
   T *dst = (T *)a->hist + (size_t)slot * 3 * a->box.n;
   hipLaunchKernelGGL(k_adj_record<T>, dim3(ceil_div(a->box.n, 256)), dim3(256), 0, e->stream,
                      geo_of(e), a->box, (const T *)e->state[e->cur], dst);
@@ -430,6 +431,7 @@ int record(hfdtd_adjoint *a, int slot) {
 template <typename T>
 int probe_t(hfdtd_adjoint *a, long long n) {
   Engine *e = a->e;
+  HF_SETTLE(e);
   const int th = a->np * a->nf;
   hipLaunchKernelGGL(k_adj_probe_dft<T>, dim3(ceil_div(th, 64)), dim3(64), 0, e->stream,
                      geo_of(e), a->np, a->nf, a->pcomp, a->pidx, a->om, (double)n * e->dt, e->dt,
@@ -441,6 +443,7 @@ int probe_t(hfdtd_adjoint *a, long long n) {
 template <typename T>
 int inject_t(hfdtd_adjoint *a, long long n) {
   Engine *e = a->e;
+  HF_SETTLE(e);
   hipLaunchKernelGGL(k_adj_inject<T>, dim3(ceil_div(a->np, 64)), dim3(64), 0, e->stream,
                      geo_of(e), a->np, a->nf, a->pcomp, a->pidx, a->om, a->coef,
                      (double)n * e->dt, e->dt, (const T *)e->coef, (T *)a->adj);
@@ -451,6 +454,7 @@ int inject_t(hfdtd_adjoint *a, long long n) {
 template <typename T>
 int reverse_step_t(hfdtd_adjoint *a, long long n, int slot) {
   Engine *e = a->e;
+  HF_SETTLE(e);
   const Geo g = geo_of(e);
   const size_t hs = (size_t)3 * a->box.n;
   const T *h1 = (const T *)a->hist + (size_t)slot * hs;
@@ -480,6 +484,7 @@ int hfdtd_adjoint_create(hfdtd_engine *h, const int *design_box, int hist_slots,
   if (!h || !design_box || !out) return fail(HFDTD_EINVAL, "NULL argument");
   *out = nullptr;
   Engine *e = &h->s;
+  HF_SETTLE(e);
   const int ext[3] = {e->nx, e->ny, e->nz};
   for (int a = 0; a < 3; ++a) {
     const int lo = design_box[2 * a], hi = design_box[2 * a + 1];
@@ -548,6 +553,7 @@ int hfdtd_adjoint_set_probes(hfdtd_adjoint *a, int n, const int *ijkc, int nfreq
                              const double *omega) {
   if (!a || n < 1 || !ijkc || nfreq < 1 || !omega) return fail(HFDTD_EINVAL, "bad probe list");
   Engine *e = a->e;
+  HF_SETTLE(e);
   std::vector<int> comp(n);
   std::vector<long long> where(n);
   for (int p = 0; p < n; ++p) {
@@ -598,6 +604,7 @@ int hfdtd_adjoint_probe_reset(hfdtd_adjoint *a) {
 int hfdtd_adjoint_probe_read(hfdtd_adjoint *a, double *out) {
   if (!a || !out || !a->np) return fail(HFDTD_EINVAL, "no probes set");
   Engine *e = a->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   HF_HIP(hipMemcpyAsync(out, a->F, (size_t)a->np * a->nf * 2 * sizeof(double),
                         hipMemcpyDeviceToHost, e->stream));
@@ -608,6 +615,7 @@ int hfdtd_adjoint_probe_read(hfdtd_adjoint *a, double *out) {
 int hfdtd_adjoint_set_objective_grad(hfdtd_adjoint *a, const double *coef) {
   if (!a || !coef || !a->np) return fail(HFDTD_EINVAL, "no probes set");
   Engine *e = a->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   HF_HIP(hipMemcpyAsync(a->coef, coef, (size_t)a->np * a->nf * 2 * sizeof(double),
                         hipMemcpyHostToDevice, e->stream));
@@ -619,6 +627,7 @@ int hfdtd_adjoint_set_objective_grad(hfdtd_adjoint *a, const double *coef) {
 int hfdtd_adjoint_save(hfdtd_adjoint *a, int c) {
   if (!a || c < 0 || c >= a->n_ckpt) return fail(HFDTD_EINVAL, "checkpoint index out of range");
   Engine *e = a->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   char *dst = (char *)a->ckpt + (size_t)c * (a->ckpt_state + a->ckpt_psi);
   HF_HIP(hipMemcpyAsync(dst, e->state[e->cur], a->ckpt_state, hipMemcpyDeviceToDevice,
@@ -631,6 +640,7 @@ int hfdtd_adjoint_save(hfdtd_adjoint *a, int c) {
 int hfdtd_adjoint_restore(hfdtd_adjoint *a, int c) {
   if (!a || c < 0 || c >= a->n_ckpt) return fail(HFDTD_EINVAL, "checkpoint index out of range");
   Engine *e = a->e;
+  HF_SETTLE(e);
   e->touch();
   HF_HIP(hipSetDevice(e->device));
   const char *src = (const char *)a->ckpt + (size_t)c * (a->ckpt_state + a->ckpt_psi);
@@ -658,6 +668,7 @@ int hfdtd_adjoint_forward(hfdtd_adjoint *a, const hfdtd_source *src, int nsrc,
     return fail(HFDTD_EINVAL, "recorded steps exceed the history slots");
   if (probe && !a->np) return fail(HFDTD_EINVAL, "no probes set");
   Engine *e = a->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   std::vector<hfdtd_source> s(src, src + nsrc);
   const bool one = path != HFDTD_PATH_SPLIT && e->fused_fits();
@@ -684,6 +695,7 @@ int hfdtd_adjoint_begin(hfdtd_adjoint *a, int64_t n_last) {
   if (!a || !a->np || !a->coef_set)
     return fail(HFDTD_EINVAL, "set the probes and the objective gradient first");
   Engine *e = a->e;
+  HF_SETTLE(e);
   e->touch();
   if (!e->lossless)
     return fail(HFDTD_EUNSUPPORTED, "the adjoint gradient needs a lossless grid (no sigma)");
@@ -706,6 +718,7 @@ int hfdtd_adjoint_reverse(hfdtd_adjoint *a, int64_t n_hi, int64_t n_lo, int slot
   if (slot_hi >= a->hist_slots || slot_hi - (n_hi - n_lo) < 0)
     return fail(HFDTD_EINVAL, "history slots do not cover the reverse range");
   Engine *e = a->e;
+  HF_SETTLE(e);
   e->touch();
   HF_HIP(hipSetDevice(e->device));
   for (long long n = n_hi; n > n_lo; --n) {
@@ -719,6 +732,7 @@ int hfdtd_adjoint_reverse(hfdtd_adjoint *a, int64_t n_hi, int64_t n_lo, int slot
 int hfdtd_adjoint_read_grad(hfdtd_adjoint *a, double *out) {
   if (!a || !out) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = a->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   HF_HIP(hipMemcpyAsync(out, a->G, (size_t)3 * a->box.n * sizeof(double), hipMemcpyDeviceToHost,
                         e->stream));
@@ -729,6 +743,7 @@ int hfdtd_adjoint_read_grad(hfdtd_adjoint *a, double *out) {
 int hfdtd_adjoint_read_volume(hfdtd_adjoint *a, int which, void *host) {
   if (!a || !host || which < 0 || which > 5) return fail(HFDTD_EINVAL, "bad volume id");
   Engine *e = a->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   HF_HIP(hipMemcpyAsync(host, (const char *)a->adj + (size_t)which * e->vstride * e->elem,
                         e->ncell * e->elem, hipMemcpyDeviceToHost, e->stream));

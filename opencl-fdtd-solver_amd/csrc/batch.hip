@@ -351,15 +351,6 @@ struct ArenaPart {
   long long slab, psi_len;             // elements of one state slab, one psi slab
 };
 
-// One row copy between an engine and the arena: rows x w floats from src
-// (row pitch sp) to dst (row pitch dp).
-struct PadJob {
-  const float *src;
-  float *dst;
-  long long rows;
-  int w, sp, dp;
-};
-
 // Job blockIdx.y, grid-stride over its elements.
 __global__ __launch_bounds__(256) void k_pad_rows(const PadJob *__restrict__ jobs) {
   const PadJob J = jobs[blockIdx.y];
@@ -551,16 +542,8 @@ static int batch_build(hfdtd_batch *b) {
 // March path: per-grid descriptors (parity origin = current step) and the
 // work list of march tiles.  The kernel reads the material index when every
 // grid has one, else the ce volumes (with ca when some grid is lossy).
-// Host copy of psi_slot (fdtd_core.h): element offset of psi slot s.
-static long long psi_slot_host(const Geo &g, int s) {
-  const long long n2 = 2ll * g.npml;
-  const long long sx = n2 * g.ny * g.sk, sy = (long long)g.nx * n2 * g.sk;
-  const long long sz = (long long)g.nx * g.ny * n2;
-  const int t = s % 6;
-  const long long in_group = t == 0 ? 0 : t == 1 ? sx : t == 2 ? 2 * sx : t == 3 ? 2 * sx + sy
-                             : t == 4 ? 2 * (sx + sy) : 2 * (sx + sy) + sz;
-  return (long long)(s / 6) * 2 * (sx + sy + sz) + in_group;
-}
+// Element offset of psi slot s in a psi slab of geometry g.
+static long long psi_slot_host(const Geo &g, int s) { return (long long)psi_slot(g, s); }
 
 // The arena's layout (in elements of T, every piece aligned to 64): per
 // engine the state slabs of both parities, the psi slabs of both parities,
@@ -636,27 +619,31 @@ static void arena_jobs(const hfdtd_batch *b, int s, int ep, int q, bool to_arena
   }
 }
 
-static int pad_run(hfdtd_batch *b, const std::vector<PadJob> &jobs) {
+int run_pad_jobs(hipStream_t stream, const std::vector<PadJob> &jobs, PadJob *&d_jobs, int &cap) {
   if (jobs.empty()) return HFDTD_OK;
-  if ((int)jobs.size() > b->n_jobs_cap) {
-    if (b->d_jobs) HF_HIP(hipFree(b->d_jobs));
-    b->d_jobs = nullptr;
-    b->n_jobs_cap = 0;
-    HF_HIP(hipMalloc((void **)&b->d_jobs, sizeof(PadJob) * jobs.size()));
-    b->n_jobs_cap = (int)jobs.size();
+  if ((int)jobs.size() > cap) {
+    if (d_jobs) HF_HIP(hipFree(d_jobs));
+    d_jobs = nullptr;
+    cap = 0;
+    HF_HIP(hipMalloc((void **)&d_jobs, sizeof(PadJob) * jobs.size()));
+    cap = (int)jobs.size();
   }
-  HF_HIP(hipMemcpyAsync(b->d_jobs, jobs.data(), sizeof(PadJob) * jobs.size(),
-                        hipMemcpyHostToDevice, b->stream));
+  HF_HIP(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(PadJob) * jobs.size(), hipMemcpyHostToDevice,
+                        stream));
   long long most = 0;
   for (const PadJob &j : jobs) most = std::max(most, j.rows * j.w);
   const unsigned gx = (unsigned)std::min<long long>((most + 255) / 256, 2048);
   for (size_t j0 = 0; j0 < jobs.size(); j0 += 65535) {  // grid y of one launch: 65535 jobs
     const unsigned gy = (unsigned)std::min<size_t>(jobs.size() - j0, 65535);
-    hipLaunchKernelGGL(k_pad_rows, dim3(gx, gy), dim3(256), 0, b->stream,
-                       (const PadJob *)b->d_jobs + j0);
+    hipLaunchKernelGGL(k_pad_rows, dim3(gx, gy), dim3(256), 0, stream,
+                       (const PadJob *)d_jobs + j0);
     HF_HIP(hipGetLastError());
   }
   return HFDTD_OK;
+}
+
+static int pad_run(hfdtd_batch *b, const std::vector<PadJob> &jobs) {
+  return run_pad_jobs(b->stream, jobs, b->d_jobs, b->n_jobs_cap);
 }
 
 // Lay the arena out, (re)allocate it when it grows (zeroed: pad columns stay
@@ -980,6 +967,7 @@ int hfdtd_probe_create(hfdtd_engine *h, int i, int j, int k, int comp, int nfreq
   if (!h || !out) return fail(HFDTD_EINVAL, "NULL argument");
   *out = nullptr;
   Engine *e = &h->s;
+  HF_SETTLE(e);
   if (i < 0 || i >= e->nx || j < 0 || j >= e->ny || k < 0 || k >= e->nz)
     return fail(HFDTD_EINVAL, "probe cell outside the grid");
   if (comp < 0 || comp > 5 || nfreq < 1) return fail(HFDTD_EINVAL, "bad component or nfreq");
@@ -1017,6 +1005,7 @@ int hfdtd_probe_destroy(hfdtd_probe *p) {
 int hfdtd_probe_accumulate(hfdtd_probe *p, const double *phase) {
   if (!p || !phase) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = p->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   const size_t idx = ((size_t)p->i * e->ny + p->j) * e->nz + p->k;
   for (int f0 = 0; f0 < p->nfreq; f0 += kProbeFreqPerLaunch) {
@@ -1160,6 +1149,7 @@ int hfdtd_batch_run(hfdtd_batch *b, int n_steps, int graph_steps) {
   if (b->nfreq > 0 && b->d_probe_acc == nullptr) return fail(HFDTD_EINVAL, "probes not set");
   HF_HIP(hipSetDevice(b->eng[0]->device));
   for (Engine *e : b->eng) {
+    HF_SETTLE(e);
     HF_HIP(hipStreamSynchronize(e->stream));
     e->touch();
   }

@@ -40,6 +40,7 @@ enum PsiSlot {
 // Axis of the stripe each slot lives in (0 = x, 1 = y, 2 = z).
 constexpr int kPsiAxis[PSI_COUNT] = {0, 0, 1, 1, 2, 2, 0, 0, 1, 1, 2, 2};
 
+constexpr int kMarchSources = 4;  // sources the march sweep applies inside the sweep
 constexpr int kMatMax = 48;  // materials of an indexed grid at most (march sweep)
 
 // Where the march sweep takes an edge's ce from on lossless FP32 grids: the
@@ -79,6 +80,9 @@ struct Engine {
   double dtm_dl = 0, dtm = 0, inv_dl = 0;  // folded on the host in FP64
   size_t ncell = 0;
   size_t vstride = 0;  // elements between volumes of a slab (ncell + 64, rounded up to 64)
+  // elements between j rows of the volumes the march sweep addresses: nz, or
+  // the padded row length during a row-padded run
+  int sk = 0;
   size_t elem = 4;     // bytes per value
   hipStream_t stream = nullptr;
 
@@ -141,6 +145,21 @@ struct Engine {
   }
 
   uint64_t bytes = 0;
+
+  // Row-padded run (hfdtd_padded_begin / hfdtd_padded_end): the march sweep
+  // steps a copy of the state and psi (and of the ce source it reads) whose
+  // rows are padded to whole 128-byte lines; state[], psi[], vstride and sk
+  // point at the copy meanwhile, and the engine's own slabs are kept here.
+  struct PaddedRun {
+    bool live = false;
+    void *buf = nullptr;
+    size_t bytes = 0;
+    void *state[2] = {}, *psi[2] = {};
+    void *coef = nullptr, *mat = nullptr, *epsv = nullptr;
+    size_t vstride = 0;
+    struct PadJob *d_jobs = nullptr;
+    int jobs_cap = 0;
+  } prun;
 
   // Advanced by every call that changes E, H or psi (or what field() reads);
   // a march step that writes the z-face record stamps it with the new value.
@@ -246,6 +265,25 @@ int launch_march_step(Engine *e, const hfdtd_source *src, int n);
 int launch_pair_steps(Engine *e, const hfdtd_source *sa, int na, const hfdtd_source *sb, int nb);
 // ensure the ping-pong partners exist (engine.hip)
 int ensure_alt(Engine *e);
+
+// One row copy between two layouts of a volume: rows x w floats from src
+// (row pitch sp) to dst (row pitch dp).
+struct PadJob {
+  const float *src;
+  float *dst;
+  long long rows;
+  int w, sp, dp;
+};
+// Run row copies on a stream (batch.hip); d_jobs / cap hold the device list.
+int run_pad_jobs(hipStream_t stream, const std::vector<PadJob> &jobs, PadJob *&d_jobs, int &cap);
+// End a row-padded run: the copy's state and psi back into the engine's own
+// slabs (engine.hip); a no-op when none is live.
+int end_padded_run(Engine *e);
+#define HF_SETTLE(e)                                     \
+  do {                                                   \
+    const int _src = ::hfdtd::end_padded_run(e);         \
+    if (_src) return _src;                               \
+  } while (0)
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 

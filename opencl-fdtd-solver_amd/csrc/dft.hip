@@ -173,6 +173,7 @@ static int direct_t(hfdtd_dft *d, const double *phase) {
     ph.im[f] = f < d->F ? A(phase[2 * f + 1]) : A(0);
   }
   Engine *e = d->e;
+  HF_SETTLE(e);
   hipLaunchKernelGGL((k_dft_direct<T, A, KAHAN>), dim3((unsigned)ceil_div(d->P, 256)), dim3(256),
                      0, e->stream, (const T *)e->field(0), (long long)e->vstride, d->pl,
                      d->P, d->F, ph, (A *)d->acc, (A *)d->cmp);
@@ -191,6 +192,7 @@ int hfdtd_dft_plane_create(hfdtd_engine *h, int axis, int index, int comp_mask, 
   if (!h || !out) return fail(HFDTD_EINVAL, "NULL argument");
   *out = nullptr;
   Engine *e = &h->s;
+  HF_SETTLE(e);
   if (axis < 0 || axis > 2) return fail(HFDTD_EINVAL, "axis must be 0 (x), 1 (y) or 2 (z)");
   const int nax = axis == 0 ? e->nx : (axis == 1 ? e->ny : e->nz);
   if (index < 0 || index >= nax) return fail(HFDTD_EINVAL, "plane index outside the grid");
@@ -263,6 +265,7 @@ int hfdtd_dft_destroy(hfdtd_dft *d) {
 int hfdtd_dft_accumulate(hfdtd_dft *d, const double *phase) {
   if (!d || !phase) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = d->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   if (d->mode == HFDTD_DFT_DIRECT) {
     if (e->fp64) return direct_t<double, double, false>(d, phase);

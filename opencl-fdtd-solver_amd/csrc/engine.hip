@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.h"
+#include "fdtd_core.h"
 
 namespace hfdtd {
 
@@ -158,6 +159,22 @@ static int dmalloc(Engine *e, void **p, size_t bytes) {
 static void release(Engine *e) {
   if (e->device >= 0) (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->prun.live) {  // the engine's own slabs back in place (their contents no longer matter)
+    for (int p = 0; p < 2; ++p) {
+      e->state[p] = e->prun.state[p];
+      e->psi[p] = e->prun.psi[p];
+    }
+    e->coef = e->prun.coef;
+    e->mat = e->prun.mat;
+    e->epsv = e->prun.epsv;
+    e->vstride = e->prun.vstride;
+    e->sk = e->nz;
+    e->prun.live = false;
+  }
+  if (e->prun.buf) (void)hipFree(e->prun.buf);
+  if (e->prun.d_jobs) (void)hipFree(e->prun.d_jobs);
+  e->prun.buf = nullptr;
+  e->prun.d_jobs = nullptr;
   auto fr = [](void *&p) {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -341,6 +358,148 @@ static void build_eps_volume(Engine *e, const S *host, const void *d_eps) {
   e->eps_ok = true;
 }
 
+// ── row-padded runs ────────────────────────────────────────────────────────
+// Where nz does not fill whole 128-byte lines, every row of a march tile
+// starts mid-line and its 64-column segment touches three lines instead of
+// two.  A row-padded run steps a copy whose rows are padded (the sweep
+// reads the row stride from Geo::sk); the engine's state[], psi[], the ce
+// source the sweep reads, vstride and sk point at the copy until it ends.
+
+// Row copies of the state and psi of parity p between the engine's own
+// slabs (dense rows) and the copy of geometry gp at buf (padded rows).
+static void padded_jobs(const Engine *e, const Geo &gp, char *buf, size_t slab, size_t psil, int p,
+                        bool to_copy, std::vector<PadJob> &jobs) {
+  const int F = (int)(e->elem / 4);  // floats per element
+  auto add = [&](const void *own, void *cp, long long rows, long long w, long long ow,
+                 long long cw) {
+    if (rows <= 0 || w <= 0) return;
+    float *o = (float *)const_cast<void *>(own), *c = (float *)cp;
+    jobs.push_back(to_copy ? PadJob{o, c, rows, (int)(w * F), (int)(ow * F), (int)(cw * F)}
+                           : PadJob{c, o, rows, (int)(w * F), (int)(cw * F), (int)(ow * F)});
+  };
+  const long long rows = (long long)e->nx * e->ny, n2 = 2ll * e->npml;
+  const void *own_state = e->prun.live ? e->prun.state[p] : e->state[p];
+  const void *own_psi = e->prun.live ? e->prun.psi[p] : e->psi[p];
+  const size_t ovs = e->prun.live ? e->prun.vstride : e->vstride;
+  for (int c = 0; c < 6; ++c)
+    add((const char *)own_state + (size_t)c * ovs * e->elem,
+        buf + ((size_t)p * slab + (size_t)c * gp.vs) * e->elem, rows, e->nz, e->nz, gp.sk);
+  for (int t = 0; t < PSI_COUNT; ++t) {
+    const void *o = (const char *)own_psi + e->psi_off[t] * e->elem;
+    void *c = buf + (2 * slab + (size_t)p * psil + psi_slot(gp, t)) * e->elem;
+    const int u = t % 6;
+    if (u < 2) add(o, c, n2 * e->ny, e->nz, e->nz, gp.sk);
+    else if (u < 4) add(o, c, (long long)e->nx * n2, e->nz, e->nz, gp.sk);
+    else add(o, c, 1, rows * n2, rows * n2, rows * n2);
+  }
+}
+
+static size_t padded_psi_len(const Geo &gp) {
+  return ((size_t)psi_slot(gp, PSI_COUNT) + 64 + 63) / 64 * 64;
+}
+
+int end_padded_run(Engine *e) {
+  if (!e->prun.live) return HFDTD_OK;
+  HF_HIP(hipSetDevice(e->device));
+  const Geo gp = geo_of(e);
+  const size_t slab = 6 * e->vstride, psil = padded_psi_len(gp);
+  std::vector<PadJob> jobs;
+  padded_jobs(e, gp, (char *)e->prun.buf, slab, psil, e->cur, false, jobs);
+  int rc = run_pad_jobs(e->stream, jobs, e->prun.d_jobs, e->prun.jobs_cap);
+  if (rc) return rc;
+  for (int p = 0; p < 2; ++p) {
+    e->state[p] = e->prun.state[p];
+    e->psi[p] = e->prun.psi[p];
+  }
+  e->coef = e->prun.coef;
+  e->mat = e->prun.mat;
+  e->epsv = e->prun.epsv;
+  e->vstride = e->prun.vstride;
+  e->sk = e->nz;
+  e->prun.live = false;
+  e->touch();
+  HF_HIP(hipStreamSynchronize(e->stream));
+  return HFDTD_OK;
+}
+
+// Start a row-padded run (see above); a no-op where the rows already fill
+// whole lines.
+static int begin_padded_run(Engine *e) {
+  if (e->prun.live) return HFDTD_OK;
+  const size_t per_line = 128 / e->elem;
+  if (e->nz % per_line == 0) return HFDTD_OK;
+  if (e->zrec.owner || e->view_shadow)
+    return fail(HFDTD_EUNSUPPORTED, "a row-padded run takes an engine whose monitors record "
+                                    "nothing inside the sweep");
+  HF_HIP(hipSetDevice(e->device));
+  Geo gp = geo_of(e);
+  gp.sk = (int)((e->nz + per_line - 1) / per_line * per_line);
+  gp.vs = ((long long)e->nx * e->ny * gp.sk + 64 + 63) / 64 * 64;
+  const size_t slab = 6 * (size_t)gp.vs, psil = padded_psi_len(gp);
+  const int ce = e->ce_source();
+  const size_t cells_p = (size_t)e->nx * e->ny * gp.sk + 64;
+  const size_t ce_elems = ce == kCeVolumes ? slab
+                          : ce == kCeEps   ? (cells_p * 4 + e->elem - 1) / e->elem
+                                           : (cells_p + e->elem - 1) / e->elem;
+  // the march sweep's 32-bit offsets: slabs below 4 GB, or volumes below
+  // 2 GB on per-volume addressing (FP32); psi slabs below 2 GB
+  const double E = (double)e->elem;
+  if ((6.0 * gp.vs * E >= 4294967296.0 && (e->fp64 || gp.vs * E >= 2147483648.0)) ||
+      psil * E >= 2147483648.0)
+    return fail(HFDTD_EUNSUPPORTED, "the padded copy exceeds the march sweep's slab limits");
+  const size_t bytes = (2 * slab + 2 * psil + (ce_elems + 63) / 64 * 64) * e->elem;
+  if (bytes > e->prun.bytes) {
+    if (e->prun.buf) HF_HIP(hipFree(e->prun.buf));
+    e->prun.buf = nullptr;
+    e->prun.bytes = 0;
+    const hipError_t err = hipMalloc(&e->prun.buf, bytes);
+    if (err != hipSuccess) {
+      (void)hipGetLastError();
+      e->prun.buf = nullptr;
+      return fail(HFDTD_ENOMEM, "no device memory for the row-padded copy");
+    }
+    HF_HIP(hipMemsetAsync(e->prun.buf, 0, bytes, e->stream));  // pad columns stay zero
+    e->prun.bytes = bytes;
+  }
+  int rc = ensure_alt(e);
+  if (rc) return rc;
+  char *buf = (char *)e->prun.buf;
+  char *cep = buf + (2 * slab + 2 * psil) * e->elem;
+  std::vector<PadJob> jobs;
+  padded_jobs(e, gp, buf, slab, psil, e->cur, true, jobs);
+  const long long rows = (long long)e->nx * e->ny;
+  if (ce == kCeVolumes) {
+    const int F = (int)(e->elem / 4);
+    for (int c = 0; c < 6; ++c)
+      jobs.push_back(PadJob{(const float *)e->coef_vol(c), (float *)(cep + (size_t)c * gp.vs * e->elem),
+                            rows, e->nz * F, e->nz * F, gp.sk * F});
+  } else if (ce == kCeEps) {
+    jobs.push_back(PadJob{(const float *)e->epsv, (float *)cep, rows, e->nz, e->nz, gp.sk});
+  } else {
+    HF_HIP(hipMemcpy2DAsync(cep, (size_t)gp.sk, e->mat, (size_t)e->nz, (size_t)e->nz, (size_t)rows,
+                            hipMemcpyDeviceToDevice, e->stream));
+  }
+  if ((rc = run_pad_jobs(e->stream, jobs, e->prun.d_jobs, e->prun.jobs_cap))) return rc;
+  for (int p = 0; p < 2; ++p) {
+    e->prun.state[p] = e->state[p];
+    e->prun.psi[p] = e->psi[p];
+    e->state[p] = buf + (size_t)p * slab * e->elem;
+    e->psi[p] = buf + (2 * slab + (size_t)p * psil) * e->elem;
+  }
+  e->prun.coef = e->coef;
+  e->prun.mat = e->mat;
+  e->prun.epsv = e->epsv;
+  e->prun.vstride = e->vstride;
+  if (ce == kCeVolumes) e->coef = cep;
+  else if (ce == kCeEps) e->epsv = cep;
+  else e->mat = cep;
+  e->vstride = (size_t)gp.vs;
+  e->sk = gp.sk;
+  e->prun.live = true;
+  e->touch();
+  return HFDTD_OK;
+}
+
 }  // namespace hfdtd
 
 using namespace hfdtd;
@@ -420,6 +579,7 @@ int hfdtd_create(int nx, int ny, int nz, int npml, double dl, double dt, int fp6
   e->nx = nx;
   e->ny = ny;
   e->nz = nz;
+  e->sk = nz;
   e->npml = npml;
   e->fp64 = fp64 ? 1 : 0;
   e->device = device;
@@ -533,6 +693,7 @@ static void *volume_ptr(Engine *e, int which) {
 int hfdtd_set_materials(hfdtd_engine *h, const void *eps_r, const void *sigma, int src_fp64) {
   if (!h || !eps_r) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = &h->s;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   const size_t sb = e->ncell * (src_fp64 ? 8 : 4);
   void *d_eps = nullptr, *d_sig = nullptr;
@@ -583,6 +744,7 @@ int hfdtd_set_materials(hfdtd_engine *h, const void *eps_r, const void *sigma, i
 int hfdtd_write_volume(hfdtd_engine *h, int which, const void *host) {
   if (!h || !host) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = &h->s;
+  HF_SETTLE(e);
   void *slot = volume_ptr(e, which);
   if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
   if (which >= HFDTD_CEX) e->mat_ok = e->eps_ok = false;  // coefficients no longer from eps_r
@@ -597,6 +759,7 @@ int hfdtd_write_volume(hfdtd_engine *h, int which, const void *host) {
 int hfdtd_read_volume(hfdtd_engine *h, int which, void *host) {
   if (!h || !host) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = &h->s;
+  HF_SETTLE(e);
   void *slot = volume_ptr(e, which);
   if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
   HF_HIP(hipSetDevice(e->device));
@@ -608,6 +771,7 @@ int hfdtd_read_volume(hfdtd_engine *h, int which, void *host) {
 int hfdtd_read_point(hfdtd_engine *h, int which, int64_t index, double *out) {
   if (!h || !out) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = &h->s;
+  HF_SETTLE(e);
   void *slot = volume_ptr(e, which);
   if (!slot) return fail(HFDTD_EINVAL, "volume id out of range");
   if (index < 0 || (size_t)index >= e->ncell) return fail(HFDTD_EINVAL, "index out of range");
@@ -631,6 +795,7 @@ int hfdtd_read_point(hfdtd_engine *h, int which, int64_t index, double *out) {
 int hfdtd_reset_fields(hfdtd_engine *h) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = &h->s;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   e->touch();
   int rc;
@@ -642,22 +807,27 @@ int hfdtd_reset_fields(hfdtd_engine *h) {
 
 int hfdtd_update_H(hfdtd_engine *h) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  HF_SETTLE(&h->s);
   return launch_update_H(&h->s);
 }
 
 int hfdtd_update_E(hfdtd_engine *h) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  HF_SETTLE(&h->s);
   return launch_update_E(&h->s);
 }
 
 int hfdtd_add_sources(hfdtd_engine *h, const hfdtd_source *src, int n) {
   if (!h || (n > 0 && !src) || n < 0) return fail(HFDTD_EINVAL, "bad source list");
+  HF_SETTLE(&h->s);
   return launch_sources(&h->s, src, n);
 }
 
 int hfdtd_step(hfdtd_engine *h, const hfdtd_source *src, int n, int path) {
   if (!h || (n > 0 && !src) || n < 0) return fail(HFDTD_EINVAL, "bad source list");
   Engine *e = &h->s;
+  // a row-padded run takes march steps whose sources the sweep applies
+  if (e->prun.live && (path != HFDTD_PATH_MARCH || n > kMarchSources)) HF_SETTLE(e);
   if (path == HFDTD_PATH_FUSED) return launch_fused_step(e, src, n);
   if (path == HFDTD_PATH_MARCH) return launch_march_step(e, src, n);
   if (path != HFDTD_PATH_SPLIT) return fail(HFDTD_EINVAL, "unknown step path");
@@ -671,13 +841,31 @@ int hfdtd_step_pair(hfdtd_engine *h, const hfdtd_source *src_a, int na, const hf
                     int nb) {
   if (!h || na < 0 || nb < 0 || (na > 0 && !src_a) || (nb > 0 && !src_b))
     return fail(HFDTD_EINVAL, "bad source list");
+  HF_SETTLE(&h->s);
   return launch_pair_steps(&h->s, src_a, na, src_b, nb);
+}
+
+int hfdtd_padded_begin(hfdtd_engine *h) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  return begin_padded_run(&h->s);
+}
+
+int hfdtd_padded_end(hfdtd_engine *h) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  return end_padded_run(&h->s);
+}
+
+int hfdtd_padded_live(const hfdtd_engine *h, int *live) {
+  if (!h || !live) return fail(HFDTD_EINVAL, "NULL argument");
+  *live = h->s.prun.live ? 1 : 0;
+  return HFDTD_OK;
 }
 
 int hfdtd_set_watch(hfdtd_engine *h, const int *boxes, int n) {
   if (!h || (n > 0 && !boxes)) return fail(HFDTD_EINVAL, "NULL argument");
   if (n < 0 || n > 8) return fail(HFDTD_EINVAL, "at most 8 watched boxes");
   Engine *e = &h->s;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   const int hi[3] = {e->nx, e->ny, e->nz};
   for (int b = 0; b < n; ++b) {

@@ -30,7 +30,7 @@ inline Geo geo_of(const Engine *e) {
   g.nx = e->nx;
   g.ny = e->ny;
   g.nz = e->nz;
-  g.sk = e->nz;
+  g.sk = e->sk;
   g.npml = e->npml;
   g.vs = (long long)e->vstride;
   g.plen = (int)e->prof_len();
@@ -575,7 +575,7 @@ __device__ __forceinline__ int dma(Rsrc rs, T *dst, const LaneOff<T> &vo, unsign
 // order H: Hy_x Hz_x Hx_y Hz_y Hx_z Hy_z, E: Ey_x Ez_x Ex_y Ez_y Ex_z Ey_z;
 // a slot spans its stripe pair).  Cheaper to rebuild from the grid than to
 // keep twelve offsets live in scalar registers.
-__device__ __forceinline__ unsigned psi_slot(const Geo &g, int s) {
+__host__ __device__ __forceinline__ unsigned psi_slot(const Geo &g, int s) {
   const unsigned n2 = 2u * (unsigned)g.npml;
   const unsigned sx = n2 * (unsigned)(g.ny * g.sk), sy = (unsigned)g.nx * n2 * (unsigned)g.sk;
   const unsigned sz = (unsigned)(g.nx * g.ny) * n2;

@@ -40,7 +40,7 @@
 namespace hfdtd {
 
 constexpr int kMarchThreads = 256;  // four wavefronts
-constexpr int kMarchSources = 4;    // sources applied inside the sweep (more: fix-up kernel)
+// kMarchSources (common.h): sources applied inside the sweep (more: fix-up kernel)
 
 // Sources of one step, passed to the sweep by value.  The block copies them
 // to LDS once and reads them there with volatile loads, so they occupy no

@@ -545,6 +545,7 @@ static int ensure_scratch(hfdtd_n2f *m, size_t n) {
 template <typename T, typename A, bool KAHAN>
 static int accumulate_t(hfdtd_n2f *m, const double *phase) {
   Engine *e = m->e;
+  HF_SETTLE(e);
   const Dims3 g{e->nx, e->ny, e->nz};
   const Box b = box_of(m);
   for (int f0 = 0; f0 < m->nfreq; f0 += kMaxFreqPerLaunch) {
@@ -584,6 +585,7 @@ static int n2f_flush(hfdtd_n2f *m) {
 
 static int gather_step(hfdtd_n2f *m, const double *phase) {
   Engine *e = m->e;
+  HF_SETTLE(e);
   PhaseRowEH ph;
   for (int q = 0; q < 32; ++q) {
     const int fr = q >> 1, c = q & 1;
@@ -670,6 +672,7 @@ template <typename G, typename A>
 static int farfield_t(hfdtd_n2f *m, int fr, const double *obs, int n_obs, double k_wave,
                       double eta0, int l_sign, double *eh, double *scratch) {
   Engine *e = m->e;
+  HF_SETTLE(e);
   const int chunk = kFarfieldChunk;
   const int nblk = std::max(1, (int)ceil_div(m->n_face, chunk));
   double *part = scratch;
@@ -733,6 +736,7 @@ int hfdtd_n2f_create(hfdtd_engine *h, const int box[6], int nfreq, int acc_mode,
   if (!h || !box || !out) return fail(HFDTD_EINVAL, "NULL argument");
   *out = nullptr;
   Engine *e = &h->s;
+  HF_SETTLE(e);
   if (nfreq < 1) return fail(HFDTD_EINVAL, "nfreq must be >= 1");
   if (acc_mode < HFDTD_ACC_REAL || acc_mode > HFDTD_ACC_KAHAN)
     return fail(HFDTD_EINVAL, "unknown accumulator mode");
@@ -799,6 +803,7 @@ int hfdtd_n2f_layout(const hfdtd_n2f *m, int *n_face, int offsets[6]) {
 int hfdtd_n2f_accumulate(hfdtd_n2f *m, const double *phase) {
   if (!m || !phase) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = m->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   if (m->K > 0) return gather_step(m, phase);
   if (e->fp64) return accumulate_t<double, double, false>(m, phase);
@@ -823,6 +828,7 @@ int hfdtd_n2f_snapshot(hfdtd_n2f *m) {
   int rc = n2f_flush(m);
   if (rc) return rc;
   Engine *e = m->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   const size_t bytes = (size_t)6 * m->nfreq * m->n_face * 2 * m->acc_elem;
   if (!m->snap) HF_HIP(hipMalloc(&m->snap, bytes));
@@ -837,6 +843,7 @@ int hfdtd_n2f_rel_change(hfdtd_n2f *m, int freq, double *num, double *den) {
   int rc = n2f_flush(m);
   if (rc) return rc;
   Engine *e = m->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   const int nblk = (int)ceil_div(m->n_face, 256);
   rc = ensure_scratch(m, (size_t)2 * nblk);
@@ -871,6 +878,7 @@ int hfdtd_n2f_read(hfdtd_n2f *m, int comp, int freq, double *host) {
   int rc = n2f_flush(m);
   if (rc) return rc;
   Engine *e = m->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   const size_t n2 = (size_t)m->n_face * 2;
   const size_t off = ((size_t)comp * m->nfreq + freq) * n2;
@@ -897,6 +905,7 @@ int hfdtd_n2f_farfield(hfdtd_n2f *m, int freq, const double *obs, int n_obs, dou
   int rc = n2f_flush(m);
   if (rc) return rc;
   Engine *e = m->e;
+  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   rc = ensure_scratch(m, farfield_scratch_doubles(m, n_obs));
   if (rc) return rc;

@@ -64,6 +64,9 @@ EXPORTS = {
         C.c_int,
         [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_char_p, C.c_int],
     ),
+    "hfdtd_padded_begin": (C.c_int, [C.c_void_p]),
+    "hfdtd_padded_end": (C.c_int, [C.c_void_p]),
+    "hfdtd_padded_live": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hfdtd_stream_probe": (
         C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, C.POINTER(C.c_double)],
     ),

@@ -119,6 +119,7 @@ class HIPFDTD(SourceMonitorMixin):
         self._watch_set = None
         self._h_done = False
         self._pending: list[_lib.Source] = []
+        self._row_padding = True  # run(): march steps on a row-padded copy where it pays
 
         self._lib = _lib.load()
         ndev = _lib.device_count()
@@ -393,9 +394,38 @@ class HIPFDTD(SourceMonitorMixin):
         for mon in self._monitors:
             mon(self)
 
+    def set_row_padding(self, on: bool) -> None:
+        """Row-padded march runs in ``run`` (on by default; extension).  Where
+        nz does not fill whole 128-byte lines, ``run`` on the march path
+        without monitors steps a copy of the state whose rows are padded to
+        whole lines (``hfdtd_padded_begin``), written back when the run
+        ends or anything else touches the engine; results are identical."""
+        self._row_padding = bool(on)
+
+    def _padded_begin(self) -> bool:
+        """Start a row-padded run where it applies; True when one is live."""
+        if not (self._row_padding and self.path == "march" and not self._monitors
+                and self.Nz % (128 // self.dtype.itemsize) != 0):  # fmt: skip
+            return False
+        if self._lib.hfdtd_padded_begin(self._handle) != _lib.OK:
+            return False  # no room for the copy, or past the sweep's limits: run as is
+        live = C.c_int(0)
+        _lib.check(self._lib.hfdtd_padded_live(self._handle, C.byref(live)), "padded_live")
+        return bool(live.value)
+
     def run(self, n_steps, progress_every=0):
-        """Run ``n_steps`` steps (on the 'pair' path two per launch where it can)."""
+        """Run ``n_steps`` steps (on the 'pair' path two per launch where it
+        can; on the march path without monitors, on a row-padded copy where
+        nz does not fill whole cache lines)."""
         n = int(n_steps)
+        padded = n > 1 and self._padded_begin()
+        try:
+            self._run(n, n_steps, progress_every)
+        finally:
+            if padded:
+                _lib.check(self._lib.hfdtd_padded_end(self._handle), "padded_end")
+
+    def _run(self, n, n_steps, progress_every):
         i = 0
         while i < n:
             if self.path == "pair" and n - i >= 2 and self._step_pair():
