@@ -1004,10 +1004,9 @@ int hfdtd_probe_destroy(hfdtd_probe *p) {
 
 int hfdtd_probe_accumulate(hfdtd_probe *p, const double *phase) {
   if (!p || !phase) return fail(HFDTD_EINVAL, "NULL argument");
-  Engine *e = p->e;
-  HF_SETTLE(e);
+  Engine *e = p->e;  // the field volume as the engine holds it (a row-padded run's too)
   HF_HIP(hipSetDevice(e->device));
-  const size_t idx = ((size_t)p->i * e->ny + p->j) * e->nz + p->k;
+  const size_t idx = ((size_t)p->i * e->ny + p->j) * e->sk + p->k;
   for (int f0 = 0; f0 < p->nfreq; f0 += kProbeFreqPerLaunch) {
     const int nf = std::min(kProbeFreqPerLaunch, p->nfreq - f0);
     ProbePhase ph;

@@ -428,9 +428,8 @@ static int begin_padded_run(Engine *e) {
   if (e->prun.live) return HFDTD_OK;
   const size_t per_line = 128 / e->elem;
   if (e->nz % per_line == 0) return HFDTD_OK;
-  if (e->zrec.owner || e->view_shadow)
-    return fail(HFDTD_EUNSUPPORTED, "a row-padded run takes an engine whose monitors record "
-                                    "nothing inside the sweep");
+  if (e->view_shadow)
+    return fail(HFDTD_EUNSUPPORTED, "a row-padded run does not take the pair sweep's view");
   HF_HIP(hipSetDevice(e->device));
   Geo gp = geo_of(e);
   gp.sk = (int)((e->nz + per_line - 1) / per_line * per_line);

@@ -79,6 +79,7 @@ __device__ __forceinline__ int decode_face(const Box &b, int s, int &i, int &j, 
 
 struct Dims3 {
   int nx, ny, nz;
+  int sk;  // elements between j rows of the field volumes (nz, or a row-padded run's)
 };
 
 template <typename T>
@@ -100,7 +101,7 @@ __device__ __forceinline__ void colocate_impl(const Dims3 &g, int f, int i, int 
                                               const T *Hx, const T *Hy, const T *Hz, T v[4],
                                               int c[4]) {
   auto at = [&](const T *F, int a, int b, int cc) {
-    return F[((size_t)a * g.ny + b) * g.nz + cc];
+    return F[((size_t)a * g.ny + b) * g.sk + cc];
   };
   if (f <= 1) {
     const int k1 = k + 1 < g.nz ? k + 1 : k;
@@ -544,9 +545,8 @@ static int ensure_scratch(hfdtd_n2f *m, size_t n) {
 
 template <typename T, typename A, bool KAHAN>
 static int accumulate_t(hfdtd_n2f *m, const double *phase) {
-  Engine *e = m->e;
-  HF_SETTLE(e);
-  const Dims3 g{e->nx, e->ny, e->nz};
+  Engine *e = m->e;  // the field volumes as the engine holds them (a row-padded run's too)
+  const Dims3 g{e->nx, e->ny, e->nz, e->sk};
   const Box b = box_of(m);
   for (int f0 = 0; f0 < m->nfreq; f0 += kMaxFreqPerLaunch) {
     const int nf = std::min(kMaxFreqPerLaunch, m->nfreq - f0);
@@ -584,15 +584,14 @@ static int n2f_flush(hfdtd_n2f *m) {
 }
 
 static int gather_step(hfdtd_n2f *m, const double *phase) {
-  Engine *e = m->e;
-  HF_SETTLE(e);
+  Engine *e = m->e;  // the field volumes as the engine holds them (a row-padded run's too)
   PhaseRowEH ph;
   for (int q = 0; q < 32; ++q) {
     const int fr = q >> 1, c = q & 1;
     ph.e[q] = fr < m->nfreq ? (float)phase[fr * 4 + c] : 0.0f;
     ph.h[q] = fr < m->nfreq ? (float)phase[fr * 4 + 2 + c] : 0.0f;
   }
-  const Dims3 g{e->nx, e->ny, e->nz};
+  const Dims3 g{e->nx, e->ny, e->nz, e->sk};
   ZRead zr{};
   if (e->zrec_valid(m)) {  // the last step recorded this box's z faces
     zr.buf = (const float4 *)e->zrec.buf;
@@ -803,7 +802,6 @@ int hfdtd_n2f_layout(const hfdtd_n2f *m, int *n_face, int offsets[6]) {
 int hfdtd_n2f_accumulate(hfdtd_n2f *m, const double *phase) {
   if (!m || !phase) return fail(HFDTD_EINVAL, "NULL argument");
   Engine *e = m->e;
-  HF_SETTLE(e);
   HF_HIP(hipSetDevice(e->device));
   if (m->K > 0) return gather_step(m, phase);
   if (e->fp64) return accumulate_t<double, double, false>(m, phase);

@@ -396,15 +396,18 @@ class HIPFDTD(SourceMonitorMixin):
 
     def set_row_padding(self, on: bool) -> None:
         """Row-padded march runs in ``run`` (on by default; extension).  Where
-        nz does not fill whole 128-byte lines, ``run`` on the march path
-        without monitors steps a copy of the state whose rows are padded to
-        whole lines (``hfdtd_padded_begin``), written back when the run
-        ends or anything else touches the engine; results are identical."""
+        nz does not fill whole 128-byte lines, ``run`` on the march path steps
+        a copy of the state whose rows are padded to whole lines
+        (``hfdtd_padded_begin``), written back when the run ends or anything
+        else touches the engine; results are identical.  Huygens-box
+        monitors and point probes sample the copy; with any other monitor
+        the run keeps the engine's own layout."""
         self._row_padding = bool(on)
 
     def _padded_begin(self) -> bool:
         """Start a row-padded run where it applies; True when one is live."""
-        if not (self._row_padding and self.path == "march" and not self._monitors
+        if not (self._row_padding and self.path == "march"
+                and all(getattr(m, "padded_ok", False) for m in self._monitors)
                 and self.Nz % (128 // self.dtype.itemsize) != 0):  # fmt: skip
             return False
         if self._lib.hfdtd_padded_begin(self._handle) != _lib.OK:
@@ -415,8 +418,8 @@ class HIPFDTD(SourceMonitorMixin):
 
     def run(self, n_steps, progress_every=0):
         """Run ``n_steps`` steps (on the 'pair' path two per launch where it
-        can; on the march path without monitors, on a row-padded copy where
-        nz does not fill whole cache lines)."""
+        can; on the march path, on a row-padded copy where nz does not fill
+        whole cache lines and every monitor samples such a copy)."""
         n = int(n_steps)
         padded = n > 1 and self._padded_begin()
         try:

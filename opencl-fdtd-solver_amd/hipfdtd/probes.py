@@ -50,6 +50,10 @@ class HIPPointProbe:
         self._buf = np.zeros(2 * len(self.freqs), dtype=np.float64)
         fdtd.add_monitor(self)
 
+    # its per-step call reads the field as the engine holds it, so a
+    # row-padded run keeps going while it samples (HIPFDTD.run)
+    padded_ok = True
+
     def __call__(self, fdtd):
         if self._phases is None:
             self._phases = np.exp(1j * self.omegas * fdtd.t) * fdtd.dt

@@ -125,3 +125,51 @@ def test_padded_run_through_the_c_abi():
     L.check(c._lib.hfdtd_padded_live(c._handle, C.byref(live)))
     assert live.value == 0  # 64 floats: rows already fill whole lines
     c.close()
+
+
+@pytest.mark.parametrize("buffer_steps,accum,probe,plane", [
+    (None, None, True, False), (0, "fp64", False, False), (0, "kahan", True, False),
+    (None, None, False, True)])  # fmt: skip
+def test_padded_run_with_box_monitor_and_probe(buffer_steps, accum, probe, plane):
+    """A Huygens box (its z faces recorded by the sweep when buffered) and a
+    point probe sample the padded copy: face DFTs, far field and probe DFT
+    equal the dense run's bit for bit.  With a plane DFT monitor the run keeps
+    the dense layout, with the same results."""
+    import hipfdtd.dft
+    import hipfdtd.probes
+
+    shape = (38, 42, 53)
+    print(f"[padded-mon] K={buffer_steps} accum={accum} probe={probe} plane={plane}", flush=True)
+    res = []
+    for pad in (False, True):
+        sim = hip().HIPFDTD(shape, 1e-3, npml=5, path="march")
+        eps, _ = _medium("index", shape)
+        sim.set_epsilon(eps)
+        sim.set_row_padding(pad)
+        lo, hi = (9, 10, 12), (29, 31, 40)
+        c = [0.5 * (a + b) * sim.dl for a, b in zip(lo, hi)]
+        size = [(b - a) * sim.dl for a, b in zip(lo, hi)]
+        kw = {} if buffer_steps is None else {"buffer_steps": buffer_steps}
+        mon = hip().HIPNear2FarMonitor(sim, c, size, freqs=[2.5e10, 3e10], accum=accum, **kw)
+        pr = hip().probes.HIPPointProbe(sim, (20, 21, 30), "Ez", [2.5e10, 3e10]) if probe else None
+        pl = (hip().dft.HIPPlaneDFTMonitor(sim, "z", 26, ("Ex", "Ey"), [3e10])
+              if plane else None)  # fmt: skip
+        if pad:
+            assert sim._padded_begin() == (not plane)
+            hip()._lib.check(sim._lib.hfdtd_padded_end(sim._handle))
+        nx, ny, nz = shape
+        sim.add_source(lambda f: f.add_source_Jx(nz // 2, fo.gaussian_sine(f.t, 3e10, 1.5e10),
+                                                 4, nx - 4, 4, ny - 4))  # fmt: skip
+        sim.run(23)
+        sim.run(18)
+        out = {k: mon.read_dft(k, 1) for k in FIELDS}
+        out["far"] = mon.get_farfields([(0.3, 0.2, 1.0), (0.0, 0.0, -1.0)], 0)
+        if pr is not None:
+            out["probe"] = pr.dft()
+        if pl is not None:
+            out["plane"] = pl.dft()
+        out.update({"f" + k: v for k, v in sim.fields().items()})
+        res.append(out)
+        sim.close()
+    for k in res[0]:
+        np.testing.assert_array_equal(res[1][k], res[0][k], err_msg=k)
