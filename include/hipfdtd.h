@@ -185,6 +185,14 @@ int hfdtd_set_march_tiling(hfdtd_engine *e, int rows_per_wave, int planes_i, int
  * (FP64 and larger grids keep 64).  Results are the same for every width. */
 int hfdtd_set_march_width(hfdtd_engine *e, int width);
 int hfdtd_set_march_addressing(hfdtd_engine *e, int per_volume);
+/* No counterpart in the reference.  1 (the default): a K-buffered
+ * near-to-far monitor's per-step gather that does not fill its K-step block
+ * is held by the engine, and the next march sweep (FP32, slabs below 4 GB)
+ * runs it in blocks after its own, in the same launch: both only read the
+ * state the step left (the sweep writes the other slab).  Any other call on
+ * the engine runs a held gather on its own first.  0: every gather is
+ * launched at once.  Results are the same either way. */
+int hfdtd_set_overlap(hfdtd_engine *e, int on);
 
 /* Row-padded runs (extension, no counterpart in the reference: its kernels
  * index the dense layout).  Where nz does not fill whole 128-byte lines

@@ -60,17 +60,24 @@ constexpr int kCeVolumes = 0, kCeIndex = 1, kCeEps = 2;
 // [face][column: k - 1 | k][i - i0][j - j0]; the monitor's gather then reads
 // them contiguously instead of one cache line per sample and field.  The
 // record matches the state while `epoch` equals the engine's state_epoch,
-// which every change of the state advances.
+// which every change of the state advances.  buf holds two records of
+// `bytes` each, one per slab parity: a sweep that writes state[p] writes
+// record p, so the gather of the previous step may still read the other.
 struct ZRec {
   const void *owner = nullptr;  // the monitor it serves (nullptr: off)
   int kf[2] = {};
   int i0 = 0, i1 = -1, j0 = 0, j1 = -1;
   float *buf = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;  // of one record
   uint64_t epoch = ~0ull;
   // quads per (face, column) array
   size_t quads() const { return (size_t)(i1 - i0 + 1) * (size_t)(j1 - j0 + 1); }
+  float *half(int parity) const {
+    return buf ? buf + (size_t)parity * (bytes / sizeof(float)) : nullptr;
+  }
 };
+
+struct GatherTail;  // n2f_gather.h
 
 struct Engine {
   int nx = 0, ny = 0, nz = 0, npml = 0;
@@ -165,6 +172,14 @@ struct Engine {
   // a march step that writes the z-face record stamps it with the new value.
   uint64_t state_epoch = 0;
   ZRec zrec;
+
+  // A K-buffered near-to-far monitor's gather of step n, held back so that
+  // the march sweep of step n + 1 runs it in blocks after its own (both read
+  // state[p] only; the sweep writes the other slab and the other record).
+  // Every other call on the engine first runs a held gather on its own
+  // (HF_SETTLE).  pgather is owned by n2f.hip.
+  bool fuse_gather = true;
+  GatherTail *pgather = nullptr;
   void touch() { ++state_epoch; }
   bool zrec_valid(const void *owner) const {
     return zrec.owner == owner && zrec.buf && zrec.epoch == state_epoch && !view_shadow;
@@ -279,10 +294,17 @@ int run_pad_jobs(hipStream_t stream, const std::vector<PadJob> &jobs, PadJob *&d
 // End a row-padded run: the copy's state and psi back into the engine's own
 // slabs (engine.hip); a no-op when none is live.
 int end_padded_run(Engine *e);
-#define HF_SETTLE(e)                                     \
-  do {                                                   \
-    const int _src = ::hfdtd::end_padded_run(e);         \
-    if (_src) return _src;                               \
+// The held gather (Engine::pgather; n2f.hip): flush_gather runs it on the
+// engine's stream if one is held; free_gather drops the holder.
+int flush_gather(Engine *e);
+void free_gather(Engine *e);
+// Before any entry point other than a march step and the per-step monitor
+// kernels: a held gather run, then the row-padded run ended.
+int settle(Engine *e);
+#define HF_SETTLE(e)                             \
+  do {                                           \
+    const int _src = ::hfdtd::settle(e);         \
+    if (_src) return _src;                       \
   } while (0)
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

@@ -159,6 +159,7 @@ static int dmalloc(Engine *e, void **p, size_t bytes) {
 static void release(Engine *e) {
   if (e->device >= 0) (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  free_gather(e);
   if (e->prun.live) {  // the engine's own slabs back in place (their contents no longer matter)
     for (int p = 0; p < 2; ++p) {
       e->state[p] = e->prun.state[p];
@@ -420,6 +421,12 @@ int end_padded_run(Engine *e) {
   e->touch();
   HF_HIP(hipStreamSynchronize(e->stream));
   return HFDTD_OK;
+}
+
+int settle(Engine *e) {
+  int rc = flush_gather(e);
+  if (rc) return rc;
+  return end_padded_run(e);
 }
 
 // Start a row-padded run (see above); a no-op where the rows already fill
@@ -825,8 +832,9 @@ int hfdtd_add_sources(hfdtd_engine *h, const hfdtd_source *src, int n) {
 int hfdtd_step(hfdtd_engine *h, const hfdtd_source *src, int n, int path) {
   if (!h || (n > 0 && !src) || n < 0) return fail(HFDTD_EINVAL, "bad source list");
   Engine *e = &h->s;
-  // a row-padded run takes march steps whose sources the sweep applies
-  if (e->prun.live && (path != HFDTD_PATH_MARCH || n > kMarchSources)) HF_SETTLE(e);
+  // a row-padded run takes march steps whose sources the sweep applies, and
+  // such a step takes a held gather into its launch
+  if (path != HFDTD_PATH_MARCH || n > kMarchSources) HF_SETTLE(e);
   if (path == HFDTD_PATH_FUSED) return launch_fused_step(e, src, n);
   if (path == HFDTD_PATH_MARCH) return launch_march_step(e, src, n);
   if (path != HFDTD_PATH_SPLIT) return fail(HFDTD_EINVAL, "unknown step path");
@@ -846,12 +854,14 @@ int hfdtd_step_pair(hfdtd_engine *h, const hfdtd_source *src_a, int na, const hf
 
 int hfdtd_padded_begin(hfdtd_engine *h) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  int rc = flush_gather(&h->s);
+  if (rc) return rc;
   return begin_padded_run(&h->s);
 }
 
 int hfdtd_padded_end(hfdtd_engine *h) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
-  return end_padded_run(&h->s);
+  return settle(&h->s);
 }
 
 int hfdtd_padded_live(const hfdtd_engine *h, int *live) {
@@ -914,6 +924,8 @@ int hfdtd_set_pair_planes(hfdtd_engine *h, int planes_i) {
 int hfdtd_synchronize(hfdtd_engine *h) {
   if (!h) return fail(HFDTD_EINVAL, "NULL argument");
   HF_HIP(hipSetDevice(h->s.device));
+  int rc = flush_gather(&h->s);
+  if (rc) return rc;
   HF_HIP(hipStreamSynchronize(h->s.stream));
   return HFDTD_OK;
 }
@@ -981,6 +993,16 @@ int hfdtd_set_march_width(hfdtd_engine *h, int width) {
   if (width != 0 && width != 64 && width != 32)
     return fail(HFDTD_EINVAL, "march width must be 64 or 32 (0 = 64)");
   h->s.march_width = width ? width : 64;
+  return HFDTD_OK;
+}
+
+int hfdtd_set_overlap(hfdtd_engine *h, int on) {
+  if (!h) return fail(HFDTD_EINVAL, "NULL argument");
+  if (on != 0 && on != 1) return fail(HFDTD_EINVAL, "on must be 0 or 1");
+  HF_HIP(hipSetDevice(h->s.device));
+  int rc = flush_gather(&h->s);
+  if (rc) return rc;
+  h->s.fuse_gather = on != 0;
   return HFDTD_OK;
 }
 

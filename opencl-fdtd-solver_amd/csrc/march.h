@@ -36,6 +36,7 @@
 #include <type_traits>
 
 #include "fdtd_core.h"
+#include "n2f_gather.h"
 
 namespace hfdtd {
 
@@ -1002,7 +1003,16 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
                                                          const void *__restrict__ mat,
                                                          const float *__restrict__ mtab,
                                                          int nmat, double cscale,
-                                                         const ZCap zc) {
+                                                         const ZCap zc, int nsweep,
+                                                         const GatherTail gt) {
+  if constexpr (sizeof(T) == 4 && !BIG) {
+    // blocks past the sweep's: a held Huygens-box gather of the step before
+    // (it reads s0 and the other record, which this sweep does not write)
+    if ((int)blockIdx.x >= nsweep) {
+      gather_block(gt, (int)blockIdx.x - nsweep, (int)threadIdx.x);
+      return;
+    }
+  }
   __shared__ MarchLds<T, RW, W> L;
   __shared__ Src<T> lsrc[kMarchSources];
   __shared__ float ltab[CEM == kCeIndex ? kMatMax * kMatMax : 1];
@@ -1013,7 +1023,7 @@ __global__ __launch_bounds__(kMarchThreads, sizeof(T) == 4 && RW == 1 ? 4 : 2) v
   if constexpr (CEM == kCeIndex) {
     for (int t = threadIdx.x; t < nmat * nmat; t += kMarchThreads) ltab[t] = mtab[t];
   }
-  const int B = xcd_runs(blockIdx.x, gridDim.x, xcd, run);
+  const int B = xcd_runs(blockIdx.x, nsweep, xcd, run);
   int k0, j0, i0, i1;
   march_tile_of<RW, W>(g, B, planes, tiles_k, tiles_j, ch, run, k0, j0, i0, i1);
   int si0 = g.nx, si1 = 0;

@@ -39,7 +39,22 @@ int march_rw(Engine *e, const SrcList<T> &sl) {
   MarchChunks ch;
   ch.split = e->npml > 0 && e->nx > 2 * e->npml ? 1 : 0;
   ch.inner = (int)ceil_div(ch.split ? e->nx - 2 * e->npml : e->nx, planes);
-  const dim3 grid(tiles_k * tiles_j * (ch.inner + 2 * ch.split));
+  // a held Huygens-box gather of the step before runs in blocks after the
+  // sweep's (FP32 slabs below 4 GB; otherwise on its own, first)
+  GatherTail gt;
+  int gblocks = 0;
+  if (e->pgather && e->pgather->held) {
+    if constexpr (sizeof(T) == 4 && !BIG) {
+      gt = *e->pgather;
+      gblocks = gt.blocks();
+      e->pgather->held = false;
+    } else {
+      int rg = flush_gather(e);
+      if (rg) return rg;
+    }
+  }
+  const int nsweep = tiles_k * tiles_j * (ch.inner + 2 * ch.split);
+  const dim3 grid(nsweep + gblocks);
   MarchSources<T> ms;
   const bool inside = sl.n <= kMarchSources;
   ms.n = inside ? sl.n : 0;
@@ -49,7 +64,7 @@ int march_rw(Engine *e, const SrcList<T> &sl) {
   const ZRec &z = e->zrec;
   const bool record = sizeof(T) == 4 && !BIG && z.owner && z.buf;
   if (record) {
-    zc.buf = z.buf;
+    zc.buf = z.half(e->cur ^ 1);  // the record of the slab this sweep writes
     zc.kf0 = z.kf[0];
     zc.kf1 = z.kf[1];
     zc.i0 = z.i0;
@@ -70,7 +85,7 @@ int march_rw(Engine *e, const SrcList<T> &sl) {
                         CEM == kCeIndex ? (const void *)e->mat
                                         : CEM == kCeEps ? (const void *)e->epsv : nullptr,
                         CEM == kCeIndex ? (const float *)e->mtab : nullptr,
-                        CEM == kCeIndex ? e->nmat : 0, e->dt / kEPS0, zc);
+                        CEM == kCeIndex ? e->nmat : 0, e->dt / kEPS0, zc, nsweep, gt);
   HF_HIP(hipGetLastError());
   // the record holds this step's values unless the fix-up kernel below
   // changes E on source cells afterwards

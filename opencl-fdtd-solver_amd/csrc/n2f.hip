@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "common.h"
+#include "n2f_gather.h"
 
 struct hfdtd_n2f {
   hfdtd::Engine *e = nullptr;
@@ -40,98 +41,15 @@ namespace hfdtd {
 
 constexpr int kMaxFreqPerLaunch = 16;
 
-struct Box {
-  int ix0, ix1, iy0, iy1, iz0, iz1;
-  int off[6];
-  int n_face;
-};
 
 template <typename A>
 struct PhaseArg {
   A v[kMaxFreqPerLaunch][4];  // e.re e.im h.re h.im
 };
 
-// Packed sample -> face id and absolute (i, j, k) (dft_farfield.cl:141-164).
-__device__ __forceinline__ int decode_face(const Box &b, int s, int &i, int &j, int &k) {
-  const int nyf = b.iy1 - b.iy0 + 1, nzf = b.iz1 - b.iz0 + 1;
-  int f, loc;
-  if (s < b.off[2]) {
-    f = s < b.off[1] ? 0 : 1;
-    loc = s - b.off[f];
-    i = f == 0 ? b.ix0 : b.ix1;
-    j = b.iy0 + loc / nzf;
-    k = b.iz0 + loc % nzf;
-  } else if (s < b.off[4]) {
-    f = s < b.off[3] ? 2 : 3;
-    loc = s - b.off[f];
-    i = b.ix0 + loc / nzf;
-    j = f == 2 ? b.iy0 : b.iy1;
-    k = b.iz0 + loc % nzf;
-  } else {
-    f = s < b.off[5] ? 4 : 5;
-    loc = s - b.off[f];
-    i = b.ix0 + loc / nyf;
-    j = b.iy0 + loc % nyf;
-    k = f == 4 ? b.iz0 : b.iz1;
-  }
-  return f;
-}
 
-struct Dims3 {
-  int nx, ny, nz;
-  int sk;  // elements between j rows of the field volumes (nz, or a row-padded run's)
-};
 
-template <typename T>
-__device__ __forceinline__ T avg2(T a, T b, bool ok) {
-  return ok ? T(0.5) * (a + b) : a;
-}
 
-template <typename T>
-__device__ __forceinline__ T avg4(T a, T b, T c, T d) {
-  return T(0.25) * (a + b + c + d);
-}
-
-// Tangential E/H of one face sample co-located at the face centre
-// (dft_farfield.cl:171-227): E as 2-point, H as 4-point averages with
-// clamped neighbours.  c[] receives the component ids (0..5 = Ex..Hz).
-template <typename T>
-__device__ __forceinline__ void colocate_impl(const Dims3 &g, int f, int i, int j, int k,
-                                              const T *Ex, const T *Ey, const T *Ez,
-                                              const T *Hx, const T *Hy, const T *Hz, T v[4],
-                                              int c[4]) {
-  auto at = [&](const T *F, int a, int b, int cc) {
-    return F[((size_t)a * g.ny + b) * g.sk + cc];
-  };
-  if (f <= 1) {
-    const int k1 = k + 1 < g.nz ? k + 1 : k;
-    const int j1 = j + 1 < g.ny ? j + 1 : j;
-    const int il = i > 0 ? i - 1 : i;
-    v[0] = avg2(at(Ey, i, j, k), at(Ey, i, j, k1), k1 != k);
-    v[1] = avg2(at(Ez, i, j, k), at(Ez, i, j1, k), j1 != j);
-    v[2] = avg4(at(Hy, i, j, k), at(Hy, il, j, k), at(Hy, i, j1, k), at(Hy, il, j1, k));
-    v[3] = avg4(at(Hz, i, j, k), at(Hz, il, j, k), at(Hz, i, j, k1), at(Hz, il, j, k1));
-    c[0] = 1; c[1] = 2; c[2] = 4; c[3] = 5;
-  } else if (f <= 3) {
-    const int i1 = i + 1 < g.nx ? i + 1 : i;
-    const int k1 = k + 1 < g.nz ? k + 1 : k;
-    const int jl = j > 0 ? j - 1 : j;
-    v[0] = avg2(at(Ex, i, j, k), at(Ex, i, j, k1), k1 != k);
-    v[1] = avg2(at(Ez, i, j, k), at(Ez, i1, j, k), i1 != i);
-    v[2] = avg4(at(Hx, i, j, k), at(Hx, i, jl, k), at(Hx, i1, j, k), at(Hx, i1, jl, k));
-    v[3] = avg4(at(Hz, i, j, k), at(Hz, i, jl, k), at(Hz, i, j, k1), at(Hz, i, jl, k1));
-    c[0] = 0; c[1] = 2; c[2] = 3; c[3] = 5;
-  } else {
-    const int i1 = i + 1 < g.nx ? i + 1 : i;
-    const int j1 = j + 1 < g.ny ? j + 1 : j;
-    const int kl = k > 0 ? k - 1 : k;
-    v[0] = avg2(at(Ex, i, j, k), at(Ex, i, j1, k), j1 != j);
-    v[1] = avg2(at(Ey, i, j, k), at(Ey, i1, j, k), i1 != i);
-    v[2] = avg4(at(Hx, i, j, k), at(Hx, i, j, kl), at(Hx, i, j1, k), at(Hx, i, j1, kl));
-    v[3] = avg4(at(Hy, i, j, k), at(Hy, i, j, kl), at(Hy, i1, j, k), at(Hy, i1, j, kl));
-    c[0] = 0; c[1] = 1; c[2] = 3; c[3] = 4;
-  }
-}
 
 // acc += val * phase, in A, optionally Kahan-compensated (cmp holds the
 // running compensation, sum_true = acc - cmp).
@@ -188,69 +106,8 @@ __global__ __launch_bounds__(256) void k_n2f_accumulate(Dims3 g, Box b, int nfre
 // writes the co-located tangential samples of one step into row k of the
 // buffer, k_n2f_contract folds the block into the sums.
 
-struct PhaseRowEH {
-  float e[32], h[32];  // cos, sin pairs (times dt); zero beyond 2 nfreq
-};
-
-// The engine's z-face record as the gather reads it (buf == nullptr: read
-// the z faces from the fields).
-struct ZRead {
-  const float4 *buf;
-  int i0, j0, nj;
-  long long quads;  // quads per (face, column) array
-};
-
-// Co-located z-face sample from the record: the same values and the same
-// operations as colocate_impl's z branch (the faces' k are at least 1, so
-// its k - 1 is the record's first column).
-__device__ __forceinline__ void colocate_zrec(const Dims3 &g, const ZRead &z, int f, int i, int j,
-                                              float v[4]) {
-  const int i1 = i + 1 < g.nx ? i + 1 : i;
-  const int j1 = j + 1 < g.ny ? j + 1 : j;
-  auto at = [&](int col, int a, int bj) {  // (Ex, Ey, Hx, Hy) at column k - 1 + col
-    return z.buf[(size_t)(2 * (f - 4) + col) * z.quads + (size_t)(a - z.i0) * z.nj + (bj - z.j0)];
-  };
-  const float4 c0 = at(1, i, j), cj = at(1, i, j1), ci = at(1, i1, j);
-  const float4 l0 = at(0, i, j), lj = at(0, i, j1), li = at(0, i1, j);
-  v[0] = avg2(c0.x, cj.x, j1 != j);
-  v[1] = avg2(c0.y, ci.y, i1 != i);
-  v[2] = avg4(c0.z, l0.z, cj.z, lj.z);
-  v[3] = avg4(c0.w, l0.w, ci.w, li.w);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_n2f_gather(Dims3 g, Box b, const T *__restrict__ Ex,
-                                                    const T *__restrict__ Ey,
-                                                    const T *__restrict__ Ez,
-                                                    const T *__restrict__ Hx,
-                                                    const T *__restrict__ Hy,
-                                                    const T *__restrict__ Hz,
-                                                    float *__restrict__ row, long long pe,
-                                                    PhaseRowEH ph, float *__restrict__ we,
-                                                    float *__restrict__ wh, ZRead zr) {
-  if (blockIdx.x == 0 && threadIdx.x < 32) {
-    we[threadIdx.x] = ph.e[threadIdx.x];
-    wh[threadIdx.x] = ph.h[threadIdx.x];
-  }
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= b.n_face) return;
-  int i, j, k;
-  const int f = decode_face(b, s, i, j, k);
-  T v[4];
-  int c[4];
-  if constexpr (sizeof(T) == 4) {
-    if (zr.buf && f >= 4) {
-      colocate_zrec(g, zr, f, i, j, v);
-    } else {
-      colocate_impl<T>(g, f, i, j, k, Ex, Ey, Ez, Hx, Hy, Hz, v, c);
-    }
-  } else {
-    colocate_impl<T>(g, f, i, j, k, Ex, Ey, Ez, Hx, Hy, Hz, v, c);
-  }
-  row[2 * s] = (float)v[0];
-  row[2 * s + 1] = (float)v[1];
-  row[pe + 2 * s] = (float)v[2];
-  row[pe + 2 * s + 1] = (float)v[3];
+__global__ __launch_bounds__(256) void k_n2f_gather(const GatherTail t) {
+  gather_block(t, blockIdx.x, threadIdx.x);
 }
 
 typedef float n2f_f32x16 __attribute__((ext_vector_type(16)));
@@ -564,51 +421,97 @@ static int accumulate_t(hfdtd_n2f *m, const double *phase) {
 }
 
 template <typename A, bool KAHAN>
-static int contract_t(hfdtd_n2f *m) {
+static int contract_t(hfdtd_n2f *m, hipStream_t st) {
   const long long waves = m->ppad / 32;
   hipLaunchKernelGGL((k_n2f_contract<A, KAHAN>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0,
-                     m->e->stream, box_of(m), m->buf, m->ppad, m->pe, m->we, m->wh, m->kfill,
-                     m->nfreq, (A *)m->acc, (A *)m->cmp);
+                     st, box_of(m), m->buf, m->ppad, m->pe, m->we, m->wh, m->kfill, m->nfreq,
+                     (A *)m->acc, (A *)m->cmp);
   HF_HIP(hipGetLastError());
   m->kfill = 0;
   return HFDTD_OK;
 }
 
-// Fold a partially filled block into the sums (before anything reads them).
-static int n2f_flush(hfdtd_n2f *m) {
-  if (m->K == 0 || m->kfill == 0) return HFDTD_OK;
-  HF_HIP(hipSetDevice(m->e->device));
-  if (m->acc_mode == HFDTD_ACC_FP64) return contract_t<double, false>(m);
-  if (m->acc_mode == HFDTD_ACC_KAHAN) return contract_t<float, true>(m);
-  return contract_t<float, false>(m);
+// Fold the buffered rows into the sums, on the stream that wrote the rows.
+static int contract(hfdtd_n2f *m, hipStream_t st) {
+  if (m->acc_mode == HFDTD_ACC_FP64) return contract_t<double, false>(m, st);
+  if (m->acc_mode == HFDTD_ACC_KAHAN) return contract_t<float, true>(m, st);
+  return contract_t<float, false>(m, st);
 }
 
+static int launch_gather(Engine *e, const GatherTail &t) {
+  hipLaunchKernelGGL(k_n2f_gather, dim3(t.blocks()), dim3(256), 0, e->stream, t);
+  HF_HIP(hipGetLastError());
+  return HFDTD_OK;
+}
+
+// Fold a partially filled block into the sums (before anything reads them);
+// a held gather first writes its row.
+static int n2f_flush(hfdtd_n2f *m) {
+  HF_HIP(hipSetDevice(m->e->device));
+  int rc = flush_gather(m->e);
+  if (rc) return rc;
+  if (m->K == 0 || m->kfill == 0) return HFDTD_OK;
+  return contract(m, m->e->stream);
+}
+
+// Drop a monitor's buffers: nothing may still run on them.
+static int quiesce(Engine *e) {
+  HF_HIP(hipSetDevice(e->device));
+  int rc = flush_gather(e);
+  if (rc) return rc;
+  HF_HIP(hipStreamSynchronize(e->stream));
+  return HFDTD_OK;
+}
+
+// One step's co-located samples into row kfill, and the contraction when
+// the block is full.  With the engine's fuse_gather on, a gather that does
+// not fill the block is held: the next march sweep runs it in blocks after
+// its own (it reads state[cur] and record cur, which that sweep only reads),
+// and any other call runs it first on its own.
 static int gather_step(hfdtd_n2f *m, const double *phase) {
   Engine *e = m->e;  // the field volumes as the engine holds them (a row-padded run's too)
-  PhaseRowEH ph;
+  int rc = flush_gather(e);  // one held at a time (another monitor's, say)
+  if (rc) return rc;
+  GatherTail t;
   for (int q = 0; q < 32; ++q) {
     const int fr = q >> 1, c = q & 1;
-    ph.e[q] = fr < m->nfreq ? (float)phase[fr * 4 + c] : 0.0f;
-    ph.h[q] = fr < m->nfreq ? (float)phase[fr * 4 + 2 + c] : 0.0f;
+    t.ph.e[q] = fr < m->nfreq ? (float)phase[fr * 4 + c] : 0.0f;
+    t.ph.h[q] = fr < m->nfreq ? (float)phase[fr * 4 + 2 + c] : 0.0f;
   }
-  const Dims3 g{e->nx, e->ny, e->nz, e->sk};
-  ZRead zr{};
+  t.g = Dims3{e->nx, e->ny, e->nz, e->sk};
+  t.b = box_of(m);
+  for (int c = 0; c < 6; ++c) t.f[c] = (const float *)e->field(c);
+  t.row = m->buf + (size_t)m->kfill * m->ppad;
+  t.pe = m->pe;
+  t.we = m->we + (size_t)m->kfill * 32;
+  t.wh = m->wh + (size_t)m->kfill * 32;
+  t.zr = ZRead{};
   if (e->zrec_valid(m)) {  // the last step recorded this box's z faces
-    zr.buf = (const float4 *)e->zrec.buf;
-    zr.i0 = e->zrec.i0;
-    zr.j0 = e->zrec.j0;
-    zr.nj = e->zrec.j1 - e->zrec.j0 + 1;
-    zr.quads = (long long)e->zrec.quads();
+    t.zr.buf = (const float4 *)e->zrec.half(e->cur);
+    t.zr.i0 = e->zrec.i0;
+    t.zr.j0 = e->zrec.j0;
+    t.zr.nj = e->zrec.j1 - e->zrec.j0 + 1;
+    t.zr.quads = (long long)e->zrec.quads();
   }
-  hipLaunchKernelGGL(k_n2f_gather<float>, dim3(ceil_div(m->n_face, 256)), dim3(256), 0,
-                     e->stream, g, box_of(m), (const float *)e->field(0),
-                     (const float *)e->field(1), (const float *)e->field(2),
-                     (const float *)e->field(3), (const float *)e->field(4),
-                     (const float *)e->field(5), m->buf + (size_t)m->kfill * m->ppad, m->pe, ph,
-                     m->we + (size_t)m->kfill * 32, m->wh + (size_t)m->kfill * 32, zr);
-  HF_HIP(hipGetLastError());
-  if (++m->kfill == m->K) return n2f_flush(m);
-  return HFDTD_OK;
+  if (++m->kfill < m->K && e->fuse_gather && !e->view_shadow) {
+    if (!e->pgather) e->pgather = new GatherTail();
+    *e->pgather = t;
+    e->pgather->held = true;
+    return HFDTD_OK;
+  }
+  if ((rc = launch_gather(e, t))) return rc;
+  return m->kfill == m->K ? contract(m, e->stream) : HFDTD_OK;
+}
+
+int flush_gather(Engine *e) {
+  if (!e->pgather || !e->pgather->held) return HFDTD_OK;
+  e->pgather->held = false;
+  return launch_gather(e, *e->pgather);
+}
+
+void free_gather(Engine *e) {
+  delete e->pgather;
+  e->pgather = nullptr;
 }
 
 // The engine's z-face record for this monitor's box: asked for with K-step
@@ -629,7 +532,7 @@ static void zrec_acquire(hfdtd_n2f *m) {
   z.j1 = std::min(b[3] + 1, e->ny - 1);
   z.bytes = 2 * 2 * z.quads() * 4 * sizeof(float);
   if (z.bytes >= 0x40000000ull) return;  // the sweep's drop offsets need a record below 1 GB
-  if (hipMalloc((void **)&z.buf, z.bytes) != hipSuccess) {
+  if (hipMalloc((void **)&z.buf, 2 * z.bytes) != hipSuccess) {  // one record per slab parity
     (void)hipGetLastError();
     return;
   }
@@ -641,12 +544,13 @@ static void zrec_acquire(hfdtd_n2f *m) {
 static void zrec_release(hfdtd_n2f *m) {
   ZRec &z = m->e->zrec;
   if (z.owner != m) return;
-  (void)hipStreamSynchronize(m->e->stream);
+  (void)quiesce(m->e);
   if (z.buf) (void)hipFree(z.buf);
   z = ZRec{};
 }
 
 static void free_buffering(hfdtd_n2f *m) {
+  (void)quiesce(m->e);
   zrec_release(m);
   for (float *p : {m->buf, m->we, m->wh})
     if (p) (void)hipFree(p);
@@ -780,8 +684,7 @@ int hfdtd_n2f_create(hfdtd_engine *h, const int box[6], int nfreq, int acc_mode,
 
 int hfdtd_n2f_destroy(hfdtd_n2f *m) {
   if (!m) return HFDTD_OK;
-  (void)hipSetDevice(m->e->device);
-  (void)hipStreamSynchronize(m->e->stream);
+  (void)quiesce(m->e);
   if (m->acc) (void)hipFree(m->acc);
   if (m->cmp) (void)hipFree(m->cmp);
   if (m->snap) (void)hipFree(m->snap);

@@ -103,6 +103,7 @@ EXPORTS = {
     "hfdtd_ce_source": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hfdtd_set_march_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "hfdtd_set_march_addressing": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfdtd_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_set_march_width": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "hfdtd_timing_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),

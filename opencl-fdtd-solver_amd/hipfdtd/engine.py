@@ -643,6 +643,13 @@ class HIPFDTD(SourceMonitorMixin):
         _lib.check(self._lib.hfdtd_set_march_addressing(self._handle, int(bool(per_volume))),
                    "set_march_addressing")  # fmt: skip
 
+    def set_overlap(self, on: bool) -> None:
+        """True (default): a K-buffered near-to-far monitor's per-step gather
+        is held and run by the next march sweep's launch, in blocks after the
+        sweep's own (one launch fewer per step); False: each gather is
+        launched at once.  The results are the same."""
+        _lib.check(self._lib.hfdtd_set_overlap(self._handle, int(bool(on))), "set_overlap")
+
     def material_count(self) -> int:
         """Materials of the march sweep's material index (0: the sweep reads
         the ce volumes).  Lossless FP32 grids with at most 48 distinct εr
