@@ -9,7 +9,9 @@ this on and off the results must be equal bit for bit (face DFTs, far
 fields, probe DFT, fields): over odd and even step counts, contractions
 every 2 and 4 steps, two monitors gathering in one step, far-field and field
 reads between runs, field writes, switches to the split path, a row-padded
-run, and a monitor released between runs.
+run, a monitor released between runs, per-volume addressing (whose sweep
+runs a held gather on its own first) and a plane DFT monitor beside the box
+(whose per-step call does the same).
 """
 
 from __future__ import annotations
@@ -27,6 +29,7 @@ PTS = [(0.3, 0.2, 1.0), (0.0, 0.0, -1.0), (1.0, 0.5, 0.1)]
 
 def hip():
     import hipfdtd
+    import hipfdtd.dft  # noqa: F401
     import hipfdtd.probes  # noqa: F401
 
     return hipfdtd
@@ -42,6 +45,8 @@ def _run(case, overlap):
         eps[lo[0]:lo[0] + 8, lo[1]:lo[1] + 8, lo[2]:lo[2] + 8] = float(rng.uniform(1.5, 4))
     sim.set_epsilon(eps)
     sim.set_row_padding(case == "padded")
+    if case == "pervolume":  # the sweep for slabs of 4 GB and more: a held gather runs first
+        sim.set_march_addressing(True)
     sim.set_overlap(overlap)
     lo, hi = (9, 8, 12), (34, 37, 52)
     c = [0.5 * (a + b) * sim.dl for a, b in zip(lo, hi)]
@@ -52,6 +57,9 @@ def _run(case, overlap):
     assert a.records_zfaces and not b.records_zfaces
     pr = hip().probes.HIPPointProbe(sim, (20, 21, 30), "Ez", freqs)
     extra = hip().HIPNear2FarMonitor(sim, c, s, freqs=freqs[:1], buffer_steps=2)
+    # a plane DFT's per-step call runs a held gather before its own
+    pl = (hip().dft.HIPPlaneDFTMonitor(sim, "z", 30, ("Ex", "Hy"), freqs[:2])
+          if case == "plane" else None)  # fmt: skip
     nx, ny, nz = shape
     sim.add_source(lambda f: f.add_source_Jx(nz // 2, fo.gaussian_sine(f.t, 3e10, 1.5e10),
                                              nx // 3, 2 * nx // 3, ny // 3, 2 * ny // 3))  # fmt: skip
@@ -82,6 +90,8 @@ def _run(case, overlap):
     out["fa"] = a.get_farfields(PTS, 0)
     out["fb"] = b.get_farfields(PTS, 2)
     out["probe"] = pr.dft()
+    if pl is not None:
+        out["plane"] = pl.dft()
     if case != "drop":
         out["extra"] = extra.read_dft("Hy", 0)
     out.update({"f" + n: v for n, v in sim.fields().items()})
@@ -89,7 +99,8 @@ def _run(case, overlap):
     return out
 
 
-@pytest.mark.parametrize("case", ["plain", "midread", "write", "switch", "padded", "drop"])
+@pytest.mark.parametrize("case", ["plain", "midread", "write", "switch", "padded", "drop",
+                                  "pervolume", "plane"])
 def test_overlapped_gather_matches_serial(case):
     print(f"[overlap] {case}", flush=True)
     want = _run(case, False)
