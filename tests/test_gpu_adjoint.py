@@ -128,6 +128,25 @@ def test_checkpoint_length_does_not_change_the_gradient():
     assert np.array_equal(grads[0], grads[2])
 
 
+def test_forward_path_does_not_change_the_gradient():
+    """The forward and recompute steps on the split, fused and march paths
+    give bit-identical fields, so the gradient is the same on each."""
+    eps_bg, design = small_eps(6)
+    A = adj()
+    grads = {}
+    for path in ("split", "fused", "march"):
+        print(f"[adjoint-path] {path}", flush=True)
+        pb = A.ProbeDFTAdjoint(SHAPE, DL, NPML, BOX, SOURCES, PROBES, FREQS, STEPS,
+                               eps_background=eps_bg, checkpoint=7)  # fmt: skip
+        pb.fdtd.set_path(path)
+        F = pb.forward(design)
+        grads[path] = (F, pb.backward(2.0 * F))
+        pb.close()
+    for path in ("fused", "march"):
+        np.testing.assert_array_equal(grads[path][0], grads["split"][0], err_msg=path)
+        np.testing.assert_array_equal(grads[path][1], grads["split"][1], err_msg=path)
+
+
 def test_autograd_function_matches_value_and_grad():
     import torch
 
